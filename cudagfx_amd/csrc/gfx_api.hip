@@ -1,0 +1,512 @@
+// gfx_api.hip -- the C ABI (include/cudagfx.h): device state of the single
+// gauge-fixing instance, host<->device transfers and the overrelaxation
+// driver loop (overrelax.cu:677-823) on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/cudagfx.h"
+#include "gfx_internal.h"
+#include "gfx_lattice.h"
+#include "gfx_state.h"
+
+using namespace gfx;
+
+extern "C" {
+int T = 0, L = 0, LX = 0, LY = 0, LZ = 0, VOLUME = 0;
+double FUNC = 0.0, DADA = 0.0, maxDADA = 0.0, PLAQ = 0.0, BETA = 0.0;
+}
+
+namespace gfx {
+State g_st;
+
+const char* err_text(int code) {
+  switch (code) {
+    case GFX_OK: return "ok";
+    case GFX_ERR_GEOMETRY: return "lattice geometry not supported (all extents must be even)";
+    case GFX_ERR_ALLOC: return "device memory allocation failed";
+    case GFX_ERR_NOCONV: return "gauge condition not reached";
+    case GFX_ERR_WRITE: return "write failed";
+    case GFX_ERR_STATE: return "library not initialised for this call";
+    case GFX_ERR_DEVICE: return "HIP runtime error";
+    case GFX_ERR_MTDAT: return "MersenneTwister.dat missing or unreadable";
+    default: return "unknown error";
+  }
+}
+
+int hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    fprintf(stderr, "cudagfx: %s: %s\n", what, hipGetErrorString(e));
+    return GFX_ERR_DEVICE;
+  }
+  return GFX_OK;
+}
+
+Geom make_geom(int l0, int l1, int l2, int l3) {
+  Geom g;
+  g.L[0] = l0;
+  g.L[1] = l1;
+  g.L[2] = l2;
+  g.L[3] = l3;
+  g.h0 = l0 / 2;
+  g.nh = (int)((long)l0 * l1 * l2 * l3 / 2);
+  g.st[0] = 1;
+  g.st[1] = g.h0;
+  g.st[2] = g.h0 * l1;
+  g.st[3] = g.h0 * l1 * l2;
+  // planes 64-element aligned; a power-of-two plane distance puts the six
+  // planes of a site on the same HBM channel, so break it
+  int s = (g.nh + 63) & ~63;
+  if ((s & (s - 1)) == 0) s += 64;
+  g.stride = s;
+  g.dh0 = make_fastdiv(g.h0);
+  g.dL1 = make_fastdiv(l1);
+  g.dL2 = make_fastdiv(l2);
+  return g;
+}
+
+bool geometry_ok(int l0, int l1, int l2, int l3) {
+  const int d[4] = {l0, l1, l2, l3};
+  long v = 1;
+  for (int k = 0; k < 4; k++) {
+    if (d[k] < 2 || (d[k] & 1)) return false;
+    v *= d[k];
+  }
+  return v <= (1L << 30);
+}
+
+int ensure_common(int need_links32) {
+  State& s = g_st;
+  if (!s.stream) {
+    if (hip_check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "stream")) return GFX_ERR_DEVICE;
+    hipEventCreate(&s.ev0);
+    hipEventCreate(&s.ev1);
+  }
+  (void)need_links32;
+  return GFX_OK;
+}
+
+static int dev_alloc(void** p, size_t bytes, const char* what) {
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    fprintf(stderr, "cudagfx: allocation of %s (%zu bytes) failed\n", what, bytes);
+    *p = nullptr;
+    return GFX_ERR_ALLOC;
+  }
+  return GFX_OK;
+}
+
+size_t field_bytes64(const Geom& g) { return (size_t)6 * g.stride * sizeof(double2); }
+size_t field_bytes32(const Geom& g) { return (size_t)3 * g.stride * sizeof(float4); }
+
+int alloc_reduction(State& s) {
+  const int nb = functional_blocks(s.geo.nh) + 8;
+  // partials: max over Landau (1 segment) and Coulomb (T segments)
+  const int nbc = functional_blocks(s.geo.nh / s.geo.L[3]) * s.geo.L[3] + 8;
+  s.npartials = (nb > nbc ? nb : nbc);
+  if (dev_alloc((void**)&s.partials, sizeof(double) * 3 * s.npartials, "partials")) return GFX_ERR_ALLOC;
+  if (dev_alloc((void**)&s.results, sizeof(double) * 4 * (s.geo.L[3] + 1), "results")) return GFX_ERR_ALLOC;
+  if (hipHostMalloc((void**)&s.h_results, sizeof(double) * 4 * (s.geo.L[3] + 1), 0) != hipSuccess) return GFX_ERR_ALLOC;
+  if (dev_alloc((void**)&s.active, (size_t)s.geo.L[3], "active")) return GFX_ERR_ALLOC;
+  return GFX_OK;
+}
+
+// staging chunk for AoS <-> SoA conversions (sites per chunk)
+static const int CHUNK = 32768;
+
+int ensure_staging(State& s) {
+  if (s.staging) return GFX_OK;
+  return dev_alloc((void**)&s.staging, (size_t)CHUNK * 4 * 18 * sizeof(double), "staging");
+}
+
+// host AoS su3 (nmat per site) -> device FP64 fields
+int upload_aos(State& s, const double* host, double2* dst, int nmat) {
+  if (ensure_staging(s)) return GFX_ERR_ALLOC;
+  const int V = 2 * s.geo.nh;
+  for (int s0 = 0; s0 < V; s0 += CHUNK) {
+    const int ns = (V - s0 < CHUNK) ? V - s0 : CHUNK;
+    const size_t bytes = (size_t)ns * nmat * 18 * sizeof(double);
+    if (hip_check(hipMemcpyAsync(s.staging, host + (size_t)s0 * nmat * 18, bytes, hipMemcpyHostToDevice, s.stream), "H2D")) return GFX_ERR_DEVICE;
+    launch_pack_f64(s.staging, dst, s.geo, s0, ns, nmat, s.stream);
+  }
+  return hip_check(hipStreamSynchronize(s.stream), "upload");
+}
+
+int download_aos(State& s, const double2* src, double* host, int nmat) {
+  if (ensure_staging(s)) return GFX_ERR_ALLOC;
+  const int V = 2 * s.geo.nh;
+  for (int s0 = 0; s0 < V; s0 += CHUNK) {
+    const int ns = (V - s0 < CHUNK) ? V - s0 : CHUNK;
+    const size_t bytes = (size_t)ns * nmat * 18 * sizeof(double);
+    launch_unpack_f64(src, s.staging, s.geo, s0, ns, nmat, s.stream);
+    if (hip_check(hipMemcpyAsync(host + (size_t)s0 * nmat * 18, s.staging, bytes, hipMemcpyDeviceToHost, s.stream), "D2H")) return GFX_ERR_DEVICE;
+    if (hip_check(hipStreamSynchronize(s.stream), "download")) return GFX_ERR_DEVICE;
+  }
+  return GFX_OK;
+}
+
+int ensure_f32_links(State& s) {
+  if (s.U32 && s.U32_valid) return GFX_OK;
+  if (!s.U32) {
+    if (dev_alloc((void**)&s.U32, 8 * field_bytes32(s.geo), "FP32 links")) return GFX_ERR_ALLOC;
+    if (dev_alloc((void**)&s.g32, 2 * field_bytes32(s.geo), "FP32 trafo")) return GFX_ERR_ALLOC;
+  }
+  launch_f64_to_f32(s.U64, s.U32, s.geo, 8, s.stream);
+  s.U32_valid = true;
+  return GFX_OK;
+}
+
+void free_all(State& s) {
+  if (s.stream) hipStreamSynchronize(s.stream);
+  hipFree(s.U64);
+  hipFree(s.g64);
+  hipFree(s.U32);
+  hipFree(s.g32);
+  hipFree(s.partials);
+  hipFree(s.results);
+  hipFree(s.active);
+  hipFree(s.staging);
+  if (s.h_results) hipHostFree(s.h_results);
+  s.U64 = nullptr;
+  s.g64 = nullptr;
+  s.U32 = nullptr;
+  s.g32 = nullptr;
+  s.partials = nullptr;
+  s.results = nullptr;
+  s.active = nullptr;
+  s.staging = nullptr;
+  s.h_results = nullptr;
+  s.U32_valid = false;
+  s.init = 0;
+}
+
+int set_lattice_from_globals(State& s) {
+  if (!geometry_ok(LX, LY, LZ, T)) {
+    printf("Error: lattice %d x %d x %d x %d not supported (extents must be even). Aborting...\n", LX, LY, LZ, T);
+    return GFX_ERR_GEOMETRY;
+  }
+  VOLUME = LX * LY * LZ * T;
+  s.geo = make_geom(LX, LY, LZ, T);
+  return GFX_OK;
+}
+
+// one functional evaluation into s.h_results[0..2] for nseg segments
+int functional_now(State& s, bool fp32, int ndir, int nseg, double eps) {
+  if (fp32)
+    launch_functional_f32(s.U32, s.g32, s.geo, ndir, nseg, s.partials, eps, s.results, s.stream);
+  else
+    launch_functional_f64(s.U64, s.g64, s.geo, ndir, nseg, s.partials, eps, s.results, s.stream);
+  if (hip_check(hipMemcpyAsync(s.h_results, s.results, sizeof(double) * 4 * nseg, hipMemcpyDeviceToHost, s.stream), "D2H results")) return GFX_ERR_DEVICE;
+  return hip_check(hipStreamSynchronize(s.stream), "functional");
+}
+
+}  // namespace gfx
+
+// ======================================================================
+extern "C" {
+
+const char* gfx_error_string(int code) { return err_text(code); }
+
+int init_gaugefixing(gfx_su3* gf, gfx_su3* trafo) {
+  State& s = g_st;
+  if (s.init) free_all(s);
+  int rc = set_lattice_from_globals(s);
+  if (rc) return rc;
+  if ((rc = ensure_common(0))) return rc;
+  if (dev_alloc((void**)&s.U64, 8 * field_bytes64(s.geo), "gauge field")) return GFX_ERR_ALLOC;
+  printf("Allocated gauge field on device\n");
+  if (dev_alloc((void**)&s.g64, 2 * field_bytes64(s.geo), "trafo field")) return GFX_ERR_ALLOC;
+  printf("Allocated trafo field 1 on device\n");
+  if ((rc = alloc_reduction(s))) return rc;
+  if ((rc = upload_aos(s, (const double*)gf, s.U64, 4))) return rc;
+  if ((rc = upload_aos(s, (const double*)trafo, s.g64, 1))) return rc;
+  s.U32_valid = false;
+  s.init = 1;
+  return GFX_OK;
+}
+
+void finalize_gaugefixing(void) { free_all(g_st); }
+
+int init_thermalization(gfx_su3* gf) {
+  State& s = g_st;
+  if (s.init) free_all(s);
+  int rc = set_lattice_from_globals(s);
+  if (rc) return rc;
+  if ((rc = ensure_common(0))) return rc;
+  if (dev_alloc((void**)&s.U64, 8 * field_bytes64(s.geo), "gauge field")) return GFX_ERR_ALLOC;
+  printf("Allocated gauge field on device\n");
+  if (dev_alloc((void**)&s.g64, 2 * field_bytes64(s.geo), "trafo field")) return GFX_ERR_ALLOC;
+  if ((rc = alloc_reduction(s))) return rc;
+  if ((rc = upload_aos(s, (const double*)gf, s.U64, 4))) return rc;
+  s.U32_valid = false;
+  s.init = 2;
+  return GFX_OK;
+}
+
+void finalize_thermalization(void) { free_all(g_st); }
+
+int gfx_set_omega(double omega) {
+  g_st.omega = omega;
+  return GFX_OK;
+}
+int gfx_set_precision(int mode, double switch_theta) {
+  if (mode != GFX_PREC_FP64 && mode != GFX_PREC_MIXED) return GFX_ERR_STATE;
+  g_st.precision = mode;
+  if (switch_theta > 0) g_st.switch_theta = switch_theta;
+  return GFX_OK;
+}
+int gfx_set_verbose(int v) {
+  g_st.verbose = v;
+  return GFX_OK;
+}
+
+double calc_functional(void) {
+  State& s = g_st;
+  if (!s.init) return -1.0;
+  if (functional_now(s, false, 4, 1, 0.0)) return -1.0;
+  FUNC = s.h_results[0];
+  DADA = s.h_results[1];
+  maxDADA = s.h_results[2];
+  return maxDADA;
+}
+
+int gfx_functional_slices(double* F, double* D, double* M) {
+  State& s = g_st;
+  if (!s.init) return GFX_ERR_STATE;
+  const int nseg = s.geo.L[3];
+  if (functional_now(s, false, 3, nseg, 0.0)) return GFX_ERR_DEVICE;
+  for (int t = 0; t < nseg; t++) {
+    if (F) F[t] = s.h_results[4 * t];
+    if (D) D[t] = s.h_results[4 * t + 1];
+    if (M) M[t] = s.h_results[4 * t + 2];
+  }
+  return GFX_OK;
+}
+
+double calc_plaquette(int outputyn) {
+  State& s = g_st;
+  if (!s.init) return -1.0;
+  launch_plaquette_f64(s.U64, s.geo, s.partials, s.results, s.stream);
+  hipMemcpyAsync(s.h_results, s.results, sizeof(double), hipMemcpyDeviceToHost, s.stream);
+  if (hip_check(hipStreamSynchronize(s.stream), "plaquette")) return -1.0;
+  const double p = s.h_results[0];
+  if (outputyn == 1) printf("PLAQ = %.16f\n", p);
+  return p;
+}
+
+int gfx_halfsweep(int parity, int ndir) {
+  State& s = g_st;
+  if (!s.init) return GFX_ERR_STATE;
+  if (s.precision == GFX_PREC_MIXED) {
+    int rc = ensure_f32_links(s);
+    if (rc) return rc;
+    launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
+    launch_sweep_f32(s.U32, s.g32, s.geo, parity, ndir, s.omega, nullptr, s.stream);
+    launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
+  } else {
+    launch_sweep_f64(s.U64, s.g64, s.geo, parity, ndir, s.omega, nullptr, s.stream);
+  }
+  return hip_check(hipStreamSynchronize(s.stream), "halfsweep");
+}
+
+int overrelax_gauge(int maxit, double eps, int checkinterval) {
+  State& s = g_st;
+  if (!s.init) return -1;
+  if (checkinterval < 1) checkinterval = 1;
+  bool fp32 = (s.precision == GFX_PREC_MIXED);
+  if (fp32) {
+    if (ensure_f32_links(s)) return -1;
+    launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
+  }
+  s.run_iters32 = 0;
+  s.run_iters64 = 0;
+  double maxdada = 0.0;
+  int i;
+  hipStreamSynchronize(s.stream);
+  const auto t0 = std::chrono::steady_clock::now();
+  auto stop_clock = [&]() {
+    hipStreamSynchronize(s.stream);
+    s.run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  };
+  for (i = 0; i < maxit; i++) {
+    if (fp32) {
+      launch_sweep_f32(s.U32, s.g32, s.geo, 0, 4, s.omega, nullptr, s.stream);
+      launch_sweep_f32(s.U32, s.g32, s.geo, 1, 4, s.omega, nullptr, s.stream);
+      s.run_iters32++;
+    } else {
+      launch_sweep_f64(s.U64, s.g64, s.geo, 0, 4, s.omega, nullptr, s.stream);
+      launch_sweep_f64(s.U64, s.g64, s.geo, 1, 4, s.omega, nullptr, s.stream);
+      s.run_iters64++;
+    }
+    if ((i % checkinterval) == 0) {
+      if (functional_now(s, fp32, 4, 1, eps)) return -1;
+      FUNC = s.h_results[0];
+      DADA = s.h_results[1];
+      maxDADA = maxdada = s.h_results[2];
+      if (s.verbose)
+        printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e\n", i, FUNC, DADA, maxdada);
+      if (fp32 && maxdada < s.switch_theta) {
+        // leave FP32: reunitarise g in FP64 and continue with FP64 sweeps
+        launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
+        fp32 = false;
+      }
+    }
+    if (maxdada < eps) {
+      if (fp32) launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
+      stop_clock();
+      if (s.verbose) {
+        printf("CONVERGENCE!\nFinal values: F = %.16e dAdA = %.16e, maxdAdA = %.16e\n", FUNC, DADA, maxDADA);
+        printf("Overrelaxation finished after %f sec\n", s.run_ms / 1000.0);
+      }
+      return i;
+    }
+  }
+  if (fp32) launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
+  stop_clock();
+  if (s.verbose) printf("Overrelaxation finished after %f sec\n", s.run_ms / 1000.0);
+  if (maxdada > eps) {
+    if (s.verbose)
+      printf("FAIL! Gauge condition not reached!\nFinal values: F = %.16e dAdA = %.16e, maxdAdA = %.16e\n", FUNC, DADA, maxDADA);
+    return -1;
+  }
+  return i;
+}
+
+int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
+  State& s = g_st;
+  if (!s.init) return GFX_ERR_STATE;
+  if (checkinterval < 1) checkinterval = 1;
+  const int nt = s.geo.L[3];
+  std::vector<unsigned char> act(nt, 1);
+  for (int t = 0; t < nt; t++) iters[t] = -1;
+  bool fp32 = (s.precision == GFX_PREC_MIXED);
+  if (fp32) {
+    if (ensure_f32_links(s)) return GFX_ERR_ALLOC;
+    launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
+  }
+  hipMemcpyAsync(s.active, act.data(), nt, hipMemcpyHostToDevice, s.stream);
+  s.run_iters32 = s.run_iters64 = 0;
+  int left = nt;
+  hipStreamSynchronize(s.stream);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < maxit && left > 0; i++) {
+    if (fp32) {
+      launch_sweep_f32(s.U32, s.g32, s.geo, 0, 3, s.omega, s.active, s.stream);
+      launch_sweep_f32(s.U32, s.g32, s.geo, 1, 3, s.omega, s.active, s.stream);
+      s.run_iters32++;
+    } else {
+      launch_sweep_f64(s.U64, s.g64, s.geo, 0, 3, s.omega, s.active, s.stream);
+      launch_sweep_f64(s.U64, s.g64, s.geo, 1, 3, s.omega, s.active, s.stream);
+      s.run_iters64++;
+    }
+    if ((i % checkinterval) == 0) {
+      if (functional_now(s, fp32, 3, nt, eps)) return GFX_ERR_DEVICE;
+      bool changed = false, all_below_switch = true;
+      for (int t = 0; t < nt; t++) {
+        if (!act[t]) continue;
+        const double m = s.h_results[4 * t + 2];
+        if (m < eps) {
+          act[t] = 0;
+          iters[t] = i;
+          left--;
+          changed = true;
+        } else if (m >= s.switch_theta) {
+          all_below_switch = false;
+        }
+      }
+      if (changed) hipMemcpyAsync(s.active, act.data(), nt, hipMemcpyHostToDevice, s.stream);
+      if (fp32 && all_below_switch) {
+        launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
+        fp32 = false;
+      }
+      if (s.verbose) {
+        double F = 0, D = 0, M = 0;
+        for (int t = 0; t < nt; t++) {
+          F += s.h_results[4 * t];
+          D += s.h_results[4 * t + 1];
+          M = fmax(M, s.h_results[4 * t + 2]);
+        }
+        printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e\t slices left = %d\n", i, F / nt, D / nt, M, left);
+      }
+    }
+  }
+  if (fp32) launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
+  hipStreamSynchronize(s.stream);
+  s.run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return left == 0 ? GFX_OK : GFX_ERR_NOCONV;
+}
+
+int get_trafo(gfx_su3* trafo) {
+  State& s = g_st;
+  if (!s.init) return GFX_ERR_STATE;
+  return download_aos(s, s.g64, (double*)trafo, 1);
+}
+int set_trafo(gfx_su3* trafo) {
+  State& s = g_st;
+  if (!s.init) return GFX_ERR_STATE;
+  return upload_aos(s, (const double*)trafo, s.g64, 1);
+}
+int get_gauge(gfx_su3* gf) {
+  State& s = g_st;
+  if (!s.init) return GFX_ERR_STATE;
+  return download_aos(s, s.U64, (double*)gf, 4);
+}
+
+int apply_trafo(gfx_su3* gf) {
+  State& s = g_st;
+  if (!s.init) return GFX_ERR_STATE;
+  if (ensure_staging(s)) return GFX_ERR_ALLOC;
+  const int V = 2 * s.geo.nh;
+  double* host = (double*)gf;
+  for (int s0 = 0; s0 < V; s0 += CHUNK) {
+    const int ns = (V - s0 < CHUNK) ? V - s0 : CHUNK;
+    const size_t bytes = (size_t)ns * 4 * 18 * sizeof(double);
+    hipMemcpyAsync(s.staging, host + (size_t)s0 * 72, bytes, hipMemcpyHostToDevice, s.stream);
+    launch_apply_trafo_aos(s.staging, s.g64, s.geo, s0, ns, s.stream);
+    hipMemcpyAsync(host + (size_t)s0 * 72, s.staging, bytes, hipMemcpyDeviceToHost, s.stream);
+    if (hip_check(hipStreamSynchronize(s.stream), "apply_trafo")) return GFX_ERR_DEVICE;
+  }
+  return GFX_OK;
+}
+
+int gfx_time_sweeps(int nsweeps, int precision, double* ms) {
+  State& s = g_st;
+  if (!s.init) return GFX_ERR_STATE;
+  const bool fp32 = (precision == GFX_PREC_MIXED);
+  if (fp32) {
+    int rc = ensure_f32_links(s);
+    if (rc) return rc;
+    launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
+  }
+  hipEventRecord(s.ev0, s.stream);
+  for (int k = 0; k < nsweeps; k++)
+    for (int p = 0; p < 2; p++) {
+      if (fp32)
+        launch_sweep_f32(s.U32, s.g32, s.geo, p, 4, s.omega, nullptr, s.stream);
+      else
+        launch_sweep_f64(s.U64, s.g64, s.geo, p, 4, s.omega, nullptr, s.stream);
+    }
+  hipEventRecord(s.ev1, s.stream);
+  if (hip_check(hipEventSynchronize(s.ev1), "timing")) return GFX_ERR_DEVICE;
+  float t = 0.f;
+  hipEventElapsedTime(&t, s.ev0, s.ev1);
+  if (fp32) {
+    launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
+    hipStreamSynchronize(s.stream);
+  }
+  *ms = (double)t / (2.0 * nsweeps);
+  return GFX_OK;
+}
+
+int gfx_last_run_stats(double* ms, int* it32, int* it64) {
+  if (ms) *ms = g_st.run_ms;
+  if (it32) *it32 = g_st.run_iters32;
+  if (it64) *it64 = g_st.run_iters64;
+  return GFX_OK;
+}
+
+}  // extern "C"

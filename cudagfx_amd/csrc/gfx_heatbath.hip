@@ -1,0 +1,288 @@
+// gfx_heatbath.hip -- stochastic updates on the checkerboard:
+//   k_sa_sweep      simulated annealing of g: Cabibbo-Marinari heatbath of
+//                   exp(Re tr(g K)/T) (simulated_annealing.cu:5-257)
+//   k_micro_sweep   microcanonical (F-preserving) reflection of g in each
+//                   SU(2) subgroup (heatbath_thermalization.cu:318-401,
+//                   applied to g)
+//   k_link_heatbath SU(3) link heatbath of the Wilson action with the
+//                   staple of heatbath_thermalization.cu:235-308
+//
+// Two SU(2) samplers:
+//   SAMPLER_REF  the reference's rule (simulated_annealing.cu:5-73): one
+//                trial with a0 = 1 - (g^2 - ln(1-u0))/(|w|^2 beta), accept if
+//                2 u1^2 <= 2 - y, otherwise alpha = w (unnormalised); the same
+//                5 numbers serve all three subgroups.  Random layout
+//                gauss[i], unif[4i+k] with i the parity index.
+//   SAMPLER_KP   Kennedy-Pendleton (Phys. Lett. B156 (1985) 393): per trial
+//                x = (g^2/2 - ln u)/(2 a), accept if v^2 <= 1 - x, a0 = 1 - 2x,
+//                a = beta*|w|; up to NT trials per subgroup, the subgroup is
+//                left unchanged if all fail (a mixture of the identity and
+//                the exact conditional, so the Boltzmann weight stays
+//                invariant).  Layout gauss[NG*i + sub*NT + t],
+//                unif[NU*i + sub*(2NT+2) + ...].
+#include <hip/hip_runtime.h>
+
+#include "gfx_lattice.h"
+#include "gfx_su3.h"
+
+namespace gfx {
+
+constexpr int HB_BLK = 256;
+constexpr int KP_NT = 4;
+constexpr int KP_NG = 3 * KP_NT;
+constexpr int KP_NU = 3 * (2 * KP_NT + 2);
+
+// quaternion product in the (a.x, a.y, b.x, b.y) component order of the
+// reference (dev_su3.cu:1385-1399); equals the product of the 2x2 matrices
+// [[q0 + i q3, q2 + i q1], [-q2 + i q1, q0 - i q3]]
+__device__ __forceinline__ void su2_mul(double r[4], const double a[4], const double b[4]) {
+  r[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+  r[1] = a[0] * b[1] + a[1] * b[0] - a[2] * b[3] + a[3] * b[2];
+  r[2] = a[0] * b[2] + a[2] * b[0] - a[3] * b[1] + a[1] * b[3];
+  r[3] = a[0] * b[3] + a[3] * b[0] - a[1] * b[2] + a[2] * b[1];
+}
+
+// simulated_annealing.cu:5-73 restated
+__device__ __forceinline__ void sample_ref(double alpha[4], const double w[4], double beta,
+                                           const float* __restrict__ gauss,
+                                           const float* __restrict__ unif, int ind) {
+  const double k = w[0] * w[0] + w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
+  const double rdet = ::rsqrt(k);
+  const double v[4] = {w[0] * rdet, w[1] * rdet, w[2] * rdet, w[3] * rdet};
+  const double kb = k * beta;
+  float a = fabsf(gauss[ind]);
+  float b = -logf(1.0f - unif[4 * ind]);
+  float y = a * a + b;
+  y = (float)((double)y / kb);
+  const float help = unif[4 * ind + 1];
+  double q[4];
+  int ret;
+  if ((2.0f * (help * help)) <= (2.0f - y)) {
+    q[0] = (double)(1.0f - y);
+    ret = 1;
+  } else {
+    q[0] = 0.0;
+    ret = 0;
+  }
+  const double norm = sqrt(1.0 - q[0] * q[0]);
+  const double ct = 2.0 * (double)unif[4 * ind + 2] - 1.0;
+  const double st = sqrt(1.0 - ct * ct);
+  const double phi = 6.2831853071795862 * (double)unif[4 * ind + 3];
+  double sp, cp;
+  sincos(phi, &sp, &cp);
+  q[1] = norm * st * cp;
+  q[2] = norm * st * sp;
+  q[3] = norm * ct;
+  if (ret == 1) {
+    su2_mul(alpha, q, v);
+  } else {
+    alpha[0] = w[0];
+    alpha[1] = w[1];
+    alpha[2] = w[2];
+    alpha[3] = w[3];
+  }
+}
+
+// Kennedy-Pendleton; returns false if every trial was rejected
+__device__ __forceinline__ bool sample_kp(double alpha[4], const double w[4], double beta,
+                                          const float* __restrict__ g, const float* __restrict__ u) {
+  const double k = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
+  if (!(k > 1e-300)) return false;
+  const double a = beta * k;
+  double a0 = 0.0;
+  bool ok = false;
+#pragma unroll
+  for (int t = 0; t < KP_NT; t++) {
+    if (!ok) {
+      const double gg = (double)g[t];
+      const double x = (0.5 * gg * gg - log((double)u[2 * t])) / (2.0 * a);
+      const double acc = (double)u[2 * t + 1];
+      if (x <= 1.0 && acc * acc <= 1.0 - x) {
+        a0 = 1.0 - 2.0 * x;
+        ok = true;
+      }
+    }
+  }
+  if (!ok) return false;
+  const double norm = sqrt(fmax(0.0, 1.0 - a0 * a0));
+  const double ct = 2.0 * (double)u[2 * KP_NT] - 1.0;
+  const double st = sqrt(fmax(0.0, 1.0 - ct * ct));
+  const double phi = 6.283185307179586 * (double)u[2 * KP_NT + 1];
+  double sp, cp;
+  sincos(phi, &sp, &cp);
+  const double q[4] = {a0, norm * st * cp, norm * st * sp, norm * ct};
+  const double v[4] = {w[0] / k, w[1] / k, w[2] / k, w[3] / k};
+  su2_mul(alpha, q, v);
+  return true;
+}
+
+template <int SAMPLER>
+__device__ __forceinline__ void cm_heatbath(M3<double>& h, const M3<double>& K, double beta,
+                                            const float* __restrict__ gauss,
+                                            const float* __restrict__ unif, int i) {
+  int sub = 0;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = a + 1; b < 3; b++) {
+      double w[4], al[4];
+      su2_subvec(h, K, a, b, w[0], w[1], w[2], w[3]);
+      if (SAMPLER == 0) {
+        sample_ref(al, w, beta, gauss, unif, i);
+        su2_left(h, a, b, al[0], al[1], al[2], al[3]);
+      } else {
+        const float* g = gauss + (size_t)KP_NG * i + sub * KP_NT;
+        const float* u = unif + (size_t)KP_NU * i + sub * (2 * KP_NT + 2);
+        if (sample_kp(al, w, beta, g, u)) su2_left(h, a, b, al[0], al[1], al[2], al[3]);
+      }
+      sub++;
+    }
+}
+
+// star K of g at site i (same as the overrelaxation sweep)
+template <int NDIR>
+__device__ __forceinline__ M3<double> star_of(const double2* __restrict__ U,
+                                              const double2* __restrict__ gop, const Geom& geo,
+                                              const Site& s, int i, int parity) {
+  const int S = geo.stride;
+  M3<double> K;
+  m_zero(K);
+#pragma unroll
+  for (int mu = 0; mu < NDIR; mu++) {
+    const int jf = nb_fwd(geo, s, i, mu);
+    const int jb = nb_bwd(geo, s, i, mu);
+    M3<double> A = load_m<double2, double>(field_ptr(U, parity * 4 + mu, S), S, i);
+    M3<double> B = load_m_dagger<double2, double>(gop, S, jf);
+    m_addmul(K, A, B);
+    A = load_m_dagger<double2, double>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb);
+    B = load_m_dagger<double2, double>(gop, S, jb);
+    m_addmul(K, A, B);
+  }
+  return K;
+}
+
+template <int SAMPLER, int NDIR>
+__global__ __launch_bounds__(HB_BLK) void k_sa_sweep(const double2* __restrict__ U,
+                                                     double2* __restrict__ g, Geom geo, int parity,
+                                                     double beta, const float* __restrict__ gauss,
+                                                     const float* __restrict__ unif) {
+  const int i = blockIdx.x * HB_BLK + threadIdx.x;
+  if (i >= geo.nh) return;
+  const Site s = site_coords(geo, i, parity);
+  const int S = geo.stride;
+  const M3<double> K = star_of<NDIR>(U, field_ptr(static_cast<const double2*>(g), 1 - parity, S),
+                                     geo, s, i, parity);
+  double2* gme = field_ptr(g, parity, S);
+  M3<double> h = load_m<double2, double>(gme, S, i);
+  cm_heatbath<SAMPLER>(h, K, beta, gauss, unif, i);
+  m_normalize(h);
+  Store<double2>::store(gme, S, i, h);
+}
+
+template <int NDIR>
+__global__ __launch_bounds__(HB_BLK) void k_micro_sweep(const double2* __restrict__ U,
+                                                        double2* __restrict__ g, Geom geo,
+                                                        int parity) {
+  const int i = blockIdx.x * HB_BLK + threadIdx.x;
+  if (i >= geo.nh) return;
+  const Site s = site_coords(geo, i, parity);
+  const int S = geo.stride;
+  const M3<double> K = star_of<NDIR>(U, field_ptr(static_cast<const double2*>(g), 1 - parity, S),
+                                     geo, s, i, parity);
+  double2* gme = field_ptr(g, parity, S);
+  M3<double> h = load_m<double2, double>(gme, S, i);
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = a + 1; b < 3; b++) {
+      double w[4];
+      su2_subvec(h, K, a, b, w[0], w[1], w[2], w[3]);
+      const double ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
+      // heatbath_thermalization.cu:318-338: alpha = 2 w0 w/|w|^2 - e0
+      if (!(fabs(ww) < 1.0e-9)) {
+        const double f = 2.0 * w[0] / ww;
+        su2_left(h, a, b, f * w[0] - 1.0, f * w[1], f * w[2], f * w[3]);
+      }
+    }
+  m_normalize(h);
+  Store<double2>::store(gme, S, i, h);
+}
+
+// staple of U_nu(x): sum_{mu != nu} U_mu(x+nu) U_nu^+(x+mu) U_mu^+(x)
+//                                 + U_mu^+(x+nu-mu) U_nu^+(x-mu) U_mu(x-mu)
+__global__ __launch_bounds__(HB_BLK) void k_link_heatbath(double2* __restrict__ U, Geom geo,
+                                                          int parity, int nu, double beta,
+                                                          const float* __restrict__ gauss,
+                                                          const float* __restrict__ unif) {
+  const int i = blockIdx.x * HB_BLK + threadIdx.x;
+  if (i >= geo.nh) return;
+  const int S = geo.stride;
+  const int p = parity, q = 1 - parity;
+  const Site s = site_coords(geo, i, p);
+  const int xpnu = nb_fwd(geo, s, i, nu);  // parity q
+  const Site snu = site_coords(geo, xpnu, q);
+  M3<double> st;
+  m_zero(st);
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) {
+    if (mu == nu) continue;
+    const int xpmu = nb_fwd(geo, s, i, mu);        // parity q
+    const int xmmu = nb_bwd(geo, s, i, mu);        // parity q
+    const int xpnumu = nb_bwd(geo, snu, xpnu, mu); // x+nu-mu, parity p
+    // forward staple
+    M3<double> a = load_m<double2, double>(field_ptr(U, q * 4 + mu, S), S, xpnu);
+    M3<double> b = load_m_dagger<double2, double>(field_ptr(U, q * 4 + nu, S), S, xpmu);
+    M3<double> c = m_mul(a, b);
+    M3<double> d = load_m_dagger<double2, double>(field_ptr(U, p * 4 + mu, S), S, i);
+    m_addmul(st, c, d);
+    // backward staple
+    a = load_m_dagger<double2, double>(field_ptr(U, p * 4 + mu, S), S, xpnumu);
+    b = load_m_dagger<double2, double>(field_ptr(U, q * 4 + nu, S), S, xmmu);
+    c = m_mul(a, b);
+    d = load_m<double2, double>(field_ptr(U, q * 4 + mu, S), S, xmmu);
+    m_addmul(st, c, d);
+  }
+  double2* Un = field_ptr(U, p * 4 + nu, S);
+  M3<double> h = load_m<double2, double>(Un, S, i);
+  cm_heatbath<1>(h, st, beta, gauss, unif, i);
+  m_normalize(h);
+  Store<double2>::store(Un, S, i, h);
+}
+
+void launch_sa_sweep(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
+                     double beta, int sampler, const float* gauss, const float* unif,
+                     hipStream_t st) {
+  dim3 grid((geo.nh + HB_BLK - 1) / HB_BLK);
+  if (sampler == 0) {
+    if (ndir == 4)
+      hipLaunchKernelGGL((k_sa_sweep<0, 4>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity, beta, gauss, unif);
+    else
+      hipLaunchKernelGGL((k_sa_sweep<0, 3>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity, beta, gauss, unif);
+  } else {
+    if (ndir == 4)
+      hipLaunchKernelGGL((k_sa_sweep<1, 4>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity, beta, gauss, unif);
+    else
+      hipLaunchKernelGGL((k_sa_sweep<1, 3>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity, beta, gauss, unif);
+  }
+}
+
+void launch_micro_sweep(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
+                        hipStream_t st) {
+  dim3 grid((geo.nh + HB_BLK - 1) / HB_BLK);
+  if (ndir == 4)
+    hipLaunchKernelGGL((k_micro_sweep<4>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity);
+  else
+    hipLaunchKernelGGL((k_micro_sweep<3>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity);
+}
+
+void launch_link_heatbath(double2* U, const Geom& geo, int parity, int nu, double beta,
+                          const float* gauss, const float* unif, hipStream_t st) {
+  dim3 grid((geo.nh + HB_BLK - 1) / HB_BLK);
+  hipLaunchKernelGGL(k_link_heatbath, grid, dim3(HB_BLK), 0, st, U, geo, parity, nu, beta, gauss,
+                     unif);
+}
+
+int kp_gauss_per_site() { return KP_NG; }
+int kp_unif_per_site() { return KP_NU; }
+
+}  // namespace gfx

@@ -1,0 +1,31 @@
+// gfx_internal.h -- launchers shared between the kernel and API translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gfx_lattice.h"
+
+namespace gfx {
+
+void launch_sweep_f64(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
+                      double omega, const unsigned char* active, hipStream_t st);
+void launch_sweep_f32(const float4* U, float4* g, const Geom& geo, int parity, int ndir,
+                      double omega, const unsigned char* active, hipStream_t st);
+int functional_blocks(int nsh);
+void launch_functional_f64(const double2* U, const double2* g, const Geom& geo, int ndir, int nseg,
+                           double* partials, double eps, double* out, hipStream_t st);
+void launch_functional_f32(const float4* U, const float4* g, const Geom& geo, int ndir, int nseg,
+                           double* partials, double eps, double* out, hipStream_t st);
+void launch_plaquette_f64(const double2* U, const Geom& geo, double* partials, double* out,
+                          hipStream_t st);
+void launch_pack_f64(const double* aos, double2* dst, const Geom& geo, int s0, int ns, int nmat,
+                     hipStream_t st);
+void launch_unpack_f64(const double2* src, double* aos, const Geom& geo, int s0, int ns, int nmat,
+                       hipStream_t st);
+void launch_apply_trafo_aos(double* aos, const double2* g, const Geom& geo, int s0, int ns,
+                            hipStream_t st);
+void launch_f64_to_f32(const double2* src, float4* dst, const Geom& geo, int nfield,
+                       hipStream_t st);
+void launch_g32_to_g64(const float4* src, double2* dst, const Geom& geo, hipStream_t st);
+
+}  // namespace gfx

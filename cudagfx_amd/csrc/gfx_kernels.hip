@@ -1,0 +1,421 @@
+// gfx_kernels.hip -- gfx950 kernels of the gauge-fixing path.
+//
+//   k_or_sweep       red-black overrelaxation half sweep (Landau: 4 dirs,
+//                    Coulomb: 3 spatial dirs with per-slice activity),
+//                    overrelax.cu:335-446
+//   k_functional     per-site F and Re tr(Delta Delta^+), wavefront-shuffle
+//                    + LDS block reduction, overrelax.cu:482-605
+//   k_finalize       grid-level pass over block partials, overrelax.cu:610-671
+//   k_plaquette      mean plaquette, overrelax.cu:34-138
+//   k_pack_* / k_unpack_*   host su3 (AoS, 18 doubles) <-> device planes,
+//                    su3manip.c:212-230, 281-299, 303-359
+//   k_apply_trafo_aos  U_mu(x) <- g(x) U_mu(x) g^+(x+mu) on host-layout
+//                    links, observables.c:17-47
+//   k_f64_to_f32 / k_g32_to_g64  precision conversions of the mixed path
+//
+// One thread per site; all SU(3) algebra in registers (gfx_su3.h).  The
+// work is HBM-bound 3x3 complex algebra, so nothing here uses MFMA.
+#include <hip/hip_runtime.h>
+
+#include "gfx_internal.h"
+#include "gfx_lattice.h"
+#include "gfx_su3.h"
+
+namespace gfx {
+
+constexpr int BLK = 256;
+
+// ------------------------------------------------------------ reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// sums a, b and takes the max of c over the block; result valid in thread 0
+__device__ __forceinline__ void block_reduce3(double& a, double& b, double& c) {
+  __shared__ double red[3][BLK / 64];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_max(c);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) {
+    red[0][w] = a;
+    red[1][w] = b;
+    red[2][w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    a = red[0][0];
+    b = red[1][0];
+    c = red[2][0];
+    for (int k = 1; k < nw; k++) {
+      a += red[0][k];
+      b += red[1][k];
+      c = fmax(c, red[2][k]);
+    }
+  }
+}
+
+// ------------------------------------------------------------ OR sweep
+template <typename V, typename R, int NDIR>
+__global__ __launch_bounds__(BLK) void k_or_sweep(const V* __restrict__ U, V* __restrict__ g,
+                                                   Geom geo, int parity, R c1, R c2,
+                                                   const unsigned char* __restrict__ active) {
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int i = b * BLK + threadIdx.x;
+  if (i >= geo.nh) return;
+  const Site s = site_coords(geo, i, parity);
+  if (NDIR == 3 && active != nullptr && !active[s.x[3]]) return;
+  const int S = geo.stride;
+  const V* gop = field_ptr(static_cast<const V*>(g), 1 - parity, S);
+  V* gme = field_ptr(g, parity, S);
+
+  // star K = sum_mu U_mu(x) g^+(x+mu) + U_mu^+(x-mu) g^+(x-mu)
+  M3<R> K;
+  m_zero(K);
+#pragma unroll
+  for (int mu = 0; mu < NDIR; mu++) {
+    const int jf = nb_fwd(geo, s, i, mu);
+    const int jb = nb_bwd(geo, s, i, mu);
+    M3<R> A = load_m<V, R>(field_ptr(U, parity * 4 + mu, S), S, i);
+    M3<R> B = load_m_dagger<V, R>(gop, S, jf);
+    m_addmul(K, A, B);
+    A = load_m_dagger<V, R>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb);
+    B = load_m_dagger<V, R>(gop, S, jb);
+    m_addmul(K, A, B);
+  }
+  M3<R> G = load_m<V, R>(gme, S, i);
+  M3<R> h = G;
+  cm_relax(h, K);
+  m_normalize(h);
+  overrelax_apply(G, h, c1, c2);
+  m_normalize(G);
+  Store<V>::store(gme, S, i, G);
+}
+
+// ------------------------------------------------------------ functional
+// Segment = whole lattice (Landau) or one time slice (Coulomb).  Within a
+// segment, sites [0, nsh) are parity 0 and [nsh, 2 nsh) parity 1; the
+// parity index of site k of segment seg is seg*nsh + (k mod nsh).
+template <typename V, int NDIR>
+__global__ __launch_bounds__(BLK) void k_functional(const V* __restrict__ U,
+                                                     const V* __restrict__ g, Geom geo, int nsh,
+                                                     double* __restrict__ partials) {
+  const int seg = blockIdx.y;
+  const int k = blockIdx.x * BLK + threadIdx.x;
+  double F = 0.0, d = 0.0, mx = 0.0;
+  if (k < 2 * nsh) {
+    const int p = (k >= nsh) ? 1 : 0;
+    const int i = seg * nsh + (k - p * nsh);
+    const int S = geo.stride;
+    const Site s = site_coords(geo, i, p);
+    const V* gme = field_ptr(g, p, S);
+    const V* gop = field_ptr(g, 1 - p, S);
+    const M3<double> Gx = load_m<V, double>(gme, S, i);
+    const M3<double> Gxd = load_m_dagger<V, double>(gme, S, i);
+    M3<double> gather;
+    m_zero(gather);
+#pragma unroll
+    for (int mu = 0; mu < NDIR; mu++) {
+      const int jf = nb_fwd(geo, s, i, mu);
+      const int jb = nb_bwd(geo, s, i, mu);
+      M3<double> Um = load_m<V, double>(field_ptr(U, p * 4 + mu, S), S, i);
+      M3<double> help = m_mul(Gx, Um);
+      M3<double> Gd = load_m_dagger<V, double>(gop, S, jf);
+      M3<double> fw = m_mul(help, Gd);
+      F += m_retrace(fw);
+      Um = load_m<V, double>(field_ptr(U, (1 - p) * 4 + mu, S), S, jb);
+      M3<double> Gb = load_m<V, double>(gop, S, jb);
+      help = m_mul(Gb, Um);
+      M3<double> bw = m_mul(help, Gxd);
+      vecpot(fw);
+      vecpot(bw);
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          Cx<double> t = csub(bw.e[a][c], fw.e[a][c]);
+          gather.e[a][c] = cadd(gather.e[a][c], t);
+        }
+    }
+    M3<double> dd = m_mul_d(gather, gather);
+    d = m_retrace(dd);
+    mx = d;
+  }
+  block_reduce3(F, d, mx);
+  if (threadIdx.x == 0) {
+    double* o = partials + 3 * ((size_t)seg * gridDim.x + blockIdx.x);
+    o[0] = F;
+    o[1] = d;
+    o[2] = mx;
+  }
+}
+
+// grid-level pass: one block per segment over its nb block partials.
+// out[4*seg + {0,1,2,3}] = F/(ndir*Vseg), dAdA/Vseg, max dAdA, converged flag
+__global__ __launch_bounds__(BLK) void k_finalize(const double* __restrict__ partials, int nb,
+                                                  double normF, double normD, double eps,
+                                                  double* __restrict__ out) {
+  const int seg = blockIdx.x;
+  double F = 0.0, d = 0.0, mx = 0.0;
+  for (int b = threadIdx.x; b < nb; b += BLK) {
+    const double* p = partials + 3 * ((size_t)seg * nb + b);
+    F += p[0];
+    d += p[1];
+    mx = fmax(mx, p[2]);
+  }
+  block_reduce3(F, d, mx);
+  if (threadIdx.x == 0) {
+    out[4 * seg + 0] = F / normF;
+    out[4 * seg + 1] = d / normD;
+    out[4 * seg + 2] = mx;
+    out[4 * seg + 3] = (mx < eps) ? 1.0 : 0.0;
+  }
+}
+
+// ------------------------------------------------------------ plaquette
+template <typename V>
+__global__ __launch_bounds__(BLK) void k_plaquette(const V* __restrict__ U, Geom geo,
+                                                   double* __restrict__ partials) {
+  const int k = blockIdx.x * BLK + threadIdx.x;
+  double acc = 0.0, dummy = 0.0, dummy2 = 0.0;
+  if (k < 2 * geo.nh) {
+    const int p = (k >= geo.nh) ? 1 : 0;
+    const int i = k - p * geo.nh;
+    const int S = geo.stride;
+    const Site s = site_coords(geo, i, p);
+#pragma unroll
+    for (int nu = 0; nu < 3; nu++) {
+      const int jnu = nb_fwd(geo, s, i, nu);
+#pragma unroll
+      for (int mu = nu + 1; mu < 4; mu++) {
+        const int jmu = nb_fwd(geo, s, i, mu);
+        M3<double> M1 = load_m<V, double>(field_ptr(U, p * 4 + mu, S), S, i);
+        M3<double> M2 = load_m<V, double>(field_ptr(U, (1 - p) * 4 + nu, S), S, jmu);
+        M3<double> M3d = load_m_dagger<V, double>(field_ptr(U, (1 - p) * 4 + mu, S), S, jnu);
+        M3<double> M4d = load_m_dagger<V, double>(field_ptr(U, p * 4 + nu, S), S, i);
+        M3<double> s1 = m_mul(M3d, M4d);
+        M3<double> s2 = m_mul(M2, s1);
+        s1 = m_mul(M1, s2);
+        acc += m_retrace(s1) / 3.0;
+      }
+    }
+  }
+  block_reduce3(acc, dummy, dummy2);
+  if (threadIdx.x == 0) partials[3 * blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(BLK) void k_sum_partials(const double* __restrict__ partials, int nb,
+                                                      double norm, double* __restrict__ out) {
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int k = threadIdx.x; k < nb; k += BLK) a += partials[3 * k];
+  block_reduce3(a, b, c);
+  if (threadIdx.x == 0) out[0] = a / norm;
+}
+
+// ------------------------------------------------------------ layout conversion
+// parity and parity index of lexicographic site s
+__device__ __forceinline__ void lex_site(const Geom& geo, FastDiv dL0, int s, int x[4], int& p) {
+  uint32_t q = fdiv((uint32_t)s, dL0);
+  x[0] = s - (int)q * geo.L[0];
+  uint32_t q2 = fdiv(q, geo.dL1);
+  x[1] = (int)(q - q2 * (uint32_t)geo.L[1]);
+  uint32_t q3 = fdiv(q2, geo.dL2);
+  x[2] = (int)(q2 - q3 * (uint32_t)geo.L[2]);
+  x[3] = (int)q3;
+  p = (x[0] + x[1] + x[2] + x[3]) & 1;
+}
+
+// nmat matrices per site (4 for links, 1 for g); staging holds sites
+// [s0, s0+ns) as AoS su3 (18 doubles each, matrix m of site s at
+// (s-s0)*nmat+m).
+template <typename V>
+__global__ __launch_bounds__(BLK) void k_pack(const double* __restrict__ aos, V* __restrict__ dst,
+                                              Geom geo, FastDiv dL0, int s0, int ns, int nmat) {
+  const int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= ns * nmat) return;
+  const int sl = t / nmat, m = t - sl * nmat;
+  const int s = s0 + sl;
+  int x[4], p;
+  lex_site(geo, dL0, s, x, p);
+  const double* a = aos + (size_t)t * 18;
+  M3<double> M;
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) M.e[r][c] = cx<double>(a[6 * r + 2 * c], a[6 * r + 2 * c + 1]);
+  Store<V>::store(field_ptr(dst, p * nmat + m, geo.stride), geo.stride, s >> 1, M);
+}
+
+template <typename V>
+__global__ __launch_bounds__(BLK) void k_unpack(const V* __restrict__ src, double* __restrict__ aos,
+                                                Geom geo, FastDiv dL0, int s0, int ns, int nmat) {
+  const int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= ns * nmat) return;
+  const int sl = t / nmat, m = t - sl * nmat;
+  const int s = s0 + sl;
+  int x[4], p;
+  lex_site(geo, dL0, s, x, p);
+  M3<double> M = load_m<V, double>(field_ptr(src, p * nmat + m, geo.stride), geo.stride, s >> 1);
+  double* a = aos + (size_t)t * 18;
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      a[6 * r + 2 * c] = M.e[r][c].re;
+      a[6 * r + 2 * c + 1] = M.e[r][c].im;
+    }
+}
+
+// host-layout links of sites [s0, s0+ns) in place: U <- g(x) U g^+(x+mu)
+__global__ __launch_bounds__(BLK) void k_apply_trafo_aos(double* __restrict__ aos,
+                                                         const double2* __restrict__ g, Geom geo,
+                                                         FastDiv dL0, int s0, int ns) {
+  const int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= ns * 4) return;
+  const int sl = t >> 2, mu = t & 3;
+  const int s = s0 + sl;
+  int x[4], p;
+  lex_site(geo, dL0, s, x, p);
+  const int i = s >> 1;
+  const Site st = site_coords(geo, i, p);
+  const int jf = nb_fwd(geo, st, i, mu);
+  double* a = aos + (size_t)t * 18;
+  M3<double> Um;
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) Um.e[r][c] = cx<double>(a[6 * r + 2 * c], a[6 * r + 2 * c + 1]);
+  const M3<double> G = load_m<double2, double>(field_ptr(g, p, geo.stride), geo.stride, i);
+  const M3<double> Gpd = load_m_dagger<double2, double>(field_ptr(g, 1 - p, geo.stride), geo.stride, jf);
+  M3<double> h1 = m_mul(G, Um);
+  M3<double> r = m_mul(h1, Gpd);
+#pragma unroll
+  for (int rr = 0; rr < 3; rr++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      a[6 * rr + 2 * c] = r.e[rr][c].re;
+      a[6 * rr + 2 * c + 1] = r.e[rr][c].im;
+    }
+}
+
+// nfield FP64 fields -> FP32 fields (12-real each)
+__global__ __launch_bounds__(BLK) void k_f64_to_f32(const double2* __restrict__ src,
+                                                    float4* __restrict__ dst, int nh, int stride,
+                                                    int nfield) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  const int f = blockIdx.y;
+  if (i >= nh || f >= nfield) return;
+  const M3<double> M = load_m<double2, double>(field_ptr(src, f, stride), stride, i);
+  Store<float4>::store(field_ptr(dst, f, stride), stride, i, M);
+}
+
+// FP32 g -> FP64 g with FP64 reunitarisation (Gram-Schmidt, dev_su3.cu:1569-1634)
+__global__ __launch_bounds__(BLK) void k_g32_to_g64(const float4* __restrict__ src,
+                                                    double2* __restrict__ dst, int nh, int stride) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  const int f = blockIdx.y;
+  if (i >= nh) return;
+  M3<double> M = load_m<float4, double>(field_ptr(src, f, stride), stride, i);
+  m_normalize(M);
+  Store<double2>::store(field_ptr(dst, f, stride), stride, i, M);
+}
+
+// ------------------------------------------------------------ launchers
+static inline dim3 grid1(long n) { return dim3((unsigned)((n + BLK - 1) / BLK)); }
+
+template <typename V, typename R>
+static void launch_sweep_t(const V* U, V* g, const Geom& geo, int parity, int ndir, double omega,
+                           const unsigned char* active, hipStream_t st) {
+  // C(w,1) = (1+w-1)/1 and C(w,2) = (1+w-2)/2 as the reference forms them
+  // (overrelax.cu:167), in double, then rounded to R.
+  const R c1 = (R)((1.0 + omega - 1) / 1);
+  const R c2 = (R)((1.0 + omega - 2) / 2);
+  dim3 grid = grid1(geo.nh);
+  if (ndir == 4)
+    hipLaunchKernelGGL((k_or_sweep<V, R, 4>), grid, dim3(BLK), 0, st, U, g, geo, parity, c1, c2,
+                       nullptr);
+  else
+    hipLaunchKernelGGL((k_or_sweep<V, R, 3>), grid, dim3(BLK), 0, st, U, g, geo, parity, c1, c2,
+                       active);
+}
+
+void launch_sweep_f64(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
+                      double omega, const unsigned char* active, hipStream_t st) {
+  launch_sweep_t<double2, double>(U, g, geo, parity, ndir, omega, active, st);
+}
+void launch_sweep_f32(const float4* U, float4* g, const Geom& geo, int parity, int ndir,
+                      double omega, const unsigned char* active, hipStream_t st) {
+  launch_sweep_t<float4, float>(U, g, geo, parity, ndir, omega, active, st);
+}
+
+int functional_blocks(int nsh) { return (2 * nsh + BLK - 1) / BLK; }
+
+template <typename V>
+static void launch_functional_t(const V* U, const V* g, const Geom& geo, int ndir, int nseg,
+                                double* partials, double eps, double* out, hipStream_t st) {
+  const int nsh = geo.nh / nseg;
+  const int nb = functional_blocks(nsh);
+  dim3 grid(nb, nseg);
+  if (ndir == 4)
+    hipLaunchKernelGGL((k_functional<V, 4>), grid, dim3(BLK), 0, st, U, g, geo, nsh, partials);
+  else
+    hipLaunchKernelGGL((k_functional<V, 3>), grid, dim3(BLK), 0, st, U, g, geo, nsh, partials);
+  const double vseg = 2.0 * nsh;
+  hipLaunchKernelGGL(k_finalize, dim3(nseg), dim3(BLK), 0, st, partials, nb, ndir * vseg, vseg,
+                     eps, out);
+}
+
+void launch_functional_f64(const double2* U, const double2* g, const Geom& geo, int ndir, int nseg,
+                           double* partials, double eps, double* out, hipStream_t st) {
+  launch_functional_t<double2>(U, g, geo, ndir, nseg, partials, eps, out, st);
+}
+void launch_functional_f32(const float4* U, const float4* g, const Geom& geo, int ndir, int nseg,
+                           double* partials, double eps, double* out, hipStream_t st) {
+  launch_functional_t<float4>(U, g, geo, ndir, nseg, partials, eps, out, st);
+}
+
+void launch_plaquette_f64(const double2* U, const Geom& geo, double* partials, double* out,
+                          hipStream_t st) {
+  const int nb = (2 * geo.nh + BLK - 1) / BLK;
+  hipLaunchKernelGGL(k_plaquette<double2>, dim3(nb), dim3(BLK), 0, st, U, geo, partials);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(BLK), 0, st, partials, nb, 6.0 * 2.0 * geo.nh,
+                     out);
+}
+
+void launch_pack_f64(const double* aos, double2* dst, const Geom& geo, int s0, int ns, int nmat,
+                     hipStream_t st) {
+  FastDiv dL0 = make_fastdiv(geo.L[0]);
+  hipLaunchKernelGGL(k_pack<double2>, grid1((long)ns * nmat), dim3(BLK), 0, st, aos, dst, geo, dL0,
+                     s0, ns, nmat);
+}
+void launch_unpack_f64(const double2* src, double* aos, const Geom& geo, int s0, int ns, int nmat,
+                       hipStream_t st) {
+  FastDiv dL0 = make_fastdiv(geo.L[0]);
+  hipLaunchKernelGGL(k_unpack<double2>, grid1((long)ns * nmat), dim3(BLK), 0, st, src, aos, geo,
+                     dL0, s0, ns, nmat);
+}
+void launch_apply_trafo_aos(double* aos, const double2* g, const Geom& geo, int s0, int ns,
+                            hipStream_t st) {
+  FastDiv dL0 = make_fastdiv(geo.L[0]);
+  hipLaunchKernelGGL(k_apply_trafo_aos, grid1((long)ns * 4), dim3(BLK), 0, st, aos, g, geo, dL0, s0,
+                     ns);
+}
+void launch_f64_to_f32(const double2* src, float4* dst, const Geom& geo, int nfield,
+                       hipStream_t st) {
+  dim3 grid((geo.nh + BLK - 1) / BLK, nfield);
+  hipLaunchKernelGGL(k_f64_to_f32, grid, dim3(BLK), 0, st, src, dst, geo.nh, geo.stride, nfield);
+}
+void launch_g32_to_g64(const float4* src, double2* dst, const Geom& geo, hipStream_t st) {
+  dim3 grid((geo.nh + BLK - 1) / BLK, 2);
+  hipLaunchKernelGGL(k_g32_to_g64, grid, dim3(BLK), 0, st, src, dst, geo.nh, geo.stride);
+}
+
+}  // namespace gfx

@@ -1,0 +1,189 @@
+// gfx_lattice.h -- checkerboard geometry and the structure-of-arrays layout.
+//
+// Sites: lexicographic s = x0 + L0*(x1 + L1*(x2 + L2*x3)) as in the
+// reference (cudagaugefix.cu:79-85).  Parity p = (x0+x1+x2+x3) & 1.  With L0
+// even, the index of a site inside its parity is i = s >> 1, which is the
+// reference's even/odd order (lexicographic filtered by parity,
+// cudagaugefix.cu:159-195), so eoidx_{even,odd}[i] == 2*i + (bit of x0).
+//
+// Device layout.  A *field* holds one 2-row SU(3) matrix (rows 0,1 = 12
+// reals, su3manip.c:212-230) for each of the nh = V/2 sites of one parity,
+// as planes of 16-byte vectors so that a wavefront reads 64 consecutive
+// vectors (1 KiB) per load:
+//   FP64: 6 planes of double2  = (u00)(u01)(u02)(u10)(u11)(u12)
+//   FP32: 3 planes of float4   = (u00,u01)(u02,u10)(u11,u12)
+// plane c of site i sits at field[c*stride + i]; stride >= nh, a multiple
+// of 64.  Links: field (p*4 + mu) holds U_mu(x) for x of parity p.  Gauge
+// transformation g: field p.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gfx_su3.h"
+
+namespace gfx {
+
+// n / d for 0 <= n < 2^31 with a precomputed multiplier ("round-up" method
+// of Granlund & Montgomery).
+struct FastDiv {
+  uint32_t d, m, s;
+};
+
+static inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f;
+  f.d = d;
+  uint32_t s = 0;
+  while ((1ull << s) < d) s++;
+  f.s = s;
+  f.m = (uint32_t)((((1ull << 32) * ((1ull << s) - d)) / d) + 1);
+  return f;
+}
+
+__host__ __device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv f) {
+#ifdef __HIP_DEVICE_COMPILE__
+  uint32_t t = __umulhi(n, f.m);
+#else
+  uint32_t t = (uint32_t)(((uint64_t)n * f.m) >> 32);
+#endif
+  return (t + n) >> f.s;
+}
+
+struct Geom {
+  int L[4];      // extents x0..x3
+  int h0;        // L0/2
+  int nh;        // sites per parity
+  int st[4];     // index strides inside a parity: 1 (x0 special), h0, h0*L1, h0*L1*L2
+  int stride;    // plane stride (elements)
+  FastDiv dh0, dL1, dL2;
+};
+
+struct Site {
+  int x[4];
+  int q;  // i / h0
+};
+
+__device__ __forceinline__ Site site_coords(const Geom& g, int i, int parity) {
+  Site s;
+  uint32_t q = fdiv((uint32_t)i, g.dh0);
+  int i0h = i - (int)q * g.h0;
+  uint32_t q2 = fdiv(q, g.dL1);
+  s.x[1] = (int)(q - q2 * (uint32_t)g.L[1]);
+  uint32_t q3 = fdiv(q2, g.dL2);
+  s.x[2] = (int)(q2 - q3 * (uint32_t)g.L[2]);
+  s.x[3] = (int)q3;
+  s.x[0] = 2 * i0h + ((parity + s.x[1] + s.x[2] + s.x[3]) & 1);
+  s.q = (int)q;
+  return s;
+}
+
+// index (in the other parity) of the forward / backward neighbour in mu
+__device__ __forceinline__ int nb_fwd(const Geom& g, const Site& s, int i, int mu) {
+  if (mu == 0) {
+    int x0 = s.x[0] + 1;
+    if (x0 == g.L[0]) x0 = 0;
+    return (x0 >> 1) + s.q * g.h0;
+  }
+  return (s.x[mu] == g.L[mu] - 1) ? i - (g.L[mu] - 1) * g.st[mu] : i + g.st[mu];
+}
+__device__ __forceinline__ int nb_bwd(const Geom& g, const Site& s, int i, int mu) {
+  if (mu == 0) {
+    int x0 = (s.x[0] == 0) ? g.L[0] - 1 : s.x[0] - 1;
+    return (x0 >> 1) + s.q * g.h0;
+  }
+  return (s.x[mu] == 0) ? i + (g.L[mu] - 1) * g.st[mu] : i - g.st[mu];
+}
+
+// ---- storage traits: 2-row matrices in planes of 16-byte vectors ----
+template <typename V>
+struct Store;
+
+template <>
+struct Store<double2> {
+  static constexpr int NPLANE = 6;
+  template <typename R>
+  __device__ __forceinline__ static void load(const double2* __restrict__ f, int stride, int i,
+                                              Cx<R> r0[3], Cx<R> r1[3]) {
+    double2 v[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) v[c] = f[(size_t)c * stride + i];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      r0[c] = cx<R>((R)v[c].x, (R)v[c].y);
+      r1[c] = cx<R>((R)v[3 + c].x, (R)v[3 + c].y);
+    }
+  }
+  template <typename R>
+  __device__ __forceinline__ static void store(double2* __restrict__ f, int stride, int i,
+                                               const M3<R>& m) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      f[(size_t)c * stride + i] = make_double2((double)m.e[0][c].re, (double)m.e[0][c].im);
+      f[(size_t)(3 + c) * stride + i] = make_double2((double)m.e[1][c].re, (double)m.e[1][c].im);
+    }
+  }
+};
+
+template <>
+struct Store<float4> {
+  static constexpr int NPLANE = 3;
+  template <typename R>
+  __device__ __forceinline__ static void load(const float4* __restrict__ f, int stride, int i,
+                                              Cx<R> r0[3], Cx<R> r1[3]) {
+    float4 a = f[i];
+    float4 b = f[(size_t)stride + i];
+    float4 c = f[(size_t)2 * stride + i];
+    r0[0] = cx<R>((R)a.x, (R)a.y);
+    r0[1] = cx<R>((R)a.z, (R)a.w);
+    r0[2] = cx<R>((R)b.x, (R)b.y);
+    r1[0] = cx<R>((R)b.z, (R)b.w);
+    r1[1] = cx<R>((R)c.x, (R)c.y);
+    r1[2] = cx<R>((R)c.z, (R)c.w);
+  }
+  template <typename R>
+  __device__ __forceinline__ static void store(float4* __restrict__ f, int stride, int i,
+                                               const M3<R>& m) {
+    f[i] = make_float4((float)m.e[0][0].re, (float)m.e[0][0].im, (float)m.e[0][1].re,
+                       (float)m.e[0][1].im);
+    f[(size_t)stride + i] = make_float4((float)m.e[0][2].re, (float)m.e[0][2].im,
+                                        (float)m.e[1][0].re, (float)m.e[1][0].im);
+    f[(size_t)2 * stride + i] = make_float4((float)m.e[1][1].re, (float)m.e[1][1].im,
+                                            (float)m.e[1][2].re, (float)m.e[1][2].im);
+  }
+};
+
+template <typename V, typename R>
+__device__ __forceinline__ M3<R> load_m(const V* f, int stride, int i) {
+  M3<R> m;
+  Store<V>::template load<R>(f, stride, i, m.e[0], m.e[1]);
+  m_row2(m);
+  return m;
+}
+
+template <typename V, typename R>
+__device__ __forceinline__ M3<R> load_m_dagger(const V* f, int stride, int i) {
+  Cx<R> r0[3], r1[3];
+  Store<V>::template load<R>(f, stride, i, r0, r1);
+  M3<R> d;
+  m_dagger_from_rows(d, r0, r1);
+  return d;
+}
+
+template <typename V>
+__device__ __forceinline__ const V* field_ptr(const V* base, int field, int stride) {
+  return base + (size_t)field * Store<V>::NPLANE * stride;
+}
+template <typename V>
+__device__ __forceinline__ V* field_ptr(V* base, int field, int stride) {
+  return base + (size_t)field * Store<V>::NPLANE * stride;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Remap so that XCD k works on a contiguous range of sites and shares
+// neighbour g(x +- mu) lines in its own L2.
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+  if ((nblk & 7) != 0) return b;
+  return (b & 7) * (nblk >> 3) + (b >> 3);
+}
+
+}  // namespace gfx

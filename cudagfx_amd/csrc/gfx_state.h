@@ -1,0 +1,59 @@
+// gfx_state.h -- device state of the single library instance (the reference
+// keeps the same state in globals, global.h:128-170, cudagaugefix.cu:46-54).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gfx_lattice.h"
+
+namespace gfx {
+
+struct State {
+  int init = 0;  // 1 gauge fixing, 2 thermalization
+  Geom geo{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // FP64 fields: links 8 fields (parity*4+mu), trafo 2 fields (parity)
+  double2* U64 = nullptr;
+  double2* g64 = nullptr;
+  // FP32 12-real copies for the mixed path
+  float4* U32 = nullptr;
+  float4* g32 = nullptr;
+  bool U32_valid = false;
+
+  // reductions
+  double* partials = nullptr;
+  int npartials = 0;
+  double* results = nullptr;    // device, 4 per segment
+  double* h_results = nullptr;  // pinned host mirror
+  unsigned char* active = nullptr;
+
+  double* staging = nullptr;  // AoS transfer chunk
+
+  // parameters
+  double omega = 1.68;  // overrelax.cu:416
+  int precision = 0;
+  double switch_theta = 1e-6;
+  int verbose = 1;
+  long mt_seed = -1;
+
+  // Mersenne Twister fields (MersenneTwister.cu:55-68)
+  uint32_t* mt_state = nullptr;   // 4096 x 19
+  uint32_t* mt_params = nullptr;  // 4096 x 4 (matrix_a, mask_b, mask_c, seed)
+  float* rnd_gauss = nullptr;
+  float* rnd_unif = nullptr;
+  int n_per_gauss = 0, n_per_unif = 0;
+
+  // last run statistics
+  double run_ms = 0.0;
+  int run_iters32 = 0, run_iters64 = 0;
+};
+
+extern State g_st;
+
+int hip_check(hipError_t e, const char* what);
+int functional_now(State& s, bool fp32, int ndir, int nseg, double eps);
+
+}  // namespace gfx

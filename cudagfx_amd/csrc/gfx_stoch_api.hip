@@ -1,0 +1,309 @@
+// gfx_stoch_api.hip -- C ABI of the stochastic parts: Mersenne Twister
+// fields (MersenneTwister.cu:236-330), simulated annealing
+// (simulated_annealing.cu:265-476) and link-heatbath thermalization
+// (heatbath_thermalization.cu:557-841).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <vector>
+
+#include "../../include/cudagfx.h"
+#include "gfx_state.h"
+
+namespace gfx {
+void launch_mt_generate(uint32_t* params, uint32_t* state, float* out, int nper, int initialized,
+                        hipStream_t st);
+void launch_box_muller(float* d, int nper, hipStream_t st);
+void launch_sa_sweep(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
+                     double beta, int sampler, const float* gauss, const float* unif,
+                     hipStream_t st);
+void launch_micro_sweep(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
+                        hipStream_t st);
+void launch_link_heatbath(double2* U, const Geom& geo, int parity, int nu, double beta,
+                          const float* gauss, const float* unif, hipStream_t st);
+int kp_gauss_per_site();
+int kp_unif_per_site();
+
+static const int MT_COUNT = 4096;
+
+struct Stoch {
+  std::vector<uint32_t> host_params;  // 4096 x 4 as loaded (seed fields from the file)
+  bool have_params = false;
+  char dat_path[1024] = "MersenneTwister.dat";
+  int sa_sampler = 0;   // 0 reference rule, 1 Kennedy-Pendleton
+  int sa_nmicro = 0;    // microcanonical sweeps after each SA sweep
+  int sa_ndir = 4;
+};
+static Stoch g_rng;
+
+// iDivUp / iAlignUp (MersenneTwister.cu:31-43)
+static int n_per_rng(int n) {
+  int q = (n % MT_COUNT) ? n / MT_COUNT + 1 : n / MT_COUNT;
+  return (q % 2) ? q + 1 : q;
+}
+
+static int load_params() {
+  if (g_rng.have_params) return GFX_OK;
+  FILE* fd = fopen(g_rng.dat_path, "rb");
+  if (!fd) {
+    printf("initMTGPU(): failed to open %s\n", g_rng.dat_path);
+    printf("FAILED\n");
+    return GFX_ERR_MTDAT;
+  }
+  g_rng.host_params.assign(4 * MT_COUNT, 0u);
+  const size_t got = fread(g_rng.host_params.data(), sizeof(uint32_t) * 4 * MT_COUNT, 1, fd);
+  fclose(fd);
+  if (got != 1) {
+    printf("initMTGPU(): failed to load %s\n", g_rng.dat_path);
+    printf("FAILED\n");
+    return GFX_ERR_MTDAT;
+  }
+  g_rng.have_params = true;
+  return GFX_OK;
+}
+
+// (re)size the random fields; keeps the twister state
+static int mt_fields(State& s, int n_gauss, int n_unif) {
+  const int ng = n_per_rng(n_gauss), nu = n_per_rng(n_unif);
+  if (ng > s.n_per_gauss || !s.rnd_gauss) {
+    hipFree(s.rnd_gauss);
+    if (hipMalloc((void**)&s.rnd_gauss, sizeof(float) * (size_t)MT_COUNT * (ng > 0 ? ng : 2)) != hipSuccess) return GFX_ERR_ALLOC;
+  }
+  if (nu > s.n_per_unif || !s.rnd_unif) {
+    hipFree(s.rnd_unif);
+    if (hipMalloc((void**)&s.rnd_unif, sizeof(float) * (size_t)MT_COUNT * (nu > 0 ? nu : 2)) != hipSuccess) return GFX_ERR_ALLOC;
+  }
+  s.n_per_gauss = ng;
+  s.n_per_unif = nu;
+  return GFX_OK;
+}
+
+// RandomGPU(gauss) -> BoxMullerGPU -> RandomGPU(unif), sharing one state
+// array (MersenneTwister.cu:272-277, 300-306)
+static void mt_refill(State& s, int initialized) {
+  launch_mt_generate(s.mt_params, s.mt_state, s.rnd_gauss, s.n_per_gauss, initialized, s.stream);
+  launch_box_muller(s.rnd_gauss, s.n_per_gauss, s.stream);
+  launch_mt_generate(s.mt_params, s.mt_state, s.rnd_unif, s.n_per_unif, initialized, s.stream);
+}
+
+int mt_init_device(State& s, int n_gauss, int n_unif) {
+  int rc = load_params();
+  if (rc) return rc;
+  if (!s.stream) {
+    if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) return GFX_ERR_DEVICE;
+    hipEventCreate(&s.ev0);
+    hipEventCreate(&s.ev1);
+  }
+  // seedMTGPU (MersenneTwister.cu:98-114): srand, then one rand() per twister
+  srand(s.mt_seed >= 0 ? (unsigned)s.mt_seed : (unsigned)time(NULL));
+  std::vector<uint32_t> p(g_rng.host_params);
+  for (int i = 0; i < MT_COUNT; i++) p[4 * i + 3] = (uint32_t)rand();
+  if (!s.mt_params && hipMalloc((void**)&s.mt_params, sizeof(uint32_t) * 4 * MT_COUNT) != hipSuccess) return GFX_ERR_ALLOC;
+  if (!s.mt_state && hipMalloc((void**)&s.mt_state, sizeof(uint32_t) * 19 * MT_COUNT) != hipSuccess) return GFX_ERR_ALLOC;
+  hipMemcpyAsync(s.mt_params, p.data(), sizeof(uint32_t) * 4 * MT_COUNT, hipMemcpyHostToDevice, s.stream);
+  if ((rc = mt_fields(s, n_gauss, n_unif))) return rc;
+  printf("No. of gauss random numbers: %d\n", MT_COUNT * s.n_per_gauss);
+  printf("No. of unif. dist. random numbers: %d\n", MT_COUNT * s.n_per_unif);
+  mt_refill(s, 0);
+  return hip_check(hipStreamSynchronize(s.stream), "init_MT");
+}
+
+void mt_free(State& s) {
+  hipFree(s.rnd_gauss);
+  hipFree(s.rnd_unif);
+  hipFree(s.mt_params);
+  hipFree(s.mt_state);
+  s.rnd_gauss = s.rnd_unif = nullptr;
+  s.mt_params = s.mt_state = nullptr;
+  s.n_per_gauss = s.n_per_unif = 0;
+}
+
+}  // namespace gfx
+
+using namespace gfx;
+
+extern "C" {
+
+// parameter globals of the input grammar (read_input.l:29-46), defined by
+// the host input parser of this library
+typedef struct {
+  double Tmin, Tmax, expo;
+  int N, checkint;
+} gfx_saparam_t;
+typedef struct {
+  int Nsweep, Nsave;
+  double beta;
+  int checkint, startcond;
+} gfx_thermparam_t;
+extern gfx_saparam_t saparam;
+extern gfx_thermparam_t thermparam;
+
+int gfx_set_mt_seed(long seed) {
+  g_st.mt_seed = seed;
+  return GFX_OK;
+}
+
+int gfx_set_mt_file(const char* path) {
+  if (!path || strlen(path) >= sizeof(g_rng.dat_path)) return GFX_ERR_STATE;
+  strcpy(g_rng.dat_path, path);
+  g_rng.have_params = false;
+  return GFX_OK;
+}
+
+// twister parameter records supplied directly: 4096 x {matrix_a, mask_b, mask_c, seed}
+int gfx_set_mt_params(const uint32_t* params) {
+  g_rng.host_params.assign(params, params + 4 * MT_COUNT);
+  g_rng.have_params = true;
+  return GFX_OK;
+}
+
+int init_MT(int n_gaussnumbers, int n_unifnumbers) {
+  printf("Initializing MT random number generator...\n");
+  return mt_init_device(g_st, n_gaussnumbers, n_unifnumbers);
+}
+
+void update_MT(void) {
+  State& s = g_st;
+  if (!s.mt_state) return;
+  mt_refill(s, 1);
+  hip_check(hipStreamSynchronize(s.stream), "update_MT");
+}
+
+void finalize_MT(void) { mt_free(g_st); }
+
+// copies of the random fields and twister state, for inspection
+int gfx_get_mt_fields(float* gauss, float* unif, uint32_t* state, uint32_t* params) {
+  State& s = g_st;
+  if (!s.mt_state) return GFX_ERR_STATE;
+  if (gauss) hipMemcpy(gauss, s.rnd_gauss, sizeof(float) * MT_COUNT * s.n_per_gauss, hipMemcpyDeviceToHost);
+  if (unif) hipMemcpy(unif, s.rnd_unif, sizeof(float) * MT_COUNT * s.n_per_unif, hipMemcpyDeviceToHost);
+  if (state) hipMemcpy(state, s.mt_state, sizeof(uint32_t) * 19 * MT_COUNT, hipMemcpyDeviceToHost);
+  if (params) hipMemcpy(params, s.mt_params, sizeof(uint32_t) * 4 * MT_COUNT, hipMemcpyDeviceToHost);
+  return GFX_OK;
+}
+
+// the MT field sizes per twister (N_PER_RNG_GAUSS, N_PER_RNG_UNIF)
+int gfx_mt_sizes(int* n_per_gauss, int* n_per_unif) {
+  if (n_per_gauss) *n_per_gauss = g_st.n_per_gauss;
+  if (n_per_unif) *n_per_unif = g_st.n_per_unif;
+  return GFX_OK;
+}
+
+// sampler 0: the reference's rule; 1: Kennedy-Pendleton.  nmicro
+// microcanonical sweeps follow every SA sweep.  ndir 4 Landau, 3 Coulomb.
+int gfx_set_sa(int sampler, int nmicro, int ndir) {
+  g_rng.sa_sampler = sampler ? 1 : 0;
+  g_rng.sa_nmicro = nmicro < 0 ? 0 : nmicro;
+  g_rng.sa_ndir = (ndir == 3) ? 3 : 4;
+  return GFX_OK;
+}
+
+// set_sa_temperature (simulated_annealing.cu:265-299): the temperature of step i
+double gfx_sa_temperature(int i, double Tmin, double Tmax, double expo, int N) {
+  double a, temperature;
+  if (Tmax == Tmin) {
+    temperature = Tmax;
+  } else if (expo == 0) {
+    a = (double)(i) / (double)(N - 1);
+    temperature = pow(Tmin / Tmax, a) * Tmax;
+  } else if (expo == -1) {
+    a = (Tmin - Tmax) / (double)(N - 1);
+    temperature = (a * i + Tmax);
+  } else {
+    a = pow(Tmin, -expo) - pow(Tmax, -expo);
+    a = a / (double)(N - 1);
+    temperature = pow((a * i + pow(Tmax, -expo)), (-1.0 / expo));
+  }
+  return temperature;
+}
+
+void simannealing_gauge(void) {
+  State& s = g_st;
+  if (s.init != 1) return;
+  const int ndir = g_rng.sa_ndir;
+  const int sampler = g_rng.sa_sampler;
+  const int nh = s.geo.nh;
+  // random numbers per half sweep: reference rule 1 gauss + 4 unif per site
+  const int need_g = (sampler == 0) ? nh : kp_gauss_per_site() * nh;
+  const int need_u = (sampler == 0) ? 4 * nh : kp_unif_per_site() * nh;
+  if (!s.mt_state) {
+    if (mt_init_device(s, need_g, need_u)) return;
+  } else if (MT_COUNT * s.n_per_gauss < need_g || MT_COUNT * s.n_per_unif < need_u) {
+    if (mt_fields(s, need_g, need_u)) return;
+    mt_refill(s, 1);
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  const int ck = saparam.checkint > 0 ? saparam.checkint : 1;
+  for (int i = 0; i < saparam.N; i++) {
+    const double T = gfx_sa_temperature(i, saparam.Tmin, saparam.Tmax, saparam.expo, saparam.N);
+    const double beta = 1.0 / T;
+    for (int p = 0; p < 2; p++) {
+      launch_sa_sweep(s.U64, s.g64, s.geo, p, ndir, beta, sampler, s.rnd_gauss, s.rnd_unif, s.stream);
+      mt_refill(s, 1);
+    }
+    for (int m = 0; m < g_rng.sa_nmicro; m++)
+      for (int p = 0; p < 2; p++) launch_micro_sweep(s.U64, s.g64, s.geo, p, ndir, s.stream);
+    if ((i % ck) == 0) {
+      if (functional_now(s, false, ndir, 1, 0.0)) return;
+      FUNC = s.h_results[0];
+      DADA = s.h_results[1];
+      maxDADA = s.h_results[2];
+      if (s.verbose)
+        printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e\n", i, FUNC, DADA, maxDADA);
+    }
+  }
+  hipStreamSynchronize(s.stream);
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (s.verbose) printf("SA finished after %f sec\n", sec);
+}
+
+// one full heatbath sweep (4 directions x 2 parities) at beta_eff = beta/3
+int gfx_heatbath_sweeps(int nsweep, double beta) {
+  State& s = g_st;
+  if (!s.init) return GFX_ERR_STATE;
+  const int nh = s.geo.nh;
+  const int need_g = kp_gauss_per_site() * nh, need_u = kp_unif_per_site() * nh;
+  if (!s.mt_state) {
+    int rc = mt_init_device(s, need_g, need_u);
+    if (rc) return rc;
+  } else if (MT_COUNT * s.n_per_gauss < need_g || MT_COUNT * s.n_per_unif < need_u) {
+    if (mt_fields(s, need_g, need_u)) return GFX_ERR_ALLOC;
+    mt_refill(s, 1);
+  }
+  for (int k = 0; k < nsweep; k++)
+    for (int nu = 0; nu < 4; nu++)
+      for (int p = 0; p < 2; p++) {
+        launch_link_heatbath(s.U64, s.geo, p, nu, beta / 3.0, s.rnd_gauss, s.rnd_unif, s.stream);
+        mt_refill(s, 1);
+      }
+  s.U32_valid = false;
+  return hip_check(hipStreamSynchronize(s.stream), "heatbath");
+}
+
+void thermalize_gauge(void) {
+  State& s = g_st;
+  if (!s.init) return;
+  printf("Setting beta to %f\n", thermparam.beta);
+  FILE* plaqfile = fopen("plaq.txt", "w");
+  if (!plaqfile) fprintf(stderr, "Could not open file 'plaq.txt'\n");
+  const auto t0 = std::chrono::steady_clock::now();
+  const int ck = thermparam.checkint > 0 ? thermparam.checkint : 1;
+  for (int i = 0; i < thermparam.Nsweep; i++) {
+    if (gfx_heatbath_sweeps(1, thermparam.beta)) break;
+    if ((i % ck) == 0) {
+      const double plaq = calc_plaquette(0);
+      if (plaqfile) fprintf(plaqfile, "%d %.16e\n", i, plaq);
+      printf("therm %6d:\t PLAQ = %.16e\n", i, plaq);
+    }
+  }
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (plaqfile) fclose(plaqfile);
+  printf("Thermalization finished after %f sec\n", sec);
+}
+
+}  // extern "C"

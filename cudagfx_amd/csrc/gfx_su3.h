@@ -1,0 +1,291 @@
+// gfx_su3.h -- SU(3) algebra for the gfx950 kernels.
+//
+// Complex 3x3 matrices live entirely in VGPRs (every loop is unrolled, no
+// dynamic indexing).  The operation order of each routine follows the
+// reference device code (dev_su3.cu, overrelax.cu) so that the FP64 path
+// reproduces its arithmetic up to fused multiply-add rounding:
+//   * products accumulate k = 0,1,2 as  s = v[i][k]*w[k][j] + s
+//     (dev_su3.cu:1418-1475);
+//   * the third row of a 2-row matrix is conj(row0 x row1)
+//     (dev_su3.cu:289-305, 1617-1632);
+//   * Gram-Schmidt normalisation with rsqrt (dev_su3.cu:1569-1634).
+// Templated on the arithmetic type R (double for the plain path, float for
+// the bulk sweeps of the mixed path).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace gfx {
+
+template <typename R>
+struct Cx {
+  R re, im;
+};
+
+template <typename R>
+__device__ __forceinline__ Cx<R> cx(R re, R im) {
+  Cx<R> c;
+  c.re = re;
+  c.im = im;
+  return c;
+}
+template <typename R>
+__device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b) {
+  return cx<R>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+// a * conj(b)
+template <typename R>
+__device__ __forceinline__ Cx<R> cmulc(Cx<R> a, Cx<R> b) {
+  return cx<R>(a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im);
+}
+template <typename R>
+__device__ __forceinline__ Cx<R> cadd(Cx<R> a, Cx<R> b) {
+  return cx<R>(a.re + b.re, a.im + b.im);
+}
+template <typename R>
+__device__ __forceinline__ Cx<R> csub(Cx<R> a, Cx<R> b) {
+  return cx<R>(a.re - b.re, a.im - b.im);
+}
+template <typename R>
+__device__ __forceinline__ Cx<R> cconj(Cx<R> a) {
+  return cx<R>(a.re, -a.im);
+}
+
+template <typename R>
+struct M3 {
+  Cx<R> e[3][3];
+};
+
+template <typename R>
+__device__ __forceinline__ void m_zero(M3<R>& m) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) m.e[i][j] = cx<R>(R(0), R(0));
+}
+
+// third row = conj(row0 x row1)
+template <typename R>
+__device__ __forceinline__ void m_row2(M3<R>& u) {
+  u.e[2][0] = cconj(csub(cmul(u.e[0][1], u.e[1][2]), cmul(u.e[0][2], u.e[1][1])));
+  u.e[2][1] = cconj(csub(cmul(u.e[0][2], u.e[1][0]), cmul(u.e[0][0], u.e[1][2])));
+  u.e[2][2] = cconj(csub(cmul(u.e[0][0], u.e[1][1]), cmul(u.e[0][1], u.e[1][0])));
+}
+
+// Dagger of a 2-row matrix whose rows sit in u.e[0..1]: the result's
+// columns 0,1 are the conjugated rows, column 2 the conjugated cross product
+// (dev_su3.cu:826-929).
+template <typename R>
+__device__ __forceinline__ void m_dagger_from_rows(M3<R>& d, const Cx<R> r0[3], const Cx<R> r1[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    d.e[i][0] = cconj(r0[i]);
+    d.e[i][1] = cconj(r1[i]);
+  }
+  d.e[0][2] = csub(cconj(cmul(d.e[1][0], d.e[2][1])), cconj(cmul(d.e[2][0], d.e[1][1])));
+  d.e[1][2] = csub(cconj(cmul(d.e[2][0], d.e[0][1])), cconj(cmul(d.e[0][0], d.e[2][1])));
+  d.e[2][2] = csub(cconj(cmul(d.e[0][0], d.e[1][1])), cconj(cmul(d.e[1][0], d.e[0][1])));
+}
+
+// u += v * w   (dev_add_su3_ti_su3)
+template <typename R>
+__device__ __forceinline__ void m_addmul(M3<R>& u, const M3<R>& v, const M3<R>& w) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      Cx<R> s = cmul(v.e[i][0], w.e[0][j]);
+      s = cadd(cmul(v.e[i][1], w.e[1][j]), s);
+      s = cadd(cmul(v.e[i][2], w.e[2][j]), s);
+      u.e[i][j].re += s.re;
+      u.e[i][j].im += s.im;
+    }
+}
+
+// u = v * w   (dev_su3_ti_su3)
+template <typename R>
+__device__ __forceinline__ M3<R> m_mul(const M3<R>& v, const M3<R>& w) {
+  M3<R> u;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      Cx<R> s = cmul(v.e[i][0], w.e[0][j]);
+      s = cadd(cmul(v.e[i][1], w.e[1][j]), s);
+      s = cadd(cmul(v.e[i][2], w.e[2][j]), s);
+      u.e[i][j] = s;
+    }
+  return u;
+}
+
+// u = v * w^+   (dev_su3_ti_su3d)
+template <typename R>
+__device__ __forceinline__ M3<R> m_mul_d(const M3<R>& v, const M3<R>& w) {
+  M3<R> u;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      Cx<R> s = cmulc(v.e[i][0], w.e[j][0]);
+      s = cadd(cmulc(v.e[i][1], w.e[j][1]), s);
+      s = cadd(cmulc(v.e[i][2], w.e[j][2]), s);
+      u.e[i][j] = s;
+    }
+  return u;
+}
+
+// element (i,j) of v * w
+template <typename R>
+__device__ __forceinline__ Cx<R> m_mul_elem(const M3<R>& v, const M3<R>& w, int i, int j) {
+  Cx<R> s = cmul(v.e[i][0], w.e[0][j]);
+  s = cadd(cmul(v.e[i][1], w.e[1][j]), s);
+  s = cadd(cmul(v.e[i][2], w.e[2][j]), s);
+  return s;
+}
+
+template <typename R>
+__device__ __forceinline__ R rsqrt_t(R x);
+template <>
+__device__ __forceinline__ double rsqrt_t<double>(double x) {
+  return ::rsqrt(x);
+}
+template <>
+__device__ __forceinline__ float rsqrt_t<float>(float x) {
+  return ::rsqrtf(x);
+}
+
+// Gram-Schmidt on rows 0,1; row 2 from the cross product (dev_su3.cu:1569-1634)
+template <typename R>
+__device__ __forceinline__ void m_normalize(M3<R>& u) {
+  R len = (u.e[0][0].re * u.e[0][0].re + u.e[0][0].im * u.e[0][0].im) +
+          (u.e[0][1].re * u.e[0][1].re + u.e[0][1].im * u.e[0][1].im) +
+          (u.e[0][2].re * u.e[0][2].re + u.e[0][2].im * u.e[0][2].im);
+  len = rsqrt_t<R>(len);
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    u.e[0][i].re = u.e[0][i].re * len;
+    u.e[0][i].im = u.e[0][i].im * len;
+  }
+  Cx<R> proj = cmulc(u.e[1][0], u.e[0][0]);
+  proj = cadd(proj, cmulc(u.e[1][1], u.e[0][1]));
+  proj = cadd(proj, cmulc(u.e[1][2], u.e[0][2]));
+#pragma unroll
+  for (int i = 0; i < 3; i++) u.e[1][i] = csub(u.e[1][i], cmul(proj, u.e[0][i]));
+  len = (u.e[1][0].re * u.e[1][0].re + u.e[1][0].im * u.e[1][0].im) +
+        (u.e[1][1].re * u.e[1][1].re + u.e[1][1].im * u.e[1][1].im) +
+        (u.e[1][2].re * u.e[1][2].re + u.e[1][2].im * u.e[1][2].im);
+  len = rsqrt_t<R>(len);
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    u.e[1][i].re = u.e[1][i].re * len;
+    u.e[1][i].im = u.e[1][i].im * len;
+  }
+  m_row2(u);
+}
+
+// Left multiplication of rows a,b of g by the SU(2) element
+// [[w0 + i w3, w2 + i w1], [-w2 + i w1, w0 - i w3]]  (overrelax.cu:244-265)
+template <typename R>
+__device__ __forceinline__ void su2_left(M3<R>& g, int a, int b, R w0, R w1, R w2, R w3) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    Cx<R> d = cadd(cmul(cx<R>(w0, w3), g.e[a][c]), cmul(cx<R>(w2, w1), g.e[b][c]));
+    g.e[b][c] = cadd(cmul(cx<R>(-w2, w1), g.e[a][c]), cmul(cx<R>(w0, -w3), g.e[b][c]));
+    g.e[a][c] = d;
+  }
+}
+
+// The SU(2) subgroup vector w of X = g*K for rows (a,b)  (overrelax.cu:217-223).
+// Only the four entries of X that enter are formed; each with the same
+// k-order as the full product, so the values equal those of the full product.
+template <typename R>
+__device__ __forceinline__ void su2_subvec(const M3<R>& g, const M3<R>& K, int a, int b,
+                                           R& w0, R& w1, R& w2, R& w3) {
+  Cx<R> xaa = m_mul_elem(g, K, a, a);
+  Cx<R> xbb = m_mul_elem(g, K, b, b);
+  Cx<R> xab = m_mul_elem(g, K, a, b);
+  Cx<R> xba = m_mul_elem(g, K, b, a);
+  w0 = xaa.re + xbb.re;
+  w3 = -xaa.im + xbb.im;
+  w1 = -xab.im - xba.im;
+  w2 = -xab.re + xba.re;
+}
+
+// Cabibbo-Marinari maximisation of Re tr(g K) over the subgroups
+// (0,1),(0,2),(1,2) (overrelax.cu:188-269)
+template <typename R>
+__device__ __forceinline__ void cm_relax(M3<R>& g, const M3<R>& K) {
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = a + 1; b < 3; b++) {
+      R w0, w1, w2, w3;
+      su2_subvec(g, K, a, b, w0, w1, w2, w3);
+      R det = w0 * w0 + w1 * w1 + w2 * w2 + w3 * w3;
+      det = rsqrt_t<R>(det);
+      su2_left(g, a, b, w0 * det, w1 * det, w2 * det, w3 * det);
+    }
+}
+
+// g <- normalize(1 + w R + w(w-1)/2 R^2) g with R = gn g^+ - 1: the
+// reference's 3-term binomial series (overrelax.cu:149-177, N = 3).
+// c1 = (1+w-1)/1 and c2 = (1+w-2)/2 are formed on the host exactly as the
+// reference forms them.  For general N the loop form is kept.
+template <typename R>
+__device__ __forceinline__ void overrelax_apply(M3<R>& g, const M3<R>& gn, R c1, R c2) {
+  M3<R> Rm = m_mul_d(gn, g);
+  Rm.e[0][0].re -= R(1);
+  Rm.e[1][1].re -= R(1);
+  Rm.e[2][2].re -= R(1);
+  // a1 = (1 * R) * c1 ; Rw = 1 + a1
+  M3<R> a1, Rw;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      a1.e[i][j].re = Rm.e[i][j].re * c1;
+      a1.e[i][j].im = Rm.e[i][j].im * c1;
+      Rw.e[i][j] = a1.e[i][j];
+    }
+  Rw.e[0][0].re = R(1) + a1.e[0][0].re;
+  Rw.e[1][1].re = R(1) + a1.e[1][1].re;
+  Rw.e[2][2].re = R(1) + a1.e[2][2].re;
+  // a2 = (a1 * R) * c2 ; Rw = Rw + a2
+  M3<R> a2 = m_mul(a1, Rm);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      Rw.e[i][j].re = Rw.e[i][j].re + a2.e[i][j].re * c2;
+      Rw.e[i][j].im = Rw.e[i][j].im + a2.e[i][j].im * c2;
+    }
+  m_normalize(Rw);
+  g = m_mul(Rw, g);
+}
+
+// traceless anti-hermitian part (i/2)(U^+ - U) - tr/3 (overrelax.cu:451-473)
+template <typename R>
+__device__ __forceinline__ void vecpot(M3<R>& U) {
+  M3<R> t;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      Cx<R> d = csub(cconj(U.e[j][i]), U.e[i][j]);
+      // (0, 0.5) * d
+      t.e[i][j] = cx<R>(R(0) * d.re - R(0.5) * d.im, R(0) * d.im + R(0.5) * d.re);
+    }
+  Cx<R> tr = cadd(cadd(t.e[0][0], t.e[1][1]), t.e[2][2]);
+  tr.re = tr.re / R(3);
+  tr.im = tr.im / R(3);
+#pragma unroll
+  for (int i = 0; i < 3; i++) t.e[i][i] = csub(t.e[i][i], tr);
+  U = t;
+}
+
+template <typename R>
+__device__ __forceinline__ R m_retrace(const M3<R>& m) {
+  return (m.e[0][0].re + m.e[1][1].re) + m.e[2][2].re;
+}
+
+}  // namespace gfx

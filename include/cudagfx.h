@@ -1,0 +1,160 @@
+/*
+ * cudagfx.h -- C ABI of libcudagfx.so, the MI355X (gfx950) implementation of
+ * the cudagfx gauge-fixing path.
+ *
+ * The first block carries the reference's own entry points, same names,
+ * same argument meaning (reference file:line cited per declaration); a
+ * driver written against the reference's cudagaugefix.cu calls them
+ * unchanged.  Differences are stated per function: entry points that the
+ * reference declared `void` and ended with exit(code) return that code as
+ * an int instead, and functions that took device pointers take none (the
+ * library owns the device fields between init_* and finalize_*).
+ *
+ * Host matrices use the reference layout: su3 = 3x3 complex double,
+ * row-major (global.h:26-33); gauge field gf[4*s+mu], trafo g[s] with
+ * s = x + LX*(y + LY*(z + LZ*t)) (cudagaugefix.cu:79-85).
+ *
+ * Single instance, global state, one host thread, as the reference.  Every
+ * value is final on return.
+ */
+#ifndef CUDAGFX_H
+#define CUDAGFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef GLOBAL_H
+/* the reference's global.h came first: use its su3 type */
+typedef su3 gfx_su3;
+#else
+typedef struct {
+  double re, im;
+} gfx_complex;
+typedef gfx_complex gfx_su3[3][3];
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: the reference's exit codes (cudagaugefix.cu, overrelax.cu) ---- */
+#define GFX_OK 0
+#define GFX_ERR_GEOMETRY 100    /* overrelax.cu:687-690: (V/2) % BLOCK, shape rules   */
+#define GFX_ERR_ALLOC 200       /* cudagaugefix.cu:216-219                            */
+#define GFX_ERR_NOCONV 300      /* cudagaugefix.cu:741-748                            */
+#define GFX_ERR_WRITE 400       /* cudagaugefix.cu:808-811                            */
+#define GFX_ERR_STATE 600       /* called before init_* or after finalize_*           */
+#define GFX_ERR_DEVICE 700      /* a HIP runtime call failed                          */
+#define GFX_ERR_MTDAT 800       /* MersenneTwister.cu:81-90 (.dat missing or short)   */
+
+/* ---- lattice globals (global.h:77, 86-94), defined by the library ---- */
+extern int T, L, LX, LY, LZ, VOLUME;
+extern double FUNC, DADA, maxDADA, PLAQ, BETA;
+
+/* ================= reference entry points ================= */
+
+/* cudagaugefix.cu:204 -- copy U (4V su3) and g (V su3) to the device in the
+ * even/odd structure-of-arrays layout.  Returns GFX_OK, GFX_ERR_GEOMETRY or
+ * GFX_ERR_ALLOC (reference: void + exit(100|200)). */
+int init_gaugefixing(gfx_su3 *gf, gfx_su3 *trafo);
+
+/* cudagaugefix.cu:381 */
+void finalize_gaugefixing(void);
+
+/* cudagaugefix.cu:420 -- device copy of U for thermalization. */
+int init_thermalization(gfx_su3 *gf);
+
+/* cudagaugefix.cu:556 */
+void finalize_thermalization(void);
+
+/* overrelax.cu:677 -- red-black overrelaxation until max_x dAdA(x) < eps,
+ * checked every `checkinterval` iterations; returns the stop iteration or
+ * -1.  Prints the reference's "iter" / "CONVERGENCE!" lines
+ * (overrelax.cu:782, 795-798) unless gfx_set_verbose(0).  Uses the
+ * precision mode of gfx_set_precision and omega of gfx_set_omega. */
+int overrelax_gauge(int maxit, double eps, int checkinterval);
+
+/* overrelax.cu:610 -- F, dAdA, max dAdA of the current g; sets FUNC, DADA,
+ * maxDADA and returns maxDADA.  (reference: took the device fields as
+ * arguments; here the library's own fields are used.) */
+double calc_functional(void);
+
+/* overrelax.cu:115 -- mean plaquette of the device links; prints
+ * "PLAQ = %.16f" when outputyn == 1. */
+double calc_plaquette(int outputyn);
+
+/* simulated_annealing.cu:309 -- SA on g with the schedule in saparam. */
+void simannealing_gauge(void);
+
+/* heatbath_thermalization.cu:568 -- link heatbath with thermparam. */
+void thermalize_gauge(void);
+
+/* MersenneTwister.cu:236 -- load MersenneTwister.dat from the working
+ * directory, seed the 4096 twisters from srand(seed)/rand() (seed:
+ * gfx_set_mt_seed, else time(NULL) as the reference) and fill the gauss and
+ * uniform fields.  Returns GFX_ERR_MTDAT if the file is missing. */
+int init_MT(int n_gaussnumbers, int n_unifnumbers);
+/* MersenneTwister.cu:295 */
+void update_MT(void);
+/* MersenneTwister.cu:327 */
+void finalize_MT(void);
+
+/* ================= replacements for direct device-pointer use ================= */
+
+/* cudagaugefix.cu:789-792 (cudaMemcpy D2H of dev_trafo1 + from2vtosu3_trafo):
+ * current g as V host su3 matrices, row 2 rebuilt from rows 0,1. */
+int get_trafo(gfx_su3 *trafo);
+/* set g from V host su3 matrices (rows 0,1 are used). */
+int set_trafo(gfx_su3 *trafo);
+/* current device links as 4V host su3 matrices (row 2 rebuilt). */
+int get_gauge(gfx_su3 *gf);
+/* observables.c:17-47 (g_trafo) on the device: gf[4s+mu] <- g(s) gf g^+(s+mu)
+ * for host links gf (all three rows used), with the current g. */
+int apply_trafo(gfx_su3 *gf);
+
+/* ================= extensions (no counterpart in the reference) ================= */
+
+#define GFX_PREC_FP64 0   /* plain FP64 path                                    */
+#define GFX_PREC_MIXED 1  /* FP32 12-real bulk sweeps, FP64 reductions + polish */
+
+#define GFX_GAUGE_LANDAU 0
+#define GFX_GAUGE_COULOMB 1
+
+/* overrelaxation parameter; the reference hard-codes 1.68 (overrelax.cu:416) */
+int gfx_set_omega(double omega);
+/* precision mode; switch_theta = max dAdA at which the mixed path leaves
+ * FP32 (default 1e-6) */
+int gfx_set_precision(int mode, double switch_theta);
+/* 0 silences the per-check log lines */
+int gfx_set_verbose(int verbose);
+/* seed for srand() before the twister seeds are drawn (negative: time(NULL)) */
+int gfx_set_mt_seed(long seed);
+
+/* Coulomb gauge: every time slice is gauge fixed over the 3 spatial
+ * directions; a slice stops updating once its own max dAdA < eps (checked
+ * every checkinterval).  iters[t] receives the stop iteration of slice t
+ * (-1 if not reached).  Returns GFX_OK or GFX_ERR_NOCONV. */
+int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int *iters);
+/* per-slice F, dAdA, max dAdA (arrays of length T) for the current g */
+int gfx_functional_slices(double *F, double *dAdA, double *maxdAdA);
+
+/* one red-black half sweep of the given parity (0 even, 1 odd) in the current
+ * precision mode; ndir 4 (Landau) or 3 (Coulomb, all slices) */
+int gfx_halfsweep(int parity, int ndir);
+
+/* Timing of the plain sweep: runs `nsweeps` full sweeps (even + odd) in
+ * `precision` and returns the mean kernel time of one half-sweep launch in
+ * ms, measured with HIP events on the library's stream. */
+int gfx_time_sweeps(int nsweeps, int precision, double *ms_per_halfsweep);
+/* wall-clock of the last overrelax_gauge / gfx_coulomb_gauge call in ms
+ * (first sweep to the stop decision), and the number of FP32 and FP64
+ * iterations it ran. */
+int gfx_last_run_stats(double *ms, int *iters_fp32, int *iters_fp64);
+
+/* last error as text */
+const char *gfx_error_string(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

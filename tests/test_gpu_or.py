@@ -1,0 +1,146 @@
+"""GPU parity of the overrelaxation path against the oracle (through the C ABI).
+
+Inputs are seeded (Lehmer RNG, rngs.c) and identical for both sides; the
+FP64 path must agree with the oracle to 1e-10 max-abs per matrix element and
+stop at the same iteration (+-2) with checkinterval 1.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPS12 = float(np.float32(1e-12))  # (double)(float)1e-12, read_input.l:189-192
+
+
+def _setup(gfx, dims, u, g, omega=1.68, prec=0):
+    gfx.set_lattice(*dims)
+    gfx.set_verbose(0)
+    gfx.set_omega(omega)
+    gfx.set_precision(prec)
+    gfx.init_gaugefixing(u, g)
+
+
+def smooth_config(orc, dims, seed, eps=0.25):
+    """A gauge copy of a near-unit field: U = h(x) W h^+(x+mu) with W close
+    to 1, built from the oracle's random SU(3) matrices."""
+    V = orc.volume(dims)
+    w = orc.random_init_gauge(V, seed)
+    unit = np.zeros(18)
+    unit[[0, 8, 16]] = 1.0
+    # pull every link towards 1 and reunitarise
+    m = (unit[None, :] + eps * w.reshape(-1, 18)).copy()
+    r12 = orc.su3_to_2v(m.ravel())
+    # Gram-Schmidt through the oracle normalisation
+    out = np.zeros_like(m)
+    for k in range(m.shape[0]):
+        a = orc.from_2v(r12[12 * k:12 * k + 12]).copy()
+        orc.lib.orc_su3_normalize(a)
+        out[k] = a
+    h = orc.random_init_trafo(V, seed + 1)
+    return orc.g_trafo(out.ravel(), h, dims)
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (8, 8, 8, 8), (4, 6, 8, 2)])
+@pytest.mark.parametrize("parity", [0, 1])
+def test_halfsweep_matches_oracle(gfx, orc, dims, parity):
+    V = orc.volume(dims)
+    u = orc.random_init_gauge(V, 4357)
+    g = orc.random_init_trafo(V, 351)
+    _setup(gfx, dims, u, g)
+    gfx.halfsweep(parity, 4)
+    g_gpu = gfx.get_trafo()
+    gfx.finalize_gaugefixing()
+    u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+    orc.halfsweep(u2, g2, dims, parity, 4, 1.68)
+    g_cpu = orc.from_2v(g2)
+    assert np.abs(g_gpu - g_cpu).max() < 1e-13
+
+
+def test_functional_matches_oracle(gfx, orc):
+    dims = (8, 8, 8, 8)
+    V = orc.volume(dims)
+    u = orc.random_init_gauge(V, 11)
+    g = orc.random_init_trafo(V, 12)
+    _setup(gfx, dims, u, g)
+    m = gfx.calc_functional()
+    F, D, M = gfx.results()
+    plaq = gfx.calc_plaquette(0)
+    gfx.finalize_gaugefixing()
+    Fo, Do, Mo = orc.functional(orc.su3_to_2v(u), orc.su3_to_2v(g), dims)
+    assert abs(F - Fo) < 1e-13 and abs(D - Do) / Do < 1e-13 and m == M
+    assert abs(M - Mo) / Mo < 1e-13
+    assert abs(plaq - orc.plaquette_2v(orc.su3_to_2v(u), dims)) < 1e-13
+
+
+@pytest.mark.parametrize("dims,omega", [((8, 8, 8, 8), 1.68), ((8, 8, 8, 8), 1.7),
+                                        ((4, 4, 4, 8), 1.0)])
+def test_overrelax_run_matches_oracle(gfx, orc, dims, omega):
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, 77)
+    g = orc.random_init_trafo(V, 351)
+    _setup(gfx, dims, u, g, omega)
+    it_gpu = gfx.overrelax_gauge(3000, EPS12, 1)
+    F_gpu = gfx.results()[0]
+    g_gpu = gfx.get_trafo()
+    fixed_gpu = gfx.apply_trafo(u)
+    gfx.finalize_gaugefixing()
+    u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+    it_cpu, hist = orc.overrelax_gauge(u2, g2, dims, 3000, EPS12, 1, omega)
+    g_cpu = orc.from_2v(g2)
+    fixed_cpu = orc.g_trafo(u, g_cpu, dims)
+    assert it_cpu > 0 and it_gpu > 0
+    assert abs(it_gpu - it_cpu) <= 2
+    assert np.abs(g_gpu - g_cpu).max() < 1e-10
+    assert np.abs(fixed_gpu - fixed_cpu).max() < 1e-10
+    assert abs(F_gpu - hist[-1, 0]) < 1e-12
+
+
+def test_cold_links_stop_at_zero(gfx, orc):
+    """U = 1 and g = 1: F = 3, dAdA = 0, so the first check stops at i = 0."""
+    dims = (4, 4, 4, 4)
+    V = orc.volume(dims)
+    u = np.zeros(18 * 4 * V)
+    u.reshape(-1, 18)[:, [0, 8, 16]] = 1.0
+    g = np.zeros(18 * V)
+    g.reshape(-1, 18)[:, [0, 8, 16]] = 1.0
+    _setup(gfx, dims, u, g)
+    assert gfx.overrelax_gauge(100, EPS12, 1) == 0
+    F, D, M = gfx.results()
+    gfx.finalize_gaugefixing()
+    assert abs(F - 3.0) < 1e-15 and D == 0.0 and M == 0.0
+
+
+def test_pure_gauge_fixes_to_unit(gfx, orc):
+    """U = h(x) h^+(x+mu) is a pure gauge: the fixed field has F = 3."""
+    dims = (8, 8, 8, 8)
+    V = orc.volume(dims)
+    unit = np.zeros(18 * 4 * V)
+    unit.reshape(-1, 18)[:, [0, 8, 16]] = 1.0
+    u = orc.g_trafo(unit, orc.random_init_trafo(V, 5), dims)
+    g = orc.random_init_trafo(V, 6)
+    _setup(gfx, dims, u, g)
+    it = gfx.overrelax_gauge(5000, EPS12, 1)
+    F, D, M = gfx.results()
+    fixed = gfx.apply_trafo(u)
+    gfx.finalize_gaugefixing()
+    assert it > 0 and M < EPS12
+    assert abs(F - 3.0) < 1e-12
+    assert abs(orc.gauge_functional(fixed, dims) - 3.0) < 1e-12
+
+
+def test_get_set_trafo_roundtrip(gfx, orc):
+    dims = (4, 4, 6, 8)
+    V = orc.volume(dims)
+    u = orc.random_init_gauge(V, 1)
+    g = orc.random_init_trafo(V, 2)
+    _setup(gfx, dims, u, g)
+    back = gfx.get_trafo()
+    links = gfx.get_gauge()
+    g2 = orc.random_init_trafo(V, 3)
+    gfx.set_trafo(g2)
+    back2 = gfx.get_trafo()
+    gfx.finalize_gaugefixing()
+    # rows 0,1 exact; row 2 rebuilt from rows 0,1 (from2vtosu3_trafo)
+    assert np.array_equal(back, orc.from_2v(orc.su3_to_2v(g)))
+    assert np.array_equal(links, orc.from_2v(orc.su3_to_2v(u)))
+    assert np.array_equal(back2, orc.from_2v(orc.su3_to_2v(g2)))

@@ -1,0 +1,312 @@
+"""Pins of the CPU oracle (oracle/oracle.c): golden vectors of the reference's
+own host C routines, the live reference build where present, known-answer
+tests and the closed forms / invariants of the algorithm."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _gold():
+    return np.load(os.path.join(GOLD, "ref_fields.npz"))
+
+
+# ------------------------------------------------------------------ golden vectors
+def test_lehmer_kat(orc):
+    """TestRandom (rngs.c:153-179): seed 1, 10000 draws -> 399268537;
+    PlantSeeds(1) -> stream 1 = 22925."""
+    lib = orc.lib
+    lib.orc_lehmer_select_stream(0)
+    lib.orc_lehmer_put_seed(1)
+    for _ in range(10000):
+        lib.orc_lehmer_random()
+    assert lib.orc_lehmer_get_seed() == 399268537
+    lib.orc_lehmer_select_stream(1)
+    lib.orc_lehmer_plant_seeds(1)
+    assert lib.orc_lehmer_get_seed() == 22925
+    lib.orc_lehmer_select_stream(0)
+    gold = json.load(open(os.path.join(GOLD, "lehmer.json")))
+    assert gold["check_after_10000"] == 399268537 and gold["stream1_after_plant1"] == 22925
+    lib.orc_lehmer_plant_seeds(gold["seed"])
+    assert [lib.orc_lehmer_random() for _ in range(20)] == gold["draws"]
+
+
+def test_random_init_bitexact(orc):
+    gd = _gold()
+    dims = tuple(gd["dims"])
+    V = orc.volume(dims)
+    assert np.array_equal(orc.random_init_trafo(V, 351), gd["trafo"])
+    assert np.array_equal(orc.random_init_gauge(V, 4357), gd["gauge"])
+
+
+def test_host_observables_bitexact(orc):
+    gd = _gold()
+    dims = tuple(gd["dims"])
+    u, g = gd["gauge"], gd["trafo"]
+    ut = orc.g_trafo(u, g, dims)
+    assert np.array_equal(ut, gd["gauge_transformed"])
+    for tag, a in (("u", u), ("ut", ut)):
+        assert orc.gauge_functional(a, dims) == gd[tag + "_F"]
+        d, m = orc.dAdA(a, dims)
+        assert d == gd[tag + "_dAdA"] and m == gd[tag + "_maxdAdA"]
+        assert orc.mean_plaq(a, dims) == gd[tag + "_plaq"]
+
+
+def test_against_live_reference_build(orc):
+    """Where oracle/_ref was built from the reference checkout: bigger lattice."""
+    import ctypes as C
+
+    ref = orc.ref_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (no reference checkout)")
+    dims = (6, 4, 4, 8)
+    V = orc.volume(dims)
+    for n, v in zip(["LX", "LY", "LZ", "T", "VOLUME"], list(dims) + [V]):
+        C.c_int.in_dll(ref, n).value = v
+    nn = orc.neighbours(dims)
+    C.c_void_p.in_dll(ref, "nn").value = nn.ctypes.data
+    vp = C.c_void_p
+    ref.PlantSeeds.argtypes = [C.c_long]
+    ref.random_init_gauge.argtypes = [vp]
+    ref.random_init_trafo.argtypes = [vp]
+    ref.g_trafo.argtypes = [vp, vp]
+    for f in ("gauge_functional", "dAdA", "mean_plaq"):
+        getattr(ref, f).restype = C.c_double
+        getattr(ref, f).argtypes = [vp]
+    ref.PlantSeeds(99)
+    u = np.zeros(18 * 4 * V)
+    ref.random_init_gauge(u.ctypes.data_as(vp))
+    ref.PlantSeeds(98)
+    g = np.zeros(18 * V)
+    ref.random_init_trafo(g.ctypes.data_as(vp))
+    assert np.array_equal(u, orc.random_init_gauge(V, 99))
+    assert np.array_equal(g, orc.random_init_trafo(V, 98))
+    ut = u.copy()
+    ref.g_trafo(ut.ctypes.data_as(vp), g.ctypes.data_as(vp))
+    assert np.array_equal(ut, orc.g_trafo(u, g, dims))
+    assert ref.dAdA(ut.ctypes.data_as(vp)) == orc.dAdA(ut, dims)[0]
+    assert ref.mean_plaq(ut.ctypes.data_as(vp)) == orc.mean_plaq(ut, dims)
+
+
+# ------------------------------------------------------------------ helpers (numpy, independent)
+def mats(a):
+    r = a.reshape(-1, 3, 3, 2)
+    return r[..., 0] + 1j * r[..., 1]
+
+
+def flat(m):
+    return np.stack([m.real, m.imag], axis=-1).reshape(-1)
+
+
+def unitary_err(m):
+    e = m @ np.conj(np.swapaxes(m, -1, -2)) - np.eye(3)
+    return np.abs(e).max(), np.abs(np.linalg.det(m) - 1).max()
+
+
+def rand_su3(orc, n, seed):
+    return mats(orc.random_init_trafo(n, seed))
+
+
+# ------------------------------------------------------------------ closed forms
+def test_reconstruct_and_normalize_give_su3(orc):
+    m = orc.random_init_trafo(64, 7)
+    for k in range(64):
+        r = orc.su3_to_2v(m[18 * k:18 * k + 18])
+        full = np.zeros(18)
+        orc.lib.orc_reconstruct(r, full)
+        dag = np.zeros(18)
+        orc.lib.orc_reconstruct_dagger(r, dag)
+        M = mats(full)[0]
+        assert np.allclose(mats(dag)[0], M.conj().T, atol=0, rtol=0)
+        ue, de = unitary_err(M)
+        assert ue < 1e-14 and de < 1e-14
+    # normalise a perturbed matrix
+    a = m[:18] + 1e-3 * np.sin(np.arange(18))
+    orc.lib.orc_su3_normalize(a)
+    ue, de = unitary_err(mats(a)[0])
+    assert ue < 1e-14 and de < 1e-14
+
+
+def test_su2_relax_is_the_maximiser(orc):
+    """One CM step on a star that lives in the (0,1) subgroup: g K becomes the
+    identity in that block (Re tr(alpha w) is maximal at alpha = w/|w|)."""
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        s = rng.uniform(0.5, 2.0)
+        su2 = np.array([[q[0] + 1j * q[3], q[2] + 1j * q[1]], [-q[2] + 1j * q[1], q[0] - 1j * q[3]]])
+        K = np.eye(3, dtype=complex)
+        K[:2, :2] = s * su2
+        g = np.eye(3, dtype=complex)
+        ga = flat(g)
+        orc.lib.orc_cm_relax(ga, flat(K))
+        gk = mats(ga)[0] @ K
+        assert np.allclose(gk[:2, :2], s * np.eye(2), atol=1e-13)
+        # and no unit SU(2) rotation does better
+        F = np.real(np.trace(gk))
+        for _ in range(50):
+            r = rng.normal(size=4)
+            r /= np.linalg.norm(r)
+            R = np.eye(3, dtype=complex)
+            R[:2, :2] = [[r[0] + 1j * r[3], r[2] + 1j * r[1]], [-r[2] + 1j * r[1], r[0] - 1j * r[3]]]
+            assert np.real(np.trace(R @ mats(ga)[0] @ K)) <= F + 1e-12
+
+
+def test_cm_relax_does_not_decrease_functional(orc):
+    g = rand_su3(orc, 30, 3)
+    K = rand_su3(orc, 30, 4) + rand_su3(orc, 30, 5) + 0.3 * rand_su3(orc, 30, 6)
+    for k in range(30):
+        ga = flat(g[k])
+        before = np.real(np.trace(g[k] @ K[k]))
+        orc.lib.orc_cm_relax(ga, flat(K[k]))
+        assert np.real(np.trace(mats(ga)[0] @ K[k])) >= before - 1e-12
+
+
+@pytest.mark.parametrize("omega", [1.0, 2.0, 1.68, 1.7])
+def test_overrelax_series_closed_form(orc, omega):
+    """overrelax.cu:149-177 with N = 3: g -> normalize(1 + w R + w(w-1)/2 R^2) g,
+    R = gn g^+ - 1.  w = 1 gives gn, w = 2 gives (gn g^+)^2 g."""
+    g = rand_su3(orc, 10, 8)
+    gn = rand_su3(orc, 10, 9)
+    for k in range(10):
+        ga = flat(g[k])
+        orc.lib.orc_overrelax_step(ga, flat(gn[k]), omega, 3)
+        P = gn[k] @ g[k].conj().T
+        R = P - np.eye(3)
+        S = np.eye(3) + omega * R + omega * (omega - 1) / 2 * R @ R
+        # Gram-Schmidt of rows 0,1, row 2 = conj cross product
+        a = S[0] / np.linalg.norm(S[0])
+        b = S[1] - np.vdot(a, S[1]) * a
+        b /= np.linalg.norm(b)
+        Sn = np.array([a, b, np.conj(np.cross(a, b))])
+        want = Sn @ g[k]
+        assert np.abs(mats(ga)[0] - want).max() < 1e-13
+        if omega == 1.0:
+            assert np.abs(mats(ga)[0] - gn[k]).max() < 1e-13
+        if omega == 2.0:
+            assert np.abs(mats(ga)[0] - P @ P @ g[k]).max() < 1e-12
+
+
+def test_halfsweep_brute_force_2x4(orc):
+    """A 2^4 lattice (every neighbour pair is doubly connected) against a
+    direct numpy evaluation of the star, CM relaxation and overrelaxation."""
+    dims = (2, 2, 2, 2)
+    V = 16
+    u = orc.random_init_gauge(V, 21)
+    g = orc.random_init_trafo(V, 22)
+    U = mats(orc.from_2v(orc.su3_to_2v(u))).reshape(V, 4, 3, 3)
+    G = mats(orc.from_2v(orc.su3_to_2v(g)))
+    g2 = orc.su3_to_2v(g)
+    orc.halfsweep(orc.su3_to_2v(u), g2, dims, 0, 4, 1.68)
+    got = mats(orc.from_2v(g2))
+
+    def coords(s):
+        return [s % 2, (s // 2) % 2, (s // 4) % 2, s // 8]
+
+    def site(x):
+        return x[0] + 2 * (x[1] + 2 * (x[2] + 2 * x[3]))
+
+    for s in range(V):
+        x = coords(s)
+        if sum(x) % 2:
+            assert np.array_equal(got[s], G[s])  # odd sites untouched
+            continue
+        K = np.zeros((3, 3), complex)
+        for mu in range(4):
+            xp = list(x)
+            xp[mu] = (xp[mu] + 1) % 2
+            xm = list(x)
+            xm[mu] = (xm[mu] - 1) % 2
+            K += U[s, mu] @ G[site(xp)].conj().T + U[site(xm), mu].conj().T @ G[site(xm)].conj().T
+        h = G[s].copy()
+        for a, b in ((0, 1), (0, 2), (1, 2)):
+            X = h @ K
+            w = np.array([X[a, a].real + X[b, b].real, -X[a, b].imag - X[b, a].imag,
+                          -X[a, b].real + X[b, a].real, -X[a, a].imag + X[b, b].imag])
+            w /= np.linalg.norm(w)
+            A = np.array([[w[0] + 1j * w[3], w[2] + 1j * w[1]], [-w[2] + 1j * w[1], w[0] - 1j * w[3]]])
+            h[[a, b]] = A @ h[[a, b]]
+
+        def gs(M):
+            a0 = M[0] / np.linalg.norm(M[0])
+            b0 = M[1] - np.vdot(a0, M[1]) * a0
+            b0 /= np.linalg.norm(b0)
+            return np.array([a0, b0, np.conj(np.cross(a0, b0))])
+
+        h = gs(h)
+        R = h @ G[s].conj().T - np.eye(3)
+        S = gs(np.eye(3) + 1.68 * R + 1.68 * 0.68 / 2 * R @ R)
+        want = gs(S @ G[s])
+        assert np.abs(got[s] - want).max() < 1e-13
+
+
+def test_functional_of_cold_and_pure_gauge(orc):
+    dims = (4, 4, 4, 4)
+    V = orc.volume(dims)
+    unit = np.zeros(18 * 4 * V)
+    unit.reshape(-1, 18)[:, [0, 8, 16]] = 1.0
+    gu = np.zeros(18 * V)
+    gu.reshape(-1, 18)[:, [0, 8, 16]] = 1.0
+    F, D, M = orc.functional(orc.su3_to_2v(unit), orc.su3_to_2v(gu), dims)
+    assert F == 3.0 and D == 0.0 and M == 0.0
+    # pure gauge U = h h^+(x+mu) transformed back with g = h^+ gives F = 3
+    h = orc.random_init_trafo(V, 5)
+    u = orc.g_trafo(unit, h, dims)
+    hd = flat(np.conj(np.swapaxes(mats(h), -1, -2)))
+    F, D, M = orc.functional(orc.su3_to_2v(u), orc.su3_to_2v(hd), dims)
+    assert abs(F - 3.0) < 1e-14 and M < 1e-25
+
+
+def test_stop_rule_quirks(orc):
+    """overrelax.cu:775-822: the stop is checked only when i % ck == 0, so the
+    stop iteration is a multiple of ck; maxit = 0 returns 0."""
+    dims = (4, 4, 4, 4)
+    V = orc.volume(dims)
+    unit = np.zeros(18 * 4 * V)
+    unit.reshape(-1, 18)[:, [0, 8, 16]] = 1.0
+    u = orc.su3_to_2v(orc.g_trafo(unit, orc.random_init_trafo(V, 5), dims))
+    g = orc.su3_to_2v(orc.random_init_trafo(V, 6))
+    it1, h1 = orc.overrelax_gauge(u, g.copy(), dims, 5000, 1e-12, 1)
+    it7, h7 = orc.overrelax_gauge(u, g.copy(), dims, 5000, 1e-12, 7)
+    assert it1 > 0 and it7 % 7 == 0 and it1 <= it7 < it1 + 7
+    n = len(h1[::7])
+    assert np.array_equal(h1[::7], h7[:n])  # same trajectory, sampled
+    it0, _ = orc.overrelax_gauge(u, g.copy(), dims, 0, 1e-12, 1)
+    assert it0 == 0
+    itf, _ = orc.overrelax_gauge(u, g.copy(), dims, 3, 1e-30, 1)
+    assert itf == -1
+
+
+def test_functional_monotone_for_omega_one(orc):
+    dims = (4, 4, 4, 4)
+    V = orc.volume(dims)
+    u = orc.su3_to_2v(orc.random_init_gauge(V, 31))
+    g = orc.su3_to_2v(orc.random_init_trafo(V, 32))
+    it, hist = orc.overrelax_gauge(u, g, dims, 40, 1e-30, 1, 1.0)
+    assert np.all(np.diff(hist[:, 0]) >= -1e-14)
+
+
+def test_plaquette_gauge_invariant(orc):
+    dims = (4, 4, 4, 4)
+    V = orc.volume(dims)
+    u = orc.random_init_gauge(V, 41)
+    ut = orc.g_trafo(u, orc.random_init_trafo(V, 42), dims)
+    assert abs(orc.mean_plaq(u, dims) - orc.mean_plaq(ut, dims)) < 1e-14
+    assert abs(orc.plaquette_2v(orc.su3_to_2v(u), dims) - orc.mean_plaq(u, dims)) < 1e-14
+
+
+def test_coulomb_slices_are_independent(orc):
+    """Coulomb gauge uses only spatial links: fixing a 4^3 x 4 lattice slice by
+    slice equals fixing each slice alone; every slice reaches the bound."""
+    dims = (4, 4, 4, 4)
+    V = orc.volume(dims)
+    u = orc.su3_to_2v(orc.random_init_gauge(V, 51))
+    g = orc.su3_to_2v(orc.random_init_trafo(V, 52))
+    rc, iters = orc.coulomb_gauge(u, g, dims, 4000, 1e-12, 1)
+    assert rc == 0 and np.all(iters > 0)
+    F, D, M = orc.functional_slices(u, g, dims, 3)
+    assert np.all(M < 1e-12)
