@@ -1,0 +1,241 @@
+"""Benchmark of the gauge-fixing hot path on one MI355X.
+
+Workload (one GPU): Landau gauge fixing of a 32^4 SU(3) configuration at
+beta = 6.0, heatbath-thermalized on the GPU from a cold start (500 sweeps,
+twister seeds from srand(4357)), omega = 1.7, initial gauge transformation
+from the Lehmer RNG with RandSeed = 351, stop at max_x dAdA(x) < 1e-12,
+checked every 10 iterations.
+
+A *step* is one complete gauge-fixing run from that fixed start to the stop
+decision in the mixed path (FP32 12-real sweeps until max dAdA < 1e-6,
+then FP64 reunitarisation and FP64 polish; F/theta reductions in FP64).
+value = site-updates/s of those steps (iterations x V / wall time).  The
+plain FP64 run to the same bound is timed beside it (time-to-theta,
+speedup), with the error of F and of the links of the mixed result against
+it.  The roofline entry is the sweep kernel that dominates the mixed run,
+timed with HIP events on the library's stream; algorithmic bytes per
+site-update are 8 links + 3 g (FP32 12-real: 528 B, FP64: 1056 B).
+
+cpu_baseline: the oracle's OpenMP half-sweep (a C restatement of the
+reference's sweep on 2-row AoS data) timed on the host cores on a bounded
+sample of the same 32^4 problem.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+DIMS = (32, 32, 32, 32)
+BETA = 6.0
+NTHERM = 500
+OMEGA = 1.7
+EPS = float(np.float32(1e-12))  # the input grammar reads eps through a C float
+SWITCH = 1e-6
+CK = 10
+RANDSEED = 351
+MTSEED = 4357
+HBM_PEAK_GBS = 8000.0
+BYTES_FP32 = 8 * 48 + 3 * 48
+BYTES_FP64 = 8 * 96 + 3 * 96
+
+
+def synthetic_mt_params():
+    """Twister parameter records (4096 x {matrix_a, mask_b, mask_c, seed}):
+    the reference loads them from MersenneTwister.dat, which does not ship
+    with this repository; these are generated from a fixed splitmix64 stream
+    with the top bit of matrix_a set and the MT19937 tempering masks."""
+    out = np.zeros((4096, 4), dtype=np.uint32)
+    x = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        for i in range(4096):
+            x = np.uint64(x + np.uint64(0x9E3779B97F4A7C15))
+            z = x
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+            out[i] = (np.uint32(int(z) & 0xFFFFFFFF) | np.uint32(0x80000000), 0x9D2C5680,
+                      0xEFC60000, 0)
+    return out.ravel()
+
+
+def thermalized_config(gfx):
+    gfx.set_lattice(*DIMS)
+    u = gfx.unit_gauge()
+    gfx.init_thermalization(u)
+    gfx.set_mt_params(synthetic_mt_params())
+    gfx.set_mt_seed(MTSEED)
+    gfx.heatbath_sweeps(NTHERM, BETA)
+    plaq = gfx.calc_plaquette(0)
+    u = gfx.get_gauge()
+    gfx.finalize_MT()
+    gfx.finalize_thermalization()
+    return u, plaq
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", "sweep_traffic_32x4.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get(kernel, {}).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(u_su3, g_su3, budget_s=12.0):
+    import oracle
+
+    u2 = oracle.su3_to_2v(u_su3)
+    g2 = oracle.su3_to_2v(g_su3)
+    V = oracle.volume(DIMS)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        oracle.halfsweep(u2, g2, DIMS, n % 2, 4, OMEGA)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 8:
+            break
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rate = n * (V // 2) / dt
+    return {"value": rate, "unit": "site-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{n} FP64 half-sweeps of the 32^4 problem ({n * (V // 2)} site-updates) "
+                      f"by oracle/oracle.c orc_or_halfsweep with OpenMP, {dt:.1f} s",
+            "GBs_at_1056B": rate * 1056 / 1e9}
+
+
+def align_global(g_a, g_b):
+    """Links are fixed up to a constant gauge rotation h (F is invariant under
+    g -> h g); return max |g_a - h g_b| with h the best unitary fit."""
+    A = g_a.reshape(-1, 3, 3, 2)
+    B = g_b.reshape(-1, 3, 3, 2)
+    A = A[..., 0] + 1j * A[..., 1]
+    B = B[..., 0] + 1j * B[..., 1]
+    M = np.einsum("nij,nkj->ik", A, B.conj())  # sum_x A(x) B(x)^+
+    U, _, Vh = np.linalg.svd(M)
+    h = U @ Vh
+    return float(np.abs(A - np.einsum("ij,njk->nik", h, B)).max())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; the path would shard by "
+                          "time-slab domain decomposition across GPUs, which is not built"}))
+        sys.exit(2)
+
+    import cudagfx_amd as gfx
+
+    gfx.load()
+    gfx.set_verbose(0)
+    V = int(np.prod(DIMS))
+    t_setup = time.perf_counter()
+    u, plaq = thermalized_config(gfx)
+    gfx.plant_seeds(RANDSEED)
+    g0 = gfx.random_init_trafo_host(V)
+    gfx.init_gaugefixing(u, g0)
+    gfx.set_omega(OMEGA)
+    setup_s = time.perf_counter() - t_setup
+
+    def run(prec):
+        gfx.set_trafo(g0)
+        gfx.set_precision(prec, SWITCH)
+        gfx.sync()
+        t0 = time.perf_counter()
+        it = gfx.overrelax_gauge(20000, EPS, CK)
+        gfx.sync()
+        wall = time.perf_counter() - t0
+        ms, i32, i64 = gfx.last_run_stats()
+        F, D, M = gfx.results()
+        return dict(iter=it, wall_s=wall, lib_ms=ms, it32=i32, it64=i64, F=F, dAdA=D, max=M)
+
+    # plain FP64 reference run
+    r64 = run(gfx.PREC_FP64)
+    g_fp64 = gfx.get_trafo()
+    # mixed steps
+    for _ in range(args.warmup):
+        run(gfx.PREC_MIXED)
+    walls, last = [], None
+    for _ in range(args.steps):
+        last = run(gfx.PREC_MIXED)
+        walls.append(last["wall_s"])
+    g_mixed = gfx.get_trafo()
+    ms_step = 1000.0 * max(walls)
+    iters = last["it32"] + last["it64"]
+    value = iters * V / (ms_step / 1000.0)
+
+    # kernel timing of the two sweeps (HIP events on the library stream)
+    gfx.set_trafo(g0)
+    t32 = gfx.time_sweeps(50, gfx.PREC_MIXED)
+    t64 = gfx.time_sweeps(50, gfx.PREC_FP64)
+    gb32 = (V // 2) * BYTES_FP32 / (t32 * 1e-3) / 1e9
+    gb64 = (V // 2) * BYTES_FP64 / (t64 * 1e-3) / 1e9
+    share32 = last["it32"] * 2 * t32
+    share64 = last["it64"] * 2 * t64
+    dom32 = share32 >= share64
+    kernel = "k_or_sweep<float4,float,4>" if dom32 else "k_or_sweep<double2,double,4>"
+    ach = gb32 if dom32 else gb64
+    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": read_traffic(kernel), "kernel": kernel,
+            "ms_per_launch": t32 if dom32 else t64,
+            "algorithmic_bytes_per_launch": (V // 2) * (BYTES_FP32 if dom32 else BYTES_FP64)}
+
+    link_err = align_global(g_mixed, g_fp64)
+    gfx.finalize_gaugefixing()
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(u, g0)
+        except Exception as e:  # the oracle is optional on a box without it
+            cpu = {"error": str(e)}
+
+    out = {
+        "metric": "site-updates/s + time-to-theta<1e-12, 32^4 Landau, mixed vs FP64; HBM GB/s",
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 sweeps + f64 reductions/polish",
+        "data": "synthetic: cold start + 500 GPU heatbath sweeps at beta=6.0 (twister params "
+                "generated, srand(4357) seeds), Lehmer RandSeed=351 initial g",
+        "config": {"workload": "Landau gauge fixing 32^4 beta=6.0, omega=1.7, theta<1e-12, ck=10",
+                   "lattice": list(DIMS), "site_updates_per_step": iters * V},
+        "time_to_theta_s": {"mixed": ms_step / 1000.0, "fp64": r64["wall_s"],
+                            "speedup": r64["wall_s"] / (ms_step / 1000.0)},
+        "iterations": {"mixed_fp32": last["it32"], "mixed_fp64": last["it64"],
+                       "fp64": r64["it64"], "stop_mixed": last["iter"], "stop_fp64": r64["iter"]},
+        "sweep_kernels": {"fp32_ms": t32, "fp32_GBs": gb32, "fp32_site_updates_per_s": (V // 2) / (t32 * 1e-3),
+                          "fp64_ms": t64, "fp64_GBs": gb64, "fp64_site_updates_per_s": (V // 2) / (t64 * 1e-3)},
+        "accuracy_mixed_vs_fp64": {"F_rel": abs(last["F"] - r64["F"]) / r64["F"],
+                                   "theta_mixed": last["max"], "theta_fp64": r64["max"],
+                                   "links_maxabs_after_global_alignment": link_err},
+        "plaquette": plaq,
+        "setup_s": setup_s,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

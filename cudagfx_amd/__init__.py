@@ -101,6 +101,12 @@ def load():
         "gfx_set_sa": (C.c_int, [C.c_int, C.c_int, C.c_int]),
         "gfx_sa_temperature": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int]),
         "gfx_heatbath_sweeps": (C.c_int, [C.c_int, C.c_double]),
+        "gfx_micro_sweeps": (C.c_int, [C.c_int, C.c_int]),
+        "gfx_set_autotune": (C.c_int, [C.c_int]),
+        "gfx_get_sweep_variant": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int),
+                                            C.POINTER(C.c_int), C.POINTER(C.c_int), vp]),
+        "gfx_set_sweep_variant": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+        "gfx_sync": (C.c_int, []),
         "gfx_coulomb_gauge": (C.c_int, [C.c_int, C.c_double, C.c_int, vp]),
         "gfx_functional_slices": (C.c_int, [vp, vp, vp]),
         "gfx_halfsweep": (C.c_int, [C.c_int, C.c_int]),
@@ -295,6 +301,33 @@ def set_sa(sampler=0, nmicro=0, ndir=4):
 
 def heatbath_sweeps(nsweep, beta):
     _check(load().gfx_heatbath_sweeps(int(nsweep), float(beta)), "heatbath")
+
+
+def set_autotune(on):
+    load().gfx_set_autotune(int(on))
+
+
+def sweep_variant(precision=PREC_FP64, ndir=4):
+    """(variant, block, rolled, ms per variant) chosen by the autotuner, or None."""
+    v, b, r = C.c_int(), C.c_int(), C.c_int()
+    ms = np.zeros(6)
+    rc = load().gfx_get_sweep_variant(int(precision), int(ndir), C.byref(v), C.byref(b),
+                                      C.byref(r), ms.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        return None
+    return v.value, b.value, bool(r.value), ms
+
+
+def set_sweep_variant(precision, ndir, variant):
+    _check(load().gfx_set_sweep_variant(int(precision), int(ndir), int(variant)), "variant")
+
+
+def micro_sweeps(n, ndir=4):
+    _check(load().gfx_micro_sweeps(int(n), int(ndir)), "micro_sweeps")
+
+
+def sync():
+    _check(load().gfx_sync(), "sync")
 
 
 def coulomb_gauge(maxit, eps, checkinterval):

@@ -204,6 +204,72 @@ int functional_now(State& s, bool fp32, int ndir, int nseg, double eps) {
   return hip_check(hipStreamSynchronize(s.stream), "functional");
 }
 
+void reset_tuning(State& s) {
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) {
+      s.tuned[a][b] = false;
+      s.variant[a][b] = 0;
+      for (int k = 0; k < 8; k++) s.tune_ms[a][b][k] = 0.f;
+    }
+}
+
+// one half sweep with the tuned variant for (precision, ndir)
+void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* active) {
+  const int v = s.variant[fp32 ? 1 : 0][ndir == 4 ? 1 : 0];
+  if (fp32)
+    launch_sweep_f32(s.U32, s.g32, s.geo, parity, ndir, s.omega, active, v, s.stream);
+  else
+    launch_sweep_f64(s.U64, s.g64, s.geo, parity, ndir, s.omega, active, v, s.stream);
+}
+
+// Times every sweep variant on a scratch copy of g (the state is not
+// touched) and keeps the fastest for this lattice, precision and ndir.
+int tune_sweep(State& s, bool fp32, int ndir) {
+  const int pi = fp32 ? 1 : 0, di = ndir == 4 ? 1 : 0;
+  if (!s.autotune || s.tuned[pi][di]) return GFX_OK;
+  const size_t bytes = 2 * (fp32 ? field_bytes32(s.geo) : field_bytes64(s.geo));
+  void* scratch = nullptr;
+  if (hipMalloc(&scratch, bytes) != hipSuccess) {
+    s.tuned[pi][di] = true;  // keep the default variant
+    return GFX_OK;
+  }
+  hipMemcpyAsync(scratch, fp32 ? (void*)s.g32 : (void*)s.g64, bytes, hipMemcpyDeviceToDevice, s.stream);
+  float best = 1e30f;
+  int bestv = s.variant[pi][di];
+  for (int v = 0; v < SWEEP_VARIANTS; v++) {
+    for (int rep = 0; rep < 2; rep++)
+      for (int p = 0; p < 2; p++) {
+        if (fp32)
+          launch_sweep_f32(s.U32, (float4*)scratch, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
+        else
+          launch_sweep_f64(s.U64, (double2*)scratch, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
+      }
+    hipEventRecord(s.ev0, s.stream);
+    for (int rep = 0; rep < 3; rep++)
+      for (int p = 0; p < 2; p++) {
+        if (fp32)
+          launch_sweep_f32(s.U32, (float4*)scratch, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
+        else
+          launch_sweep_f64(s.U64, (double2*)scratch, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
+      }
+    hipEventRecord(s.ev1, s.stream);
+    if (hip_check(hipEventSynchronize(s.ev1), "tune")) break;
+    float t = 0.f;
+    hipEventElapsedTime(&t, s.ev0, s.ev1);
+    t /= 6.0f;
+    s.tune_ms[pi][di][v] = t;
+    if (t < best) {
+      best = t;
+      bestv = v;
+    }
+  }
+  hipStreamSynchronize(s.stream);
+  hipFree(scratch);
+  s.variant[pi][di] = bestv;
+  s.tuned[pi][di] = true;
+  return GFX_OK;
+}
+
 }  // namespace gfx
 
 // ======================================================================
@@ -225,6 +291,7 @@ int init_gaugefixing(gfx_su3* gf, gfx_su3* trafo) {
   if ((rc = upload_aos(s, (const double*)gf, s.U64, 4))) return rc;
   if ((rc = upload_aos(s, (const double*)trafo, s.g64, 1))) return rc;
   s.U32_valid = false;
+  reset_tuning(s);
   s.init = 1;
   return GFX_OK;
 }
@@ -243,6 +310,7 @@ int init_thermalization(gfx_su3* gf) {
   if ((rc = alloc_reduction(s))) return rc;
   if ((rc = upload_aos(s, (const double*)gf, s.U64, 4))) return rc;
   s.U32_valid = false;
+  reset_tuning(s);
   s.init = 2;
   return GFX_OK;
 }
@@ -305,10 +373,10 @@ int gfx_halfsweep(int parity, int ndir) {
     int rc = ensure_f32_links(s);
     if (rc) return rc;
     launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
-    launch_sweep_f32(s.U32, s.g32, s.geo, parity, ndir, s.omega, nullptr, s.stream);
+    do_sweep(s, true, parity, ndir, nullptr);
     launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
   } else {
-    launch_sweep_f64(s.U64, s.g64, s.geo, parity, ndir, s.omega, nullptr, s.stream);
+    do_sweep(s, false, parity, ndir, nullptr);
   }
   return hip_check(hipStreamSynchronize(s.stream), "halfsweep");
 }
@@ -321,7 +389,9 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
   if (fp32) {
     if (ensure_f32_links(s)) return -1;
     launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
+    tune_sweep(s, true, 4);
   }
+  tune_sweep(s, false, 4);
   s.run_iters32 = 0;
   s.run_iters64 = 0;
   double maxdada = 0.0;
@@ -334,12 +404,12 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
   };
   for (i = 0; i < maxit; i++) {
     if (fp32) {
-      launch_sweep_f32(s.U32, s.g32, s.geo, 0, 4, s.omega, nullptr, s.stream);
-      launch_sweep_f32(s.U32, s.g32, s.geo, 1, 4, s.omega, nullptr, s.stream);
+      do_sweep(s, true, 0, 4, nullptr);
+      do_sweep(s, true, 1, 4, nullptr);
       s.run_iters32++;
     } else {
-      launch_sweep_f64(s.U64, s.g64, s.geo, 0, 4, s.omega, nullptr, s.stream);
-      launch_sweep_f64(s.U64, s.g64, s.geo, 1, 4, s.omega, nullptr, s.stream);
+      do_sweep(s, false, 0, 4, nullptr);
+      do_sweep(s, false, 1, 4, nullptr);
       s.run_iters64++;
     }
     if ((i % checkinterval) == 0) {
@@ -387,7 +457,9 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   if (fp32) {
     if (ensure_f32_links(s)) return GFX_ERR_ALLOC;
     launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
+    tune_sweep(s, true, 3);
   }
+  tune_sweep(s, false, 3);
   hipMemcpyAsync(s.active, act.data(), nt, hipMemcpyHostToDevice, s.stream);
   s.run_iters32 = s.run_iters64 = 0;
   int left = nt;
@@ -395,12 +467,12 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   const auto t0 = std::chrono::steady_clock::now();
   for (int i = 0; i < maxit && left > 0; i++) {
     if (fp32) {
-      launch_sweep_f32(s.U32, s.g32, s.geo, 0, 3, s.omega, s.active, s.stream);
-      launch_sweep_f32(s.U32, s.g32, s.geo, 1, 3, s.omega, s.active, s.stream);
+      do_sweep(s, true, 0, 3, s.active);
+      do_sweep(s, true, 1, 3, s.active);
       s.run_iters32++;
     } else {
-      launch_sweep_f64(s.U64, s.g64, s.geo, 0, 3, s.omega, s.active, s.stream);
-      launch_sweep_f64(s.U64, s.g64, s.geo, 1, 3, s.omega, s.active, s.stream);
+      do_sweep(s, false, 0, 3, s.active);
+      do_sweep(s, false, 1, 3, s.active);
       s.run_iters64++;
     }
     if ((i % checkinterval) == 0) {
@@ -482,13 +554,14 @@ int gfx_time_sweeps(int nsweeps, int precision, double* ms) {
     if (rc) return rc;
     launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
   }
+  tune_sweep(s, fp32, 4);
   hipEventRecord(s.ev0, s.stream);
   for (int k = 0; k < nsweeps; k++)
     for (int p = 0; p < 2; p++) {
       if (fp32)
-        launch_sweep_f32(s.U32, s.g32, s.geo, p, 4, s.omega, nullptr, s.stream);
+        do_sweep(s, true, p, 4, nullptr);
       else
-        launch_sweep_f64(s.U64, s.g64, s.geo, p, 4, s.omega, nullptr, s.stream);
+        do_sweep(s, false, p, 4, nullptr);
     }
   hipEventRecord(s.ev1, s.stream);
   if (hip_check(hipEventSynchronize(s.ev1), "timing")) return GFX_ERR_DEVICE;
@@ -506,6 +579,33 @@ int gfx_last_run_stats(double* ms, int* it32, int* it64) {
   if (ms) *ms = g_st.run_ms;
   if (it32) *it32 = g_st.run_iters32;
   if (it64) *it64 = g_st.run_iters64;
+  return GFX_OK;
+}
+
+int gfx_sync(void) { return hip_check(hipDeviceSynchronize(), "sync"); }
+
+int gfx_set_autotune(int on) {
+  g_st.autotune = on ? 1 : 0;
+  return GFX_OK;
+}
+
+int gfx_get_sweep_variant(int precision, int ndir, int* variant, int* block, int* rolled,
+                          double* ms_per_variant) {
+  const int pi = precision == GFX_PREC_MIXED ? 1 : 0, di = ndir == 4 ? 1 : 0;
+  const int v = g_st.variant[pi][di];
+  if (variant) *variant = v;
+  if (block) *block = sweep_variant_block(v);
+  if (rolled) *rolled = sweep_variant_rolled(v) ? 1 : 0;
+  if (ms_per_variant)
+    for (int k = 0; k < SWEEP_VARIANTS; k++) ms_per_variant[k] = g_st.tune_ms[pi][di][k];
+  return g_st.tuned[pi][di] ? GFX_OK : GFX_ERR_STATE;
+}
+
+int gfx_set_sweep_variant(int precision, int ndir, int variant) {
+  if (variant < 0 || variant >= SWEEP_VARIANTS) return GFX_ERR_STATE;
+  const int pi = precision == GFX_PREC_MIXED ? 1 : 0, di = ndir == 4 ? 1 : 0;
+  g_st.variant[pi][di] = variant;
+  g_st.tuned[pi][di] = true;
   return GFX_OK;
 }
 

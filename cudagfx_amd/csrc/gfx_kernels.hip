@@ -64,12 +64,18 @@ __device__ __forceinline__ void block_reduce3(double& a, double& b, double& c) {
 }
 
 // ------------------------------------------------------------ OR sweep
-template <typename V, typename R, int NDIR>
-__global__ __launch_bounds__(BLK) void k_or_sweep(const V* __restrict__ U, V* __restrict__ g,
-                                                   Geom geo, int parity, R c1, R c2,
-                                                   const unsigned char* __restrict__ active) {
+// Variants (chosen per lattice and precision by the autotuner in gfx_api.hip):
+//   BLOCK  workgroup size (64, 128, 256)
+//   ROLLED the direction loop of the star is kept as a loop: one direction's
+//          operands are live at a time (FP64: 130 VGPRs, 3 waves/SIMD)
+//          instead of the fully unrolled form (more load-level parallelism
+//          per wave, 332 registers, 1 wave/SIMD).
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED>
+__global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* __restrict__ g,
+                                                     Geom geo, int parity, R c1, R c2,
+                                                     const unsigned char* __restrict__ active) {
   const int b = xcd_block(blockIdx.x, gridDim.x);
-  const int i = b * BLK + threadIdx.x;
+  const int i = b * BLOCK + threadIdx.x;
   if (i >= geo.nh) return;
   const Site s = site_coords(geo, i, parity);
   if (NDIR == 3 && active != nullptr && !active[s.x[3]]) return;
@@ -80,16 +86,34 @@ __global__ __launch_bounds__(BLK) void k_or_sweep(const V* __restrict__ U, V* __
   // star K = sum_mu U_mu(x) g^+(x+mu) + U_mu^+(x-mu) g^+(x-mu)
   M3<R> K;
   m_zero(K);
+  if (ROLLED) {
+    int jf[NDIR], jb[NDIR];
 #pragma unroll
-  for (int mu = 0; mu < NDIR; mu++) {
-    const int jf = nb_fwd(geo, s, i, mu);
-    const int jb = nb_bwd(geo, s, i, mu);
-    M3<R> A = load_m<V, R>(field_ptr(U, parity * 4 + mu, S), S, i);
-    M3<R> B = load_m_dagger<V, R>(gop, S, jf);
-    m_addmul(K, A, B);
-    A = load_m_dagger<V, R>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb);
-    B = load_m_dagger<V, R>(gop, S, jb);
-    m_addmul(K, A, B);
+    for (int mu = 0; mu < NDIR; mu++) {
+      jf[mu] = nb_fwd(geo, s, i, mu);
+      jb[mu] = nb_bwd(geo, s, i, mu);
+    }
+#pragma unroll 1
+    for (int mu = 0; mu < NDIR; mu++) {
+      M3<R> A = load_m<V, R>(field_ptr(U, parity * 4 + mu, S), S, i);
+      M3<R> B = load_m_dagger<V, R>(gop, S, jf[mu]);
+      m_addmul(K, A, B);
+      A = load_m_dagger<V, R>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb[mu]);
+      B = load_m_dagger<V, R>(gop, S, jb[mu]);
+      m_addmul(K, A, B);
+    }
+  } else {
+#pragma unroll
+    for (int mu = 0; mu < NDIR; mu++) {
+      const int jf = nb_fwd(geo, s, i, mu);
+      const int jb = nb_bwd(geo, s, i, mu);
+      M3<R> A = load_m<V, R>(field_ptr(U, parity * 4 + mu, S), S, i);
+      M3<R> B = load_m_dagger<V, R>(gop, S, jf);
+      m_addmul(K, A, B);
+      A = load_m_dagger<V, R>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb);
+      B = load_m_dagger<V, R>(gop, S, jb);
+      m_addmul(K, A, B);
+    }
   }
   M3<R> G = load_m<V, R>(gme, S, i);
   M3<R> h = G;
@@ -331,29 +355,55 @@ __global__ __launch_bounds__(BLK) void k_g32_to_g64(const float4* __restrict__ s
 // ------------------------------------------------------------ launchers
 static inline dim3 grid1(long n) { return dim3((unsigned)((n + BLK - 1) / BLK)); }
 
+// sweep variant v: block = {256,128,64}[v % 3], rolled direction loop if v < 3
+int sweep_variant_block(int v) { return v % 3 == 0 ? 256 : (v % 3 == 1 ? 128 : 64); }
+bool sweep_variant_rolled(int v) { return v < 3; }
+
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED>
+static void launch_sweep_v(const V* U, V* g, const Geom& geo, int parity, R c1, R c2,
+                           const unsigned char* active, hipStream_t st) {
+  dim3 grid((geo.nh + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL((k_or_sweep<V, R, NDIR, BLOCK, ROLLED>), grid, dim3(BLOCK), 0, st, U, g, geo,
+                     parity, c1, c2, active);
+}
+
+template <typename V, typename R, int NDIR>
+static void launch_sweep_nd(const V* U, V* g, const Geom& geo, int parity, R c1, R c2,
+                            const unsigned char* active, int variant, hipStream_t st) {
+  switch (variant) {
+    case 0: launch_sweep_v<V, R, NDIR, 256, true>(U, g, geo, parity, c1, c2, active, st); break;
+    case 1: launch_sweep_v<V, R, NDIR, 128, true>(U, g, geo, parity, c1, c2, active, st); break;
+    case 2: launch_sweep_v<V, R, NDIR, 64, true>(U, g, geo, parity, c1, c2, active, st); break;
+    case 3: launch_sweep_v<V, R, NDIR, 256, false>(U, g, geo, parity, c1, c2, active, st); break;
+    case 4: launch_sweep_v<V, R, NDIR, 128, false>(U, g, geo, parity, c1, c2, active, st); break;
+    default: launch_sweep_v<V, R, NDIR, 64, false>(U, g, geo, parity, c1, c2, active, st); break;
+  }
+}
+
 template <typename V, typename R>
 static void launch_sweep_t(const V* U, V* g, const Geom& geo, int parity, int ndir, double omega,
-                           const unsigned char* active, hipStream_t st) {
+                           const unsigned char* active, int variant, hipStream_t st) {
   // C(w,1) = (1+w-1)/1 and C(w,2) = (1+w-2)/2 as the reference forms them
   // (overrelax.cu:167), in double, then rounded to R.
   const R c1 = (R)((1.0 + omega - 1) / 1);
   const R c2 = (R)((1.0 + omega - 2) / 2);
-  dim3 grid = grid1(geo.nh);
   if (ndir == 4)
-    hipLaunchKernelGGL((k_or_sweep<V, R, 4>), grid, dim3(BLK), 0, st, U, g, geo, parity, c1, c2,
-                       nullptr);
+    launch_sweep_nd<V, R, 4>(U, g, geo, parity, c1, c2, nullptr, variant, st);
   else
-    hipLaunchKernelGGL((k_or_sweep<V, R, 3>), grid, dim3(BLK), 0, st, U, g, geo, parity, c1, c2,
-                       active);
+    launch_sweep_nd<V, R, 3>(U, g, geo, parity, c1, c2, active, variant, st);
 }
 
 void launch_sweep_f64(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
-                      double omega, const unsigned char* active, hipStream_t st) {
-  launch_sweep_t<double2, double>(U, g, geo, parity, ndir, omega, active, st);
+                      double omega, const unsigned char* active, int variant, hipStream_t st) {
+  launch_sweep_t<double2, double>(U, g, geo, parity, ndir, omega, active, variant, st);
 }
 void launch_sweep_f32(const float4* U, float4* g, const Geom& geo, int parity, int ndir,
-                      double omega, const unsigned char* active, hipStream_t st) {
-  launch_sweep_t<float4, float>(U, g, geo, parity, ndir, omega, active, st);
+                      double omega, const unsigned char* active, int variant, hipStream_t st) {
+  launch_sweep_t<float4, float>(U, g, geo, parity, ndir, omega, active, variant, st);
+}
+
+void launch_copy_fields(const void* src, void* dst, size_t bytes, hipStream_t st) {
+  hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
 }
 
 int functional_blocks(int nsh) { return (2 * nsh + BLK - 1) / BLK; }

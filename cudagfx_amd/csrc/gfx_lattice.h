@@ -106,7 +106,7 @@ struct Store<double2> {
                                               Cx<R> r0[3], Cx<R> r1[3]) {
     double2 v[6];
 #pragma unroll
-    for (int c = 0; c < 6; c++) v[c] = f[(size_t)c * stride + i];
+    for (int c = 0; c < 6; c++) v[c] = (f + (size_t)c * stride)[(unsigned)i];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       r0[c] = cx<R>((R)v[c].x, (R)v[c].y);
@@ -118,8 +118,9 @@ struct Store<double2> {
                                                const M3<R>& m) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      f[(size_t)c * stride + i] = make_double2((double)m.e[0][c].re, (double)m.e[0][c].im);
-      f[(size_t)(3 + c) * stride + i] = make_double2((double)m.e[1][c].re, (double)m.e[1][c].im);
+      (f + (size_t)c * stride)[(unsigned)i] = make_double2((double)m.e[0][c].re, (double)m.e[0][c].im);
+      (f + (size_t)(3 + c) * stride)[(unsigned)i] =
+          make_double2((double)m.e[1][c].re, (double)m.e[1][c].im);
     }
   }
 };
@@ -130,9 +131,9 @@ struct Store<float4> {
   template <typename R>
   __device__ __forceinline__ static void load(const float4* __restrict__ f, int stride, int i,
                                               Cx<R> r0[3], Cx<R> r1[3]) {
-    float4 a = f[i];
-    float4 b = f[(size_t)stride + i];
-    float4 c = f[(size_t)2 * stride + i];
+    float4 a = f[(unsigned)i];
+    float4 b = (f + (size_t)stride)[(unsigned)i];
+    float4 c = (f + (size_t)2 * stride)[(unsigned)i];
     r0[0] = cx<R>((R)a.x, (R)a.y);
     r0[1] = cx<R>((R)a.z, (R)a.w);
     r0[2] = cx<R>((R)b.x, (R)b.y);
@@ -143,11 +144,11 @@ struct Store<float4> {
   template <typename R>
   __device__ __forceinline__ static void store(float4* __restrict__ f, int stride, int i,
                                                const M3<R>& m) {
-    f[i] = make_float4((float)m.e[0][0].re, (float)m.e[0][0].im, (float)m.e[0][1].re,
+    f[(unsigned)i] = make_float4((float)m.e[0][0].re, (float)m.e[0][0].im, (float)m.e[0][1].re,
                        (float)m.e[0][1].im);
-    f[(size_t)stride + i] = make_float4((float)m.e[0][2].re, (float)m.e[0][2].im,
+    (f + (size_t)stride)[(unsigned)i] = make_float4((float)m.e[0][2].re, (float)m.e[0][2].im,
                                         (float)m.e[1][0].re, (float)m.e[1][0].im);
-    f[(size_t)2 * stride + i] = make_float4((float)m.e[1][1].re, (float)m.e[1][1].im,
+    (f + (size_t)2 * stride)[(unsigned)i] = make_float4((float)m.e[1][1].re, (float)m.e[1][1].im,
                                             (float)m.e[1][2].re, (float)m.e[1][2].im);
   }
 };

@@ -46,6 +46,13 @@ struct State {
   float* rnd_unif = nullptr;
   int n_per_gauss = 0, n_per_unif = 0;
 
+  // sweep kernel variant per [precision fp64/fp32][ndir 3/4], picked by the
+  // autotuner on first use for the current lattice
+  int variant[2][2] = {{0, 0}, {0, 0}};
+  bool tuned[2][2] = {{false, false}, {false, false}};
+  float tune_ms[2][2][8] = {};
+  int autotune = 1;
+
   // last run statistics
   double run_ms = 0.0;
   int run_iters32 = 0, run_iters64 = 0;

@@ -262,6 +262,15 @@ void simannealing_gauge(void) {
   if (s.verbose) printf("SA finished after %f sec\n", sec);
 }
 
+// n microcanonical sweeps (even + odd) of g; F is unchanged by each
+int gfx_micro_sweeps(int n, int ndir) {
+  State& s = g_st;
+  if (s.init != 1) return GFX_ERR_STATE;
+  for (int k = 0; k < n; k++)
+    for (int p = 0; p < 2; p++) launch_micro_sweep(s.U64, s.g64, s.geo, p, ndir == 3 ? 3 : 4, s.stream);
+  return hip_check(hipStreamSynchronize(s.stream), "micro");
+}
+
 // one full heatbath sweep (4 directions x 2 parities) at beta_eff = beta/3
 int gfx_heatbath_sweeps(int nsweep, double beta) {
   State& s = g_st;

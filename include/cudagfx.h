@@ -150,6 +150,43 @@ int gfx_time_sweeps(int nsweeps, int precision, double *ms_per_halfsweep);
  * (first sweep to the stop decision), and the number of FP32 and FP64
  * iterations it ran. */
 int gfx_last_run_stats(double *ms, int *iters_fp32, int *iters_fp64);
+/* wait for all device work of the library */
+int gfx_sync(void);
+
+/* Sweep-kernel autotuner: on the first sweep of a lattice in a precision and
+ * gauge, every variant (workgroup size 256/128/64, star loop rolled or
+ * unrolled) is timed on a scratch copy of g and the fastest is kept.  on = 0
+ * keeps variant 0 (256, rolled) or the one set by gfx_set_sweep_variant. */
+int gfx_set_autotune(int on);
+/* the chosen variant and the measured ms per half sweep of each of the 6
+ * variants; GFX_ERR_STATE if not tuned yet for this lattice */
+int gfx_get_sweep_variant(int precision, int ndir, int *variant, int *block, int *rolled,
+                          double *ms_per_variant);
+int gfx_set_sweep_variant(int precision, int ndir, int variant);
+
+/* ---- stochastic parts (MersenneTwister.cu, simulated_annealing.cu,
+ *      heatbath_thermalization.cu extensions) ---- */
+/* path of the twister parameter file (default "MersenneTwister.dat" in the
+ * working directory, as MersenneTwister.cu:258) */
+int gfx_set_mt_file(const char *path);
+/* parameter records given directly: 4096 x {matrix_a, mask_b, mask_c, seed} */
+int gfx_set_mt_params(const uint32_t *params);
+/* copies of the gauss / uniform fields, the 4096 x 19 states and the params */
+int gfx_get_mt_fields(float *gauss, float *unif, uint32_t *state, uint32_t *params);
+/* numbers per twister of the gauss and uniform fields */
+int gfx_mt_sizes(int *n_per_gauss, int *n_per_unif);
+/* SA options: sampler 0 = the reference's rule, 1 = Kennedy-Pendleton;
+ * nmicro microcanonical sweeps after each SA sweep; ndir 4 Landau / 3 Coulomb */
+int gfx_set_sa(int sampler, int nmicro, int ndir);
+/* n microcanonical sweeps of g (even + odd): in every SU(2) subgroup g is
+ * reflected so that Re tr(g K) is unchanged (heatbath_thermalization.cu:318-338
+ * rule applied to g) */
+int gfx_micro_sweeps(int n, int ndir);
+/* SA temperature of step i (simulated_annealing.cu:265-299) */
+double gfx_sa_temperature(int i, double Tmin, double Tmax, double expo, int N);
+/* nsweep link-heatbath sweeps of the Wilson action at beta (Kennedy-Pendleton
+ * SU(2) subgroups, random numbers from the twisters) */
+int gfx_heatbath_sweeps(int nsweep, double beta);
 
 /* last error as text */
 const char *gfx_error_string(int code);

@@ -144,3 +144,55 @@ def test_get_set_trafo_roundtrip(gfx, orc):
     assert np.array_equal(back, orc.from_2v(orc.su3_to_2v(g)))
     assert np.array_equal(links, orc.from_2v(orc.su3_to_2v(u)))
     assert np.array_equal(back2, orc.from_2v(orc.su3_to_2v(g2)))
+
+
+def test_mixed_precision_matches_fp64(gfx, orc):
+    """FP32 12-real bulk sweeps to max dAdA < 1e-6, then FP64 polish: the same
+    stop bound, F within 1e-10 relative of the FP64 run (stated error)."""
+    dims = (8, 8, 8, 16)
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, 90, eps=0.35)
+    g = orc.random_init_trafo(V, 351)
+    _setup(gfx, dims, u, g, 1.7)
+    it64 = gfx.overrelax_gauge(5000, EPS12, 10)
+    F64, _, M64 = gfx.results()
+    g64 = gfx.get_trafo()
+    gfx.set_trafo(g)
+    gfx.set_precision(gfx.PREC_MIXED, 1e-6)
+    itm = gfx.overrelax_gauge(5000, EPS12, 10)
+    Fm, _, Mm = gfx.results()
+    ms, i32, i64 = gfx.last_run_stats()
+    gm = gfx.get_trafo()
+    gfx.set_precision(gfx.PREC_FP64)
+    gfx.finalize_gaugefixing()
+    assert it64 > 0 and itm > 0 and M64 < EPS12 and Mm < EPS12
+    assert i32 > 0 and i64 > 0 and i32 + i64 == itm + 1
+    assert abs(Fm - F64) / F64 < 1e-10
+    # g agrees up to a constant rotation (F is invariant under g -> h g)
+    A = gm.reshape(-1, 3, 3, 2)
+    B = g64.reshape(-1, 3, 3, 2)
+    A = A[..., 0] + 1j * A[..., 1]
+    B = B[..., 0] + 1j * B[..., 1]
+    W, _, Vh = np.linalg.svd(np.einsum("nij,nkj->ik", A, B.conj()))
+    err = np.abs(A - np.einsum("ij,njk->nik", W @ Vh, B)).max()
+    assert err < 1e-5
+
+
+def test_coulomb_matches_oracle(gfx, orc):
+    dims = (8, 8, 8, 4)
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, 31, eps=0.3)
+    g = orc.random_init_trafo(V, 32)
+    _setup(gfx, dims, u, g, 1.68)
+    rc, it_gpu = gfx.coulomb_gauge(3000, EPS12, 1)
+    g_gpu = gfx.get_trafo()
+    F, D, M = gfx.functional_slices()
+    gfx.finalize_gaugefixing()
+    u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+    rc_o, it_cpu = orc.coulomb_gauge(u2, g2, dims, 3000, EPS12, 1, 1.68)
+    assert rc == 0 and rc_o == 0
+    assert np.all(np.abs(it_gpu - it_cpu) <= 2), (it_gpu, it_cpu)
+    assert np.all(M < EPS12)
+    assert np.abs(g_gpu - orc.from_2v(g2)).max() < 1e-10
+    Fo, Do, Mo = orc.functional_slices(u2, g2, dims, 3)
+    assert np.abs(F - Fo).max() < 1e-12

@@ -1,0 +1,182 @@
+"""GPU tests of the stochastic parts: the Mersenne Twister fields bit for bit
+against the oracle, Box-Muller to its error bound, the link heatbath against
+known plaquette values, simulated annealing invariants."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LIBC = C.CDLL("libc.so.6")
+
+
+def synthetic_params(seed=12345):
+    """Twister records for the test: the reference reads them from
+    MersenneTwister.dat, which is not shipped; any matrix_a with the top bit
+    set and any tempering masks exercise the same recurrence."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros((4096, 4), dtype=np.uint32)
+    p[:, 0] = rng.integers(0, 2 ** 32, 4096, dtype=np.uint64).astype(np.uint32) | np.uint32(0x80000000)
+    p[:, 1] = rng.integers(0, 2 ** 32, 4096, dtype=np.uint64).astype(np.uint32)
+    p[:, 2] = rng.integers(0, 2 ** 32, 4096, dtype=np.uint64).astype(np.uint32)
+    return p
+
+
+def oracle_params(orc, p, srand_seed):
+    """seedMTGPU (MersenneTwister.cu:98-114): srand(seed), one rand() per twister."""
+    LIBC.srand(srand_seed)
+    arr = (orc.MTParams * 4096)()
+    for i in range(4096):
+        arr[i].matrix_a, arr[i].mask_b, arr[i].mask_c = int(p[i, 0]), int(p[i, 1]), int(p[i, 2])
+        arr[i].seed = LIBC.rand() & 0xFFFFFFFF
+    return arr
+
+
+@pytest.mark.parametrize("n_gauss,n_unif", [(4096 * 6, 4096 * 20), (5000, 19 * 4096 + 7)])
+def test_mt_fields_bitexact(gfx, orc, n_gauss, n_unif):
+    p = synthetic_params()
+    gfx.set_mt_params(p.ravel())
+    gfx.set_mt_seed(4357)
+    gfx.init_MT(n_gauss, n_unif)
+    params = oracle_params(orc, p, 4357)
+    state = np.zeros(4096 * 19, dtype=np.uint32)
+    ng = orc.lib.orc_mt_nper(n_gauss, 4096)
+    nu = orc.lib.orc_mt_nper(n_unif, 4096)
+    for step in range(3):
+        init = 0 if step == 0 else 1
+        ug = orc.mt_generate(params, state, ng, init)
+        uu = orc.mt_generate(params, state, nu, init)
+        if step > 0:
+            gfx.update_MT()
+        gauss, unif, st, pr = gfx.get_mt_fields()
+        assert np.array_equal(unif, uu), step
+        assert np.array_equal(st, state), step
+        assert np.array_equal(pr.reshape(-1, 4)[:, 3], np.array([q.seed for q in params],
+                                                               dtype=np.uint32))
+        # Box-Muller against the exact transform of the same uniforms
+        u1 = ug.reshape(ng, 4096)[0::2].astype(np.float64)
+        u2 = ug.reshape(ng, 4096)[1::2].astype(np.float64)
+        r = np.sqrt(-2.0 * np.log(u1))
+        z1, z2 = r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)
+        g = gauss.reshape(ng, 4096)
+        assert np.abs(g[0::2] - z1).max() < 2e-5 and np.abs(g[1::2] - z2).max() < 2e-5
+    gfx.finalize_MT()
+
+
+def test_mt_uniform_range_and_moments(gfx):
+    gfx.set_mt_params(synthetic_params(7).ravel())
+    gfx.set_mt_seed(1)
+    gfx.init_MT(4096 * 2, 4096 * 256)
+    gauss, unif, _, _ = gfx.get_mt_fields()
+    gfx.finalize_MT()
+    assert unif.min() > 0.0 and unif.max() <= 1.0
+    assert abs(unif.mean() - 0.5) < 2e-3 and abs(unif.var() - 1 / 12) < 2e-3
+
+
+def _thermalize(gfx, dims, beta, nsweep, seed, start_hot=False):
+    gfx.set_lattice(*dims)
+    u = gfx.unit_gauge()
+    if start_hot:
+        gfx.plant_seeds(seed)
+        u = gfx.random_init_gauge_host()
+    gfx.init_thermalization(u)
+    gfx.set_mt_params(synthetic_params(seed).ravel())
+    gfx.set_mt_seed(seed)
+    return u
+
+
+@pytest.mark.parametrize("beta,want,tol", [(6.0, 0.5937, 0.006), (0.5, 0.5 / 18 + 0.25 / 216, 0.003)])
+def test_heatbath_plaquette(gfx, orc, beta, want, tol):
+    """SU(3) Wilson plaquette: beta = 6.0 gives 0.5937 (large volume; 8^4 is
+    within the tolerance); strong coupling <P> = beta/18 + beta^2/216 + ..."""
+    dims = (8, 8, 8, 8)
+    _thermalize(gfx, dims, beta, 0, 4357)
+    gfx.heatbath_sweeps(150, beta)
+    vals = []
+    for _ in range(60):
+        gfx.heatbath_sweeps(1, beta)
+        vals.append(gfx.calc_plaquette(0))
+    u = gfx.get_gauge()
+    gfx.finalize_MT()
+    gfx.finalize_thermalization()
+    assert abs(np.mean(vals) - want) < tol, np.mean(vals)
+    # links stay in SU(3); the host plaquette of the downloaded field agrees
+    m = u.reshape(-1, 3, 3, 2)
+    m = m[..., 0] + 1j * m[..., 1]
+    assert np.abs(m @ np.conj(np.swapaxes(m, -1, -2)) - np.eye(3)).max() < 1e-13
+    assert abs(orc.mean_plaq(u, dims) - vals[-1]) < 1e-13
+
+
+def test_heatbath_cold_and_hot_agree(gfx):
+    """Equilibrium does not depend on the start."""
+    dims = (8, 8, 8, 8)
+    res = []
+    for hot in (False, True):
+        _thermalize(gfx, dims, 5.7, 0, 11, start_hot=hot)
+        gfx.heatbath_sweeps(150, 5.7)
+        v = []
+        for _ in range(40):
+            gfx.heatbath_sweeps(1, 5.7)
+            v.append(gfx.calc_plaquette(0))
+        gfx.finalize_MT()
+        gfx.finalize_thermalization()
+        res.append(np.mean(v))
+    assert abs(res[0] - res[1]) < 0.006
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_simulated_annealing_raises_functional(gfx, orc, sampler):
+    dims = (8, 8, 8, 8)
+    V = orc.volume(dims)
+    _thermalize(gfx, dims, 6.0, 0, 99)
+    gfx.heatbath_sweeps(60, 6.0)
+    u = gfx.get_gauge()
+    gfx.finalize_MT()
+    gfx.finalize_thermalization()
+    g = orc.random_init_trafo(V, 351)
+    gfx.set_lattice(*dims)
+    gfx.init_gaugefixing(u, g)
+    gfx.calc_functional()
+    F0 = gfx.results()[0]
+    sa = gfx.saparam()
+    sa.Tmin, sa.Tmax, sa.expo, sa.N, sa.checkint = 0.01, 0.45, -1.0, 300, 50
+    gfx.set_mt_params(synthetic_params(3).ravel())
+    gfx.set_mt_seed(3)
+    gfx.set_sa(sampler, 3, 4)
+    gfx.simannealing_gauge()
+    gfx.calc_functional()
+    F1 = gfx.results()[0]
+    gsa = gfx.get_trafo()
+    it = gfx.overrelax_gauge(5000, float(np.float32(1e-12)), 10)
+    F2 = gfx.results()[0]
+    gfx.finalize_MT()
+    gfx.finalize_gaugefixing()
+    m = gsa.reshape(-1, 3, 3, 2)
+    m = m[..., 0] + 1j * m[..., 1]
+    assert np.abs(m @ np.conj(np.swapaxes(m, -1, -2)) - np.eye(3)).max() < 1e-13
+    assert F1 > F0 + 0.3 and it > 0 and F2 >= F1 - 1e-3
+
+
+def test_micro_steps_preserve_functional(gfx, orc):
+    """The microcanonical reflection changes g but keeps F (per subgroup
+    Re tr(alpha X) is unchanged, heatbath_thermalization.cu:318-338)."""
+    dims = (8, 8, 8, 8)
+    V = orc.volume(dims)
+    _thermalize(gfx, dims, 6.0, 0, 5)
+    gfx.heatbath_sweeps(30, 6.0)
+    u = gfx.get_gauge()
+    gfx.finalize_MT()
+    gfx.finalize_thermalization()
+    gfx.set_lattice(*dims)
+    gfx.init_gaugefixing(u, orc.random_init_trafo(V, 8))
+    gfx.calc_functional()
+    F0 = gfx.results()[0]
+    g0 = gfx.get_trafo()
+    gfx.micro_sweeps(3, 4)
+    gfx.calc_functional()
+    F1 = gfx.results()[0]
+    g1 = gfx.get_trafo()
+    gfx.finalize_gaugefixing()
+    assert abs(F1 - F0) < 1e-12 * abs(F0)
+    assert np.abs(g1 - g0).max() > 1e-2
