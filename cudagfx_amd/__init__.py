@@ -103,6 +103,7 @@ def load():
         "gfx_heatbath_sweeps": (C.c_int, [C.c_int, C.c_double]),
         "gfx_micro_sweeps": (C.c_int, [C.c_int, C.c_int]),
         "gfx_set_autotune": (C.c_int, [C.c_int]),
+        "gfx_stream_copy": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
         "gfx_get_sweep_variant": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int),
                                             C.POINTER(C.c_int), C.POINTER(C.c_int), vp]),
         "gfx_set_sweep_variant": (C.c_int, [C.c_int, C.c_int, C.c_int]),
@@ -301,6 +302,12 @@ def set_sa(sampler=0, nmicro=0, ndir=4):
 
 def heatbath_sweeps(nsweep, beta):
     _check(load().gfx_heatbath_sweeps(int(nsweep), float(beta)), "heatbath")
+
+
+def stream_copy(nbytes, nrep=1):
+    ms = C.c_double()
+    _check(load().gfx_stream_copy(int(nbytes), int(nrep), C.byref(ms)), "stream_copy")
+    return ms.value
 
 
 def set_autotune(on):

@@ -584,6 +584,32 @@ int gfx_last_run_stats(double* ms, int* it32, int* it64) {
 
 int gfx_sync(void) { return hip_check(hipDeviceSynchronize(), "sync"); }
 
+// A device-to-device copy of `bytes` bytes with a plain streaming kernel,
+// nrep times; returns the mean kernel time in ms.  The reference point for
+// calibrating hardware byte counters (FETCH_SIZE / WRITE_SIZE).
+int gfx_stream_copy(size_t bytes, int nrep, double* ms) {
+  State& s = g_st;
+  if (ensure_common(0)) return GFX_ERR_DEVICE;
+  bytes &= ~(size_t)15;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+    hipFree(a);
+    return GFX_ERR_ALLOC;
+  }
+  hipMemsetAsync(a, 0, bytes, s.stream);
+  launch_stream_copy(a, b, bytes, s.stream);
+  hipEventRecord(s.ev0, s.stream);
+  for (int k = 0; k < nrep; k++) launch_stream_copy(a, b, bytes, s.stream);
+  hipEventRecord(s.ev1, s.stream);
+  int rc = hip_check(hipEventSynchronize(s.ev1), "copy");
+  float t = 0.f;
+  hipEventElapsedTime(&t, s.ev0, s.ev1);
+  if (ms) *ms = t / (nrep > 0 ? nrep : 1);
+  hipFree(a);
+  hipFree(b);
+  return rc;
+}
+
 int gfx_set_autotune(int on) {
   g_st.autotune = on ? 1 : 0;
   return GFX_OK;

@@ -16,6 +16,19 @@ void launch_sweep_f64(const double2* U, double2* g, const Geom& geo, int parity,
 void launch_sweep_f32(const float4* U, float4* g, const Geom& geo, int parity, int ndir,
                       double omega, const unsigned char* active, int variant, hipStream_t st);
 int functional_blocks(int nsh);
+int functional_block_size();
+void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
+
+// batched lattices (gfx_batch_api.hip)
+void launch_batch_sweep(bool fp32, const BatchProblem* probs, const int2* bmap, int nblocks,
+                        const int* active, int parity, int ndir, double omega, int variant,
+                        hipStream_t st);
+void launch_batch_functional(bool fp32, const BatchProblem* probs, const int2* fmap, int nblocks,
+                             const int* active, int ndir, double* partials, hipStream_t st);
+void launch_batch_finalize(const double* partials, const int* poff, const BatchProblem* probs,
+                           int nprob, int ndir, double thr, int* active, double* results,
+                           const int* iter, hipStream_t st);
+void launch_batch_advance(int* iter, int step, hipStream_t st);
 void launch_functional_f64(const double2* U, const double2* g, const Geom& geo, int ndir, int nseg,
                            double* partials, double eps, double* out, hipStream_t st);
 void launch_functional_f32(const float4* U, const float4* g, const Geom& geo, int ndir, int nseg,

@@ -70,15 +70,10 @@ __device__ __forceinline__ void block_reduce3(double& a, double& b, double& c) {
 //          operands are live at a time (FP64: 130 VGPRs, 3 waves/SIMD)
 //          instead of the fully unrolled form (more load-level parallelism
 //          per wave, 332 registers, 1 wave/SIMD).
-template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED>
-__global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* __restrict__ g,
-                                                     Geom geo, int parity, R c1, R c2,
-                                                     const unsigned char* __restrict__ active) {
-  const int b = xcd_block(blockIdx.x, gridDim.x);
-  const int i = b * BLOCK + threadIdx.x;
-  if (i >= geo.nh) return;
-  const Site s = site_coords(geo, i, parity);
-  if (NDIR == 3 && active != nullptr && !active[s.x[3]]) return;
+template <typename V, typename R, int NDIR, bool ROLLED>
+__device__ __forceinline__ void sweep_site(const V* __restrict__ U, V* __restrict__ g,
+                                           const Geom& geo, const Site& s, int i, int parity, R c1,
+                                           R c2) {
   const int S = geo.stride;
   const V* gop = field_ptr(static_cast<const V*>(g), 1 - parity, S);
   V* gme = field_ptr(g, parity, S);
@@ -124,10 +119,89 @@ __global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* 
   Store<V>::store(gme, S, i, G);
 }
 
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED>
+__global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* __restrict__ g,
+                                                     Geom geo, int parity, R c1, R c2,
+                                                     const unsigned char* __restrict__ active) {
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int i = b * BLOCK + threadIdx.x;
+  if (i >= geo.nh) return;
+  const Site s = site_coords(geo, i, parity);
+  if (NDIR == 3 && active != nullptr && !active[s.x[3]]) return;
+  sweep_site<V, R, NDIR, ROLLED>(U, g, geo, s, i, parity, c1, c2);
+}
+
+// ------------------------------------------------------------ batched sweep
+// Many lattices (Gribov copies of one configuration sharing its links, or
+// configurations of different sizes) in one launch.  bmap[block] = (problem,
+// block inside the problem); a problem whose active flag is 0 has converged
+// and its blocks return at once.
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED>
+__global__ __launch_bounds__(BLOCK) void k_batch_sweep(const BatchProblem* __restrict__ probs,
+                                                        const int2* __restrict__ bmap,
+                                                        const int* __restrict__ active,
+                                                        int parity, R c1, R c2) {
+  const int2 m = bmap[blockIdx.x];
+  if (!active[m.x]) return;
+  const Geom geo = probs[m.x].geo;
+  const int i = m.y * BLOCK + threadIdx.x;
+  if (i >= geo.nh) return;
+  const Site s = site_coords(geo, i, parity);
+  sweep_site<V, R, NDIR, ROLLED>(static_cast<const V*>(probs[m.x].U), static_cast<V*>(probs[m.x].g),
+                                 geo, s, i, parity, c1, c2);
+}
+
 // ------------------------------------------------------------ functional
 // Segment = whole lattice (Landau) or one time slice (Coulomb).  Within a
 // segment, sites [0, nsh) are parity 0 and [nsh, 2 nsh) parity 1; the
 // parity index of site k of segment seg is seg*nsh + (k mod nsh).
+// F contribution (sum over mu of Re tr g(x) U_mu(x) g^+(x+mu)) and
+// Re tr(Delta Delta^+) of site i of parity p (overrelax.cu:498-586), in FP64
+// whatever the storage precision.  The direction loop is kept rolled.
+template <typename V, int NDIR>
+__device__ __forceinline__ void functional_site(const V* __restrict__ U, const V* __restrict__ g,
+                                                const Geom& geo, int p, int i, double& F,
+                                                double& d) {
+  const int S = geo.stride;
+  const Site s = site_coords(geo, i, p);
+  const V* gme = field_ptr(g, p, S);
+  const V* gop = field_ptr(g, 1 - p, S);
+  int jf[NDIR], jb[NDIR];
+#pragma unroll
+  for (int mu = 0; mu < NDIR; mu++) {
+    jf[mu] = nb_fwd(geo, s, i, mu);
+    jb[mu] = nb_bwd(geo, s, i, mu);
+  }
+  M3<double> gather;
+  m_zero(gather);
+  F = 0.0;
+#pragma unroll 1
+  for (int mu = 0; mu < NDIR; mu++) {
+    const M3<double> Gx = load_m<V, double>(gme, S, i);
+    M3<double> Um = load_m<V, double>(field_ptr(U, p * 4 + mu, S), S, i);
+    M3<double> help = m_mul(Gx, Um);
+    M3<double> Gd = load_m_dagger<V, double>(gop, S, jf[mu]);
+    M3<double> fw = m_mul(help, Gd);
+    F += m_retrace(fw);
+    Um = load_m<V, double>(field_ptr(U, (1 - p) * 4 + mu, S), S, jb[mu]);
+    M3<double> Gb = load_m<V, double>(gop, S, jb[mu]);
+    help = m_mul(Gb, Um);
+    const M3<double> Gxd = load_m_dagger<V, double>(gme, S, i);
+    M3<double> bw = m_mul(help, Gxd);
+    vecpot(fw);
+    vecpot(bw);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        Cx<double> t = csub(bw.e[a][c], fw.e[a][c]);
+        gather.e[a][c] = cadd(gather.e[a][c], t);
+      }
+  }
+  M3<double> dd = m_mul_d(gather, gather);
+  d = m_retrace(dd);
+}
+
 template <typename V, int NDIR>
 __global__ __launch_bounds__(BLK) void k_functional(const V* __restrict__ U,
                                                      const V* __restrict__ g, Geom geo, int nsh,
@@ -138,39 +212,7 @@ __global__ __launch_bounds__(BLK) void k_functional(const V* __restrict__ U,
   if (k < 2 * nsh) {
     const int p = (k >= nsh) ? 1 : 0;
     const int i = seg * nsh + (k - p * nsh);
-    const int S = geo.stride;
-    const Site s = site_coords(geo, i, p);
-    const V* gme = field_ptr(g, p, S);
-    const V* gop = field_ptr(g, 1 - p, S);
-    const M3<double> Gx = load_m<V, double>(gme, S, i);
-    const M3<double> Gxd = load_m_dagger<V, double>(gme, S, i);
-    M3<double> gather;
-    m_zero(gather);
-#pragma unroll
-    for (int mu = 0; mu < NDIR; mu++) {
-      const int jf = nb_fwd(geo, s, i, mu);
-      const int jb = nb_bwd(geo, s, i, mu);
-      M3<double> Um = load_m<V, double>(field_ptr(U, p * 4 + mu, S), S, i);
-      M3<double> help = m_mul(Gx, Um);
-      M3<double> Gd = load_m_dagger<V, double>(gop, S, jf);
-      M3<double> fw = m_mul(help, Gd);
-      F += m_retrace(fw);
-      Um = load_m<V, double>(field_ptr(U, (1 - p) * 4 + mu, S), S, jb);
-      M3<double> Gb = load_m<V, double>(gop, S, jb);
-      help = m_mul(Gb, Um);
-      M3<double> bw = m_mul(help, Gxd);
-      vecpot(fw);
-      vecpot(bw);
-#pragma unroll
-      for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          Cx<double> t = csub(bw.e[a][c], fw.e[a][c]);
-          gather.e[a][c] = cadd(gather.e[a][c], t);
-        }
-    }
-    M3<double> dd = m_mul_d(gather, gather);
-    d = m_retrace(dd);
+    functional_site<V, NDIR>(U, g, geo, p, i, F, d);
     mx = d;
   }
   block_reduce3(F, d, mx);
@@ -180,6 +222,68 @@ __global__ __launch_bounds__(BLK) void k_functional(const V* __restrict__ U,
     o[1] = d;
     o[2] = mx;
   }
+}
+
+// batched functional: fmap[block] = (problem, block over the 2*nh sites of
+// the problem); partials per launched block
+template <typename V, int NDIR>
+__global__ __launch_bounds__(BLK) void k_batch_functional(const BatchProblem* __restrict__ probs,
+                                                           const int2* __restrict__ fmap,
+                                                           const int* __restrict__ active,
+                                                           double* __restrict__ partials) {
+  const int2 m = fmap[blockIdx.x];
+  if (!active[m.x]) return;
+  const Geom geo = probs[m.x].geo;
+  const int k = m.y * BLK + threadIdx.x;
+  double F = 0.0, d = 0.0, mx = 0.0;
+  if (k < 2 * geo.nh) {
+    const int p = (k >= geo.nh) ? 1 : 0;
+    functional_site<V, NDIR>(static_cast<const V*>(probs[m.x].U), static_cast<const V*>(probs[m.x].g),
+                             geo, p, k - p * geo.nh, F, d);
+    mx = d;
+  }
+  block_reduce3(F, d, mx);
+  if (threadIdx.x == 0) {
+    double* o = partials + 3 * (size_t)blockIdx.x;
+    o[0] = F;
+    o[1] = d;
+    o[2] = mx;
+  }
+}
+
+// one block per problem over its block partials [poff[q], poff[q+1]):
+// results[4q..] = F/(ndir V), dAdA/V, max, stop iteration; a problem whose
+// max < thr is deactivated at the iteration held in *iter
+__global__ __launch_bounds__(BLK) void k_batch_finalize(const double* __restrict__ partials,
+                                                        const int* __restrict__ poff,
+                                                        const BatchProblem* __restrict__ probs,
+                                                        int ndir, double thr,
+                                                        int* __restrict__ active,
+                                                        double* __restrict__ results,
+                                                        const int* __restrict__ iter) {
+  const int q = blockIdx.x;
+  if (!active[q]) return;
+  double F = 0.0, d = 0.0, mx = 0.0;
+  for (int b = poff[q] + threadIdx.x; b < poff[q + 1]; b += BLK) {
+    F += partials[3 * (size_t)b];
+    d += partials[3 * (size_t)b + 1];
+    mx = fmax(mx, partials[3 * (size_t)b + 2]);
+  }
+  block_reduce3(F, d, mx);
+  if (threadIdx.x == 0) {
+    const double V = 2.0 * probs[q].geo.nh;
+    results[4 * q + 0] = F / (ndir * V);
+    results[4 * q + 1] = d / V;
+    results[4 * q + 2] = mx;
+    if (mx < thr) {
+      results[4 * q + 3] = (double)(*iter);
+      active[q] = 0;
+    }
+  }
+}
+
+__global__ void k_batch_advance(int* iter, int step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *iter += step;
 }
 
 // grid-level pass: one block per segment over its nb block partials.
@@ -287,7 +391,10 @@ __global__ __launch_bounds__(BLK) void k_unpack(const V* __restrict__ src, doubl
   const int s = s0 + sl;
   int x[4], p;
   lex_site(geo, dL0, s, x, p);
-  M3<double> M = load_m<V, double>(field_ptr(src, p * nmat + m, geo.stride), geo.stride, s >> 1);
+  M3<double> M;
+  Store<V>::template load<double>(field_ptr(src, p * nmat + m, geo.stride), geo.stride, s >> 1,
+                                  M.e[0], M.e[1]);
+  m_row2_unfused(M);
   double* a = aos + (size_t)t * 18;
 #pragma unroll
   for (int r = 0; r < 3; r++)
@@ -404,6 +511,86 @@ void launch_sweep_f32(const float4* U, float4* g, const Geom& geo, int parity, i
 
 void launch_copy_fields(const void* src, void* dst, size_t bytes, hipStream_t st) {
   hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
+}
+
+// ---- batch launchers: nblocks entries of bmap built for the variant's block size
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED>
+static void launch_bsweep_v(const BatchProblem* probs, const int2* bmap, int nblocks,
+                            const int* active, int parity, R c1, R c2, hipStream_t st) {
+  hipLaunchKernelGGL((k_batch_sweep<V, R, NDIR, BLOCK, ROLLED>), dim3(nblocks), dim3(BLOCK), 0, st,
+                     probs, bmap, active, parity, c1, c2);
+}
+
+template <typename V, typename R, int NDIR>
+static void launch_bsweep_nd(const BatchProblem* probs, const int2* bmap, int nblocks,
+                             const int* active, int parity, R c1, R c2, int variant,
+                             hipStream_t st) {
+  switch (variant) {
+    case 0: launch_bsweep_v<V, R, NDIR, 256, true>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+    case 1: launch_bsweep_v<V, R, NDIR, 128, true>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+    case 2: launch_bsweep_v<V, R, NDIR, 64, true>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+    case 3: launch_bsweep_v<V, R, NDIR, 256, false>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+    case 4: launch_bsweep_v<V, R, NDIR, 128, false>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+    default: launch_bsweep_v<V, R, NDIR, 64, false>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+  }
+}
+
+void launch_batch_sweep(bool fp32, const BatchProblem* probs, const int2* bmap, int nblocks,
+                        const int* active, int parity, int ndir, double omega, int variant,
+                        hipStream_t st) {
+  const double c1 = (1.0 + omega - 1) / 1, c2 = (1.0 + omega - 2) / 2;
+  if (fp32) {
+    if (ndir == 4)
+      launch_bsweep_nd<float4, float, 4>(probs, bmap, nblocks, active, parity, (float)c1, (float)c2, variant, st);
+    else
+      launch_bsweep_nd<float4, float, 3>(probs, bmap, nblocks, active, parity, (float)c1, (float)c2, variant, st);
+  } else {
+    if (ndir == 4)
+      launch_bsweep_nd<double2, double, 4>(probs, bmap, nblocks, active, parity, c1, c2, variant, st);
+    else
+      launch_bsweep_nd<double2, double, 3>(probs, bmap, nblocks, active, parity, c1, c2, variant, st);
+  }
+}
+
+void launch_batch_functional(bool fp32, const BatchProblem* probs, const int2* fmap, int nblocks,
+                             const int* active, int ndir, double* partials, hipStream_t st) {
+  if (fp32) {
+    if (ndir == 4)
+      hipLaunchKernelGGL((k_batch_functional<float4, 4>), dim3(nblocks), dim3(BLK), 0, st, probs, fmap, active, partials);
+    else
+      hipLaunchKernelGGL((k_batch_functional<float4, 3>), dim3(nblocks), dim3(BLK), 0, st, probs, fmap, active, partials);
+  } else {
+    if (ndir == 4)
+      hipLaunchKernelGGL((k_batch_functional<double2, 4>), dim3(nblocks), dim3(BLK), 0, st, probs, fmap, active, partials);
+    else
+      hipLaunchKernelGGL((k_batch_functional<double2, 3>), dim3(nblocks), dim3(BLK), 0, st, probs, fmap, active, partials);
+  }
+}
+
+void launch_batch_finalize(const double* partials, const int* poff, const BatchProblem* probs,
+                           int nprob, int ndir, double thr, int* active, double* results,
+                           const int* iter, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_finalize, dim3(nprob), dim3(BLK), 0, st, partials, poff, probs, ndir,
+                     thr, active, results, iter);
+}
+
+void launch_batch_advance(int* iter, int step, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, st, iter, step);
+}
+
+int functional_block_size() { return BLK; }
+
+// plain streaming copy (16-byte vectors), the calibration reference for HBM
+// byte counters
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ src,
+                                                     float4* __restrict__ dst, size_t n) {
+  for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256)
+    dst[k] = src[k];
+}
+
+void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t st) {
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(k_stream_copy, dim3(4096), dim3(256), 0, st, (const float4*)src, (float4*)dst, n);
 }
 
 int functional_blocks(int nsh) { return (2 * nsh + BLK - 1) / BLK; }

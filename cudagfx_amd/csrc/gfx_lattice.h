@@ -179,6 +179,14 @@ __device__ __forceinline__ V* field_ptr(V* base, int field, int stride) {
   return base + (size_t)field * Store<V>::NPLANE * stride;
 }
 
+// One lattice of a batch: its geometry and its fields (FP64 double2 or FP32
+// float4 storage, by the phase of the run).  Gribov copies share U.
+struct BatchProblem {
+  Geom geo;
+  const void* U;
+  void* g;
+};
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2).  Remap so that XCD k works on a contiguous range of sites and shares
 // neighbour g(x +- mu) lines in its own L2.

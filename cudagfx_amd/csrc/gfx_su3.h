@@ -72,6 +72,26 @@ __device__ __forceinline__ void m_row2(M3<R>& u) {
   u.e[2][2] = cconj(csub(cmul(u.e[0][0], u.e[1][1]), cmul(u.e[0][1], u.e[1][0])));
 }
 
+// The same third row without fused multiply-adds: bit for bit what the
+// reference's host conversion from2vtosu3_trafo (su3manip.c:341-356)
+// computes from the stored rows.  Used where matrices go back to the host.
+template <typename R>
+__device__ __forceinline__ void m_row2_unfused(M3<R>& u) {
+#pragma clang fp contract(off)
+  const int ia[3] = {1, 2, 0}, ib[3] = {2, 0, 1};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const Cx<R> a1 = u.e[0][ia[c]], b2 = u.e[1][ib[c]];
+    const Cx<R> a2 = u.e[0][ib[c]], b1 = u.e[1][ia[c]];
+    const R h1re = a1.re * b2.re - a1.im * b2.im;
+    const R h1im = a1.re * b2.im + a1.im * b2.re;
+    const R h2re = a2.re * b1.re - a2.im * b1.im;
+    const R h2im = a2.re * b1.im + a2.im * b1.re;
+    u.e[2][c].re = h1re - h2re;
+    u.e[2][c].im = R(-1) * (h1im - h2im);
+  }
+}
+
 // Dagger of a 2-row matrix whose rows sit in u.e[0..1]: the result's
 // columns 0,1 are the conjugated rows, column 2 the conjugated cross product
 // (dev_su3.cu:826-929).

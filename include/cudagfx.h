@@ -152,6 +152,9 @@ int gfx_time_sweeps(int nsweeps, int precision, double *ms_per_halfsweep);
 int gfx_last_run_stats(double *ms, int *iters_fp32, int *iters_fp64);
 /* wait for all device work of the library */
 int gfx_sync(void);
+/* nrep device copies of `bytes` bytes by a plain streaming kernel; mean ms
+ * per copy.  Reference point for calibrating HBM byte counters. */
+int gfx_stream_copy(size_t bytes, int nrep, double *ms);
 
 /* Sweep-kernel autotuner: on the first sweep of a lattice in a precision and
  * gauge, every variant (workgroup size 256/128/64, star loop rolled or
