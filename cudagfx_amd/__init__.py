@@ -104,6 +104,20 @@ def load():
         "gfx_micro_sweeps": (C.c_int, [C.c_int, C.c_int]),
         "gfx_set_autotune": (C.c_int, [C.c_int]),
         "gfx_stream_copy": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+        "gfx_batch_create": (C.c_int, []),
+        "gfx_batch_destroy": (None, []),
+        "gfx_batch_size": (C.c_int, []),
+        "gfx_batch_add": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]),
+        "gfx_batch_add_copy": (C.c_int, [C.c_int, vp]),
+        "gfx_batch_add_random_copy": (C.c_int, [C.c_int]),
+        "gfx_batch_run": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int]),
+        "gfx_batch_result": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+        "gfx_batch_get_trafo": (C.c_int, [C.c_int, vp]),
+        "gfx_batch_stats": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_double)]),
+        "gfx_batch_variant": (C.c_int, [C.c_int, C.POINTER(C.c_int), vp]),
         "gfx_get_sweep_variant": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int),
                                             C.POINTER(C.c_int), C.POINTER(C.c_int), vp]),
         "gfx_set_sweep_variant": (C.c_int, [C.c_int, C.c_int, C.c_int]),
@@ -302,6 +316,68 @@ def set_sa(sampler=0, nmicro=0, ndir=4):
 
 def heatbath_sweeps(nsweep, beta):
     _check(load().gfx_heatbath_sweeps(int(nsweep), float(beta)), "heatbath")
+
+
+class Batch:
+    """The batching form (gfx_batch_*): Gribov copies sharing links and
+    lattices of different sizes fixed together."""
+
+    def __init__(self):
+        _check(load().gfx_batch_create(), "batch_create")
+        self.dims = []
+
+    def add(self, dims, gf, trafo):
+        q = load().gfx_batch_add(*[int(d) for d in dims], _ptr(gf), _ptr(trafo))
+        if q < 0:
+            _check(-q, "batch_add")
+        self.dims.append(tuple(dims))
+        return q
+
+    def add_copy(self, base, trafo=None):
+        q = load().gfx_batch_add_copy(int(base), None if trafo is None else _ptr(trafo))
+        if q < 0:
+            _check(-q, "batch_add_copy")
+        self.dims.append(self.dims[base])
+        return q
+
+    def add_random_copy(self, base):
+        q = load().gfx_batch_add_random_copy(int(base))
+        if q < 0:
+            _check(-q, "batch_add_random_copy")
+        self.dims.append(self.dims[base])
+        return q
+
+    def run(self, maxit, eps, checkinterval, precision=PREC_FP64, switch_theta=1e-6,
+            periods_per_graph=4):
+        return load().gfx_batch_run(int(maxit), float(eps), int(checkinterval), int(precision),
+                                    float(switch_theta), int(periods_per_graph))
+
+    def result(self, q):
+        F, D, M, it = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+        _check(load().gfx_batch_result(int(q), C.byref(F), C.byref(D), C.byref(M), C.byref(it)),
+               "batch_result")
+        return F.value, D.value, M.value, it.value
+
+    def trafo(self, q):
+        LX, LY, LZ, LT = self.dims[q]
+        g = np.zeros(18 * LX * LY * LZ * LT)
+        _check(load().gfx_batch_get_trafo(int(q), _ptr(g)), "batch_get_trafo")
+        return g
+
+    def stats(self):
+        ms, gl, a, b, su = C.c_double(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        load().gfx_batch_stats(C.byref(ms), C.byref(gl), C.byref(a), C.byref(b), C.byref(su))
+        return dict(ms=ms.value, graph_launches=gl.value, iters_fp32=a.value, iters_fp64=b.value,
+                    site_updates=su.value)
+
+    def variant(self, precision=PREC_FP64):
+        v = C.c_int()
+        ms = np.zeros(6)
+        rc = load().gfx_batch_variant(int(precision), C.byref(v), ms.ctypes.data_as(C.c_void_p))
+        return (v.value, ms) if rc == 0 else None
+
+    def close(self):
+        load().gfx_batch_destroy()
 
 
 def stream_copy(nbytes, nrep=1):

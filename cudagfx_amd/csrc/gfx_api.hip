@@ -91,7 +91,7 @@ int ensure_common(int need_links32) {
   return GFX_OK;
 }
 
-static int dev_alloc(void** p, size_t bytes, const char* what) {
+int dev_alloc(void** p, size_t bytes, const char* what) {
   if (hipMalloc(p, bytes) != hipSuccess) {
     fprintf(stderr, "cudagfx: allocation of %s (%zu bytes) failed\n", what, bytes);
     *p = nullptr;
@@ -123,30 +123,37 @@ int ensure_staging(State& s) {
   return dev_alloc((void**)&s.staging, (size_t)CHUNK * 4 * 18 * sizeof(double), "staging");
 }
 
-// host AoS su3 (nmat per site) -> device FP64 fields
-int upload_aos(State& s, const double* host, double2* dst, int nmat) {
+// host AoS su3 (nmat per site) -> device FP64 fields of lattice geo
+int upload_aos_geo(State& s, const Geom& geo, const double* host, double2* dst, int nmat) {
   if (ensure_staging(s)) return GFX_ERR_ALLOC;
-  const int V = 2 * s.geo.nh;
+  const int V = 2 * geo.nh;
   for (int s0 = 0; s0 < V; s0 += CHUNK) {
     const int ns = (V - s0 < CHUNK) ? V - s0 : CHUNK;
     const size_t bytes = (size_t)ns * nmat * 18 * sizeof(double);
     if (hip_check(hipMemcpyAsync(s.staging, host + (size_t)s0 * nmat * 18, bytes, hipMemcpyHostToDevice, s.stream), "H2D")) return GFX_ERR_DEVICE;
-    launch_pack_f64(s.staging, dst, s.geo, s0, ns, nmat, s.stream);
+    launch_pack_f64(s.staging, dst, geo, s0, ns, nmat, s.stream);
   }
   return hip_check(hipStreamSynchronize(s.stream), "upload");
 }
 
-int download_aos(State& s, const double2* src, double* host, int nmat) {
+int download_aos_geo(State& s, const Geom& geo, const double2* src, double* host, int nmat) {
   if (ensure_staging(s)) return GFX_ERR_ALLOC;
-  const int V = 2 * s.geo.nh;
+  const int V = 2 * geo.nh;
   for (int s0 = 0; s0 < V; s0 += CHUNK) {
     const int ns = (V - s0 < CHUNK) ? V - s0 : CHUNK;
     const size_t bytes = (size_t)ns * nmat * 18 * sizeof(double);
-    launch_unpack_f64(src, s.staging, s.geo, s0, ns, nmat, s.stream);
+    launch_unpack_f64(src, s.staging, geo, s0, ns, nmat, s.stream);
     if (hip_check(hipMemcpyAsync(host + (size_t)s0 * nmat * 18, s.staging, bytes, hipMemcpyDeviceToHost, s.stream), "D2H")) return GFX_ERR_DEVICE;
     if (hip_check(hipStreamSynchronize(s.stream), "download")) return GFX_ERR_DEVICE;
   }
   return GFX_OK;
+}
+
+int upload_aos(State& s, const double* host, double2* dst, int nmat) {
+  return upload_aos_geo(s, s.geo, host, dst, nmat);
+}
+int download_aos(State& s, const double2* src, double* host, int nmat) {
+  return download_aos_geo(s, s.geo, src, host, nmat);
 }
 
 int ensure_f32_links(State& s) {

@@ -1,0 +1,455 @@
+// gfx_batch_api.hip -- the batching form: many lattices gauge fixed together
+// (Gribov copies of one configuration, or configurations of different sizes)
+// with one sweep launch per half sweep covering every lattice, device-side
+// convergence flags, and the sweeps of a check period captured in a HIP graph
+// that is replayed until every lattice has converged.
+//
+// No counterpart in the reference (it fixes one configuration per run); each
+// lattice follows overrelax_gauge (overrelax.cu:677-823) exactly: checks at
+// i % ck == 0 after sweep i, stop at max dAdA < eps, the stop iteration is
+// recorded per lattice.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/cudagfx.h"
+#include "gfx_internal.h"
+#include "gfx_lattice.h"
+#include "gfx_state.h"
+
+namespace gfx {
+
+struct BProbHost {
+  Geom geo{};
+  double2* U64 = nullptr;
+  float4* U32 = nullptr;
+  double2* g64 = nullptr;
+  float4* g32 = nullptr;
+  int ushare = 0;  // problem whose links this one uses (itself if it owns them)
+  int stop_iter[2] = {-1, -1};  // per phase (FP32 bulk, FP64)
+};
+
+struct Batch {
+  std::vector<BProbHost> probs;
+  BatchProblem* d_probs = nullptr;
+  int2* d_bmap[3] = {nullptr, nullptr, nullptr};
+  int nbmap[3] = {0, 0, 0};
+  int2* d_fmap = nullptr;
+  int nfmap = 0;
+  int* d_poff = nullptr;
+  int* d_active = nullptr;
+  int* h_active = nullptr;
+  double* d_results = nullptr;
+  double* h_results = nullptr;
+  int* d_iter = nullptr;
+  double* d_partials = nullptr;
+  bool built = false;
+  int variant[2] = {0, 0};
+  bool tuned[2] = {false, false};
+  float tune_ms[2][8] = {};
+  // statistics of the last run
+  double run_ms = 0.0;
+  int graph_launches = 0, iters32 = 0, iters64 = 0;
+  double site_updates = 0.0;
+};
+static Batch g_b;
+
+static const int BLOCKS[3] = {256, 128, 64};
+
+static int vblock_index(int variant) {
+  const int b = sweep_variant_block(variant);
+  return b == 256 ? 0 : (b == 128 ? 1 : 2);
+}
+
+static void batch_free_maps(Batch& b) {
+  for (int k = 0; k < 3; k++) {
+    hipFree(b.d_bmap[k]);
+    b.d_bmap[k] = nullptr;
+  }
+  hipFree(b.d_probs);
+  hipFree(b.d_fmap);
+  hipFree(b.d_poff);
+  hipFree(b.d_active);
+  hipFree(b.d_results);
+  hipFree(b.d_iter);
+  hipFree(b.d_partials);
+  if (b.h_active) hipHostFree(b.h_active);
+  if (b.h_results) hipHostFree(b.h_results);
+  b.d_probs = nullptr;
+  b.d_fmap = nullptr;
+  b.d_poff = nullptr;
+  b.d_active = nullptr;
+  b.d_results = nullptr;
+  b.d_iter = nullptr;
+  b.d_partials = nullptr;
+  b.h_active = nullptr;
+  b.h_results = nullptr;
+  b.built = false;
+}
+
+static void batch_free(Batch& b) {
+  if (g_st.stream) hipStreamSynchronize(g_st.stream);
+  for (auto& p : b.probs) {
+    hipFree(p.g64);
+    hipFree(p.g32);
+    if (&p - &b.probs[0] == p.ushare) {
+      hipFree(p.U64);
+      hipFree(p.U32);
+    }
+  }
+  b.probs.clear();
+  batch_free_maps(b);
+  for (int k = 0; k < 2; k++) b.tuned[k] = false;
+}
+
+// block maps: Gribov copies sharing links are interleaved so that the
+// blocks covering the same sites of every copy run together and share the
+// link lines in L2 / Infinity Cache; lattices with their own links follow
+// one another.
+static int batch_build(Batch& b) {
+  batch_free_maps(b);
+  const int n = (int)b.probs.size();
+  for (int k = 0; k < 3; k++) {
+    std::vector<int2> map;
+    std::vector<bool> done(n, false);
+    for (int q0 = 0; q0 < n; q0++) {
+      if (done[q0]) continue;
+      std::vector<int> group;
+      for (int q = q0; q < n; q++)
+        if (b.probs[q].ushare == b.probs[q0].ushare) {
+          group.push_back(q);
+          done[q] = true;
+        }
+      int nbmax = 0;
+      for (int q : group) nbmax = std::max(nbmax, (b.probs[q].geo.nh + BLOCKS[k] - 1) / BLOCKS[k]);
+      for (int blk = 0; blk < nbmax; blk++)
+        for (int q : group)
+          if (blk < (b.probs[q].geo.nh + BLOCKS[k] - 1) / BLOCKS[k]) map.push_back(make_int2(q, blk));
+    }
+    b.nbmap[k] = (int)map.size();
+    if (dev_alloc((void**)&b.d_bmap[k], sizeof(int2) * map.size(), "batch map")) return GFX_ERR_ALLOC;
+    hipMemcpy(b.d_bmap[k], map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice);
+  }
+  std::vector<int2> fmap;
+  std::vector<int> poff(n + 1, 0);
+  const int fb = functional_block_size();
+  for (int q = 0; q < n; q++) {
+    poff[q] = (int)fmap.size();
+    const int nb = (2 * b.probs[q].geo.nh + fb - 1) / fb;
+    for (int blk = 0; blk < nb; blk++) fmap.push_back(make_int2(q, blk));
+  }
+  poff[n] = (int)fmap.size();
+  b.nfmap = (int)fmap.size();
+  if (dev_alloc((void**)&b.d_fmap, sizeof(int2) * fmap.size(), "batch fmap") ||
+      dev_alloc((void**)&b.d_poff, sizeof(int) * (n + 1), "batch poff") ||
+      dev_alloc((void**)&b.d_probs, sizeof(BatchProblem) * n, "batch problems") ||
+      dev_alloc((void**)&b.d_active, sizeof(int) * n, "batch active") ||
+      dev_alloc((void**)&b.d_results, sizeof(double) * 4 * n, "batch results") ||
+      dev_alloc((void**)&b.d_iter, sizeof(int), "batch iter") ||
+      dev_alloc((void**)&b.d_partials, sizeof(double) * 3 * fmap.size(), "batch partials"))
+    return GFX_ERR_ALLOC;
+  if (hipHostMalloc((void**)&b.h_active, sizeof(int) * n, 0) != hipSuccess ||
+      hipHostMalloc((void**)&b.h_results, sizeof(double) * 4 * n, 0) != hipSuccess)
+    return GFX_ERR_ALLOC;
+  hipMemcpy(b.d_fmap, fmap.data(), sizeof(int2) * fmap.size(), hipMemcpyHostToDevice);
+  hipMemcpy(b.d_poff, poff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice);
+  b.built = true;
+  return GFX_OK;
+}
+
+static void set_phase_pointers(Batch& b, bool fp32) {
+  std::vector<BatchProblem> h(b.probs.size());
+  for (size_t q = 0; q < b.probs.size(); q++) {
+    const BProbHost& p = b.probs[q];
+    const BProbHost& o = b.probs[p.ushare];
+    h[q].geo = p.geo;
+    h[q].U = fp32 ? (const void*)o.U32 : (const void*)o.U64;
+    h[q].g = fp32 ? (void*)p.g32 : (void*)p.g64;
+  }
+  hipStreamSynchronize(g_st.stream);
+  hipMemcpy(b.d_probs, h.data(), sizeof(BatchProblem) * h.size(), hipMemcpyHostToDevice);
+}
+
+static int ensure_fp32(Batch& b) {
+  State& s = g_st;
+  for (size_t q = 0; q < b.probs.size(); q++) {
+    BProbHost& p = b.probs[q];
+    if ((int)q == p.ushare) {
+      if (!p.U32 && dev_alloc((void**)&p.U32, 8 * field_bytes32(p.geo), "batch FP32 links")) return GFX_ERR_ALLOC;
+      launch_f64_to_f32(p.U64, p.U32, p.geo, 8, s.stream);
+    }
+    if (!p.g32 && dev_alloc((void**)&p.g32, 2 * field_bytes32(p.geo), "batch FP32 trafo")) return GFX_ERR_ALLOC;
+    launch_f64_to_f32(p.g64, p.g32, p.geo, 2, s.stream);
+  }
+  return GFX_OK;
+}
+
+static void batch_check(Batch& b, bool fp32, int ndir, double thr) {
+  State& s = g_st;
+  launch_batch_functional(fp32, b.d_probs, b.d_fmap, b.nfmap, b.d_active, ndir, b.d_partials, s.stream);
+  launch_batch_finalize(b.d_partials, b.d_poff, b.d_probs, (int)b.probs.size(), ndir, thr,
+                        b.d_active, b.d_results, b.d_iter, s.stream);
+}
+
+static void batch_sweep(Batch& b, bool fp32, int ndir, int variant) {
+  State& s = g_st;
+  const int k = vblock_index(variant);
+  for (int p = 0; p < 2; p++)
+    launch_batch_sweep(fp32, b.d_probs, b.d_bmap[k], b.nbmap[k], b.d_active, p, ndir, s.omega,
+                       variant, s.stream);
+}
+
+static int count_active(Batch& b) {
+  State& s = g_st;
+  hipMemcpyAsync(b.h_active, b.d_active, sizeof(int) * b.probs.size(), hipMemcpyDeviceToHost, s.stream);
+  if (hip_check(hipStreamSynchronize(s.stream), "batch flags")) return -1;
+  int n = 0;
+  for (size_t q = 0; q < b.probs.size(); q++) n += b.h_active[q] ? 1 : 0;
+  return n;
+}
+
+// Runs one phase (FP32 bulk or FP64) from global iteration *it until every
+// lattice has max dAdA < thr or maxit is reached.  The first iterations run
+// eagerly (the sweep variants are timed on them when not tuned yet) up to a
+// check boundary; then graphs of `periods` check periods are replayed.
+static int batch_phase(Batch& b, bool fp32, int ndir, double thr, int ck, int maxit, int periods,
+                       int* it) {
+  State& s = g_st;
+  const int n = (int)b.probs.size();
+  const int pi = fp32 ? 1 : 0;
+  set_phase_pointers(b, fp32);
+  launch_fill_i32(b.d_active, n, 1, s.stream);
+  int i = *it;
+  int tune_v = b.tuned[pi] ? -1 : 0;
+  // eager head: tune, then reach an iteration with i % ck == 1 (graph periods
+  // end on a check)
+  while (i < maxit) {
+    const int v = (tune_v >= 0) ? tune_v : b.variant[pi];
+    if (tune_v >= 0) hipEventRecord(s.ev0, s.stream);
+    batch_sweep(b, fp32, ndir, v);
+    if (tune_v >= 0) {
+      hipEventRecord(s.ev1, s.stream);
+      hipEventSynchronize(s.ev1);
+      float t = 0.f;
+      hipEventElapsedTime(&t, s.ev0, s.ev1);
+      b.tune_ms[pi][tune_v] = t;
+      if (++tune_v == SWEEP_VARIANTS) {
+        int best = 0;
+        for (int k = 1; k < SWEEP_VARIANTS; k++)
+          if (b.tune_ms[pi][k] < b.tune_ms[pi][best]) best = k;
+        b.variant[pi] = best;
+        b.tuned[pi] = true;
+        tune_v = -1;
+      }
+    }
+    if (fp32) b.iters32++; else b.iters64++;
+    if (i % ck == 0) {
+      launch_fill_i32(b.d_iter, 1, i, s.stream);
+      batch_check(b, fp32, ndir, thr);
+      if (count_active(b) == 0) {
+        *it = i + 1;
+        return GFX_OK;
+      }
+    }
+    i++;
+    if (tune_v < 0 && (ck == 1 || i % ck == 1)) break;
+  }
+  if (i >= maxit) {
+    *it = i;
+    return GFX_OK;
+  }
+  // graph of `periods` check periods: ck sweeps, check, advance the counter
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const int v = b.variant[pi];
+  if (hip_check(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal), "capture")) return GFX_ERR_DEVICE;
+  for (int per = 0; per < periods; per++) {
+    for (int k = 0; k < ck; k++) batch_sweep(b, fp32, ndir, v);
+    batch_check(b, fp32, ndir, thr);
+    launch_batch_advance(b.d_iter, ck, s.stream);
+  }
+  if (hip_check(hipStreamEndCapture(s.stream, &graph), "end capture")) return GFX_ERR_DEVICE;
+  if (hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate")) {
+    hipGraphDestroy(graph);
+    return GFX_ERR_DEVICE;
+  }
+  launch_fill_i32(b.d_iter, 1, i + ck - 1, s.stream);
+  int rc = GFX_OK;
+  while (i < maxit) {
+    if (hip_check(hipGraphLaunch(exec, s.stream), "graph launch")) {
+      rc = GFX_ERR_DEVICE;
+      break;
+    }
+    b.graph_launches++;
+    i += periods * ck;
+    const int na = count_active(b);
+    if (fp32) b.iters32 += periods * ck; else b.iters64 += periods * ck;
+    if (na <= 0) break;
+  }
+  hipGraphExecDestroy(exec);
+  hipGraphDestroy(graph);
+  *it = i;
+  return rc;
+}
+
+}  // namespace gfx
+
+using namespace gfx;
+
+extern "C" {
+
+int gfx_batch_create(void) {
+  batch_free(g_b);
+  return ensure_common(0);
+}
+
+void gfx_batch_destroy(void) { batch_free(g_b); }
+
+int gfx_batch_size(void) { return (int)g_b.probs.size(); }
+
+// a lattice with its own links; returns its index (>= 0) or -error
+int gfx_batch_add(int l0, int l1, int l2, int l3, gfx_su3* gf, gfx_su3* trafo) {
+  State& s = g_st;
+  if (ensure_common(0)) return -GFX_ERR_DEVICE;
+  if (!geometry_ok(l0, l1, l2, l3)) return -GFX_ERR_GEOMETRY;
+  BProbHost p;
+  p.geo = make_geom(l0, l1, l2, l3);
+  p.ushare = (int)g_b.probs.size();
+  if (dev_alloc((void**)&p.U64, 8 * field_bytes64(p.geo), "batch links")) return -GFX_ERR_ALLOC;
+  if (dev_alloc((void**)&p.g64, 2 * field_bytes64(p.geo), "batch trafo")) {
+    hipFree(p.U64);
+    return -GFX_ERR_ALLOC;
+  }
+  if (upload_aos_geo(s, p.geo, (const double*)gf, p.U64, 4) ||
+      upload_aos_geo(s, p.geo, (const double*)trafo, p.g64, 1)) {
+    hipFree(p.U64);
+    hipFree(p.g64);
+    return -GFX_ERR_DEVICE;
+  }
+  g_b.probs.push_back(p);
+  g_b.built = false;
+  return p.ushare;
+}
+
+// a Gribov copy: the links of lattice `base`, its own initial g (V su3)
+int gfx_batch_add_copy(int base, gfx_su3* trafo) {
+  State& s = g_st;
+  if (base < 0 || base >= (int)g_b.probs.size()) return -GFX_ERR_STATE;
+  BProbHost p;
+  const BProbHost& o = g_b.probs[base];
+  p.geo = o.geo;
+  p.ushare = o.ushare;
+  p.U64 = g_b.probs[o.ushare].U64;
+  if (dev_alloc((void**)&p.g64, 2 * field_bytes64(p.geo), "batch trafo")) return -GFX_ERR_ALLOC;
+  if (trafo && upload_aos_geo(s, p.geo, (const double*)trafo, p.g64, 1)) {
+    hipFree(p.g64);
+    return -GFX_ERR_DEVICE;
+  }
+  g_b.probs.push_back(p);
+  g_b.built = false;
+  return (int)g_b.probs.size() - 1;
+}
+
+// device pointer and geometry of lattice q's FP64 g (used by the random copy initialiser)
+int gfx_batch_trafo_field(int q, void** g64, int dims[4]) {
+  if (q < 0 || q >= (int)g_b.probs.size()) return GFX_ERR_STATE;
+  if (g64) *g64 = g_b.probs[q].g64;
+  if (dims)
+    for (int k = 0; k < 4; k++) dims[k] = g_b.probs[q].geo.L[k];
+  return GFX_OK;
+}
+
+int gfx_batch_run(int maxit, double eps, int checkinterval, int precision, double switch_theta,
+                  int periods_per_graph) {
+  State& s = g_st;
+  Batch& b = g_b;
+  if (b.probs.empty()) return GFX_ERR_STATE;
+  if (checkinterval < 1) checkinterval = 1;
+  if (periods_per_graph < 1) periods_per_graph = 1;
+  if (!b.built) {
+    int rc = batch_build(b);
+    if (rc) return rc;
+  }
+  const bool mixed = (precision == GFX_PREC_MIXED);
+  if (mixed && ensure_fp32(b)) return GFX_ERR_ALLOC;
+  b.graph_launches = b.iters32 = b.iters64 = 0;
+  for (auto& p : b.probs) p.stop_iter[0] = p.stop_iter[1] = -1;
+  hipStreamSynchronize(s.stream);
+  const auto t0 = std::chrono::steady_clock::now();
+  int it = 0, rc = GFX_OK;
+  if (mixed) {
+    launch_fill_f64(b.d_results, 4 * (int)b.probs.size(), -1.0, s.stream);
+    rc = batch_phase(b, true, 4, std::max(switch_theta > 0 ? switch_theta : s.switch_theta, eps),
+                     checkinterval, maxit, periods_per_graph, &it);
+    hipMemcpyAsync(b.h_results, b.d_results, sizeof(double) * 4 * b.probs.size(), hipMemcpyDeviceToHost, s.stream);
+    hipStreamSynchronize(s.stream);
+    for (size_t q = 0; q < b.probs.size(); q++) {
+      b.probs[q].stop_iter[0] = (int)b.h_results[4 * q + 3];
+      launch_g32_to_g64(b.probs[q].g32, b.probs[q].g64, b.probs[q].geo, s.stream);
+    }
+  }
+  const int it_fp64_start = it;
+  if (rc == GFX_OK) {
+    launch_fill_f64(b.d_results, 4 * (int)b.probs.size(), -1.0, s.stream);
+    rc = batch_phase(b, false, 4, eps, checkinterval, maxit, periods_per_graph, &it);
+  }
+  hipMemcpyAsync(b.h_results, b.d_results, sizeof(double) * 4 * b.probs.size(), hipMemcpyDeviceToHost, s.stream);
+  hipStreamSynchronize(s.stream);
+  b.run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  int nconv = 0;
+  b.site_updates = 0.0;
+  for (size_t q = 0; q < b.probs.size(); q++) {
+    BProbHost& p = b.probs[q];
+    p.stop_iter[1] = (int)b.h_results[4 * q + 3];
+    if (p.stop_iter[1] >= 0) nconv++;
+    // iterations this lattice was updated in each phase
+    const int n32 = mixed ? (p.stop_iter[0] >= 0 ? p.stop_iter[0] + 1 : it_fp64_start) : 0;
+    const int n64 = (p.stop_iter[1] >= 0 ? p.stop_iter[1] + 1 : it) - it_fp64_start;
+    b.site_updates += 2.0 * p.geo.nh * (double)(n32 + n64);
+  }
+  if (rc != GFX_OK) return rc;
+  return nconv == (int)b.probs.size() ? GFX_OK : GFX_ERR_NOCONV;
+}
+
+// F, dAdA, max dAdA and the stop iteration (-1: not converged) of lattice q
+int gfx_batch_result(int q, double* F, double* dAdA, double* maxdAdA, int* iter) {
+  if (q < 0 || q >= (int)g_b.probs.size() || !g_b.h_results) return GFX_ERR_STATE;
+  if (F) *F = g_b.h_results[4 * q];
+  if (dAdA) *dAdA = g_b.h_results[4 * q + 1];
+  if (maxdAdA) *maxdAdA = g_b.h_results[4 * q + 2];
+  if (iter) *iter = g_b.probs[q].stop_iter[1];
+  return GFX_OK;
+}
+
+int gfx_batch_get_trafo(int q, gfx_su3* trafo) {
+  if (q < 0 || q >= (int)g_b.probs.size()) return GFX_ERR_STATE;
+  return download_aos_geo(g_st, g_b.probs[q].geo, g_b.probs[q].g64, (double*)trafo, 1);
+}
+
+// wall time of the last gfx_batch_run, graph launches, iterations in each
+// precision and the site updates done (sum over lattices of V x iterations
+// while the lattice was being updated)
+int gfx_batch_stats(double* ms, int* graph_launches, int* iters32, int* iters64,
+                    double* site_updates) {
+  if (ms) *ms = g_b.run_ms;
+  if (graph_launches) *graph_launches = g_b.graph_launches;
+  if (iters32) *iters32 = g_b.iters32;
+  if (iters64) *iters64 = g_b.iters64;
+  if (site_updates) *site_updates = g_b.site_updates;
+  return GFX_OK;
+}
+
+// the sweep variant picked for the batch in each precision (index 0 FP64, 1 FP32)
+int gfx_batch_variant(int precision, int* variant, double* ms_per_variant) {
+  const int pi = precision == GFX_PREC_MIXED ? 1 : 0;
+  if (variant) *variant = g_b.variant[pi];
+  if (ms_per_variant)
+    for (int k = 0; k < SWEEP_VARIANTS; k++) ms_per_variant[k] = g_b.tune_ms[pi][k];
+  return g_b.tuned[pi] ? GFX_OK : GFX_ERR_STATE;
+}
+
+}  // extern "C"

@@ -249,6 +249,37 @@ __global__ __launch_bounds__(HB_BLK) void k_link_heatbath(double2* __restrict__ 
   Store<double2>::store(Un, S, i, h);
 }
 
+// Haar-random SU(3) g(x): rows 0,1 from 12 standard normals per site (two
+// complex Gaussian vectors), Gram-Schmidt, row 2 = conj(row0 x row1).  The
+// first two rows of a Haar unitary matrix are two orthonormalised complex
+// Gaussian vectors, and fixing row 2 this way puts det = 1.
+__global__ __launch_bounds__(HB_BLK) void k_haar_trafo(double2* __restrict__ g, Geom geo,
+                                                       FastDiv dL0, const float* __restrict__ gauss) {
+  const int s = blockIdx.x * HB_BLK + threadIdx.x;
+  if (s >= 2 * geo.nh) return;
+  uint32_t q = fdiv((uint32_t)s, dL0);
+  const int x0 = s - (int)q * geo.L[0];
+  uint32_t q2 = fdiv(q, geo.dL1);
+  const int x1 = (int)(q - q2 * (uint32_t)geo.L[1]);
+  uint32_t q3 = fdiv(q2, geo.dL2);
+  const int x2 = (int)(q2 - q3 * (uint32_t)geo.L[2]);
+  const int p = (x0 + x1 + x2 + (int)q3) & 1;
+  const float* z = gauss + 12 * (size_t)s;
+  M3<double> m;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    m.e[0][c] = cx<double>((double)z[2 * c], (double)z[2 * c + 1]);
+    m.e[1][c] = cx<double>((double)z[6 + 2 * c], (double)z[6 + 2 * c + 1]);
+  }
+  m_normalize(m);
+  Store<double2>::store(field_ptr(g, p, geo.stride), geo.stride, s >> 1, m);
+}
+
+void launch_haar_trafo(double2* g, const Geom& geo, const float* gauss, hipStream_t st) {
+  hipLaunchKernelGGL(k_haar_trafo, dim3((2 * geo.nh + HB_BLK - 1) / HB_BLK), dim3(HB_BLK), 0, st,
+                     g, geo, make_fastdiv(geo.L[0]), gauss);
+}
+
 void launch_sa_sweep(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
                      double beta, int sampler, const float* gauss, const float* unif,
                      hipStream_t st) {

@@ -29,6 +29,8 @@ void launch_batch_finalize(const double* partials, const int* poff, const BatchP
                            int nprob, int ndir, double thr, int* active, double* results,
                            const int* iter, hipStream_t st);
 void launch_batch_advance(int* iter, int step, hipStream_t st);
+void launch_fill_i32(int* p, int n, int v, hipStream_t st);
+void launch_fill_f64(double* p, int n, double v, hipStream_t st);
 void launch_functional_f64(const double2* U, const double2* g, const Geom& geo, int ndir, int nseg,
                            double* partials, double eps, double* out, hipStream_t st);
 void launch_functional_f32(const float4* U, const float4* g, const Geom& geo, int ndir, int nseg,

@@ -286,6 +286,16 @@ __global__ void k_batch_advance(int* iter, int step) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *iter += step;
 }
 
+// fills used instead of memsets (values travel as kernel arguments)
+__global__ void k_fill_i32(int* __restrict__ p, int n, int v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+__global__ void k_fill_f64(double* __restrict__ p, int n, double v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
 // grid-level pass: one block per segment over its nb block partials.
 // out[4*seg + {0,1,2,3}] = F/(ndir*Vseg), dAdA/Vseg, max dAdA, converged flag
 __global__ __launch_bounds__(BLK) void k_finalize(const double* __restrict__ partials, int nb,
@@ -576,6 +586,13 @@ void launch_batch_finalize(const double* partials, const int* poff, const BatchP
 
 void launch_batch_advance(int* iter, int step, hipStream_t st) {
   hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, st, iter, step);
+}
+
+void launch_fill_i32(int* p, int n, int v, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+}
+void launch_fill_f64(double* p, int n, double v, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
 }
 
 int functional_block_size() { return BLK; }

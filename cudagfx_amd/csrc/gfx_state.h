@@ -62,5 +62,13 @@ extern State g_st;
 
 int hip_check(hipError_t e, const char* what);
 int functional_now(State& s, bool fp32, int ndir, int nseg, double eps);
+Geom make_geom(int l0, int l1, int l2, int l3);
+bool geometry_ok(int l0, int l1, int l2, int l3);
+int ensure_common(int need_links32);
+int dev_alloc(void** p, size_t bytes, const char* what);
+size_t field_bytes64(const Geom& g);
+size_t field_bytes32(const Geom& g);
+int upload_aos_geo(State& s, const Geom& geo, const double* host, double2* dst, int nmat);
+int download_aos_geo(State& s, const Geom& geo, const double2* src, double* host, int nmat);
 
 }  // namespace gfx

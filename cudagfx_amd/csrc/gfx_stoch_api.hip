@@ -28,6 +28,7 @@ void launch_link_heatbath(double2* U, const Geom& geo, int parity, int nu, doubl
                           const float* gauss, const float* unif, hipStream_t st);
 int kp_gauss_per_site();
 int kp_unif_per_site();
+void launch_haar_trafo(double2* g, const Geom& geo, const float* gauss, hipStream_t st);
 
 static const int MT_COUNT = 4096;
 
@@ -260,6 +261,32 @@ void simannealing_gauge(void) {
   hipStreamSynchronize(s.stream);
   const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (s.verbose) printf("SA finished after %f sec\n", sec);
+}
+
+int gfx_batch_add_copy(int base, gfx_su3* trafo);
+int gfx_batch_trafo_field(int q, void** g64, int dims[4]);
+
+// A Gribov copy of lattice `base` whose initial g is a Haar-random gauge
+// transformation drawn from the twister streams (12 normals per site).
+// Fixing U from g = h is fixing the transformed field U^h from g = 1.
+int gfx_batch_add_random_copy(int base) {
+  State& s = g_st;
+  const int q = gfx_batch_add_copy(base, nullptr);
+  if (q < 0) return q;
+  void* gp = nullptr;
+  int d[4];
+  gfx_batch_trafo_field(q, &gp, d);
+  const Geom geo = make_geom(d[0], d[1], d[2], d[3]);
+  const int need = 12 * 2 * geo.nh;
+  if (!s.mt_state) {
+    const int rc = mt_init_device(s, need, 2 * MT_COUNT);
+    if (rc) return -rc;
+  } else {
+    if (MT_COUNT * s.n_per_gauss < need && mt_fields(s, need, MT_COUNT * s.n_per_unif)) return -GFX_ERR_ALLOC;
+    mt_refill(s, 1);
+  }
+  launch_haar_trafo((double2*)gp, geo, s.rnd_gauss, s.stream);
+  return hip_check(hipStreamSynchronize(s.stream), "haar") ? -GFX_ERR_DEVICE : q;
 }
 
 // n microcanonical sweeps (even + odd) of g; F is unchanged by each

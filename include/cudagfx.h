@@ -191,6 +191,35 @@ double gfx_sa_temperature(int i, double Tmin, double Tmax, double expo, int N);
  * SU(2) subgroups, random numbers from the twisters) */
 int gfx_heatbath_sweeps(int nsweep, double beta);
 
+/* ================= batching form (no counterpart in the reference) =================
+ * Many lattices gauge fixed together: Gribov copies of one configuration
+ * (sharing its links) and configurations of different sizes.  Every half sweep
+ * is one launch over all lattices; converged lattices drop out through device
+ * flags; after a short eager start (which also times the sweep variants and
+ * keeps the fastest), `periods_per_graph` check periods of `checkinterval`
+ * iterations each are captured in a HIP graph and replayed.  Each lattice
+ * follows overrelax_gauge's stop rule and records its own stop iteration. */
+int gfx_batch_create(void);
+void gfx_batch_destroy(void);
+int gfx_batch_size(void);
+/* a lattice with its own links (4V su3) and initial g (V su3); returns its
+ * index or a negative status */
+int gfx_batch_add(int lx, int ly, int lz, int lt, gfx_su3 *gf, gfx_su3 *trafo);
+/* a copy sharing the links of lattice `base`, initial g from `trafo` */
+int gfx_batch_add_copy(int base, gfx_su3 *trafo);
+/* a copy of `base` with a Haar-random initial g drawn from the twisters
+ * (needs MersenneTwister.dat or gfx_set_mt_params) */
+int gfx_batch_add_random_copy(int base);
+/* precision GFX_PREC_FP64 or GFX_PREC_MIXED (FP32 bulk to switch_theta, then
+ * FP64); returns GFX_OK when every lattice reached eps, else GFX_ERR_NOCONV */
+int gfx_batch_run(int maxit, double eps, int checkinterval, int precision, double switch_theta,
+                  int periods_per_graph);
+int gfx_batch_result(int q, double *F, double *dAdA, double *maxdAdA, int *iter);
+int gfx_batch_get_trafo(int q, gfx_su3 *trafo);
+int gfx_batch_stats(double *ms, int *graph_launches, int *iters_fp32, int *iters_fp64,
+                    double *site_updates);
+int gfx_batch_variant(int precision, int *variant, double *ms_per_variant);
+
 /* last error as text */
 const char *gfx_error_string(int code);
 
