@@ -64,6 +64,7 @@ static int vblock_index(int variant) {
   const int b = sweep_variant_block(variant);
   return b == 256 ? 0 : (b == 128 ? 1 : 2);
 }
+static_assert(SWEEP_VARIANTS <= 8, "tune_ms holds 8 variants");
 
 static void batch_free_maps(Batch& b) {
   for (int k = 0; k < 3; k++) {

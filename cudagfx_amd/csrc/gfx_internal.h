@@ -7,10 +7,12 @@
 
 namespace gfx {
 
-// sweep variants 0..5 (see k_or_sweep): block size and rolled/unrolled star loop
-constexpr int SWEEP_VARIANTS = 6;
+// sweep variants 0..7 (see k_or_sweep): block size, rolled/unrolled star
+// loop, non-temporal link loads
+constexpr int SWEEP_VARIANTS = 8;
 int sweep_variant_block(int v);
 bool sweep_variant_rolled(int v);
+bool sweep_variant_nt(int v);
 void launch_sweep_f64(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
                       double omega, const unsigned char* active, int variant, hipStream_t st);
 void launch_sweep_f32(const float4* U, float4* g, const Geom& geo, int parity, int ndir,

@@ -65,12 +65,13 @@ __device__ __forceinline__ void block_reduce3(double& a, double& b, double& c) {
 
 // ------------------------------------------------------------ OR sweep
 // Variants (chosen per lattice and precision by the autotuner in gfx_api.hip):
-//   BLOCK  workgroup size (64, 128, 256)
+//   BLOCK  workgroup size (256 or 64)
+//   NT     non-temporal loads of the links
 //   ROLLED the direction loop of the star is kept as a loop: one direction's
 //          operands are live at a time (FP64: 130 VGPRs, 3 waves/SIMD)
 //          instead of the fully unrolled form (more load-level parallelism
 //          per wave, 332 registers, 1 wave/SIMD).
-template <typename V, typename R, int NDIR, bool ROLLED>
+template <typename V, typename R, int NDIR, bool ROLLED, bool NT>
 __device__ __forceinline__ void sweep_site(const V* __restrict__ U, V* __restrict__ g,
                                            const Geom& geo, const Site& s, int i, int parity, R c1,
                                            R c2) {
@@ -90,10 +91,10 @@ __device__ __forceinline__ void sweep_site(const V* __restrict__ U, V* __restric
     }
 #pragma unroll 1
     for (int mu = 0; mu < NDIR; mu++) {
-      M3<R> A = load_m<V, R>(field_ptr(U, parity * 4 + mu, S), S, i);
+      M3<R> A = load_m<V, R, NT>(field_ptr(U, parity * 4 + mu, S), S, i);
       M3<R> B = load_m_dagger<V, R>(gop, S, jf[mu]);
       m_addmul(K, A, B);
-      A = load_m_dagger<V, R>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb[mu]);
+      A = load_m_dagger<V, R, NT>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb[mu]);
       B = load_m_dagger<V, R>(gop, S, jb[mu]);
       m_addmul(K, A, B);
     }
@@ -102,10 +103,10 @@ __device__ __forceinline__ void sweep_site(const V* __restrict__ U, V* __restric
     for (int mu = 0; mu < NDIR; mu++) {
       const int jf = nb_fwd(geo, s, i, mu);
       const int jb = nb_bwd(geo, s, i, mu);
-      M3<R> A = load_m<V, R>(field_ptr(U, parity * 4 + mu, S), S, i);
+      M3<R> A = load_m<V, R, NT>(field_ptr(U, parity * 4 + mu, S), S, i);
       M3<R> B = load_m_dagger<V, R>(gop, S, jf);
       m_addmul(K, A, B);
-      A = load_m_dagger<V, R>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb);
+      A = load_m_dagger<V, R, NT>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb);
       B = load_m_dagger<V, R>(gop, S, jb);
       m_addmul(K, A, B);
     }
@@ -119,7 +120,7 @@ __device__ __forceinline__ void sweep_site(const V* __restrict__ U, V* __restric
   Store<V>::store(gme, S, i, G);
 }
 
-template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED>
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* __restrict__ g,
                                                      Geom geo, int parity, R c1, R c2,
                                                      const unsigned char* __restrict__ active) {
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* 
   if (i >= geo.nh) return;
   const Site s = site_coords(geo, i, parity);
   if (NDIR == 3 && active != nullptr && !active[s.x[3]]) return;
-  sweep_site<V, R, NDIR, ROLLED>(U, g, geo, s, i, parity, c1, c2);
+  sweep_site<V, R, NDIR, ROLLED, NT>(U, g, geo, s, i, parity, c1, c2);
 }
 
 // ------------------------------------------------------------ batched sweep
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* 
 // configurations of different sizes) in one launch.  bmap[block] = (problem,
 // block inside the problem); a problem whose active flag is 0 has converged
 // and its blocks return at once.
-template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED>
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_batch_sweep(const BatchProblem* __restrict__ probs,
                                                         const int2* __restrict__ bmap,
                                                         const int* __restrict__ active,
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_sweep(const BatchProblem* __res
   const int i = m.y * BLOCK + threadIdx.x;
   if (i >= geo.nh) return;
   const Site s = site_coords(geo, i, parity);
-  sweep_site<V, R, NDIR, ROLLED>(static_cast<const V*>(probs[m.x].U), static_cast<V*>(probs[m.x].g),
+  sweep_site<V, R, NDIR, ROLLED, NT>(static_cast<const V*>(probs[m.x].U), static_cast<V*>(probs[m.x].g),
                                  geo, s, i, parity, c1, c2);
 }
 
@@ -472,28 +473,32 @@ __global__ __launch_bounds__(BLK) void k_g32_to_g64(const float4* __restrict__ s
 // ------------------------------------------------------------ launchers
 static inline dim3 grid1(long n) { return dim3((unsigned)((n + BLK - 1) / BLK)); }
 
-// sweep variant v: block = {256,128,64}[v % 3], rolled direction loop if v < 3
-int sweep_variant_block(int v) { return v % 3 == 0 ? 256 : (v % 3 == 1 ? 128 : 64); }
-bool sweep_variant_rolled(int v) { return v < 3; }
+// sweep variant v: block 64 if bit 0 else 256; unrolled star loop if bit 1;
+// non-temporal link loads if bit 2
+int sweep_variant_block(int v) { return (v & 1) ? 64 : 256; }
+bool sweep_variant_rolled(int v) { return (v & 2) == 0; }
+bool sweep_variant_nt(int v) { return (v & 4) != 0; }
 
-template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED>
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT>
 static void launch_sweep_v(const V* U, V* g, const Geom& geo, int parity, R c1, R c2,
                            const unsigned char* active, hipStream_t st) {
   dim3 grid((geo.nh + BLOCK - 1) / BLOCK);
-  hipLaunchKernelGGL((k_or_sweep<V, R, NDIR, BLOCK, ROLLED>), grid, dim3(BLOCK), 0, st, U, g, geo,
-                     parity, c1, c2, active);
+  hipLaunchKernelGGL((k_or_sweep<V, R, NDIR, BLOCK, ROLLED, NT>), grid, dim3(BLOCK), 0, st, U, g,
+                     geo, parity, c1, c2, active);
 }
 
 template <typename V, typename R, int NDIR>
 static void launch_sweep_nd(const V* U, V* g, const Geom& geo, int parity, R c1, R c2,
                             const unsigned char* active, int variant, hipStream_t st) {
-  switch (variant) {
-    case 0: launch_sweep_v<V, R, NDIR, 256, true>(U, g, geo, parity, c1, c2, active, st); break;
-    case 1: launch_sweep_v<V, R, NDIR, 128, true>(U, g, geo, parity, c1, c2, active, st); break;
-    case 2: launch_sweep_v<V, R, NDIR, 64, true>(U, g, geo, parity, c1, c2, active, st); break;
-    case 3: launch_sweep_v<V, R, NDIR, 256, false>(U, g, geo, parity, c1, c2, active, st); break;
-    case 4: launch_sweep_v<V, R, NDIR, 128, false>(U, g, geo, parity, c1, c2, active, st); break;
-    default: launch_sweep_v<V, R, NDIR, 64, false>(U, g, geo, parity, c1, c2, active, st); break;
+  switch (variant & 7) {
+    case 0: launch_sweep_v<V, R, NDIR, 256, true, false>(U, g, geo, parity, c1, c2, active, st); break;
+    case 1: launch_sweep_v<V, R, NDIR, 64, true, false>(U, g, geo, parity, c1, c2, active, st); break;
+    case 2: launch_sweep_v<V, R, NDIR, 256, false, false>(U, g, geo, parity, c1, c2, active, st); break;
+    case 3: launch_sweep_v<V, R, NDIR, 64, false, false>(U, g, geo, parity, c1, c2, active, st); break;
+    case 4: launch_sweep_v<V, R, NDIR, 256, true, true>(U, g, geo, parity, c1, c2, active, st); break;
+    case 5: launch_sweep_v<V, R, NDIR, 64, true, true>(U, g, geo, parity, c1, c2, active, st); break;
+    case 6: launch_sweep_v<V, R, NDIR, 256, false, true>(U, g, geo, parity, c1, c2, active, st); break;
+    default: launch_sweep_v<V, R, NDIR, 64, false, true>(U, g, geo, parity, c1, c2, active, st); break;
   }
 }
 
@@ -524,24 +529,26 @@ void launch_copy_fields(const void* src, void* dst, size_t bytes, hipStream_t st
 }
 
 // ---- batch launchers: nblocks entries of bmap built for the variant's block size
-template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED>
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT>
 static void launch_bsweep_v(const BatchProblem* probs, const int2* bmap, int nblocks,
                             const int* active, int parity, R c1, R c2, hipStream_t st) {
-  hipLaunchKernelGGL((k_batch_sweep<V, R, NDIR, BLOCK, ROLLED>), dim3(nblocks), dim3(BLOCK), 0, st,
-                     probs, bmap, active, parity, c1, c2);
+  hipLaunchKernelGGL((k_batch_sweep<V, R, NDIR, BLOCK, ROLLED, NT>), dim3(nblocks), dim3(BLOCK), 0,
+                     st, probs, bmap, active, parity, c1, c2);
 }
 
 template <typename V, typename R, int NDIR>
 static void launch_bsweep_nd(const BatchProblem* probs, const int2* bmap, int nblocks,
                              const int* active, int parity, R c1, R c2, int variant,
                              hipStream_t st) {
-  switch (variant) {
-    case 0: launch_bsweep_v<V, R, NDIR, 256, true>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
-    case 1: launch_bsweep_v<V, R, NDIR, 128, true>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
-    case 2: launch_bsweep_v<V, R, NDIR, 64, true>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
-    case 3: launch_bsweep_v<V, R, NDIR, 256, false>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
-    case 4: launch_bsweep_v<V, R, NDIR, 128, false>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
-    default: launch_bsweep_v<V, R, NDIR, 64, false>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+  switch (variant & 7) {
+    case 0: launch_bsweep_v<V, R, NDIR, 256, true, false>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+    case 1: launch_bsweep_v<V, R, NDIR, 64, true, false>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+    case 2: launch_bsweep_v<V, R, NDIR, 256, false, false>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+    case 3: launch_bsweep_v<V, R, NDIR, 64, false, false>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+    case 4: launch_bsweep_v<V, R, NDIR, 256, true, true>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+    case 5: launch_bsweep_v<V, R, NDIR, 64, true, true>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+    case 6: launch_bsweep_v<V, R, NDIR, 256, false, true>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+    default: launch_bsweep_v<V, R, NDIR, 64, false, true>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
   }
 }
 

@@ -95,18 +95,41 @@ __device__ __forceinline__ int nb_bwd(const Geom& g, const Site& s, int i, int m
 }
 
 // ---- storage traits: 2-row matrices in planes of 16-byte vectors ----
+// NT = true issues non-temporal (streaming) loads: used for the links, which
+// a half sweep reads exactly once, so that they do not evict the neighbour g
+// lines that other sites still need from L2.
+typedef double nat_d2 __attribute__((ext_vector_type(2)));
+typedef float nat_f4 __attribute__((ext_vector_type(4)));
+
+template <bool NT>
+__device__ __forceinline__ double2 ld16(const double2* p) {
+  if (NT) {
+    const nat_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nat_d2*>(p));
+    return make_double2(v.x, v.y);
+  }
+  return *p;
+}
+template <bool NT>
+__device__ __forceinline__ float4 ld16(const float4* p) {
+  if (NT) {
+    const nat_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nat_f4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+  }
+  return *p;
+}
+
 template <typename V>
 struct Store;
 
 template <>
 struct Store<double2> {
   static constexpr int NPLANE = 6;
-  template <typename R>
+  template <typename R, bool NT = false>
   __device__ __forceinline__ static void load(const double2* __restrict__ f, int stride, int i,
                                               Cx<R> r0[3], Cx<R> r1[3]) {
     double2 v[6];
 #pragma unroll
-    for (int c = 0; c < 6; c++) v[c] = (f + (size_t)c * stride)[(unsigned)i];
+    for (int c = 0; c < 6; c++) v[c] = ld16<NT>(f + (size_t)c * stride + (unsigned)i);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       r0[c] = cx<R>((R)v[c].x, (R)v[c].y);
@@ -128,12 +151,12 @@ struct Store<double2> {
 template <>
 struct Store<float4> {
   static constexpr int NPLANE = 3;
-  template <typename R>
+  template <typename R, bool NT = false>
   __device__ __forceinline__ static void load(const float4* __restrict__ f, int stride, int i,
                                               Cx<R> r0[3], Cx<R> r1[3]) {
-    float4 a = f[(unsigned)i];
-    float4 b = (f + (size_t)stride)[(unsigned)i];
-    float4 c = (f + (size_t)2 * stride)[(unsigned)i];
+    float4 a = ld16<NT>(f + (unsigned)i);
+    float4 b = ld16<NT>(f + (size_t)stride + (unsigned)i);
+    float4 c = ld16<NT>(f + (size_t)2 * stride + (unsigned)i);
     r0[0] = cx<R>((R)a.x, (R)a.y);
     r0[1] = cx<R>((R)a.z, (R)a.w);
     r0[2] = cx<R>((R)b.x, (R)b.y);
@@ -153,18 +176,18 @@ struct Store<float4> {
   }
 };
 
-template <typename V, typename R>
+template <typename V, typename R, bool NT = false>
 __device__ __forceinline__ M3<R> load_m(const V* f, int stride, int i) {
   M3<R> m;
-  Store<V>::template load<R>(f, stride, i, m.e[0], m.e[1]);
+  Store<V>::template load<R, NT>(f, stride, i, m.e[0], m.e[1]);
   m_row2(m);
   return m;
 }
 
-template <typename V, typename R>
+template <typename V, typename R, bool NT = false>
 __device__ __forceinline__ M3<R> load_m_dagger(const V* f, int stride, int i) {
   Cx<R> r0[3], r1[3];
-  Store<V>::template load<R>(f, stride, i, r0, r1);
+  Store<V>::template load<R, NT>(f, stride, i, r0, r1);
   M3<R> d;
   m_dagger_from_rows(d, r0, r1);
   return d;
