@@ -33,7 +33,7 @@ ABI_FUNCTIONS = [
     "init_gaugefixing", "finalize_gaugefixing", "init_thermalization", "finalize_thermalization",
     "overrelax_gauge", "calc_functional", "calc_plaquette", "simannealing_gauge",
     "thermalize_gauge", "init_MT", "update_MT", "finalize_MT", "get_trafo", "set_trafo",
-    "get_gauge", "apply_trafo", "gfx_set_omega", "gfx_set_precision", "gfx_set_verbose",
+    "get_gauge", "apply_trafo", "gfx_set_omega", "gfx_set_precision", "gfx_set_fp32_stall", "gfx_set_verbose",
     "gfx_set_mt_seed", "gfx_coulomb_gauge", "gfx_functional_slices", "gfx_halfsweep",
     "gfx_time_sweeps", "gfx_last_run_stats", "gfx_error_string",
     "read_input", "read_gf_ildg", "write_gf_ildg", "Random", "PlantSeeds", "PutSeed", "GetSeed",
@@ -92,6 +92,7 @@ def load():
         "apply_trafo": (C.c_int, [vp]),
         "gfx_set_omega": (C.c_int, [C.c_double]),
         "gfx_set_precision": (C.c_int, [C.c_int, C.c_double]),
+        "gfx_set_fp32_stall": (C.c_int, [C.c_int, C.c_double]),
         "gfx_set_verbose": (C.c_int, [C.c_int]),
         "gfx_set_mt_seed": (C.c_int, [C.c_long]),
         "gfx_set_mt_file": (C.c_int, [C.c_char_p]),
@@ -280,6 +281,12 @@ def set_omega(omega):
 
 def set_precision(mode, switch_theta=1e-6):
     load().gfx_set_precision(int(mode), float(switch_theta))
+
+
+def set_fp32_stall(iters=300, gate=1e-8):
+    """The mixed path also leaves FP32 once its best max dAdA is below `gate`
+    and has not halved for `iters` iterations; iters=0 turns this off."""
+    _check(load().gfx_set_fp32_stall(int(iters), float(gate)), "set_fp32_stall")
 
 
 def set_verbose(v):

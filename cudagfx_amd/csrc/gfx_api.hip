@@ -112,6 +112,8 @@ int alloc_reduction(State& s) {
   if (dev_alloc((void**)&s.results, sizeof(double) * 4 * (s.geo.L[3] + 1), "results")) return GFX_ERR_ALLOC;
   if (hipHostMalloc((void**)&s.h_results, sizeof(double) * 4 * (s.geo.L[3] + 1), 0) != hipSuccess) return GFX_ERR_ALLOC;
   if (dev_alloc((void**)&s.active, (size_t)s.geo.L[3], "active")) return GFX_ERR_ALLOC;
+  s.ncheck = (s.geo.nh + 63) / 64;  // blocks of the smallest sweep workgroup
+  if (dev_alloc((void**)&s.cpart, sizeof(double) * 6 * s.ncheck, "check partials")) return GFX_ERR_ALLOC;
   return GFX_OK;
 }
 
@@ -160,7 +162,7 @@ int ensure_f32_links(State& s) {
   if (s.U32 && s.U32_valid) return GFX_OK;
   if (!s.U32) {
     if (dev_alloc((void**)&s.U32, 8 * field_bytes32(s.geo), "FP32 links")) return GFX_ERR_ALLOC;
-    if (dev_alloc((void**)&s.g32, 2 * field_bytes32(s.geo), "FP32 trafo")) return GFX_ERR_ALLOC;
+    if (dev_alloc((void**)&s.g32, 3 * field_bytes32(s.geo), "FP32 trafo")) return GFX_ERR_ALLOC;
   }
   launch_f64_to_f32(s.U64, s.U32, s.geo, 8, s.stream);
   s.U32_valid = true;
@@ -174,6 +176,7 @@ void free_all(State& s) {
   hipFree(s.U32);
   hipFree(s.g32);
   hipFree(s.partials);
+  hipFree(s.cpart);
   hipFree(s.results);
   hipFree(s.active);
   hipFree(s.staging);
@@ -183,6 +186,7 @@ void free_all(State& s) {
   s.U32 = nullptr;
   s.g32 = nullptr;
   s.partials = nullptr;
+  s.cpart = nullptr;
   s.results = nullptr;
   s.active = nullptr;
   s.staging = nullptr;
@@ -220,13 +224,40 @@ void reset_tuning(State& s) {
     }
 }
 
-// one half sweep with the tuned variant for (precision, ndir)
-void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* active) {
+double2* field64(State& s, int k) { return s.g64 + (size_t)k * 6 * s.geo.stride; }
+float4* field32(State& s, int k) { return s.g32 + (size_t)k * 3 * s.geo.stride; }
+
+// one half sweep with the tuned variant for (precision, ndir): in place on
+// the even field in slot `even` (0 or 2) and the odd field (slot 1); chk as
+// in launch_sweep_f64, writing the new even field to slot 2 - even and the
+// block partials to cpart (odd sweep first half, even sweep second half)
+void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* active, int even,
+              int chk) {
   const int v = s.variant[fp32 ? 1 : 0][ndir == 4 ? 1 : 0];
+  double* part = (chk == 2) ? s.cpart + 3 * (size_t)s.ncheck : s.cpart;
   if (fp32)
-    launch_sweep_f32(s.U32, s.g32, s.geo, parity, ndir, s.omega, active, v, s.stream);
+    launch_sweep_f32(s.U32, field32(s, even), field32(s, 1), s.geo, parity, ndir, s.omega, active,
+                     v, s.stream, chk, field32(s, 2 - even), part);
   else
-    launch_sweep_f64(s.U64, s.g64, s.geo, parity, ndir, s.omega, active, v, s.stream);
+    launch_sweep_f64(s.U64, field64(s, even), field64(s, 1), s.geo, parity, ndir, s.omega, active,
+                     v, s.stream, chk, field64(s, 2 - even), part);
+}
+void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* active) {
+  do_sweep(s, fp32, parity, ndir, active, 0, 0);
+}
+
+static void stop_clock_ms(State& s, std::chrono::steady_clock::time_point t0) {
+  hipStreamSynchronize(s.stream);
+  s.run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// bring the even field back to slot 0
+static void canonical_even(State& s, bool fp32, int even) {
+  if (even == 0) return;
+  if (fp32)
+    hipMemcpyAsync(field32(s, 0), field32(s, 2), field_bytes32(s.geo), hipMemcpyDeviceToDevice, s.stream);
+  else
+    hipMemcpyAsync(field64(s, 0), field64(s, 2), field_bytes64(s.geo), hipMemcpyDeviceToDevice, s.stream);
 }
 
 // Times every sweep variant on a scratch copy of g (the state is not
@@ -241,23 +272,25 @@ int tune_sweep(State& s, bool fp32, int ndir) {
     return GFX_OK;
   }
   hipMemcpyAsync(scratch, fp32 ? (void*)s.g32 : (void*)s.g64, bytes, hipMemcpyDeviceToDevice, s.stream);
+  float4* sc32 = static_cast<float4*>(scratch);
+  double2* sc64 = static_cast<double2*>(scratch);
   float best = 1e30f;
   int bestv = s.variant[pi][di];
   for (int v = 0; v < SWEEP_VARIANTS; v++) {
     for (int rep = 0; rep < 2; rep++)
       for (int p = 0; p < 2; p++) {
         if (fp32)
-          launch_sweep_f32(s.U32, (float4*)scratch, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
+          launch_sweep_f32(s.U32, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
         else
-          launch_sweep_f64(s.U64, (double2*)scratch, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
+          launch_sweep_f64(s.U64, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
       }
     hipEventRecord(s.ev0, s.stream);
     for (int rep = 0; rep < 3; rep++)
       for (int p = 0; p < 2; p++) {
         if (fp32)
-          launch_sweep_f32(s.U32, (float4*)scratch, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
+          launch_sweep_f32(s.U32, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
         else
-          launch_sweep_f64(s.U64, (double2*)scratch, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
+          launch_sweep_f64(s.U64, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
       }
     hipEventRecord(s.ev1, s.stream);
     if (hip_check(hipEventSynchronize(s.ev1), "tune")) break;
@@ -292,7 +325,7 @@ int init_gaugefixing(gfx_su3* gf, gfx_su3* trafo) {
   if ((rc = ensure_common(0))) return rc;
   if (dev_alloc((void**)&s.U64, 8 * field_bytes64(s.geo), "gauge field")) return GFX_ERR_ALLOC;
   printf("Allocated gauge field on device\n");
-  if (dev_alloc((void**)&s.g64, 2 * field_bytes64(s.geo), "trafo field")) return GFX_ERR_ALLOC;
+  if (dev_alloc((void**)&s.g64, 3 * field_bytes64(s.geo), "trafo field")) return GFX_ERR_ALLOC;
   printf("Allocated trafo field 1 on device\n");
   if ((rc = alloc_reduction(s))) return rc;
   if ((rc = upload_aos(s, (const double*)gf, s.U64, 4))) return rc;
@@ -313,7 +346,7 @@ int init_thermalization(gfx_su3* gf) {
   if ((rc = ensure_common(0))) return rc;
   if (dev_alloc((void**)&s.U64, 8 * field_bytes64(s.geo), "gauge field")) return GFX_ERR_ALLOC;
   printf("Allocated gauge field on device\n");
-  if (dev_alloc((void**)&s.g64, 2 * field_bytes64(s.geo), "trafo field")) return GFX_ERR_ALLOC;
+  if (dev_alloc((void**)&s.g64, 3 * field_bytes64(s.geo), "trafo field")) return GFX_ERR_ALLOC;
   if ((rc = alloc_reduction(s))) return rc;
   if ((rc = upload_aos(s, (const double*)gf, s.U64, 4))) return rc;
   s.U32_valid = false;
@@ -332,6 +365,12 @@ int gfx_set_precision(int mode, double switch_theta) {
   if (mode != GFX_PREC_FP64 && mode != GFX_PREC_MIXED) return GFX_ERR_STATE;
   g_st.precision = mode;
   if (switch_theta > 0) g_st.switch_theta = switch_theta;
+  return GFX_OK;
+}
+int gfx_set_fp32_stall(int iters, double gate) {
+  if (iters < 0 || !(gate > 0)) return GFX_ERR_STATE;
+  g_st.stall_iters = iters;
+  g_st.stall_gate = gate;
   return GFX_OK;
 }
 int gfx_set_verbose(int v) {
@@ -401,49 +440,75 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
   tune_sweep(s, false, 4);
   s.run_iters32 = 0;
   s.run_iters64 = 0;
-  double maxdada = 0.0;
+  const double V = (double)VOLUME;
+  const int stall_checks = (s.stall_iters + checkinterval - 1) / checkinterval;
+  double maxdada = 0.0, best = 1e300;
+  int stall = 0;
+  int even = 0;            // slot of the current even field (0 or 2)
+  bool even_done = false;  // the even half of iteration i ran with the last check
+  bool converged = false;
   int i;
   hipStreamSynchronize(s.stream);
   const auto t0 = std::chrono::steady_clock::now();
-  auto stop_clock = [&]() {
-    hipStreamSynchronize(s.stream);
-    s.run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  auto to_fp64 = [&]() {
+    // leave FP32 at the checked state: reunitarise g in FP64
+    canonical_even(s, true, even);
+    launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
+    even = 0;
+    fp32 = false;
   };
   for (i = 0; i < maxit; i++) {
-    if (fp32) {
-      do_sweep(s, true, 0, 4, nullptr);
-      do_sweep(s, true, 1, 4, nullptr);
-      s.run_iters32++;
-    } else {
-      do_sweep(s, false, 0, 4, nullptr);
-      do_sweep(s, false, 1, 4, nullptr);
-      s.run_iters64++;
-    }
-    if ((i % checkinterval) == 0) {
-      if (functional_now(s, fp32, 4, 1, eps)) return -1;
-      FUNC = s.h_results[0];
-      DADA = s.h_results[1];
-      maxDADA = maxdada = s.h_results[2];
-      if (s.verbose)
-        printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e\n", i, FUNC, DADA, maxdada);
-      if (fp32 && maxdada < s.switch_theta) {
-        // leave FP32: reunitarise g in FP64 and continue with FP64 sweeps
-        launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
-        fp32 = false;
-      }
-    }
+    const bool chk = (i % checkinterval) == 0;
+    if (!even_done) do_sweep(s, fp32, 0, 4, nullptr, even, 0);
+    do_sweep(s, fp32, 1, 4, nullptr, even, chk ? 1 : 0);
+    even_done = false;
+    if (fp32) s.run_iters32++; else s.run_iters64++;
+    if (!chk) continue;
+    // the check of state i rides on the even sweep of iteration i+1
+    do_sweep(s, fp32, 0, 4, nullptr, even, 2);
+    const int v = s.variant[fp32 ? 1 : 0][1];
+    launch_check_finalize(s.cpart, s.cpart + 3 * (size_t)s.ncheck, sweep_blocks(s.geo.nh, v),
+                          4.0 * V, V, eps, s.results, s.stream);
+    if (hip_check(hipMemcpyAsync(s.h_results, s.results, sizeof(double) * 4, hipMemcpyDeviceToHost, s.stream), "D2H results") ||
+        hip_check(hipStreamSynchronize(s.stream), "check"))
+      return -1;
+    FUNC = s.h_results[0];
+    DADA = s.h_results[1];
+    maxDADA = maxdada = s.h_results[2];
+    if (s.verbose)
+      printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e\n", i, FUNC, DADA, maxdada);
     if (maxdada < eps) {
-      if (fp32) launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
-      stop_clock();
-      if (s.verbose) {
-        printf("CONVERGENCE!\nFinal values: F = %.16e dAdA = %.16e, maxdAdA = %.16e\n", FUNC, DADA, maxDADA);
-        printf("Overrelaxation finished after %f sec\n", s.run_ms / 1000.0);
+      converged = true;
+      break;
+    }
+    if (fp32) {
+      bool stalled = false;
+      if (maxdada < 0.5 * best) {
+        best = maxdada;
+        stall = 0;
+      } else if (best < s.stall_gate && ++stall >= stall_checks) {
+        stalled = true;
       }
-      return i;
+      if (maxdada < s.switch_theta || stalled) {
+        to_fp64();
+        continue;
+      }
+    }
+    if (i + 1 < maxit) {
+      even = 2 - even;
+      even_done = true;
     }
   }
-  if (fp32) launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
-  stop_clock();
+  if (fp32) to_fp64();
+  canonical_even(s, false, even);
+  stop_clock_ms(s, t0);
+  if (converged) {
+    if (s.verbose) {
+      printf("CONVERGENCE!\nFinal values: F = %.16e dAdA = %.16e, maxdAdA = %.16e\n", FUNC, DADA, maxDADA);
+      printf("Overrelaxation finished after %f sec\n", s.run_ms / 1000.0);
+    }
+    return i;
+  }
   if (s.verbose) printf("Overrelaxation finished after %f sec\n", s.run_ms / 1000.0);
   if (maxdada > eps) {
     if (s.verbose)

@@ -1,8 +1,9 @@
 // gfx_batch_api.hip -- the batching form: many lattices gauge fixed together
 // (Gribov copies of one configuration, or configurations of different sizes)
 // with one sweep launch per half sweep covering every lattice, device-side
-// convergence flags, and the sweeps of a check period captured in a HIP graph
-// that is replayed until every lattice has converged.
+// convergence flags, checks fused into the sweeps (see k_or_sweep), and the
+// sweeps of check periods captured in a HIP graph that is replayed until
+// every lattice has converged.
 //
 // No counterpart in the reference (it fixes one configuration per run); each
 // lattice follows overrelax_gauge (overrelax.cu:677-823) exactly: checks at
@@ -23,6 +24,8 @@
 
 namespace gfx {
 
+// g64 / g32: three fields each (even, odd, spare even field of the checks);
+// between phases the even field is field 0
 struct BProbHost {
   Geom geo{};
   double2* U64 = nullptr;
@@ -38,15 +41,16 @@ struct Batch {
   BatchProblem* d_probs = nullptr;
   int2* d_bmap[3] = {nullptr, nullptr, nullptr};
   int nbmap[3] = {0, 0, 0};
-  int2* d_fmap = nullptr;
-  int nfmap = 0;
-  int* d_poff = nullptr;
+  int* d_poff[3] = {nullptr, nullptr, nullptr};  // check-partial offsets per problem
+  double* d_podd = nullptr;   // check partials of the odd sweeps
+  double* d_peven = nullptr;  // and of the even sweeps
+  double* d_best = nullptr;   // per problem: best max dAdA of the phase
+  int* d_stall = nullptr;     // checks since it last halved
   int* d_active = nullptr;
   int* h_active = nullptr;
   double* d_results = nullptr;
   double* h_results = nullptr;
   int* d_iter = nullptr;
-  double* d_partials = nullptr;
   bool built = false;
   int variant[2] = {0, 0};
   bool tuned[2] = {false, false};
@@ -69,24 +73,28 @@ static_assert(SWEEP_VARIANTS <= 8, "tune_ms holds 8 variants");
 static void batch_free_maps(Batch& b) {
   for (int k = 0; k < 3; k++) {
     hipFree(b.d_bmap[k]);
+    hipFree(b.d_poff[k]);
     b.d_bmap[k] = nullptr;
+    b.d_poff[k] = nullptr;
   }
   hipFree(b.d_probs);
-  hipFree(b.d_fmap);
-  hipFree(b.d_poff);
+  hipFree(b.d_podd);
+  hipFree(b.d_peven);
+  hipFree(b.d_best);
+  hipFree(b.d_stall);
   hipFree(b.d_active);
   hipFree(b.d_results);
   hipFree(b.d_iter);
-  hipFree(b.d_partials);
   if (b.h_active) hipHostFree(b.h_active);
   if (b.h_results) hipHostFree(b.h_results);
   b.d_probs = nullptr;
-  b.d_fmap = nullptr;
-  b.d_poff = nullptr;
+  b.d_podd = nullptr;
+  b.d_peven = nullptr;
+  b.d_best = nullptr;
+  b.d_stall = nullptr;
   b.d_active = nullptr;
   b.d_results = nullptr;
   b.d_iter = nullptr;
-  b.d_partials = nullptr;
   b.h_active = nullptr;
   b.h_results = nullptr;
   b.built = false;
@@ -132,36 +140,40 @@ static int batch_build(Batch& b) {
           if (blk < (b.probs[q].geo.nh + BLOCKS[k] - 1) / BLOCKS[k]) map.push_back(make_int2(q, blk));
     }
     b.nbmap[k] = (int)map.size();
-    if (dev_alloc((void**)&b.d_bmap[k], sizeof(int2) * map.size(), "batch map")) return GFX_ERR_ALLOC;
+    std::vector<int> poff(n + 1, 0);
+    for (int q = 0; q < n; q++) poff[q + 1] = poff[q] + (b.probs[q].geo.nh + BLOCKS[k] - 1) / BLOCKS[k];
+    if (dev_alloc((void**)&b.d_bmap[k], sizeof(int2) * map.size(), "batch map") ||
+        dev_alloc((void**)&b.d_poff[k], sizeof(int) * (n + 1), "batch offsets"))
+      return GFX_ERR_ALLOC;
     hipMemcpy(b.d_bmap[k], map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice);
+    hipMemcpy(b.d_poff[k], poff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice);
   }
-  std::vector<int2> fmap;
-  std::vector<int> poff(n + 1, 0);
-  const int fb = functional_block_size();
-  for (int q = 0; q < n; q++) {
-    poff[q] = (int)fmap.size();
-    const int nb = (2 * b.probs[q].geo.nh + fb - 1) / fb;
-    for (int blk = 0; blk < nb; blk++) fmap.push_back(make_int2(q, blk));
-  }
-  poff[n] = (int)fmap.size();
-  b.nfmap = (int)fmap.size();
-  if (dev_alloc((void**)&b.d_fmap, sizeof(int2) * fmap.size(), "batch fmap") ||
-      dev_alloc((void**)&b.d_poff, sizeof(int) * (n + 1), "batch poff") ||
+  // partial slots: as many as blocks of the smallest workgroup
+  size_t npart = 0;
+  for (int q = 0; q < n; q++) npart += (b.probs[q].geo.nh + 63) / 64;
+  if (dev_alloc((void**)&b.d_podd, sizeof(double) * 3 * npart, "batch partials") ||
+      dev_alloc((void**)&b.d_peven, sizeof(double) * 3 * npart, "batch partials") ||
       dev_alloc((void**)&b.d_probs, sizeof(BatchProblem) * n, "batch problems") ||
+      dev_alloc((void**)&b.d_best, sizeof(double) * n, "batch best") ||
+      dev_alloc((void**)&b.d_stall, sizeof(int) * n, "batch stall") ||
       dev_alloc((void**)&b.d_active, sizeof(int) * n, "batch active") ||
       dev_alloc((void**)&b.d_results, sizeof(double) * 4 * n, "batch results") ||
-      dev_alloc((void**)&b.d_iter, sizeof(int), "batch iter") ||
-      dev_alloc((void**)&b.d_partials, sizeof(double) * 3 * fmap.size(), "batch partials"))
+      dev_alloc((void**)&b.d_iter, sizeof(int), "batch iter"))
     return GFX_ERR_ALLOC;
   if (hipHostMalloc((void**)&b.h_active, sizeof(int) * n, 0) != hipSuccess ||
       hipHostMalloc((void**)&b.h_results, sizeof(double) * 4 * n, 0) != hipSuccess)
     return GFX_ERR_ALLOC;
-  hipMemcpy(b.d_fmap, fmap.data(), sizeof(int2) * fmap.size(), hipMemcpyHostToDevice);
-  hipMemcpy(b.d_poff, poff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice);
   b.built = true;
   return GFX_OK;
 }
 
+template <typename V>
+static V* bfield(V* g, const Geom& geo, int k) {
+  return g + (size_t)k * Store<V>::NPLANE * geo.stride;
+}
+
+// canonical field pointers of the phase's precision: even = field 0, odd =
+// field 1, spare = field 2
 static void set_phase_pointers(Batch& b, bool fp32) {
   std::vector<BatchProblem> h(b.probs.size());
   for (size_t q = 0; q < b.probs.size(); q++) {
@@ -169,10 +181,29 @@ static void set_phase_pointers(Batch& b, bool fp32) {
     const BProbHost& o = b.probs[p.ushare];
     h[q].geo = p.geo;
     h[q].U = fp32 ? (const void*)o.U32 : (const void*)o.U64;
-    h[q].g = fp32 ? (void*)p.g32 : (void*)p.g64;
+    for (int k = 0; k < 2; k++)
+      h[q].g[k] = fp32 ? (void*)bfield(p.g32, p.geo, k) : (void*)bfield(p.g64, p.geo, k);
+    h[q].galt = fp32 ? (void*)bfield(p.g32, p.geo, 2) : (void*)bfield(p.g64, p.geo, 2);
   }
   hipStreamSynchronize(g_st.stream);
   hipMemcpy(b.d_probs, h.data(), sizeof(BatchProblem) * h.size(), hipMemcpyHostToDevice);
+}
+
+// after a phase: copy each even field the checks left in the spare slot back
+// to field 0
+static void batch_canonical(Batch& b, bool fp32) {
+  State& s = g_st;
+  std::vector<BatchProblem> h(b.probs.size());
+  hipStreamSynchronize(s.stream);
+  hipMemcpy(h.data(), b.d_probs, sizeof(BatchProblem) * h.size(), hipMemcpyDeviceToHost);
+  for (size_t q = 0; q < b.probs.size(); q++) {
+    const BProbHost& p = b.probs[q];
+    void* f0 = fp32 ? (void*)p.g32 : (void*)p.g64;
+    if (h[q].g[0] != f0)
+      hipMemcpyAsync(f0, h[q].g[0], fp32 ? field_bytes32(p.geo) : field_bytes64(p.geo),
+                     hipMemcpyDeviceToDevice, s.stream);
+  }
+  hipStreamSynchronize(s.stream);
 }
 
 static int ensure_fp32(Batch& b) {
@@ -183,25 +214,29 @@ static int ensure_fp32(Batch& b) {
       if (!p.U32 && dev_alloc((void**)&p.U32, 8 * field_bytes32(p.geo), "batch FP32 links")) return GFX_ERR_ALLOC;
       launch_f64_to_f32(p.U64, p.U32, p.geo, 8, s.stream);
     }
-    if (!p.g32 && dev_alloc((void**)&p.g32, 2 * field_bytes32(p.geo), "batch FP32 trafo")) return GFX_ERR_ALLOC;
+    if (!p.g32 && dev_alloc((void**)&p.g32, 3 * field_bytes32(p.geo), "batch FP32 trafo")) return GFX_ERR_ALLOC;
     launch_f64_to_f32(p.g64, p.g32, p.geo, 2, s.stream);
   }
   return GFX_OK;
 }
 
-static void batch_check(Batch& b, bool fp32, int ndir, double thr) {
-  State& s = g_st;
-  launch_batch_functional(fp32, b.d_probs, b.d_fmap, b.nfmap, b.d_active, ndir, b.d_partials, s.stream);
-  launch_batch_finalize(b.d_partials, b.d_poff, b.d_probs, (int)b.probs.size(), ndir, thr,
-                        b.d_active, b.d_results, b.d_iter, s.stream);
-}
-
-static void batch_sweep(Batch& b, bool fp32, int ndir, int variant) {
+// one half sweep of every active lattice; chk as in launch_batch_sweep
+static void batch_half(Batch& b, bool fp32, int parity, int variant, int chk) {
   State& s = g_st;
   const int k = vblock_index(variant);
-  for (int p = 0; p < 2; p++)
-    launch_batch_sweep(fp32, b.d_probs, b.d_bmap[k], b.nbmap[k], b.d_active, p, ndir, s.omega,
-                       variant, s.stream);
+  launch_batch_sweep(fp32, b.d_probs, b.d_bmap[k], b.nbmap[k], b.d_active, b.d_poff[k], parity,
+                     s.omega, variant, chk, chk == 2 ? b.d_peven : b.d_podd, s.stream);
+}
+
+// the end of a check at iteration *d_iter: the even sweep of the next
+// iteration carrying the check, then the per-lattice grid pass
+static void batch_check_tail(Batch& b, bool fp32, int variant, double thr, int maxit,
+                             int stall_checks) {
+  State& s = g_st;
+  batch_half(b, fp32, 0, variant, 2);
+  launch_batch_finalize(b.d_podd, b.d_peven, b.d_poff[vblock_index(variant)], b.d_probs,
+                        (int)b.probs.size(), 4, thr, b.d_active, b.d_results, b.d_iter, maxit,
+                        b.d_best, b.d_stall, stall_checks, s.stall_gate, s.stream);
 }
 
 static int count_active(Batch& b) {
@@ -214,85 +249,115 @@ static int count_active(Batch& b) {
 }
 
 // Runs one phase (FP32 bulk or FP64) from global iteration *it until every
-// lattice has max dAdA < thr or maxit is reached.  The first iterations run
-// eagerly (the sweep variants are timed on them when not tuned yet) up to a
-// check boundary; then graphs of `periods` check periods are replayed.
-static int batch_phase(Batch& b, bool fp32, int ndir, double thr, int ck, int maxit, int periods,
-                       int* it) {
+// lattice has stopped (max dAdA < thr, or an FP32 stall) or maxit is reached.
+// The first iterations run eagerly (the sweep variants are timed on them when
+// not tuned yet) up to the iteration after a check; then graphs of `periods`
+// check periods are replayed while they end before maxit, and the rest runs
+// eagerly again.
+static int batch_phase(Batch& b, bool fp32, double thr, int ck, int maxit, int periods, int* it,
+                       int stall_checks) {
   State& s = g_st;
   const int n = (int)b.probs.size();
   const int pi = fp32 ? 1 : 0;
   set_phase_pointers(b, fp32);
   launch_fill_i32(b.d_active, n, 1, s.stream);
+  launch_fill_i32(b.d_stall, n, 0, s.stream);
+  launch_fill_f64(b.d_best, n, 1e300, s.stream);
   int i = *it;
+  bool even_done = false;  // the even half of iteration i ran with the last check
   int tune_v = b.tuned[pi] ? -1 : 0;
-  // eager head: tune, then reach an iteration with i % ck == 1 (graph periods
-  // end on a check)
-  while (i < maxit) {
-    const int v = (tune_v >= 0) ? tune_v : b.variant[pi];
-    if (tune_v >= 0) hipEventRecord(s.ev0, s.stream);
-    batch_sweep(b, fp32, ndir, v);
-    if (tune_v >= 0) {
+  int rc = GFX_OK;
+  auto count_iter = [&]() {
+    if (fp32) b.iters32++; else b.iters64++;
+  };
+  // one eager iteration; false when every lattice has stopped
+  auto eager = [&](int v, bool timed) {
+    const bool chk = (i % ck) == 0;
+    const bool both = !even_done;
+    if (timed && both) hipEventRecord(s.ev0, s.stream);
+    if (both) batch_half(b, fp32, 0, v, 0);
+    if (timed && !both) hipEventRecord(s.ev0, s.stream);
+    batch_half(b, fp32, 1, v, chk ? 1 : 0);
+    if (timed) {
       hipEventRecord(s.ev1, s.stream);
       hipEventSynchronize(s.ev1);
       float t = 0.f;
       hipEventElapsedTime(&t, s.ev0, s.ev1);
-      b.tune_ms[pi][tune_v] = t;
-      if (++tune_v == SWEEP_VARIANTS) {
-        int best = 0;
-        for (int k = 1; k < SWEEP_VARIANTS; k++)
-          if (b.tune_ms[pi][k] < b.tune_ms[pi][best]) best = k;
-        b.variant[pi] = best;
-        b.tuned[pi] = true;
-        tune_v = -1;
-      }
+      b.tune_ms[pi][tune_v] = both ? 0.5f * t : t;
     }
-    if (fp32) b.iters32++; else b.iters64++;
-    if (i % ck == 0) {
+    even_done = false;
+    count_iter();
+    bool go_on = true;
+    if (chk) {
       launch_fill_i32(b.d_iter, 1, i, s.stream);
-      batch_check(b, fp32, ndir, thr);
-      if (count_active(b) == 0) {
-        *it = i + 1;
-        return GFX_OK;
-      }
+      batch_check_tail(b, fp32, v, thr, maxit, stall_checks);
+      even_done = true;
+      go_on = count_active(b) > 0;
     }
     i++;
-    if (tune_v < 0 && (ck == 1 || i % ck == 1)) break;
-  }
-  if (i >= maxit) {
-    *it = i;
-    return GFX_OK;
-  }
-  // graph of `periods` check periods: ck sweeps, check, advance the counter
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  const int v = b.variant[pi];
-  if (hip_check(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal), "capture")) return GFX_ERR_DEVICE;
-  for (int per = 0; per < periods; per++) {
-    for (int k = 0; k < ck; k++) batch_sweep(b, fp32, ndir, v);
-    batch_check(b, fp32, ndir, thr);
-    launch_batch_advance(b.d_iter, ck, s.stream);
-  }
-  if (hip_check(hipStreamEndCapture(s.stream, &graph), "end capture")) return GFX_ERR_DEVICE;
-  if (hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate")) {
-    hipGraphDestroy(graph);
-    return GFX_ERR_DEVICE;
-  }
-  launch_fill_i32(b.d_iter, 1, i + ck - 1, s.stream);
-  int rc = GFX_OK;
-  while (i < maxit) {
-    if (hip_check(hipGraphLaunch(exec, s.stream), "graph launch")) {
-      rc = GFX_ERR_DEVICE;
-      break;
+    return go_on;
+  };
+  bool go_on = true;
+  while (go_on && i < maxit) {
+    int v = b.variant[pi];
+    bool timed = false;
+    if (tune_v >= 0) {
+      // time full non-check iterations when there are any (ck >= 3), else
+      // the odd half sweeps
+      const bool chk = (i % ck) == 0;
+      timed = (ck >= 3) ? (!even_done && !chk) : true;
+      if (timed) v = tune_v;
     }
-    b.graph_launches++;
-    i += periods * ck;
-    const int na = count_active(b);
-    if (fp32) b.iters32 += periods * ck; else b.iters64 += periods * ck;
-    if (na <= 0) break;
+    go_on = eager(v, timed);
+    if (timed && ++tune_v == SWEEP_VARIANTS) {
+      int best = 0;
+      for (int k = 1; k < SWEEP_VARIANTS; k++)
+        if (b.tune_ms[pi][k] < b.tune_ms[pi][best]) best = k;
+      b.variant[pi] = best;
+      b.tuned[pi] = true;
+      tune_v = -1;
+    }
+    if (tune_v < 0 && even_done) break;  // just after a check: periods start here
   }
-  hipGraphExecDestroy(exec);
-  hipGraphDestroy(graph);
+  if (go_on && i + periods * ck <= maxit) {
+    // graph of `periods` check periods over iterations i .. i + ck - 1, ...:
+    // the even half of the first iteration ran with the previous check
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const int v = b.variant[pi];
+    if (hip_check(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal), "capture")) return GFX_ERR_DEVICE;
+    for (int per = 0; per < periods; per++) {
+      for (int j = 0; j < ck; j++) {
+        if (j > 0) batch_half(b, fp32, 0, v, 0);
+        batch_half(b, fp32, 1, v, j == ck - 1 ? 1 : 0);
+      }
+      batch_check_tail(b, fp32, v, thr, maxit, stall_checks);
+      launch_batch_advance(b.d_iter, ck, s.stream);
+    }
+    if (hip_check(hipStreamEndCapture(s.stream, &graph), "end capture")) return GFX_ERR_DEVICE;
+    if (hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate")) {
+      hipGraphDestroy(graph);
+      return GFX_ERR_DEVICE;
+    }
+    launch_fill_i32(b.d_iter, 1, i + ck - 1, s.stream);
+    while (go_on && i + periods * ck <= maxit) {
+      if (hip_check(hipGraphLaunch(exec, s.stream), "graph launch")) {
+        rc = GFX_ERR_DEVICE;
+        break;
+      }
+      b.graph_launches++;
+      i += periods * ck;
+      if (fp32) b.iters32 += periods * ck; else b.iters64 += periods * ck;
+      const int na = count_active(b);
+      if (na < 0) rc = GFX_ERR_DEVICE;
+      go_on = na > 0;
+    }
+    hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph);
+  }
+  // tail: the iterations left before maxit
+  while (rc == GFX_OK && go_on && i < maxit) go_on = eager(b.variant[pi], false);
+  batch_canonical(b, fp32);
   *it = i;
   return rc;
 }
@@ -321,7 +386,7 @@ int gfx_batch_add(int l0, int l1, int l2, int l3, gfx_su3* gf, gfx_su3* trafo) {
   p.geo = make_geom(l0, l1, l2, l3);
   p.ushare = (int)g_b.probs.size();
   if (dev_alloc((void**)&p.U64, 8 * field_bytes64(p.geo), "batch links")) return -GFX_ERR_ALLOC;
-  if (dev_alloc((void**)&p.g64, 2 * field_bytes64(p.geo), "batch trafo")) {
+  if (dev_alloc((void**)&p.g64, 3 * field_bytes64(p.geo), "batch trafo")) {
     hipFree(p.U64);
     return -GFX_ERR_ALLOC;
   }
@@ -345,7 +410,7 @@ int gfx_batch_add_copy(int base, gfx_su3* trafo) {
   p.geo = o.geo;
   p.ushare = o.ushare;
   p.U64 = g_b.probs[o.ushare].U64;
-  if (dev_alloc((void**)&p.g64, 2 * field_bytes64(p.geo), "batch trafo")) return -GFX_ERR_ALLOC;
+  if (dev_alloc((void**)&p.g64, 3 * field_bytes64(p.geo), "batch trafo")) return -GFX_ERR_ALLOC;
   if (trafo && upload_aos_geo(s, p.geo, (const double*)trafo, p.g64, 1)) {
     hipFree(p.g64);
     return -GFX_ERR_DEVICE;
@@ -384,8 +449,9 @@ int gfx_batch_run(int maxit, double eps, int checkinterval, int precision, doubl
   int it = 0, rc = GFX_OK;
   if (mixed) {
     launch_fill_f64(b.d_results, 4 * (int)b.probs.size(), -1.0, s.stream);
-    rc = batch_phase(b, true, 4, std::max(switch_theta > 0 ? switch_theta : s.switch_theta, eps),
-                     checkinterval, maxit, periods_per_graph, &it);
+    rc = batch_phase(b, true, std::max(switch_theta > 0 ? switch_theta : s.switch_theta, eps),
+                     checkinterval, maxit, periods_per_graph, &it,
+                     (s.stall_iters + checkinterval - 1) / checkinterval);
     hipMemcpyAsync(b.h_results, b.d_results, sizeof(double) * 4 * b.probs.size(), hipMemcpyDeviceToHost, s.stream);
     hipStreamSynchronize(s.stream);
     for (size_t q = 0; q < b.probs.size(); q++) {
@@ -396,7 +462,7 @@ int gfx_batch_run(int maxit, double eps, int checkinterval, int precision, doubl
   const int it_fp64_start = it;
   if (rc == GFX_OK) {
     launch_fill_f64(b.d_results, 4 * (int)b.probs.size(), -1.0, s.stream);
-    rc = batch_phase(b, false, 4, eps, checkinterval, maxit, periods_per_graph, &it);
+    rc = batch_phase(b, false, eps, checkinterval, maxit, periods_per_graph, &it, 0);
   }
   hipMemcpyAsync(b.h_results, b.d_results, sizeof(double) * 4 * b.probs.size(), hipMemcpyDeviceToHost, s.stream);
   hipStreamSynchronize(s.stream);

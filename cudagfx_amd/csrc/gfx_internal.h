@@ -13,23 +13,35 @@ constexpr int SWEEP_VARIANTS = 8;
 int sweep_variant_block(int v);
 bool sweep_variant_rolled(int v);
 bool sweep_variant_nt(int v);
-void launch_sweep_f64(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
-                      double omega, const unsigned char* active, int variant, hipStream_t st);
-void launch_sweep_f32(const float4* U, float4* g, const Geom& geo, int parity, int ndir,
-                      double omega, const unsigned char* active, int variant, hipStream_t st);
+// one half sweep over the fields ge (even) and go (odd) of g.  chk (Landau
+// only): 0 plain in-place sweep; 1 check after the update (odd sweep of a
+// check iteration); 2 check before the update, new field written to galt
+// (even sweep that follows it).  Checks write 3 doubles per block to
+// partials (sweep_blocks(nh, variant) blocks).
+void launch_sweep_f64(const double2* U, double2* ge, double2* go, const Geom& geo, int parity,
+                      int ndir, double omega, const unsigned char* active, int variant,
+                      hipStream_t st, int chk = 0, double2* galt = nullptr,
+                      double* partials = nullptr);
+void launch_sweep_f32(const float4* U, float4* ge, float4* go, const Geom& geo, int parity,
+                      int ndir, double omega, const unsigned char* active, int variant,
+                      hipStream_t st, int chk = 0, float4* galt = nullptr,
+                      double* partials = nullptr);
+int sweep_blocks(int nh, int variant);
+// grid pass of a fused check: out[0..3] = F/normF, dAdA/normD, max, max < eps
+void launch_check_finalize(const double* podd, const double* peven, int nb, double normF,
+                           double normD, double eps, double* out, hipStream_t st);
 int functional_blocks(int nsh);
 int functional_block_size();
 void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
 
 // batched lattices (gfx_batch_api.hip)
 void launch_batch_sweep(bool fp32, const BatchProblem* probs, const int2* bmap, int nblocks,
-                        const int* active, int parity, int ndir, double omega, int variant,
-                        hipStream_t st);
-void launch_batch_functional(bool fp32, const BatchProblem* probs, const int2* fmap, int nblocks,
-                             const int* active, int ndir, double* partials, hipStream_t st);
-void launch_batch_finalize(const double* partials, const int* poff, const BatchProblem* probs,
-                           int nprob, int ndir, double thr, int* active, double* results,
-                           const int* iter, hipStream_t st);
+                        const int* active, const int* poff, int parity, double omega, int variant,
+                        int chk, double* partials, hipStream_t st);
+void launch_batch_finalize(const double* podd, const double* peven, const int* poff,
+                           BatchProblem* probs, int nprob, int ndir, double thr, int* active,
+                           double* results, const int* iter, int maxit, double* best, int* stall,
+                           int stall_checks, double stall_gate, hipStream_t st);
 void launch_batch_advance(int* iter, int step, hipStream_t st);
 void launch_fill_i32(int* p, int n, int v, hipStream_t st);
 void launch_fill_f64(double* p, int n, double v, hipStream_t st);

@@ -71,14 +71,39 @@ __device__ __forceinline__ void block_reduce3(double& a, double& b, double& c) {
 //          operands are live at a time (FP64: 130 VGPRs, 3 waves/SIMD)
 //          instead of the fully unrolled form (more load-level parallelism
 //          per wave, 332 registers, 1 wave/SIMD).
-template <typename V, typename R, int NDIR, bool ROLLED, bool NT>
-__device__ __forceinline__ void sweep_site(const V* __restrict__ U, V* __restrict__ g,
-                                           const Geom& geo, const Site& s, int i, int parity, R c1,
-                                           R c2) {
-  const int S = geo.stride;
-  const V* gop = field_ptr(static_cast<const V*>(g), 1 - parity, S);
-  V* gme = field_ptr(g, parity, S);
+//
+// Fused checks (Landau).  With W_mu(x) = g(x) U_mu(x) g^+(x+mu), the star of a
+// site gives g(x) K(x) = sum_mu [W_mu(x) + W_mu(x-mu)^+], so
+//   Re tr g(x)K(x) summed over the sites of one parity = sum of Re tr W over
+//   all links (every link has one even and one odd end), and
+//   Delta(x) = sum_mu [A(W_mu(x-mu)) - A(W_mu(x))] = -A(g(x)K(x))
+// with A the traceless hermitian part of overrelax.cu:451-476.  The check
+// after iteration i therefore needs no pass of its own:
+//   CHK = 1  odd half sweep of iteration i: after the update, g(x)K(x) of the
+//            new g(x) -> F and Delta of the odd sites of state i;
+//   CHK = 2  even half sweep of iteration i+1: before the update, g(x)K(x)
+//            -> Delta of the even sites of state i; the new even field goes
+//            to another buffer (gout), so a stop leaves state i intact.
+// Each block writes (sum F, sum dAdA, max dAdA) of its sites to
+// partials[3 * block].
+template <typename R>
+__device__ __forceinline__ void site_check(const M3<R>& G, const M3<R>& K, double& F, double& d) {
+  M3<R> M = m_mul(G, K);
+  F = (double)m_retrace(M);
+  vecpot(M);
+  R acc = R(0);
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc += M.e[a][c].re * M.e[a][c].re + M.e[a][c].im * M.e[a][c].im;
+  d = (double)acc;
+}
 
+template <typename V, typename R, int NDIR, bool ROLLED, bool NT, int CHK>
+__device__ __forceinline__ void sweep_site(const V* __restrict__ U, const V* __restrict__ gop,
+                                           const V* gin, V* gout, const Geom& geo, const Site& s,
+                                           int i, int parity, R c1, R c2, double& F, double& d) {
+  const int S = geo.stride;
   // star K = sum_mu U_mu(x) g^+(x+mu) + U_mu^+(x-mu) g^+(x-mu)
   M3<R> K;
   m_zero(K);
@@ -111,45 +136,111 @@ __device__ __forceinline__ void sweep_site(const V* __restrict__ U, V* __restric
       m_addmul(K, A, B);
     }
   }
-  M3<R> G = load_m<V, R>(gme, S, i);
+  M3<R> G = load_m<V, R>(gin, S, i);
+  if (CHK == 2) site_check(G, K, F, d);
   M3<R> h = G;
   cm_relax(h, K);
   m_normalize(h);
   overrelax_apply(G, h, c1, c2);
   m_normalize(G);
-  Store<V>::store(gme, S, i, G);
+  if (CHK == 1) site_check(G, K, F, d);
+  Store<V>::store(gout, S, i, G);
 }
 
-template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT>
-__global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* __restrict__ g,
+template <int CHK>
+__device__ __forceinline__ void store_check_partials(double F, double d, double* partials, int slot) {
+  double mx = d;
+  block_reduce3(F, d, mx);
+  if (threadIdx.x == 0) {
+    double* o = partials + 3 * (size_t)slot;
+    o[0] = (CHK == 1) ? F : 0.0;
+    o[1] = d;
+    o[2] = mx;
+  }
+}
+
+// ge / go: even and odd fields of g.  CHK = 2 writes the new own field to
+// galt; otherwise the sweep is in place.
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK>
+__global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* ge, V* go, V* galt,
                                                      Geom geo, int parity, R c1, R c2,
-                                                     const unsigned char* __restrict__ active) {
+                                                     const unsigned char* __restrict__ active,
+                                                     double* __restrict__ partials) {
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const int i = b * BLOCK + threadIdx.x;
-  if (i >= geo.nh) return;
-  const Site s = site_coords(geo, i, parity);
-  if (NDIR == 3 && active != nullptr && !active[s.x[3]]) return;
-  sweep_site<V, R, NDIR, ROLLED, NT>(U, g, geo, s, i, parity, c1, c2);
+  V* gme = parity ? go : ge;
+  const V* gop = parity ? ge : go;
+  V* gout = (CHK == 2) ? galt : gme;
+  double F = 0.0, d = 0.0;
+  if (CHK == 0) {
+    if (i >= geo.nh) return;
+    const Site s = site_coords(geo, i, parity);
+    if (NDIR == 3 && active != nullptr && !active[s.x[3]]) return;
+    sweep_site<V, R, NDIR, ROLLED, NT, 0>(U, gop, gme, gout, geo, s, i, parity, c1, c2, F, d);
+  } else {
+    if (i < geo.nh) {
+      const Site s = site_coords(geo, i, parity);
+      sweep_site<V, R, NDIR, ROLLED, NT, CHK>(U, gop, gme, gout, geo, s, i, parity, c1, c2, F, d);
+    }
+    store_check_partials<CHK>(F, d, partials, b);
+  }
 }
 
 // ------------------------------------------------------------ batched sweep
 // Many lattices (Gribov copies of one configuration sharing its links, or
 // configurations of different sizes) in one launch.  bmap[block] = (problem,
 // block inside the problem); a problem whose active flag is 0 has converged
-// and its blocks return at once.
-template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT>
+// and its blocks return at once.  Check partials of block (q, k) go to
+// partials[3 * (poff[q] + k)].
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK>
 __global__ __launch_bounds__(BLOCK) void k_batch_sweep(const BatchProblem* __restrict__ probs,
                                                         const int2* __restrict__ bmap,
                                                         const int* __restrict__ active,
-                                                        int parity, R c1, R c2) {
+                                                        const int* __restrict__ poff, int parity,
+                                                        R c1, R c2, double* __restrict__ partials) {
   const int2 m = bmap[blockIdx.x];
   if (!active[m.x]) return;
-  const Geom geo = probs[m.x].geo;
+  const BatchProblem& P = probs[m.x];
+  const Geom geo = P.geo;
+  const V* U = static_cast<const V*>(P.U);
+  V* gme = static_cast<V*>(P.g[parity]);
+  const V* gop = static_cast<const V*>(P.g[1 - parity]);
+  V* gout = (CHK == 2) ? static_cast<V*>(P.galt) : gme;
   const int i = m.y * BLOCK + threadIdx.x;
-  if (i >= geo.nh) return;
-  const Site s = site_coords(geo, i, parity);
-  sweep_site<V, R, NDIR, ROLLED, NT>(static_cast<const V*>(probs[m.x].U), static_cast<V*>(probs[m.x].g),
-                                 geo, s, i, parity, c1, c2);
+  double F = 0.0, d = 0.0;
+  if (CHK == 0) {
+    if (i >= geo.nh) return;
+    const Site s = site_coords(geo, i, parity);
+    sweep_site<V, R, NDIR, ROLLED, NT, 0>(U, gop, gme, gout, geo, s, i, parity, c1, c2, F, d);
+  } else {
+    if (i < geo.nh) {
+      const Site s = site_coords(geo, i, parity);
+      sweep_site<V, R, NDIR, ROLLED, NT, CHK>(U, gop, gme, gout, geo, s, i, parity, c1, c2, F, d);
+    }
+    store_check_partials<CHK>(F, d, partials, poff[m.x] + m.y);
+  }
+}
+
+// grid-level pass of a fused check: podd / peven hold nb block partials
+// each.  out = F/normF (F from the odd sweep), dAdA/normD, max dAdA, and
+// 1.0 if max < eps.
+__global__ __launch_bounds__(BLK) void k_check_finalize(const double* __restrict__ podd,
+                                                        const double* __restrict__ peven, int nb,
+                                                        double normF, double normD, double eps,
+                                                        double* __restrict__ out) {
+  double F = 0.0, d = 0.0, mx = 0.0;
+  for (int b = threadIdx.x; b < nb; b += BLK) {
+    F += podd[3 * (size_t)b];
+    d += podd[3 * (size_t)b + 1] + peven[3 * (size_t)b + 1];
+    mx = fmax(mx, fmax(podd[3 * (size_t)b + 2], peven[3 * (size_t)b + 2]));
+  }
+  block_reduce3(F, d, mx);
+  if (threadIdx.x == 0) {
+    out[0] = F / normF;
+    out[1] = d / normD;
+    out[2] = mx;
+    out[3] = (mx < eps) ? 1.0 : 0.0;
+  }
 }
 
 // ------------------------------------------------------------ functional
@@ -225,50 +316,28 @@ __global__ __launch_bounds__(BLK) void k_functional(const V* __restrict__ U,
   }
 }
 
-// batched functional: fmap[block] = (problem, block over the 2*nh sites of
-// the problem); partials per launched block
-template <typename V, int NDIR>
-__global__ __launch_bounds__(BLK) void k_batch_functional(const BatchProblem* __restrict__ probs,
-                                                           const int2* __restrict__ fmap,
-                                                           const int* __restrict__ active,
-                                                           double* __restrict__ partials) {
-  const int2 m = fmap[blockIdx.x];
-  if (!active[m.x]) return;
-  const Geom geo = probs[m.x].geo;
-  const int k = m.y * BLK + threadIdx.x;
-  double F = 0.0, d = 0.0, mx = 0.0;
-  if (k < 2 * geo.nh) {
-    const int p = (k >= geo.nh) ? 1 : 0;
-    functional_site<V, NDIR>(static_cast<const V*>(probs[m.x].U), static_cast<const V*>(probs[m.x].g),
-                             geo, p, k - p * geo.nh, F, d);
-    mx = d;
-  }
-  block_reduce3(F, d, mx);
-  if (threadIdx.x == 0) {
-    double* o = partials + 3 * (size_t)blockIdx.x;
-    o[0] = F;
-    o[1] = d;
-    o[2] = mx;
-  }
-}
-
-// one block per problem over its block partials [poff[q], poff[q+1]):
-// results[4q..] = F/(ndir V), dAdA/V, max, stop iteration; a problem whose
-// max < thr is deactivated at the iteration held in *iter
-__global__ __launch_bounds__(BLK) void k_batch_finalize(const double* __restrict__ partials,
-                                                        const int* __restrict__ poff,
-                                                        const BatchProblem* __restrict__ probs,
-                                                        int ndir, double thr,
-                                                        int* __restrict__ active,
-                                                        double* __restrict__ results,
-                                                        const int* __restrict__ iter) {
+// Grid-level pass of a batch's fused check, one block per problem over its
+// block partials [poff[q], poff[q+1]) of the odd (podd) and even (peven)
+// check sweeps: results[4q..] = F/(ndir V), dAdA/V, max dAdA.  A problem
+// stops (active = 0, results[4q+3] = *iter) when max < thr, or, with
+// stall_checks > 0, when its best max dAdA has stayed below stall_gate
+// without halving for stall_checks checks (the FP32 phase has reached its
+// rounding floor).  A problem that goes on takes the even field of the
+// check sweep (swap g[0] and galt) unless *iter + 1 == maxit, where the run
+// ends with the state of the check.
+__global__ __launch_bounds__(BLK) void k_batch_finalize(
+    const double* __restrict__ podd, const double* __restrict__ peven,
+    const int* __restrict__ poff, BatchProblem* __restrict__ probs, int ndir, double thr,
+    int* __restrict__ active, double* __restrict__ results, const int* __restrict__ iter,
+    int maxit, double* __restrict__ best, int* __restrict__ stall, int stall_checks,
+    double stall_gate) {
   const int q = blockIdx.x;
   if (!active[q]) return;
   double F = 0.0, d = 0.0, mx = 0.0;
   for (int b = poff[q] + threadIdx.x; b < poff[q + 1]; b += BLK) {
-    F += partials[3 * (size_t)b];
-    d += partials[3 * (size_t)b + 1];
-    mx = fmax(mx, partials[3 * (size_t)b + 2]);
+    F += podd[3 * (size_t)b];
+    d += podd[3 * (size_t)b + 1] + peven[3 * (size_t)b + 1];
+    mx = fmax(mx, fmax(podd[3 * (size_t)b + 2], peven[3 * (size_t)b + 2]));
   }
   block_reduce3(F, d, mx);
   if (threadIdx.x == 0) {
@@ -276,9 +345,22 @@ __global__ __launch_bounds__(BLK) void k_batch_finalize(const double* __restrict
     results[4 * q + 0] = F / (ndir * V);
     results[4 * q + 1] = d / V;
     results[4 * q + 2] = mx;
-    if (mx < thr) {
+    bool stop = mx < thr;
+    if (!stop && stall_checks > 0) {
+      if (mx < 0.5 * best[q]) {
+        best[q] = mx;
+        stall[q] = 0;
+      } else if (best[q] < stall_gate && ++stall[q] >= stall_checks) {
+        stop = true;
+      }
+    }
+    if (stop) {
       results[4 * q + 3] = (double)(*iter);
       active[q] = 0;
+    } else if (*iter + 1 < maxit) {
+      void* t = probs[q].g[0];
+      probs[q].g[0] = probs[q].galt;
+      probs[q].galt = t;
     }
   }
 }
@@ -479,116 +561,146 @@ int sweep_variant_block(int v) { return (v & 1) ? 64 : 256; }
 bool sweep_variant_rolled(int v) { return (v & 2) == 0; }
 bool sweep_variant_nt(int v) { return (v & 4) != 0; }
 
-template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT>
-static void launch_sweep_v(const V* U, V* g, const Geom& geo, int parity, R c1, R c2,
-                           const unsigned char* active, hipStream_t st) {
-  dim3 grid((geo.nh + BLOCK - 1) / BLOCK);
-  hipLaunchKernelGGL((k_or_sweep<V, R, NDIR, BLOCK, ROLLED, NT>), grid, dim3(BLOCK), 0, st, U, g,
-                     geo, parity, c1, c2, active);
+template <typename V>
+struct SweepArgs {
+  const V* U;
+  V* ge;
+  V* go;
+  V* galt;
+  Geom geo;
+  int parity;
+  const unsigned char* active;
+  double* partials;
+};
+
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK>
+static void launch_sweep_v(const SweepArgs<V>& a, R c1, R c2, hipStream_t st) {
+  dim3 grid((a.geo.nh + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL((k_or_sweep<V, R, NDIR, BLOCK, ROLLED, NT, CHK>), grid, dim3(BLOCK), 0, st,
+                     a.U, a.ge, a.go, a.galt, a.geo, a.parity, c1, c2, a.active, a.partials);
 }
 
-template <typename V, typename R, int NDIR>
-static void launch_sweep_nd(const V* U, V* g, const Geom& geo, int parity, R c1, R c2,
-                            const unsigned char* active, int variant, hipStream_t st) {
+template <typename V, typename R, int NDIR, int CHK>
+static void launch_sweep_chk(const SweepArgs<V>& a, R c1, R c2, int variant, hipStream_t st) {
   switch (variant & 7) {
-    case 0: launch_sweep_v<V, R, NDIR, 256, true, false>(U, g, geo, parity, c1, c2, active, st); break;
-    case 1: launch_sweep_v<V, R, NDIR, 64, true, false>(U, g, geo, parity, c1, c2, active, st); break;
-    case 2: launch_sweep_v<V, R, NDIR, 256, false, false>(U, g, geo, parity, c1, c2, active, st); break;
-    case 3: launch_sweep_v<V, R, NDIR, 64, false, false>(U, g, geo, parity, c1, c2, active, st); break;
-    case 4: launch_sweep_v<V, R, NDIR, 256, true, true>(U, g, geo, parity, c1, c2, active, st); break;
-    case 5: launch_sweep_v<V, R, NDIR, 64, true, true>(U, g, geo, parity, c1, c2, active, st); break;
-    case 6: launch_sweep_v<V, R, NDIR, 256, false, true>(U, g, geo, parity, c1, c2, active, st); break;
-    default: launch_sweep_v<V, R, NDIR, 64, false, true>(U, g, geo, parity, c1, c2, active, st); break;
+    case 0: launch_sweep_v<V, R, NDIR, 256, true, false, CHK>(a, c1, c2, st); break;
+    case 1: launch_sweep_v<V, R, NDIR, 64, true, false, CHK>(a, c1, c2, st); break;
+    case 2: launch_sweep_v<V, R, NDIR, 256, false, false, CHK>(a, c1, c2, st); break;
+    case 3: launch_sweep_v<V, R, NDIR, 64, false, false, CHK>(a, c1, c2, st); break;
+    case 4: launch_sweep_v<V, R, NDIR, 256, true, true, CHK>(a, c1, c2, st); break;
+    case 5: launch_sweep_v<V, R, NDIR, 64, true, true, CHK>(a, c1, c2, st); break;
+    case 6: launch_sweep_v<V, R, NDIR, 256, false, true, CHK>(a, c1, c2, st); break;
+    default: launch_sweep_v<V, R, NDIR, 64, false, true, CHK>(a, c1, c2, st); break;
   }
 }
 
 template <typename V, typename R>
-static void launch_sweep_t(const V* U, V* g, const Geom& geo, int parity, int ndir, double omega,
-                           const unsigned char* active, int variant, hipStream_t st) {
+static void launch_sweep_t(const SweepArgs<V>& a, int ndir, double omega, int variant, int chk,
+                           hipStream_t st) {
   // C(w,1) = (1+w-1)/1 and C(w,2) = (1+w-2)/2 as the reference forms them
   // (overrelax.cu:167), in double, then rounded to R.
   const R c1 = (R)((1.0 + omega - 1) / 1);
   const R c2 = (R)((1.0 + omega - 2) / 2);
-  if (ndir == 4)
-    launch_sweep_nd<V, R, 4>(U, g, geo, parity, c1, c2, nullptr, variant, st);
-  else
-    launch_sweep_nd<V, R, 3>(U, g, geo, parity, c1, c2, active, variant, st);
-}
-
-void launch_sweep_f64(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
-                      double omega, const unsigned char* active, int variant, hipStream_t st) {
-  launch_sweep_t<double2, double>(U, g, geo, parity, ndir, omega, active, variant, st);
-}
-void launch_sweep_f32(const float4* U, float4* g, const Geom& geo, int parity, int ndir,
-                      double omega, const unsigned char* active, int variant, hipStream_t st) {
-  launch_sweep_t<float4, float>(U, g, geo, parity, ndir, omega, active, variant, st);
-}
-
-void launch_copy_fields(const void* src, void* dst, size_t bytes, hipStream_t st) {
-  hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
-}
-
-// ---- batch launchers: nblocks entries of bmap built for the variant's block size
-template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT>
-static void launch_bsweep_v(const BatchProblem* probs, const int2* bmap, int nblocks,
-                            const int* active, int parity, R c1, R c2, hipStream_t st) {
-  hipLaunchKernelGGL((k_batch_sweep<V, R, NDIR, BLOCK, ROLLED, NT>), dim3(nblocks), dim3(BLOCK), 0,
-                     st, probs, bmap, active, parity, c1, c2);
-}
-
-template <typename V, typename R, int NDIR>
-static void launch_bsweep_nd(const BatchProblem* probs, const int2* bmap, int nblocks,
-                             const int* active, int parity, R c1, R c2, int variant,
-                             hipStream_t st) {
-  switch (variant & 7) {
-    case 0: launch_bsweep_v<V, R, NDIR, 256, true, false>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
-    case 1: launch_bsweep_v<V, R, NDIR, 64, true, false>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
-    case 2: launch_bsweep_v<V, R, NDIR, 256, false, false>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
-    case 3: launch_bsweep_v<V, R, NDIR, 64, false, false>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
-    case 4: launch_bsweep_v<V, R, NDIR, 256, true, true>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
-    case 5: launch_bsweep_v<V, R, NDIR, 64, true, true>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
-    case 6: launch_bsweep_v<V, R, NDIR, 256, false, true>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
-    default: launch_bsweep_v<V, R, NDIR, 64, false, true>(probs, bmap, nblocks, active, parity, c1, c2, st); break;
+  if (ndir == 4) {
+    SweepArgs<V> b = a;
+    b.active = nullptr;
+    if (chk == 1)
+      launch_sweep_chk<V, R, 4, 1>(b, c1, c2, variant, st);
+    else if (chk == 2)
+      launch_sweep_chk<V, R, 4, 2>(b, c1, c2, variant, st);
+    else
+      launch_sweep_chk<V, R, 4, 0>(b, c1, c2, variant, st);
+  } else {
+    launch_sweep_chk<V, R, 3, 0>(a, c1, c2, variant, st);
   }
+}
+
+void launch_sweep_f64(const double2* U, double2* ge, double2* go, const Geom& geo, int parity,
+                      int ndir, double omega, const unsigned char* active, int variant,
+                      hipStream_t st, int chk, double2* galt, double* partials) {
+  launch_sweep_t<double2, double>({U, ge, go, galt, geo, parity, active, partials}, ndir, omega,
+                                  variant, chk, st);
+}
+void launch_sweep_f32(const float4* U, float4* ge, float4* go, const Geom& geo, int parity,
+                      int ndir, double omega, const unsigned char* active, int variant,
+                      hipStream_t st, int chk, float4* galt, double* partials) {
+  launch_sweep_t<float4, float>({U, ge, go, galt, geo, parity, active, partials}, ndir, omega,
+                                variant, chk, st);
+}
+
+int sweep_blocks(int nh, int variant) {
+  const int b = sweep_variant_block(variant);
+  return (nh + b - 1) / b;
+}
+
+void launch_check_finalize(const double* podd, const double* peven, int nb, double normF,
+                           double normD, double eps, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_check_finalize, dim3(1), dim3(BLK), 0, st, podd, peven, nb, normF, normD,
+                     eps, out);
+}
+
+// ---- batch launchers (Landau): nblocks entries of bmap built for the
+// variant's block size, poff the partial offsets of that map
+struct BSweepArgs {
+  const BatchProblem* probs;
+  const int2* bmap;
+  int nblocks;
+  const int* active;
+  const int* poff;
+  int parity;
+  double* partials;
+};
+
+template <typename V, typename R, int BLOCK, bool ROLLED, bool NT, int CHK>
+static void launch_bsweep_v(const BSweepArgs& a, R c1, R c2, hipStream_t st) {
+  hipLaunchKernelGGL((k_batch_sweep<V, R, 4, BLOCK, ROLLED, NT, CHK>), dim3(a.nblocks), dim3(BLOCK),
+                     0, st, a.probs, a.bmap, a.active, a.poff, a.parity, c1, c2, a.partials);
+}
+
+template <typename V, typename R, int CHK>
+static void launch_bsweep_chk(const BSweepArgs& a, R c1, R c2, int variant, hipStream_t st) {
+  switch (variant & 7) {
+    case 0: launch_bsweep_v<V, R, 256, true, false, CHK>(a, c1, c2, st); break;
+    case 1: launch_bsweep_v<V, R, 64, true, false, CHK>(a, c1, c2, st); break;
+    case 2: launch_bsweep_v<V, R, 256, false, false, CHK>(a, c1, c2, st); break;
+    case 3: launch_bsweep_v<V, R, 64, false, false, CHK>(a, c1, c2, st); break;
+    case 4: launch_bsweep_v<V, R, 256, true, true, CHK>(a, c1, c2, st); break;
+    case 5: launch_bsweep_v<V, R, 64, true, true, CHK>(a, c1, c2, st); break;
+    case 6: launch_bsweep_v<V, R, 256, false, true, CHK>(a, c1, c2, st); break;
+    default: launch_bsweep_v<V, R, 64, false, true, CHK>(a, c1, c2, st); break;
+  }
+}
+
+template <typename V, typename R>
+static void launch_bsweep_t(const BSweepArgs& a, double omega, int variant, int chk,
+                            hipStream_t st) {
+  const R c1 = (R)((1.0 + omega - 1) / 1);
+  const R c2 = (R)((1.0 + omega - 2) / 2);
+  if (chk == 1)
+    launch_bsweep_chk<V, R, 1>(a, c1, c2, variant, st);
+  else if (chk == 2)
+    launch_bsweep_chk<V, R, 2>(a, c1, c2, variant, st);
+  else
+    launch_bsweep_chk<V, R, 0>(a, c1, c2, variant, st);
 }
 
 void launch_batch_sweep(bool fp32, const BatchProblem* probs, const int2* bmap, int nblocks,
-                        const int* active, int parity, int ndir, double omega, int variant,
-                        hipStream_t st) {
-  const double c1 = (1.0 + omega - 1) / 1, c2 = (1.0 + omega - 2) / 2;
-  if (fp32) {
-    if (ndir == 4)
-      launch_bsweep_nd<float4, float, 4>(probs, bmap, nblocks, active, parity, (float)c1, (float)c2, variant, st);
-    else
-      launch_bsweep_nd<float4, float, 3>(probs, bmap, nblocks, active, parity, (float)c1, (float)c2, variant, st);
-  } else {
-    if (ndir == 4)
-      launch_bsweep_nd<double2, double, 4>(probs, bmap, nblocks, active, parity, c1, c2, variant, st);
-    else
-      launch_bsweep_nd<double2, double, 3>(probs, bmap, nblocks, active, parity, c1, c2, variant, st);
-  }
+                        const int* active, const int* poff, int parity, double omega, int variant,
+                        int chk, double* partials, hipStream_t st) {
+  const BSweepArgs a{probs, bmap, nblocks, active, poff, parity, partials};
+  if (fp32)
+    launch_bsweep_t<float4, float>(a, omega, variant, chk, st);
+  else
+    launch_bsweep_t<double2, double>(a, omega, variant, chk, st);
 }
 
-void launch_batch_functional(bool fp32, const BatchProblem* probs, const int2* fmap, int nblocks,
-                             const int* active, int ndir, double* partials, hipStream_t st) {
-  if (fp32) {
-    if (ndir == 4)
-      hipLaunchKernelGGL((k_batch_functional<float4, 4>), dim3(nblocks), dim3(BLK), 0, st, probs, fmap, active, partials);
-    else
-      hipLaunchKernelGGL((k_batch_functional<float4, 3>), dim3(nblocks), dim3(BLK), 0, st, probs, fmap, active, partials);
-  } else {
-    if (ndir == 4)
-      hipLaunchKernelGGL((k_batch_functional<double2, 4>), dim3(nblocks), dim3(BLK), 0, st, probs, fmap, active, partials);
-    else
-      hipLaunchKernelGGL((k_batch_functional<double2, 3>), dim3(nblocks), dim3(BLK), 0, st, probs, fmap, active, partials);
-  }
-}
-
-void launch_batch_finalize(const double* partials, const int* poff, const BatchProblem* probs,
-                           int nprob, int ndir, double thr, int* active, double* results,
-                           const int* iter, hipStream_t st) {
-  hipLaunchKernelGGL(k_batch_finalize, dim3(nprob), dim3(BLK), 0, st, partials, poff, probs, ndir,
-                     thr, active, results, iter);
+void launch_batch_finalize(const double* podd, const double* peven, const int* poff,
+                           BatchProblem* probs, int nprob, int ndir, double thr, int* active,
+                           double* results, const int* iter, int maxit, double* best, int* stall,
+                           int stall_checks, double stall_gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_finalize, dim3(nprob), dim3(BLK), 0, st, podd, peven, poff, probs,
+                     ndir, thr, active, results, iter, maxit, best, stall, stall_checks,
+                     stall_gate);
 }
 
 void launch_batch_advance(int* iter, int step, hipStream_t st) {

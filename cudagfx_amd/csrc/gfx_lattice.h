@@ -203,11 +203,15 @@ __device__ __forceinline__ V* field_ptr(V* base, int field, int stride) {
 }
 
 // One lattice of a batch: its geometry and its fields (FP64 double2 or FP32
-// float4 storage, by the phase of the run).  Gribov copies share U.
+// float4 storage, by the phase of the run).  Gribov copies share U.  g[0] and
+// g[1] are the even and odd fields of the gauge transformation; galt is a
+// third field that a check sweep writes the new even field into, and the
+// check's finalize swaps g[0] and galt unless the lattice stops there.
 struct BatchProblem {
   Geom geo;
   const void* U;
-  void* g;
+  void* g[2];
+  void* galt;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own

@@ -15,13 +15,19 @@ struct State {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-  // FP64 fields: links 8 fields (parity*4+mu), trafo 2 fields (parity)
+  // FP64 fields: links 8 fields (parity*4+mu), trafo 3 fields: even, odd,
+  // and the spare even field that a check sweep writes into (between calls
+  // the even field is always field 0)
   double2* U64 = nullptr;
   double2* g64 = nullptr;
-  // FP32 12-real copies for the mixed path
+  // FP32 12-real copies for the mixed path (same field layout)
   float4* U32 = nullptr;
   float4* g32 = nullptr;
   bool U32_valid = false;
+
+  // fused-check partials: odd sweep [0, ncheck), even sweep [ncheck, 2 ncheck)
+  double* cpart = nullptr;
+  int ncheck = 0;
 
   // reductions
   double* partials = nullptr;
@@ -36,6 +42,10 @@ struct State {
   double omega = 1.68;  // overrelax.cu:416
   int precision = 0;
   double switch_theta = 1e-6;
+  // the FP32 phase also ends when its best max dAdA has stayed below
+  // stall_gate without halving for stall_iters iterations (rounding floor)
+  int stall_iters = 300;
+  double stall_gate = 1e-8;
   int verbose = 1;
   long mt_seed = -1;
 

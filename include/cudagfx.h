@@ -125,6 +125,10 @@ int gfx_set_omega(double omega);
 /* precision mode; switch_theta = max dAdA at which the mixed path leaves
  * FP32 (default 1e-6) */
 int gfx_set_precision(int mode, double switch_theta);
+/* the mixed path also leaves FP32 when its best max dAdA is below `gate`
+ * and has not halved for `iters` iterations (the FP32 rounding floor, near
+ * 5e-11 on 32^4); iters = 0 turns this off.  Default 300, 1e-8. */
+int gfx_set_fp32_stall(int iters, double gate);
 /* 0 silences the per-check log lines */
 int gfx_set_verbose(int verbose);
 /* seed for srand() before the twister seeds are drawn (negative: time(NULL)) */

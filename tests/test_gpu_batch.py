@@ -112,3 +112,30 @@ def test_random_gribov_copies(gfx, orc):
     # Haar: the mean of Re tr g over sites is ~0 (it is 3 for the identity)
     tr = g_start[0].reshape(-1, 18)[:, [0, 8, 16]].sum(axis=1)
     assert abs(tr.mean()) < 0.1
+
+
+@pytest.mark.parametrize("maxit,ck,periods", [(41, 5, 2), (9, 1, 3)])
+def test_batch_state_at_maxit(gfx, orc, maxit, ck, periods):
+    """Runs cut at maxit (eager head, graph periods, eager tail) leave every
+    lattice in the state after iteration maxit-1, as the single-run oracle."""
+    gfx.set_omega(1.7)
+    shapes = [(8, 8, 4, 4), (4, 6, 4, 8)]
+    b = gfx.Batch()
+    cases = []
+    for k, dims in enumerate(shapes):
+        u = orc.random_init_gauge(orc.volume(dims), 920 + k)
+        g = orc.random_init_trafo(orc.volume(dims), 930 + k)
+        b.add(dims, u, g)
+        cases.append((dims, u, g))
+    rc = b.run(maxit, EPS12, ck, gfx.PREC_FP64, 1e-6, periods)
+    res = [b.result(q) for q in range(len(shapes))]
+    gs = [b.trafo(q) for q in range(len(shapes))]
+    b.close()
+    assert rc != 0  # nothing converges in so few iterations
+    for q, (dims, u, g) in enumerate(cases):
+        u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+        it, hist = orc.overrelax_gauge(u2, g2, dims, maxit, EPS12, ck, 1.7)
+        assert it == -1 and res[q][3] == -1
+        assert np.abs(gs[q] - orc.from_2v(g2)).max() < 1e-12
+        assert abs(res[q][0] - hist[-1, 0]) < 1e-13
+        assert abs(res[q][2] - hist[-1, 2]) / hist[-1, 2] < 1e-11

@@ -196,3 +196,69 @@ def test_coulomb_matches_oracle(gfx, orc):
     assert np.abs(g_gpu - orc.from_2v(g2)).max() < 1e-10
     Fo, Do, Mo = orc.functional_slices(u2, g2, dims, 3)
     assert np.abs(F - Fo).max() < 1e-12
+
+
+@pytest.mark.parametrize("maxit,ck", [(7, 3), (12, 1), (30, 10)])
+def test_fused_check_state_at_maxit(gfx, orc, maxit, ck):
+    """The checks ride on the sweeps (odd sweep of iteration i, even sweep of
+    i+1 into a spare field).  A run cut at maxit must still leave the state
+    after iteration maxit-1, and its last check must report F, dAdA and max
+    dAdA of the state it checked, as the oracle's per-link functional does."""
+    dims = (8, 4, 6, 8)
+    V = orc.volume(dims)
+    u = orc.random_init_gauge(V, 901)
+    g = orc.random_init_trafo(V, 902)
+    _setup(gfx, dims, u, g, 1.7)
+    it = gfx.overrelax_gauge(maxit, EPS12, ck)
+    F, D, M = gfx.results()
+    g_gpu = gfx.get_trafo()
+    gfx.finalize_gaugefixing()
+    u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+    it_o, hist = orc.overrelax_gauge(u2, g2, dims, maxit, EPS12, ck, 1.7)
+    assert it == -1 and it_o == -1
+    assert np.abs(g_gpu - orc.from_2v(g2)).max() < 1e-12
+    last = hist[-1]
+    assert abs(F - last[0]) < 1e-13
+    assert abs(D - last[1]) / last[1] < 1e-11 and abs(M - last[2]) / last[2] < 1e-11
+
+
+def test_fused_check_matches_functional_at_stop(gfx, orc):
+    """At the stop (max dAdA ~ 1e-12) the fused check and the standalone
+    per-link functional (calc_functional) of the returned g agree."""
+    dims = (8, 8, 8, 8)
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, 907, eps=0.3)
+    g = orc.random_init_trafo(V, 908)
+    _setup(gfx, dims, u, g, 1.7)
+    it = gfx.overrelax_gauge(5000, EPS12, 5)
+    F, D, M = gfx.results()
+    m2 = gfx.calc_functional()
+    F2, D2, M2 = gfx.results()
+    gfx.finalize_gaugefixing()
+    assert it > 0 and M < EPS12 and m2 == M2
+    assert abs(F - F2) < 1e-14
+    assert abs(D - D2) / D2 < 1e-6 and abs(M - M2) / M2 < 1e-6
+
+
+def test_mixed_stall_guard_leaves_fp32(gfx, orc):
+    """A switch bound below the FP32 rounding floor: the stall guard ends the
+    FP32 phase, the FP64 polish reaches the bound, F agrees with FP64."""
+    dims = (8, 8, 8, 8)
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, 911, eps=0.35)
+    g = orc.random_init_trafo(V, 912)
+    _setup(gfx, dims, u, g, 1.7)
+    it64 = gfx.overrelax_gauge(8000, EPS12, 10)
+    F64 = gfx.results()[0]
+    gfx.set_trafo(g)
+    gfx.set_precision(gfx.PREC_MIXED, 1e-15)
+    gfx.set_fp32_stall(200, 1e-8)
+    itm = gfx.overrelax_gauge(8000, EPS12, 10)
+    Fm, _, Mm = gfx.results()
+    _, i32, i64 = gfx.last_run_stats()
+    gfx.set_fp32_stall(300, 1e-8)
+    gfx.set_precision(gfx.PREC_FP64)
+    gfx.finalize_gaugefixing()
+    assert it64 > 0 and itm > 0 and Mm < EPS12
+    assert i32 > 0 and i64 > 0
+    assert abs(Fm - F64) / F64 < 1e-10
