@@ -379,7 +379,7 @@ class Batch:
 
     def variant(self, precision=PREC_FP64):
         v = C.c_int()
-        ms = np.zeros(8)
+        ms = np.zeros(32)
         rc = load().gfx_batch_variant(int(precision), C.byref(v), ms.ctypes.data_as(C.c_void_p))
         return (v.value, ms) if rc == 0 else None
 
@@ -400,7 +400,7 @@ def set_autotune(on):
 def sweep_variant(precision=PREC_FP64, ndir=4):
     """(variant, block, rolled, ms per variant) chosen by the autotuner, or None."""
     v, b, r = C.c_int(), C.c_int(), C.c_int()
-    ms = np.zeros(8)
+    ms = np.zeros(32)
     rc = load().gfx_get_sweep_variant(int(precision), int(ndir), C.byref(v), C.byref(b),
                                       C.byref(r), ms.ctypes.data_as(C.c_void_p))
     if rc != 0:

@@ -3,6 +3,7 @@
 // driver loop (overrelax.cu:677-823) on one HIP stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -103,6 +104,34 @@ int dev_alloc(void** p, size_t bytes, const char* what) {
 size_t field_bytes64(const Geom& g) { return (size_t)6 * g.stride * sizeof(double2); }
 size_t field_bytes32(const Geom& g) { return (size_t)3 * g.stride * sizeof(float4); }
 
+bool block_order(const Geom& geo, int block, int tile, std::vector<int>& order) {
+  const int nblk = (geo.nh + block - 1) / block;
+  order.resize(nblk);
+  for (int b = 0; b < nblk; b++) order[b] = b;
+  if (tile <= 0) return true;
+  const int nrow = geo.h0 * geo.L[1];
+  int a = nrow, c = block;
+  while (c) {
+    const int t = a % c;
+    a = c;
+    c = t;
+  }
+  const int G = block / a;  // rows per superrow: G * nrow is a multiple of block
+  if (geo.L[2] % G != 0) return false;
+  const int ns2 = geo.L[2] / G, l3 = geo.L[3];
+  const int bps = G * nrow / block;  // blocks per superrow
+  const int C = 1 << tile;
+  if (C >= ns2) return false;  // one chunk: the lexicographic order
+  int k = 0;
+  for (int c0 = 0; c0 < ns2; c0 += C) {
+    const int cw = std::min(C, ns2 - c0);
+    for (int x3 = 0; x3 < l3; x3++)
+      for (int x2s = c0; x2s < c0 + cw; x2s++)
+        for (int sub = 0; sub < bps; sub++) order[k++] = (x2s + ns2 * x3) * bps + sub;
+  }
+  return k == nblk;
+}
+
 int alloc_reduction(State& s) {
   const int nb = functional_blocks(s.geo.nh) + 8;
   // partials: max over Landau (1 segment) and Coulomb (T segments)
@@ -114,6 +143,14 @@ int alloc_reduction(State& s) {
   if (dev_alloc((void**)&s.active, (size_t)s.geo.L[3], "active")) return GFX_ERR_ALLOC;
   s.ncheck = (s.geo.nh + 63) / 64;  // blocks of the smallest sweep workgroup
   if (dev_alloc((void**)&s.cpart, sizeof(double) * 6 * s.ncheck, "check partials")) return GFX_ERR_ALLOC;
+  for (int bi = 0; bi < 2; bi++)
+    for (int t = 0; t < TILINGS; t++) {
+      std::vector<int> order, map;
+      s.tiling_ok[bi][t] = block_order(s.geo, bi == 0 ? 256 : 64, t, order);
+      deal_to_xcds(order, map);
+      if (dev_alloc((void**)&s.bmap[bi][t], sizeof(int) * map.size(), "workgroup map")) return GFX_ERR_ALLOC;
+      hipMemcpy(s.bmap[bi][t], map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice);
+    }
   return GFX_OK;
 }
 
@@ -177,6 +214,11 @@ void free_all(State& s) {
   hipFree(s.g32);
   hipFree(s.partials);
   hipFree(s.cpart);
+  for (int bi = 0; bi < 2; bi++)
+    for (int t = 0; t < TILINGS; t++) {
+      hipFree(s.bmap[bi][t]);
+      s.bmap[bi][t] = nullptr;
+    }
   hipFree(s.results);
   hipFree(s.active);
   hipFree(s.staging);
@@ -220,8 +262,12 @@ void reset_tuning(State& s) {
     for (int b = 0; b < 2; b++) {
       s.tuned[a][b] = false;
       s.variant[a][b] = 0;
-      for (int k = 0; k < 8; k++) s.tune_ms[a][b][k] = 0.f;
+      for (int k = 0; k < SWEEP_VARIANTS; k++) s.tune_ms[a][b][k] = 0.f;
     }
+}
+
+const int* sweep_map(State& s, int v) {
+  return s.bmap[block_index(sweep_variant_block(v))][sweep_variant_tiling(v)];
 }
 
 double2* field64(State& s, int k) { return s.g64 + (size_t)k * 6 * s.geo.stride; }
@@ -235,12 +281,13 @@ void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* ac
               int chk) {
   const int v = s.variant[fp32 ? 1 : 0][ndir == 4 ? 1 : 0];
   double* part = (chk == 2) ? s.cpart + 3 * (size_t)s.ncheck : s.cpart;
+  const int* map = sweep_map(s, v);
   if (fp32)
     launch_sweep_f32(s.U32, field32(s, even), field32(s, 1), s.geo, parity, ndir, s.omega, active,
-                     v, s.stream, chk, field32(s, 2 - even), part);
+                     v, map, s.stream, chk, field32(s, 2 - even), part);
   else
     launch_sweep_f64(s.U64, field64(s, even), field64(s, 1), s.geo, parity, ndir, s.omega, active,
-                     v, s.stream, chk, field64(s, 2 - even), part);
+                     v, map, s.stream, chk, field64(s, 2 - even), part);
 }
 void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* active) {
   do_sweep(s, fp32, parity, ndir, active, 0, 0);
@@ -277,20 +324,22 @@ int tune_sweep(State& s, bool fp32, int ndir) {
   float best = 1e30f;
   int bestv = s.variant[pi][di];
   for (int v = 0; v < SWEEP_VARIANTS; v++) {
+    if (!s.tiling_ok[block_index(sweep_variant_block(v))][sweep_variant_tiling(v)]) continue;
+    const int* map = sweep_map(s, v);
     for (int rep = 0; rep < 2; rep++)
       for (int p = 0; p < 2; p++) {
         if (fp32)
-          launch_sweep_f32(s.U32, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
+          launch_sweep_f32(s.U32, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
         else
-          launch_sweep_f64(s.U64, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
+          launch_sweep_f64(s.U64, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
       }
     hipEventRecord(s.ev0, s.stream);
     for (int rep = 0; rep < 3; rep++)
       for (int p = 0; p < 2; p++) {
         if (fp32)
-          launch_sweep_f32(s.U32, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
+          launch_sweep_f32(s.U32, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
         else
-          launch_sweep_f64(s.U64, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, s.stream);
+          launch_sweep_f64(s.U64, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
       }
     hipEventRecord(s.ev1, s.stream);
     if (hip_check(hipEventSynchronize(s.ev1), "tune")) break;

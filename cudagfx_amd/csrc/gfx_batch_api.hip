@@ -39,9 +39,9 @@ struct BProbHost {
 struct Batch {
   std::vector<BProbHost> probs;
   BatchProblem* d_probs = nullptr;
-  int2* d_bmap[3] = {nullptr, nullptr, nullptr};
-  int nbmap[3] = {0, 0, 0};
-  int* d_poff[3] = {nullptr, nullptr, nullptr};  // check-partial offsets per problem
+  int2* d_bmap[2][TILINGS] = {};  // per [block 256/64][tiling]
+  int nbmap[2] = {0, 0};
+  int* d_poff[2] = {nullptr, nullptr};  // check-partial offsets per problem
   double* d_podd = nullptr;   // check partials of the odd sweeps
   double* d_peven = nullptr;  // and of the even sweeps
   double* d_best = nullptr;   // per problem: best max dAdA of the phase
@@ -54,7 +54,7 @@ struct Batch {
   bool built = false;
   int variant[2] = {0, 0};
   bool tuned[2] = {false, false};
-  float tune_ms[2][8] = {};
+  float tune_ms[2][SWEEP_VARIANTS] = {};
   // statistics of the last run
   double run_ms = 0.0;
   int graph_launches = 0, iters32 = 0, iters64 = 0;
@@ -62,19 +62,17 @@ struct Batch {
 };
 static Batch g_b;
 
-static const int BLOCKS[3] = {256, 128, 64};
+static const int BLOCKS[2] = {256, 64};
 
-static int vblock_index(int variant) {
-  const int b = sweep_variant_block(variant);
-  return b == 256 ? 0 : (b == 128 ? 1 : 2);
-}
-static_assert(SWEEP_VARIANTS <= 8, "tune_ms holds 8 variants");
+static int vblock_index(int variant) { return block_index(sweep_variant_block(variant)); }
 
 static void batch_free_maps(Batch& b) {
-  for (int k = 0; k < 3; k++) {
-    hipFree(b.d_bmap[k]);
+  for (int k = 0; k < 2; k++) {
+    for (int t = 0; t < TILINGS; t++) {
+      hipFree(b.d_bmap[k][t]);
+      b.d_bmap[k][t] = nullptr;
+    }
     hipFree(b.d_poff[k]);
-    b.d_bmap[k] = nullptr;
     b.d_poff[k] = nullptr;
   }
   hipFree(b.d_probs);
@@ -115,37 +113,40 @@ static void batch_free(Batch& b) {
   for (int k = 0; k < 2; k++) b.tuned[k] = false;
 }
 
-// block maps: Gribov copies sharing links are interleaved so that the
+// block maps: every lattice's blocks in the workgroup order of the tiling
+// (block_order); Gribov copies sharing links are interleaved so that the
 // blocks covering the same sites of every copy run together and share the
-// link lines in L2 / Infinity Cache; lattices with their own links follow
-// one another.
+// link lines in L2 / Infinity Cache; lattices with their own links follow one
+// another; the whole list is dealt to the XCDs in contiguous eighths.
 static int batch_build(Batch& b) {
   batch_free_maps(b);
   const int n = (int)b.probs.size();
-  for (int k = 0; k < 3; k++) {
-    std::vector<int2> map;
-    std::vector<bool> done(n, false);
-    for (int q0 = 0; q0 < n; q0++) {
-      if (done[q0]) continue;
-      std::vector<int> group;
-      for (int q = q0; q < n; q++)
-        if (b.probs[q].ushare == b.probs[q0].ushare) {
-          group.push_back(q);
-          done[q] = true;
-        }
-      int nbmax = 0;
-      for (int q : group) nbmax = std::max(nbmax, (b.probs[q].geo.nh + BLOCKS[k] - 1) / BLOCKS[k]);
-      for (int blk = 0; blk < nbmax; blk++)
-        for (int q : group)
-          if (blk < (b.probs[q].geo.nh + BLOCKS[k] - 1) / BLOCKS[k]) map.push_back(make_int2(q, blk));
+  for (int k = 0; k < 2; k++) {
+    for (int t = 0; t < TILINGS; t++) {
+      std::vector<int2> list;
+      std::vector<bool> done(n, false);
+      for (int q0 = 0; q0 < n; q0++) {
+        if (done[q0]) continue;
+        std::vector<int> group;
+        for (int q = q0; q < n; q++)
+          if (b.probs[q].ushare == b.probs[q0].ushare) {
+            group.push_back(q);
+            done[q] = true;
+          }
+        std::vector<int> order;
+        block_order(b.probs[q0].geo, BLOCKS[k], t, order);
+        for (int blk : order)
+          for (int q : group) list.push_back(make_int2(q, blk));
+      }
+      std::vector<int2> map;
+      deal_to_xcds(list, map);
+      b.nbmap[k] = (int)map.size();
+      if (dev_alloc((void**)&b.d_bmap[k][t], sizeof(int2) * map.size(), "batch map")) return GFX_ERR_ALLOC;
+      hipMemcpy(b.d_bmap[k][t], map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice);
     }
-    b.nbmap[k] = (int)map.size();
     std::vector<int> poff(n + 1, 0);
     for (int q = 0; q < n; q++) poff[q + 1] = poff[q] + (b.probs[q].geo.nh + BLOCKS[k] - 1) / BLOCKS[k];
-    if (dev_alloc((void**)&b.d_bmap[k], sizeof(int2) * map.size(), "batch map") ||
-        dev_alloc((void**)&b.d_poff[k], sizeof(int) * (n + 1), "batch offsets"))
-      return GFX_ERR_ALLOC;
-    hipMemcpy(b.d_bmap[k], map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice);
+    if (dev_alloc((void**)&b.d_poff[k], sizeof(int) * (n + 1), "batch offsets")) return GFX_ERR_ALLOC;
     hipMemcpy(b.d_poff[k], poff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice);
   }
   // partial slots: as many as blocks of the smallest workgroup
@@ -224,8 +225,9 @@ static int ensure_fp32(Batch& b) {
 static void batch_half(Batch& b, bool fp32, int parity, int variant, int chk) {
   State& s = g_st;
   const int k = vblock_index(variant);
-  launch_batch_sweep(fp32, b.d_probs, b.d_bmap[k], b.nbmap[k], b.d_active, b.d_poff[k], parity,
-                     s.omega, variant, chk, chk == 2 ? b.d_peven : b.d_podd, s.stream);
+  launch_batch_sweep(fp32, b.d_probs, b.d_bmap[k][sweep_variant_tiling(variant)], b.nbmap[k],
+                     b.d_active, b.d_poff[k], parity, s.omega, variant, chk,
+                     chk == 2 ? b.d_peven : b.d_podd, s.stream);
 }
 
 // the end of a check at iteration *d_iter: the even sweep of the next

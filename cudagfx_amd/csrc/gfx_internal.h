@@ -3,28 +3,60 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "gfx_lattice.h"
 
 namespace gfx {
 
-// sweep variants 0..7 (see k_or_sweep): block size, rolled/unrolled star
-// loop, non-temporal link loads
-constexpr int SWEEP_VARIANTS = 8;
+// sweep variants: bits 0-2 pick the kernel (see k_or_sweep: block size,
+// rolled/unrolled star loop, non-temporal link loads), bits 3-4 the
+// workgroup order (block_order tiling 0..3)
+constexpr int KERNEL_VARIANTS = 8;
+constexpr int TILINGS = 4;
+constexpr int SWEEP_VARIANTS = KERNEL_VARIANTS * TILINGS;
 int sweep_variant_block(int v);
 bool sweep_variant_rolled(int v);
 bool sweep_variant_nt(int v);
+inline int sweep_variant_tiling(int v) { return v >> 3; }
+inline int block_index(int block) { return block == 256 ? 0 : 1; }
+
+// Workgroup order of a sweep over one lattice (local block indices, block
+// sites each).  tile 0: lexicographic.  tile t > 0: a superrow is the G
+// consecutive (x2, x3) rows (h0*L1 sites each) that make a whole number of
+// blocks; superrows are taken in chunks of 2^t consecutive x2-superrows,
+// x3 outermost inside a chunk, so that the x3 neighbours of a block were
+// read 2^t superrows earlier and are still in L2.  Returns false (and the
+// lexicographic order) when the lattice does not split into superrows.
+bool block_order(const Geom& geo, int block, int tile, std::vector<int>& order);
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2): launch position p runs on XCD p % 8.  Deal `in` so that XCD k works
+// through the k-th contiguous eighth of it in order.
+template <typename T>
+void deal_to_xcds(const std::vector<T>& in, std::vector<T>& out) {
+  const size_t n = in.size();
+  out.resize(n);
+  size_t start = 0;
+  for (size_t k = 0; k < 8; k++) {
+    const size_t cnt = n / 8 + (k < n % 8 ? 1 : 0);
+    for (size_t j = 0; j < cnt; j++) out[k + 8 * j] = in[start + j];
+    start += cnt;
+  }
+}
 // one half sweep over the fields ge (even) and go (odd) of g.  chk (Landau
 // only): 0 plain in-place sweep; 1 check after the update (odd sweep of a
 // check iteration); 2 check before the update, new field written to galt
 // (even sweep that follows it).  Checks write 3 doubles per block to
 // partials (sweep_blocks(nh, variant) blocks).
+// bmap: launch position -> local block (block_order dealt to the XCDs)
 void launch_sweep_f64(const double2* U, double2* ge, double2* go, const Geom& geo, int parity,
                       int ndir, double omega, const unsigned char* active, int variant,
-                      hipStream_t st, int chk = 0, double2* galt = nullptr,
+                      const int* bmap, hipStream_t st, int chk = 0, double2* galt = nullptr,
                       double* partials = nullptr);
 void launch_sweep_f32(const float4* U, float4* ge, float4* go, const Geom& geo, int parity,
                       int ndir, double omega, const unsigned char* active, int variant,
-                      hipStream_t st, int chk = 0, float4* galt = nullptr,
+                      const int* bmap, hipStream_t st, int chk = 0, float4* galt = nullptr,
                       double* partials = nullptr);
 int sweep_blocks(int nh, int variant);
 // grid pass of a fused check: out[0..3] = F/normF, dAdA/normD, max, max < eps

@@ -161,12 +161,14 @@ __device__ __forceinline__ void store_check_partials(double F, double d, double*
 
 // ge / go: even and odd fields of g.  CHK = 2 writes the new own field to
 // galt; otherwise the sweep is in place.
+// bmap[launch position] = block of the lattice (workgroup order)
 template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK>
 __global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* ge, V* go, V* galt,
                                                      Geom geo, int parity, R c1, R c2,
                                                      const unsigned char* __restrict__ active,
+                                                     const int* __restrict__ bmap,
                                                      double* __restrict__ partials) {
-  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int b = bmap[blockIdx.x];
   const int i = b * BLOCK + threadIdx.x;
   V* gme = parity ? go : ge;
   const V* gop = parity ? ge : go;
@@ -570,6 +572,7 @@ struct SweepArgs {
   Geom geo;
   int parity;
   const unsigned char* active;
+  const int* bmap;
   double* partials;
 };
 
@@ -577,7 +580,8 @@ template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int
 static void launch_sweep_v(const SweepArgs<V>& a, R c1, R c2, hipStream_t st) {
   dim3 grid((a.geo.nh + BLOCK - 1) / BLOCK);
   hipLaunchKernelGGL((k_or_sweep<V, R, NDIR, BLOCK, ROLLED, NT, CHK>), grid, dim3(BLOCK), 0, st,
-                     a.U, a.ge, a.go, a.galt, a.geo, a.parity, c1, c2, a.active, a.partials);
+                     a.U, a.ge, a.go, a.galt, a.geo, a.parity, c1, c2, a.active, a.bmap,
+                     a.partials);
 }
 
 template <typename V, typename R, int NDIR, int CHK>
@@ -617,15 +621,15 @@ static void launch_sweep_t(const SweepArgs<V>& a, int ndir, double omega, int va
 
 void launch_sweep_f64(const double2* U, double2* ge, double2* go, const Geom& geo, int parity,
                       int ndir, double omega, const unsigned char* active, int variant,
-                      hipStream_t st, int chk, double2* galt, double* partials) {
-  launch_sweep_t<double2, double>({U, ge, go, galt, geo, parity, active, partials}, ndir, omega,
-                                  variant, chk, st);
+                      const int* bmap, hipStream_t st, int chk, double2* galt, double* partials) {
+  launch_sweep_t<double2, double>({U, ge, go, galt, geo, parity, active, bmap, partials}, ndir,
+                                  omega, variant, chk, st);
 }
 void launch_sweep_f32(const float4* U, float4* ge, float4* go, const Geom& geo, int parity,
                       int ndir, double omega, const unsigned char* active, int variant,
-                      hipStream_t st, int chk, float4* galt, double* partials) {
-  launch_sweep_t<float4, float>({U, ge, go, galt, geo, parity, active, partials}, ndir, omega,
-                                variant, chk, st);
+                      const int* bmap, hipStream_t st, int chk, float4* galt, double* partials) {
+  launch_sweep_t<float4, float>({U, ge, go, galt, geo, parity, active, bmap, partials}, ndir,
+                                omega, variant, chk, st);
 }
 
 int sweep_blocks(int nh, int variant) {

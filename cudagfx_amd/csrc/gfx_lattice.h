@@ -214,12 +214,4 @@ struct BatchProblem {
   void* galt;
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-// L2).  Remap so that XCD k works on a contiguous range of sites and shares
-// neighbour g(x +- mu) lines in its own L2.
-__device__ __forceinline__ int xcd_block(int b, int nblk) {
-  if ((nblk & 7) != 0) return b;
-  return (b & 7) * (nblk >> 3) + (b >> 3);
-}
-
 }  // namespace gfx

@@ -60,8 +60,12 @@ struct State {
   // autotuner on first use for the current lattice
   int variant[2][2] = {{0, 0}, {0, 0}};
   bool tuned[2][2] = {{false, false}, {false, false}};
-  float tune_ms[2][2][8] = {};
+  float tune_ms[2][2][32] = {};
   int autotune = 1;
+  // workgroup maps per [block 256/64][tiling] (device), and whether the
+  // tiling applies to this lattice
+  int* bmap[2][4] = {};
+  bool tiling_ok[2][4] = {};
 
   // last run statistics
   double run_ms = 0.0;

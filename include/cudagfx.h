@@ -162,12 +162,14 @@ int gfx_stream_copy(size_t bytes, int nrep, double *ms);
 
 /* Sweep-kernel autotuner: on the first sweep of a lattice in a precision and
  * gauge, every variant is timed on a scratch copy of g and the fastest is
- * kept.  Variant v (0..7): workgroup size 64 if bit 0 else 256, star loop
- * unrolled if bit 1 else rolled, non-temporal link loads if bit 2.  on = 0
- * keeps variant 0 (256, rolled) or the one set by gfx_set_sweep_variant. */
+ * kept.  Variant v (0..31): workgroup size 64 if bit 0 else 256, star loop
+ * unrolled if bit 1 else rolled, non-temporal link loads if bit 2, and bits
+ * 3-4 the workgroup order: 0 lexicographic, t > 0 chunks of 2^t x2-rows
+ * with x3 outermost (variants whose order does not fit the lattice are not
+ * timed).  on = 0 keeps variant 0 or the one set by gfx_set_sweep_variant. */
 int gfx_set_autotune(int on);
-/* the chosen variant and the measured ms per half sweep of each of the 8
- * variants; GFX_ERR_STATE if not tuned yet for this lattice */
+/* the chosen variant and the measured ms per half sweep of each of the 32
+ * variants (0 when not timed); GFX_ERR_STATE if not tuned yet */
 int gfx_get_sweep_variant(int precision, int ndir, int *variant, int *block, int *rolled,
                           double *ms_per_variant);
 int gfx_set_sweep_variant(int precision, int ndir, int variant);

@@ -262,3 +262,26 @@ def test_mixed_stall_guard_leaves_fp32(gfx, orc):
     assert it64 > 0 and itm > 0 and Mm < EPS12
     assert i32 > 0 and i64 > 0
     assert abs(Fm - F64) / F64 < 1e-10
+
+
+def test_every_sweep_variant_matches_oracle(gfx, orc):
+    """All 32 sweep variants (kernel form x workgroup order) give the same
+    half sweeps; 16^4 admits every workgroup order."""
+    dims = (16, 16, 16, 16)
+    V = orc.volume(dims)
+    u = orc.random_init_gauge(V, 4361)
+    g = orc.random_init_trafo(V, 4362)
+    u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+    for parity in (0, 1):
+        orc.halfsweep(u2, g2, dims, parity, 4, 1.68)
+    g_cpu = orc.from_2v(g2)
+    _setup(gfx, dims, u, g)
+    worst = 0.0
+    for v in range(32):
+        gfx.set_sweep_variant(gfx.PREC_FP64, 4, v)
+        gfx.set_trafo(g)
+        gfx.halfsweep(0, 4)
+        gfx.halfsweep(1, 4)
+        worst = max(worst, np.abs(gfx.get_trafo() - g_cpu).max())
+        assert worst < 1e-13, v
+    gfx.finalize_gaugefixing()
