@@ -150,8 +150,11 @@ def align_global(g_a, g_b):
 
 def profile_run(gfx):
     """The kernels of the timed region in isolation, for counter collection:
-    autotuned FP64 and FP32 sweeps (20 each), FP64 checks, and a plain copy of
-    512 MiB whose bytes are known."""
+    the autotuner's variant timing for both precisions, then 20 FP64 and 20
+    FP32 sweeps of the picked variants (the last 40 launches of each picked
+    kernel), 5 standalone FP64 checks, a 31-iteration FP64 run with fused
+    checks every 10 iterations, and a plain copy of 512 MiB whose bytes are
+    known."""
     global NTHERM
     NTHERM = 20  # counters do not depend on the field values
     u, _ = thermalized_config(gfx)
@@ -160,10 +163,15 @@ def profile_run(gfx):
     g0 = gfx.random_init_trafo_host(V)
     gfx.init_gaugefixing(u, g0)
     gfx.set_omega(OMEGA)
+    gfx.time_sweeps(1, gfx.PREC_FP64)
+    gfx.time_sweeps(1, gfx.PREC_MIXED)
     t64 = gfx.time_sweeps(20, gfx.PREC_FP64)
     t32 = gfx.time_sweeps(20, gfx.PREC_MIXED)
     for _ in range(5):
         gfx.calc_functional()
+    gfx.set_trafo(g0)
+    gfx.set_precision(gfx.PREC_FP64)
+    gfx.overrelax_gauge(31, EPS, CK)
     tc = gfx.stream_copy(512 << 20, 5)
     out = {"profile": True, "fp64_sweep_ms": t64, "fp32_sweep_ms": t32, "copy_512MiB_ms": tc,
            "variant_fp64": list(gfx.sweep_variant(gfx.PREC_FP64, 4)[:3]),
@@ -179,8 +187,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
-                    help="short run for rocprofv3: 20 sweeps of each precision, 5 checks, "
-                         "and a 512 MiB streaming copy as the byte-counter reference")
+                    help="short run for rocprofv3: autotuning, 20 sweeps of each precision, "
+                         "checks, and a 512 MiB streaming copy as the byte-counter reference")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; the path would shard by "
@@ -199,7 +207,10 @@ def main():
     u, plaq = thermalized_config(gfx)
     gfx.plant_seeds(RANDSEED)
     g0 = gfx.random_init_trafo_host(V)
+    t_up = time.perf_counter()
     gfx.init_gaugefixing(u, g0)
+    gfx.sync()
+    t_up = time.perf_counter() - t_up
     gfx.set_omega(OMEGA)
     # pick the sweep variants (autotuner) before anything is timed
     gfx.time_sweeps(1, gfx.PREC_FP64)
@@ -228,7 +239,9 @@ def main():
     for _ in range(args.steps):
         last = run(gfx.PREC_MIXED)
         walls.append(last["wall_s"])
+    t_down = time.perf_counter()
     g_mixed = gfx.get_trafo()
+    t_down = time.perf_counter() - t_down
     ms_step = 1000.0 * max(walls)
     # the same run with the switch at 1e-10 (FP32 rounding floor of this
     # lattice ~5e-11; the stall guard covers lattices whose floor is higher)
@@ -298,6 +311,9 @@ def main():
                                    "theta_mixed": last["max"], "theta_fp64": r64["max"],
                                    "g_maxabs_after_global_rotation": link_err},
         "autotuned_sweep": variants,
+        "host_boundary": {"upload_links_and_g_s": t_up, "download_g_s": t_down,
+                          "pcie_inclusive_site_updates_per_s":
+                              iters * V / (ms_step / 1000.0 + t_up + t_down)},
         "plaquette": plaq,
         "setup_s": setup_s,
         "roofline": roof,
