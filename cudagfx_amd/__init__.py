@@ -119,6 +119,9 @@ def load():
                                       C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_double)]),
         "gfx_batch_variant": (C.c_int, [C.c_int, C.POINTER(C.c_int), vp]),
+        "gfx_batch_anneal": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                       C.c_int, C.c_int]),
+        "gfx_batch_best_copy": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
         "gfx_get_sweep_variant": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int),
                                             C.POINTER(C.c_int), C.POINTER(C.c_int), vp]),
         "gfx_set_sweep_variant": (C.c_int, [C.c_int, C.c_int, C.c_int]),
@@ -382,6 +385,18 @@ class Batch:
         ms = np.zeros(32)
         rc = load().gfx_batch_variant(int(precision), C.byref(v), ms.ctypes.data_as(C.c_void_p))
         return (v.value, ms) if rc == 0 else None
+
+    def anneal(self, nsteps, Tmin, Tmax, expo=-1.0, nmicro=0, sampler=0, steps_per_graph=8):
+        """Simulated annealing of every lattice's g (see gfx_batch_anneal)."""
+        _check(load().gfx_batch_anneal(int(nsteps), float(Tmin), float(Tmax), float(expo),
+                                       int(nmicro), int(sampler), int(steps_per_graph)),
+               "batch_anneal")
+
+    def best_copy(self, base):
+        """(index, F) of the lattice with the largest F among the copies of base."""
+        q, F = C.c_int(), C.c_double()
+        _check(load().gfx_batch_best_copy(int(base), C.byref(q), C.byref(F)), "best_copy")
+        return q.value, F.value
 
     def close(self):
         load().gfx_batch_destroy()

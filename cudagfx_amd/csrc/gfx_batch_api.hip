@@ -44,6 +44,8 @@ struct Batch {
   int* d_poff[2] = {nullptr, nullptr};  // check-partial offsets per problem
   double* d_podd = nullptr;   // check partials of the odd sweeps
   double* d_peven = nullptr;  // and of the even sweeps
+  int* d_roff = nullptr;      // per problem: first site of its random numbers
+  int nh_total = 0;           // sites per parity over all problems
   double* d_best = nullptr;   // per problem: best max dAdA of the phase
   int* d_stall = nullptr;     // checks since it last halved
   int* d_active = nullptr;
@@ -79,6 +81,7 @@ static void batch_free_maps(Batch& b) {
   hipFree(b.d_podd);
   hipFree(b.d_peven);
   hipFree(b.d_best);
+  hipFree(b.d_roff);
   hipFree(b.d_stall);
   hipFree(b.d_active);
   hipFree(b.d_results);
@@ -89,6 +92,7 @@ static void batch_free_maps(Batch& b) {
   b.d_podd = nullptr;
   b.d_peven = nullptr;
   b.d_best = nullptr;
+  b.d_roff = nullptr;
   b.d_stall = nullptr;
   b.d_active = nullptr;
   b.d_results = nullptr;
@@ -149,6 +153,11 @@ static int batch_build(Batch& b) {
     if (dev_alloc((void**)&b.d_poff[k], sizeof(int) * (n + 1), "batch offsets")) return GFX_ERR_ALLOC;
     hipMemcpy(b.d_poff[k], poff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice);
   }
+  std::vector<int> roff(n + 1, 0);
+  for (int q = 0; q < n; q++) roff[q + 1] = roff[q] + b.probs[q].geo.nh;
+  b.nh_total = roff[n];
+  if (dev_alloc((void**)&b.d_roff, sizeof(int) * (n + 1), "batch random offsets")) return GFX_ERR_ALLOC;
+  hipMemcpy(b.d_roff, roff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice);
   // partial slots: as many as blocks of the smallest workgroup
   size_t npart = 0;
   for (int q = 0; q < n; q++) npart += (b.probs[q].geo.nh + 63) / 64;
@@ -482,6 +491,103 @@ int gfx_batch_run(int maxit, double eps, int checkinterval, int precision, doubl
   }
   if (rc != GFX_OK) return rc;
   return nconv == (int)b.probs.size() ? GFX_OK : GFX_ERR_NOCONV;
+}
+
+// Simulated annealing of every lattice's g (simulated_annealing.cu:309-476
+// per lattice): nsteps steps of an even and an odd heatbath half sweep at
+// T(i) = gfx_sa_temperature(i, Tmin, Tmax, expo, nsteps), each half sweep
+// followed by a refill of the twister fields, then nmicro microcanonical
+// sweeps.  One launch per half sweep covers the batch; the random numbers of
+// lattice q start at its site offset.  Steps run as replays of a graph of
+// steps_per_graph steps, the temperature schedule and step counter on the
+// device.
+int gfx_batch_anneal(int nsteps, double Tmin, double Tmax, double expo, int nmicro, int sampler,
+                     int steps_per_graph) {
+  State& s = g_st;
+  Batch& b = g_b;
+  if (b.probs.empty() || nsteps < 0) return GFX_ERR_STATE;
+  if (steps_per_graph < 1) steps_per_graph = 1;
+  if (!b.built) {
+    int rc = batch_build(b);
+    if (rc) return rc;
+  }
+  sampler = sampler ? 1 : 0;
+  nmicro = nmicro < 0 ? 0 : nmicro;
+  const long ng = (long)(sampler ? kp_gauss_per_site() : 1) * b.nh_total;
+  const long nu = (long)(sampler ? kp_unif_per_site() : 4) * b.nh_total;
+  if (nu > (1L << 31) - 4096 * 2) return GFX_ERR_GEOMETRY;
+  int rc = mt_prepare(s, (int)ng, (int)nu);
+  if (rc) return rc;
+  set_phase_pointers(b, false);
+  std::vector<double> betas(nsteps > 0 ? nsteps : 1);
+  for (int i = 0; i < nsteps; i++) betas[i] = 1.0 / gfx_sa_temperature(i, Tmin, Tmax, expo, nsteps);
+  double* d_betas = nullptr;
+  int* d_step = nullptr;
+  if (dev_alloc((void**)&d_betas, sizeof(double) * betas.size(), "SA schedule") ||
+      dev_alloc((void**)&d_step, sizeof(int), "SA step")) {
+    hipFree(d_betas);
+    return GFX_ERR_ALLOC;
+  }
+  hipMemcpy(d_betas, betas.data(), sizeof(double) * betas.size(), hipMemcpyHostToDevice);
+  launch_fill_i32(d_step, 1, 0, s.stream);
+  const int k = block_index(sa_block_size());
+  const int2* map = b.d_bmap[k][0];
+  const int nblk = b.nbmap[k];
+  auto one_step = [&]() {
+    for (int p = 0; p < 2; p++) {
+      launch_batch_sa_sweep(b.d_probs, map, nblk, b.d_roff, p, d_betas, d_step, sampler,
+                            s.rnd_gauss, s.rnd_unif, s.stream);
+      mt_refill(s, 1);
+    }
+    for (int m = 0; m < nmicro; m++)
+      for (int p = 0; p < 2; p++) launch_batch_micro_sweep(b.d_probs, map, nblk, p, s.stream);
+    launch_batch_advance(d_step, 1, s.stream);
+  };
+  const auto t0 = std::chrono::steady_clock::now();
+  int done = 0;
+  if (nsteps >= steps_per_graph) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (hip_check(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal), "capture")) rc = GFX_ERR_DEVICE;
+    if (rc == GFX_OK) {
+      for (int k2 = 0; k2 < steps_per_graph; k2++) one_step();
+      if (hip_check(hipStreamEndCapture(s.stream, &graph), "end capture") ||
+          hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate"))
+        rc = GFX_ERR_DEVICE;
+    }
+    while (rc == GFX_OK && done + steps_per_graph <= nsteps) {
+      if (hip_check(hipGraphLaunch(exec, s.stream), "graph launch")) rc = GFX_ERR_DEVICE;
+      done += steps_per_graph;
+      b.graph_launches++;
+    }
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+  }
+  while (rc == GFX_OK && done < nsteps) {
+    one_step();
+    done++;
+  }
+  if (hip_check(hipStreamSynchronize(s.stream), "anneal") && rc == GFX_OK) rc = GFX_ERR_DEVICE;
+  b.run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  hipFree(d_betas);
+  hipFree(d_step);
+  return rc;
+}
+
+// the lattice with the largest F among those sharing lattice base's links
+// (after gfx_batch_run): the Gribov copy to select
+int gfx_batch_best_copy(int base, int* q_best, double* F_best) {
+  if (base < 0 || base >= (int)g_b.probs.size() || !g_b.h_results) return GFX_ERR_STATE;
+  int best = -1;
+  double fb = -1e300;
+  for (int q = 0; q < (int)g_b.probs.size(); q++)
+    if (g_b.probs[q].ushare == g_b.probs[base].ushare && g_b.h_results[4 * q] > fb) {
+      fb = g_b.h_results[4 * q];
+      best = q;
+    }
+  if (q_best) *q_best = best;
+  if (F_best) *F_best = fb;
+  return GFX_OK;
 }
 
 // F, dAdA, max dAdA and the stop iteration (-1: not converged) of lattice q

@@ -6,6 +6,9 @@
 //                   applied to g)
 //   k_link_heatbath SU(3) link heatbath of the Wilson action with the
 //                   staple of heatbath_thermalization.cu:235-308
+//   k_batch_sa_sweep / k_batch_micro_sweep   the same updates of g over
+//                   every lattice of a batch in one launch
+//   k_haar_trafo    Haar-random g from the twisters' normals
 //
 // Two SU(2) samplers:
 //   SAMPLER_REF  the reference's rule (simulated_annealing.cu:5-73): one
@@ -179,6 +182,23 @@ __global__ __launch_bounds__(HB_BLK) void k_sa_sweep(const double2* __restrict__
   Store<double2>::store(gme, S, i, h);
 }
 
+// microcanonical reflection in each SU(2) subgroup:
+// heatbath_thermalization.cu:318-338, alpha = 2 w0 w/|w|^2 - e0
+__device__ __forceinline__ void micro_reflect(M3<double>& h, const M3<double>& K) {
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = a + 1; b < 3; b++) {
+      double w[4];
+      su2_subvec(h, K, a, b, w[0], w[1], w[2], w[3]);
+      const double ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
+      if (!(fabs(ww) < 1.0e-9)) {
+        const double f = 2.0 * w[0] / ww;
+        su2_left(h, a, b, f * w[0] - 1.0, f * w[1], f * w[2], f * w[3]);
+      }
+    }
+}
+
 template <int NDIR>
 __global__ __launch_bounds__(HB_BLK) void k_micro_sweep(const double2* __restrict__ U,
                                                         double2* __restrict__ g, Geom geo,
@@ -191,21 +211,55 @@ __global__ __launch_bounds__(HB_BLK) void k_micro_sweep(const double2* __restric
                                      geo, s, i, parity);
   double2* gme = field_ptr(g, parity, S);
   M3<double> h = load_m<double2, double>(gme, S, i);
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = a + 1; b < 3; b++) {
-      double w[4];
-      su2_subvec(h, K, a, b, w[0], w[1], w[2], w[3]);
-      const double ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
-      // heatbath_thermalization.cu:318-338: alpha = 2 w0 w/|w|^2 - e0
-      if (!(fabs(ww) < 1.0e-9)) {
-        const double f = 2.0 * w[0] / ww;
-        su2_left(h, a, b, f * w[0] - 1.0, f * w[1], f * w[2], f * w[3]);
-      }
-    }
+  micro_reflect(h, K);
   m_normalize(h);
   Store<double2>::store(gme, S, i, h);
+}
+
+// ---- batched annealing of many lattices (Gribov copies, batches):
+// bmap[block] = (lattice, block of HB_BLK sites of the lattice); the random
+// numbers of lattice q start at site roff[q] of the fields; the inverse
+// temperature of the current step is betas[*step] (step counter on the
+// device, so a captured graph of several steps replays unchanged).
+template <int SAMPLER>
+__global__ __launch_bounds__(HB_BLK) void k_batch_sa_sweep(const BatchProblem* __restrict__ probs,
+                                                           const int2* __restrict__ bmap,
+                                                           const int* __restrict__ roff, int parity,
+                                                           const double* __restrict__ betas,
+                                                           const int* __restrict__ step,
+                                                           const float* __restrict__ gauss,
+                                                           const float* __restrict__ unif) {
+  const int2 m = bmap[blockIdx.x];
+  const BatchProblem& P = probs[m.x];
+  const Geom geo = P.geo;
+  const int i = m.y * HB_BLK + threadIdx.x;
+  if (i >= geo.nh) return;
+  const Site s = site_coords(geo, i, parity);
+  double2* gme = static_cast<double2*>(P.g[parity]);
+  const M3<double> K = star_of<4>(static_cast<const double2*>(P.U),
+                                  static_cast<const double2*>(P.g[1 - parity]), geo, s, i, parity);
+  M3<double> h = load_m<double2, double>(gme, geo.stride, i);
+  cm_heatbath<SAMPLER>(h, K, betas[*step], gauss, unif, roff[m.x] + i);
+  m_normalize(h);
+  Store<double2>::store(gme, geo.stride, i, h);
+}
+
+__global__ __launch_bounds__(HB_BLK) void k_batch_micro_sweep(const BatchProblem* __restrict__ probs,
+                                                              const int2* __restrict__ bmap,
+                                                              int parity) {
+  const int2 m = bmap[blockIdx.x];
+  const BatchProblem& P = probs[m.x];
+  const Geom geo = P.geo;
+  const int i = m.y * HB_BLK + threadIdx.x;
+  if (i >= geo.nh) return;
+  const Site s = site_coords(geo, i, parity);
+  double2* gme = static_cast<double2*>(P.g[parity]);
+  const M3<double> K = star_of<4>(static_cast<const double2*>(P.U),
+                                  static_cast<const double2*>(P.g[1 - parity]), geo, s, i, parity);
+  M3<double> h = load_m<double2, double>(gme, geo.stride, i);
+  micro_reflect(h, K);
+  m_normalize(h);
+  Store<double2>::store(gme, geo.stride, i, h);
 }
 
 // staple of U_nu(x): sum_{mu != nu} U_mu(x+nu) U_nu^+(x+mu) U_mu^+(x)
@@ -313,6 +367,23 @@ void launch_link_heatbath(double2* U, const Geom& geo, int parity, int nu, doubl
                      unif);
 }
 
+void launch_batch_sa_sweep(const BatchProblem* probs, const int2* bmap, int nblocks,
+                           const int* roff, int parity, const double* betas, const int* step,
+                           int sampler, const float* gauss, const float* unif, hipStream_t st) {
+  if (sampler == 0)
+    hipLaunchKernelGGL(k_batch_sa_sweep<0>, dim3(nblocks), dim3(HB_BLK), 0, st, probs, bmap, roff,
+                       parity, betas, step, gauss, unif);
+  else
+    hipLaunchKernelGGL(k_batch_sa_sweep<1>, dim3(nblocks), dim3(HB_BLK), 0, st, probs, bmap, roff,
+                       parity, betas, step, gauss, unif);
+}
+
+void launch_batch_micro_sweep(const BatchProblem* probs, const int2* bmap, int nblocks, int parity,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_micro_sweep, dim3(nblocks), dim3(HB_BLK), 0, st, probs, bmap, parity);
+}
+
+int sa_block_size() { return HB_BLK; }
 int kp_gauss_per_site() { return KP_NG; }
 int kp_unif_per_site() { return KP_NU; }
 

@@ -75,6 +75,15 @@ void launch_batch_finalize(const double* podd, const double* peven, const int* p
                            double* results, const int* iter, int maxit, double* best, int* stall,
                            int stall_checks, double stall_gate, hipStream_t st);
 void launch_batch_advance(int* iter, int step, hipStream_t st);
+// batched annealing (gfx_heatbath.hip); bmap of sa_block_size() blocks
+void launch_batch_sa_sweep(const BatchProblem* probs, const int2* bmap, int nblocks,
+                           const int* roff, int parity, const double* betas, const int* step,
+                           int sampler, const float* gauss, const float* unif, hipStream_t st);
+void launch_batch_micro_sweep(const BatchProblem* probs, const int2* bmap, int nblocks, int parity,
+                              hipStream_t st);
+int sa_block_size();
+int kp_gauss_per_site();
+int kp_unif_per_site();
 void launch_fill_i32(int* p, int n, int v, hipStream_t st);
 void launch_fill_f64(double* p, int n, double v, hipStream_t st);
 void launch_functional_f64(const double2* U, const double2* g, const Geom& geo, int ndir, int nseg,

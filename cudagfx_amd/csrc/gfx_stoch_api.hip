@@ -4,6 +4,8 @@
 // (heatbath_thermalization.cu:557-841).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -86,7 +88,7 @@ static int mt_fields(State& s, int n_gauss, int n_unif) {
 
 // RandomGPU(gauss) -> BoxMullerGPU -> RandomGPU(unif), sharing one state
 // array (MersenneTwister.cu:272-277, 300-306)
-static void mt_refill(State& s, int initialized) {
+void mt_refill(State& s, int initialized) {
   launch_mt_generate(s.mt_params, s.mt_state, s.rnd_gauss, s.n_per_gauss, initialized, s.stream);
   launch_box_muller(s.rnd_gauss, s.n_per_gauss, s.stream);
   launch_mt_generate(s.mt_params, s.mt_state, s.rnd_unif, s.n_per_unif, initialized, s.stream);
@@ -112,6 +114,19 @@ int mt_init_device(State& s, int n_gauss, int n_unif) {
   printf("No. of unif. dist. random numbers: %d\n", MT_COUNT * s.n_per_unif);
   mt_refill(s, 0);
   return hip_check(hipStreamSynchronize(s.stream), "init_MT");
+}
+
+// twisters seeded and fields holding at least need_g normals and need_u
+// uniforms (grown and refilled if they were smaller)
+int mt_prepare(State& s, int need_g, int need_u) {
+  if (!s.mt_state) return mt_init_device(s, need_g, need_u);
+  if (MT_COUNT * s.n_per_gauss < need_g || MT_COUNT * s.n_per_unif < need_u) {
+    const int rc = mt_fields(s, std::max(need_g, MT_COUNT * s.n_per_gauss),
+                             std::max(need_u, MT_COUNT * s.n_per_unif));
+    if (rc) return rc;
+    mt_refill(s, 1);
+  }
+  return GFX_OK;
 }
 
 void mt_free(State& s) {

@@ -226,6 +226,17 @@ int gfx_batch_get_trafo(int q, gfx_su3 *trafo);
 int gfx_batch_stats(double *ms, int *graph_launches, int *iters_fp32, int *iters_fp64,
                     double *site_updates);
 int gfx_batch_variant(int precision, int *variant, double *ms_per_variant);
+/* simulated annealing of every lattice's g (simannealing_gauge per lattice,
+ * simulated_annealing.cu:309-476): nsteps steps at gfx_sa_temperature(i,
+ * Tmin, Tmax, expo, nsteps), each an even and an odd heatbath half sweep
+ * (sampler 0: the reference rule, 1: Kennedy-Pendleton) with a twister refill
+ * after each, then nmicro microcanonical sweeps; one launch per half sweep
+ * over the batch, steps replayed as a HIP graph of steps_per_graph steps */
+int gfx_batch_anneal(int nsteps, double Tmin, double Tmax, double expo, int nmicro, int sampler,
+                     int steps_per_graph);
+/* after gfx_batch_run: the lattice with the largest F among those sharing
+ * the links of `base` (the Gribov copy to keep) */
+int gfx_batch_best_copy(int base, int *q_best, double *F_best);
 
 /* last error as text */
 const char *gfx_error_string(int code);

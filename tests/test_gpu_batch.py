@@ -139,3 +139,114 @@ def test_batch_state_at_maxit(gfx, orc, maxit, ck, periods):
         assert np.abs(gs[q] - orc.from_2v(g2)).max() < 1e-12
         assert abs(res[q][0] - hist[-1, 0]) < 1e-13
         assert abs(res[q][2] - hist[-1, 2]) / hist[-1, 2] < 1e-11
+
+
+def _thermalized(gfx, dims, beta, nsweep, seed):
+    gfx.set_lattice(*dims)
+    gfx.init_thermalization(gfx.unit_gauge())
+    gfx.set_mt_params(synthetic_params(seed).ravel())
+    gfx.set_mt_seed(seed)
+    gfx.heatbath_sweeps(nsweep, beta)
+    u = gfx.get_gauge()
+    gfx.finalize_MT()
+    gfx.finalize_thermalization()
+    return u
+
+
+def _host_F(orc, u, g, dims):
+    return orc.gauge_functional(orc.g_trafo(u, g, dims), dims)
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_batch_anneal_of_one_lattice_equals_simannealing(gfx, orc, sampler):
+    """A batch holding one lattice draws the same twister numbers in the same
+    order as simannealing_gauge, so both give the same g."""
+    dims = (8, 8, 8, 8)
+    V = orc.volume(dims)
+    u = _thermalized(gfx, dims, 6.0, 30, 41)
+    g0 = orc.random_init_trafo(V, 42)
+    steps, Tmin, Tmax, nmicro = 40, 0.01, 0.45, 3
+    gfx.set_mt_params(synthetic_params(43).ravel())
+    gfx.set_mt_seed(43)
+    b = gfx.Batch()
+    b.add(dims, u, g0)
+    b.anneal(steps, Tmin, Tmax, -1.0, nmicro, sampler, steps_per_graph=8)
+    g_batch = b.trafo(0)
+    b.close()
+    gfx.finalize_MT()
+    gfx.set_lattice(*dims)
+    gfx.set_verbose(0)
+    gfx.init_gaugefixing(u, g0)
+    sa = gfx.saparam()
+    sa.Tmin, sa.Tmax, sa.expo, sa.N, sa.checkint = Tmin, Tmax, -1.0, steps, 1000
+    gfx.set_mt_seed(43)
+    gfx.set_sa(sampler, nmicro, 4)
+    gfx.simannealing_gauge()
+    g_single = gfx.get_trafo()
+    gfx.finalize_MT()
+    gfx.finalize_gaugefixing()
+    assert np.abs(g_batch - g_single).max() < 1e-12
+    assert _host_F(orc, u, g_batch, dims) > _host_F(orc, u, g0, dims) + 0.2
+
+
+def test_gribov_batch_anneal_then_fix(gfx, orc):
+    """Haar-random copies of one configuration, annealed and fixed together;
+    every copy reaches the bound (checked on the host from the fixed links)
+    and the best copy is the one with the largest F."""
+    dims = (8, 8, 8, 8)
+    V = orc.volume(dims)
+    u = _thermalized(gfx, dims, 6.0, 40, 51)
+    gfx.set_omega(1.7)
+    gfx.set_mt_params(synthetic_params(52).ravel())
+    gfx.set_mt_seed(52)
+    b = gfx.Batch()
+    base = b.add(dims, u, orc.random_init_trafo(V, 53))
+    for _ in range(7):
+        b.add_random_copy(base)
+    starts = [b.trafo(q) for q in range(8)]
+    b.anneal(200, 0.01, 0.45, -1.0, 3, 0, steps_per_graph=16)
+    annealed = [b.trafo(q) for q in range(8)]
+    rc = b.run(8000, EPS12, 10, gfx.PREC_MIXED, 1e-6, 4)
+    res = [b.result(q) for q in range(8)]
+    fixed = [b.trafo(q) for q in range(8)]
+    qb, Fb = b.best_copy(base)
+    b.close()
+    gfx.finalize_MT()
+    assert rc == 0
+    for q in range(8):
+        assert _host_F(orc, u, annealed[q], dims) > _host_F(orc, u, starts[q], dims) + 0.3
+        uf = orc.g_trafo(u, fixed[q], dims)
+        _, mx = orc.dAdA(uf, dims)
+        assert mx < 2 * EPS12 and res[q][2] < EPS12
+        assert abs(orc.gauge_functional(uf, dims) - res[q][0]) < 1e-12
+    assert Fb == max(r[0] for r in res) and res[qb][0] == Fb
+
+
+def test_gribov_batch_full_shape(gfx, orc):
+    """The Gribov-copy workload at its real shape: 64 Haar-random copies of
+    one 24^4 beta = 6.0 configuration, annealed from T = 0.45 to 0.01 with 3
+    microcanonical steps per step (400 of the 4000 steps of the full schedule,
+    to keep the test short), then mixed-precision OR to theta < 1e-12 in one
+    batch; every copy converges and the best copy is selected."""
+    dims = (24, 24, 24, 24)
+    u = _thermalized(gfx, dims, 6.0, 100, 61)
+    gfx.set_omega(1.7)
+    gfx.set_mt_params(synthetic_params(62).ravel())
+    gfx.set_mt_seed(62)
+    b = gfx.Batch()
+    base = b.add(dims, u, gfx.unit_trafo())
+    for _ in range(63):
+        b.add_random_copy(base)
+    b.anneal(400, 0.01, 0.45, -1.0, 3, 0, steps_per_graph=20)
+    rc = b.run(20000, EPS12, 10, gfx.PREC_MIXED, 1e-6, 8)
+    res = [b.result(q) for q in range(64)]
+    st = b.stats()
+    qb, Fb = b.best_copy(base)
+    g_best = b.trafo(qb)
+    b.close()
+    gfx.finalize_MT()
+    assert rc == 0 and st["graph_launches"] > 0
+    assert all(r[2] < EPS12 and r[3] >= 0 for r in res)
+    assert Fb == max(r[0] for r in res)
+    # the selected copy's fixed links have the reported F
+    assert abs(_host_F(orc, u, g_best, dims) - Fb) < 1e-10
