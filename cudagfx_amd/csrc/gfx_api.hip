@@ -567,11 +567,19 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
   return i;
 }
 
+// Coulomb gauge: every time slice is fixed with the 3 spatial directions and
+// stops on its own (per-slice max dAdA < eps, checked every checkinterval
+// iterations); a stopped slice is no longer updated.  When the slices hold
+// a whole number of sweep blocks the checks ride on the sweeps as for
+// Landau (per-slice partials); a slice that stops keeps the state of its
+// check: its even sites are copied back over the check sweep's new values
+// before the fields swap.
 int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   State& s = g_st;
   if (!s.init) return GFX_ERR_STATE;
   if (checkinterval < 1) checkinterval = 1;
   const int nt = s.geo.L[3];
+  const int nsh = s.geo.nh / nt;  // sites of one parity in a slice
   std::vector<unsigned char> act(nt, 1);
   for (int t = 0; t < nt; t++) iters[t] = -1;
   bool fp32 = (s.precision == GFX_PREC_MIXED);
@@ -583,51 +591,102 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   tune_sweep(s, false, 3);
   hipMemcpyAsync(s.active, act.data(), nt, hipMemcpyHostToDevice, s.stream);
   s.run_iters32 = s.run_iters64 = 0;
+  auto fused_ok = [&](bool f32) {
+    const int blk = sweep_variant_block(s.variant[f32 ? 1 : 0][0]);
+    return nsh % blk == 0;
+  };
+  const int stall_checks = (s.stall_iters + checkinterval - 1) / checkinterval;
+  double best = 1e300;
+  int stall = 0;
   int left = nt;
+  int even = 0;
+  bool even_done = false;
   hipStreamSynchronize(s.stream);
   const auto t0 = std::chrono::steady_clock::now();
+  auto to_fp64 = [&]() {
+    canonical_even(s, true, even);
+    launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
+    even = 0;
+    fp32 = false;
+  };
   for (int i = 0; i < maxit && left > 0; i++) {
-    if (fp32) {
-      do_sweep(s, true, 0, 3, s.active);
-      do_sweep(s, true, 1, 3, s.active);
-      s.run_iters32++;
-    } else {
-      do_sweep(s, false, 0, 3, s.active);
-      do_sweep(s, false, 1, 3, s.active);
-      s.run_iters64++;
+    const bool chk = (i % checkinterval) == 0;
+    const bool fused = chk && fused_ok(fp32);
+    if (!even_done) do_sweep(s, fp32, 0, 3, s.active, even, 0);
+    do_sweep(s, fp32, 1, 3, s.active, even, fused ? 1 : 0);
+    even_done = false;
+    if (fp32) s.run_iters32++; else s.run_iters64++;
+    if (!chk) continue;
+    if (fused) {
+      do_sweep(s, fp32, 0, 3, s.active, even, 2);
+      const int bps = nsh / sweep_variant_block(s.variant[fp32 ? 1 : 0][0]);
+      launch_slice_finalize(s.cpart, s.cpart + 3 * (size_t)s.ncheck, bps, nt, s.active,
+                            3.0 * 2 * nsh, 2.0 * nsh, eps, s.results, s.stream);
+      if (hip_check(hipMemcpyAsync(s.h_results, s.results, sizeof(double) * 4 * nt, hipMemcpyDeviceToHost, s.stream), "D2H results") ||
+          hip_check(hipStreamSynchronize(s.stream), "check"))
+        return GFX_ERR_DEVICE;
+    } else if (functional_now(s, fp32, 3, nt, eps)) {
+      return GFX_ERR_DEVICE;
     }
-    if ((i % checkinterval) == 0) {
-      if (functional_now(s, fp32, 3, nt, eps)) return GFX_ERR_DEVICE;
-      bool changed = false, all_below_switch = true;
+    std::vector<int> stopped;
+    bool all_below_switch = true;
+    double mx = 0.0;
+    for (int t = 0; t < nt; t++) {
+      if (!act[t]) continue;
+      const double m = s.h_results[4 * t + 2];
+      mx = fmax(mx, m);
+      if (m < eps) {
+        act[t] = 0;
+        iters[t] = i;
+        left--;
+        stopped.push_back(t);
+      } else if (m >= s.switch_theta) {
+        all_below_switch = false;
+      }
+    }
+    bool stalled = false;
+    if (fp32 && left > 0) {
+      if (mx < 0.5 * best) {
+        best = mx;
+        stall = 0;
+      } else if (best < s.stall_gate && ++stall >= stall_checks) {
+        stalled = true;
+      }
+    }
+    if (fp32 && left > 0 && (all_below_switch || stalled)) {
+      // leave FP32 at the checked state (the check sweep's values are dropped)
+      if (!stopped.empty()) hipMemcpyAsync(s.active, act.data(), nt, hipMemcpyHostToDevice, s.stream);
+      to_fp64();
+    } else if (fused && i + 1 < maxit && left > 0) {
+      // slices that stop keep their checked even sites; the others take the
+      // check sweep's update
+      const int np = fp32 ? 3 : 6;
+      const size_t vb = fp32 ? sizeof(float4) : sizeof(double2);
+      char* cur = fp32 ? (char*)field32(s, even) : (char*)field64(s, even);
+      char* nxt = fp32 ? (char*)field32(s, 2 - even) : (char*)field64(s, 2 - even);
+      for (int t : stopped)
+        for (int c = 0; c < np; c++) {
+          const size_t off = ((size_t)c * s.geo.stride + (size_t)t * nsh) * vb;
+          hipMemcpyAsync(nxt + off, cur + off, (size_t)nsh * vb, hipMemcpyDeviceToDevice, s.stream);
+        }
+      if (!stopped.empty()) hipMemcpyAsync(s.active, act.data(), nt, hipMemcpyHostToDevice, s.stream);
+      even = 2 - even;
+      even_done = true;
+    } else if (!stopped.empty()) {
+      hipMemcpyAsync(s.active, act.data(), nt, hipMemcpyHostToDevice, s.stream);
+    }
+    if (s.verbose) {
+      double F = 0, D = 0, M = 0;
       for (int t = 0; t < nt; t++) {
-        if (!act[t]) continue;
-        const double m = s.h_results[4 * t + 2];
-        if (m < eps) {
-          act[t] = 0;
-          iters[t] = i;
-          left--;
-          changed = true;
-        } else if (m >= s.switch_theta) {
-          all_below_switch = false;
-        }
+        F += s.h_results[4 * t];
+        D += s.h_results[4 * t + 1];
+        M = fmax(M, s.h_results[4 * t + 2]);
       }
-      if (changed) hipMemcpyAsync(s.active, act.data(), nt, hipMemcpyHostToDevice, s.stream);
-      if (fp32 && all_below_switch) {
-        launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
-        fp32 = false;
-      }
-      if (s.verbose) {
-        double F = 0, D = 0, M = 0;
-        for (int t = 0; t < nt; t++) {
-          F += s.h_results[4 * t];
-          D += s.h_results[4 * t + 1];
-          M = fmax(M, s.h_results[4 * t + 2]);
-        }
-        printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e\t slices left = %d\n", i, F / nt, D / nt, M, left);
-      }
+      printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e\t slices left = %d\n", i, F / nt, D / nt, M, left);
     }
   }
-  if (fp32) launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
+  if (fp32) to_fp64();
+  canonical_even(s, false, even);
   hipStreamSynchronize(s.stream);
   s.run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return left == 0 ? GFX_OK : GFX_ERR_NOCONV;

@@ -59,6 +59,11 @@ void launch_sweep_f32(const float4* U, float4* ge, float4* go, const Geom& geo, 
                       const int* bmap, hipStream_t st, int chk = 0, float4* galt = nullptr,
                       double* partials = nullptr);
 int sweep_blocks(int nh, int variant);
+// per-slice grid pass of a fused Coulomb check (bps blocks per slice; only
+// active slices are written)
+void launch_slice_finalize(const double* podd, const double* peven, int bps, int nt,
+                           const unsigned char* active, double normF, double normD, double eps,
+                           double* out, hipStream_t st);
 // grid pass of a fused check: out[0..3] = F/normF, dAdA/normD, max, max < eps
 void launch_check_finalize(const double* podd, const double* peven, int nb, double normF,
                            double normD, double eps, double* out, hipStream_t st);

@@ -182,6 +182,9 @@ __global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* 
   } else {
     if (i < geo.nh) {
       const Site s = site_coords(geo, i, parity);
+      // Coulomb: a block lies in one time slice (slices hold a whole number
+      // of blocks when checks are fused), so the whole block leaves together
+      if (NDIR == 3 && !active[s.x[3]]) return;
       sweep_site<V, R, NDIR, ROLLED, NT, CHK>(U, gop, gme, gout, geo, s, i, parity, c1, c2, F, d);
     }
     store_check_partials<CHK>(F, d, partials, b);
@@ -220,6 +223,33 @@ __global__ __launch_bounds__(BLOCK) void k_batch_sweep(const BatchProblem* __res
       sweep_site<V, R, NDIR, ROLLED, NT, CHK>(U, gop, gme, gout, geo, s, i, parity, c1, c2, F, d);
     }
     store_check_partials<CHK>(F, d, partials, poff[m.x] + m.y);
+  }
+}
+
+// grid-level pass of a fused Coulomb check, one block per active time slice
+// t over its bps block partials [t bps, (t+1) bps) of each sweep:
+// out[4t..] = F/normF, dAdA/normD, max dAdA, max < eps.  Inactive slices
+// keep their last values.
+__global__ __launch_bounds__(BLK) void k_slice_finalize(const double* __restrict__ podd,
+                                                        const double* __restrict__ peven, int bps,
+                                                        const unsigned char* __restrict__ active,
+                                                        double normF, double normD, double eps,
+                                                        double* __restrict__ out) {
+  const int t = blockIdx.x;
+  if (!active[t]) return;
+  double F = 0.0, d = 0.0, mx = 0.0;
+  for (int k = threadIdx.x; k < bps; k += BLK) {
+    const size_t b = (size_t)t * bps + k;
+    F += podd[3 * b];
+    d += podd[3 * b + 1] + peven[3 * b + 1];
+    mx = fmax(mx, fmax(podd[3 * b + 2], peven[3 * b + 2]));
+  }
+  block_reduce3(F, d, mx);
+  if (threadIdx.x == 0) {
+    out[4 * t + 0] = F / normF;
+    out[4 * t + 1] = d / normD;
+    out[4 * t + 2] = mx;
+    out[4 * t + 3] = (mx < eps) ? 1.0 : 0.0;
   }
 }
 
@@ -615,7 +645,12 @@ static void launch_sweep_t(const SweepArgs<V>& a, int ndir, double omega, int va
     else
       launch_sweep_chk<V, R, 4, 0>(b, c1, c2, variant, st);
   } else {
-    launch_sweep_chk<V, R, 3, 0>(a, c1, c2, variant, st);
+    if (chk == 1)
+      launch_sweep_chk<V, R, 3, 1>(a, c1, c2, variant, st);
+    else if (chk == 2)
+      launch_sweep_chk<V, R, 3, 2>(a, c1, c2, variant, st);
+    else
+      launch_sweep_chk<V, R, 3, 0>(a, c1, c2, variant, st);
   }
 }
 
@@ -635,6 +670,13 @@ void launch_sweep_f32(const float4* U, float4* ge, float4* go, const Geom& geo, 
 int sweep_blocks(int nh, int variant) {
   const int b = sweep_variant_block(variant);
   return (nh + b - 1) / b;
+}
+
+void launch_slice_finalize(const double* podd, const double* peven, int bps, int nt,
+                           const unsigned char* active, double normF, double normD, double eps,
+                           double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_slice_finalize, dim3(nt), dim3(BLK), 0, st, podd, peven, bps, active, normF,
+                     normD, eps, out);
 }
 
 void launch_check_finalize(const double* podd, const double* peven, int nb, double normF,

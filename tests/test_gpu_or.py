@@ -285,3 +285,25 @@ def test_every_sweep_variant_matches_oracle(gfx, orc):
         worst = max(worst, np.abs(gfx.get_trafo() - g_cpu).max())
         assert worst < 1e-13, v
     gfx.finalize_gaugefixing()
+
+
+@pytest.mark.parametrize("ck", [3, 7])
+def test_coulomb_fused_checks_match_oracle(gfx, orc, ck):
+    """Slices stop at different checks; with the checks fused into the sweeps
+    each stopped slice keeps the state of its check."""
+    dims = (8, 8, 8, 6)
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, 41 + ck, eps=0.35)
+    g = orc.random_init_trafo(V, 42 + ck)
+    _setup(gfx, dims, u, g, 1.7)
+    rc, it_gpu = gfx.coulomb_gauge(5000, EPS12, ck)
+    g_gpu = gfx.get_trafo()
+    F, D, M = gfx.functional_slices()
+    gfx.finalize_gaugefixing()
+    u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+    rc_o, it_cpu = orc.coulomb_gauge(u2, g2, dims, 5000, EPS12, ck, 1.7)
+    assert rc == 0 and rc_o == 0
+    assert len(set(it_cpu.tolist())) > 1  # the slices do stop at different checks
+    assert np.all(np.abs(it_gpu - it_cpu) <= 2), (it_gpu, it_cpu)
+    assert np.abs(g_gpu - orc.from_2v(g2)).max() < 1e-10
+    assert np.all(M < EPS12)
