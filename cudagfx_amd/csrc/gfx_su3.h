@@ -283,6 +283,58 @@ __device__ __forceinline__ void overrelax_apply(M3<R>& g, const M3<R>& gn, R c1,
   g = m_mul(Rw, g);
 }
 
+// ---- shorter arithmetic for the FP32 sweep (the FP64 sweep keeps the
+// reference's operation order).  Both use that a product of SU(3) matrices
+// is in SU(3): two rows are formed and the third is conj(row0 x row1).
+
+// a[0..1] are rows 0,1 of A, b[0..2] the rows of B: rows 0,1 of A B^+
+template <typename R>
+__device__ __forceinline__ void rows_mul_dagger(Cx<R> p[2][3], const Cx<R> a[2][3],
+                                                const Cx<R> b[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      p[i][j] = cadd(cadd(cmulc(a[i][0], b[j][0]), cmulc(a[i][1], b[j][1])), cmulc(a[i][2], b[j][2]));
+}
+// rows 0,1 of A B
+template <typename R>
+__device__ __forceinline__ void rows_mul(Cx<R> p[2][3], const Cx<R> a[2][3], const Cx<R> b[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      p[i][j] = cadd(cadd(cmul(a[i][0], b[0][j]), cmul(a[i][1], b[1][j])), cmul(a[i][2], b[2][j]));
+}
+// r2 = conj(r0 x r1)
+template <typename R>
+__device__ __forceinline__ void row2_of(Cx<R> r2[3], const Cx<R> r0[3], const Cx<R> r1[3]) {
+  r2[0] = cconj(csub(cmul(r0[1], r1[2]), cmul(r0[2], r1[1])));
+  r2[1] = cconj(csub(cmul(r0[2], r1[0]), cmul(r0[0], r1[2])));
+  r2[2] = cconj(csub(cmul(r0[0], r1[1]), cmul(r0[1], r1[0])));
+}
+
+// g <- normalize(Rw g) with Rw = 1 + c1 R + c1 c2 R^2, R = h g^+ - 1, in the
+// form Rw g = g + c1 (h - g) + c1 c2 (h g^+ h - 2 h + g): two 2x3 products
+// instead of three 3x3 ones.  Gram-Schmidt commutes with multiplying by the
+// unitary g from the right, so normalize(Rw) g = normalize(Rw g).
+template <typename R>
+__device__ __forceinline__ void overrelax_apply_short(M3<R>& g, const M3<R>& h, R c1, R c2) {
+  Cx<R> t[2][3], w[2][3];
+  rows_mul_dagger(t, h.e, g.e);
+  rows_mul(w, t, h.e);
+  const R cw = c1 * c2;
+  const R cg = R(1) - c1 + cw, ch = c1 - R(2) * cw;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      g.e[i][j].re = cg * g.e[i][j].re + ch * h.e[i][j].re + cw * w[i][j].re;
+      g.e[i][j].im = cg * g.e[i][j].im + ch * h.e[i][j].im + cw * w[i][j].im;
+    }
+  m_normalize(g);
+}
+
 // traceless anti-hermitian part (i/2)(U^+ - U) - tr/3 (overrelax.cu:451-473)
 template <typename R>
 __device__ __forceinline__ void vecpot(M3<R>& U) {

@@ -307,3 +307,26 @@ def test_coulomb_fused_checks_match_oracle(gfx, orc, ck):
     assert np.all(np.abs(it_gpu - it_cpu) <= 2), (it_gpu, it_cpu)
     assert np.abs(g_gpu - orc.from_2v(g2)).max() < 1e-10
     assert np.all(M < EPS12)
+
+
+@pytest.mark.parametrize("rolled", [True, False])
+def test_fp32_halfsweep_close_to_oracle(gfx, orc, rolled):
+    """The FP32 sweep uses a shorter but equal arithmetic (two rows of each
+    SU(3) product, the series applied to g directly); a half sweep agrees
+    with the oracle's FP64 one to FP32 rounding."""
+    dims = (8, 8, 8, 8)
+    V = orc.volume(dims)
+    u = orc.random_init_gauge(V, 4371)
+    g = orc.random_init_trafo(V, 4372)
+    _setup(gfx, dims, u, g, 1.7, prec=gfx.PREC_MIXED)
+    gfx.set_sweep_variant(gfx.PREC_MIXED, 4, 0 if rolled else 2)
+    gfx.halfsweep(0, 4)
+    gfx.halfsweep(1, 4)
+    g_gpu = gfx.get_trafo()
+    gfx.set_precision(gfx.PREC_FP64)
+    gfx.finalize_gaugefixing()
+    u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+    orc.halfsweep(u2, g2, dims, 0, 4, 1.7)
+    orc.halfsweep(u2, g2, dims, 1, 4, 1.7)
+    err = np.abs(g_gpu - orc.from_2v(g2)).max()
+    assert 1e-9 < err < 2e-5, err
