@@ -119,6 +119,7 @@ def load():
                                       C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_double)]),
         "gfx_batch_variant": (C.c_int, [C.c_int, C.POINTER(C.c_int), vp]),
+        "gfx_su2_sample": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int, vp]),
         "gfx_batch_anneal": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                        C.c_int, C.c_int]),
         "gfx_batch_best_copy": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
@@ -425,6 +426,14 @@ def sweep_variant(precision=PREC_FP64, ndir=4):
 
 def set_sweep_variant(precision, ndir, variant):
     _check(load().gfx_set_sweep_variant(int(precision), int(ndir), int(variant)), "variant")
+
+
+def su2_sample(sampler, beta, wnorm, n):
+    """n draws of the SA sweep's SU(2) sampler (alpha_0 for w = (wnorm,0,0,0))."""
+    out = np.zeros(int(n))
+    _check(load().gfx_su2_sample(int(sampler), float(beta), float(wnorm), int(n), _ptr(out)),
+           "su2_sample")
+    return out
 
 
 def micro_sweeps(n, ndir=4):

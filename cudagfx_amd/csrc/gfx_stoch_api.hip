@@ -31,6 +31,8 @@ void launch_link_heatbath(double2* U, const Geom& geo, int parity, int nu, doubl
 int kp_gauss_per_site();
 int kp_unif_per_site();
 void launch_haar_trafo(double2* g, const Geom& geo, const float* gauss, hipStream_t st);
+void launch_su2_sample(int sampler, double beta, double wnorm, int n, const float* gauss,
+                       const float* unif, double* out, hipStream_t st);
 
 static const int MT_COUNT = 4096;
 
@@ -302,6 +304,27 @@ int gfx_batch_add_random_copy(int base) {
   }
   launch_haar_trafo((double2*)gp, geo, s.rnd_gauss, s.stream);
   return hip_check(hipStreamSynchronize(s.stream), "haar") ? -GFX_ERR_DEVICE : q;
+}
+
+// n draws of the SU(2) sampler (0 reference rule, 1 Kennedy-Pendleton) for
+// w = (wnorm, 0, 0, 0) at inverse temperature beta from the current twister
+// fields (grown if needed), then one refill; a0[i] = alpha_0
+int gfx_su2_sample(int sampler, double beta, double wnorm, int n, double* a0) {
+  State& s = g_st;
+  if (n <= 0 || !a0) return GFX_ERR_STATE;
+  const long ng = (long)(sampler ? kp_gauss_per_site() : 1) * n;
+  const long nu = (long)(sampler ? kp_unif_per_site() : 4) * n;
+  if (nu > (1L << 30)) return GFX_ERR_GEOMETRY;
+  int rc = mt_prepare(s, (int)ng, (int)nu);
+  if (rc) return rc;
+  double* d = nullptr;
+  if (hipMalloc((void**)&d, sizeof(double) * n) != hipSuccess) return GFX_ERR_ALLOC;
+  launch_su2_sample(sampler ? 1 : 0, beta, wnorm, n, s.rnd_gauss, s.rnd_unif, d, s.stream);
+  hipMemcpyAsync(a0, d, sizeof(double) * n, hipMemcpyDeviceToHost, s.stream);
+  mt_refill(s, 1);
+  rc = hip_check(hipStreamSynchronize(s.stream), "su2 sample");
+  hipFree(d);
+  return rc;
 }
 
 // n microcanonical sweeps (even + odd) of g; F is unchanged by each

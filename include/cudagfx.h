@@ -194,6 +194,12 @@ int gfx_set_sa(int sampler, int nmicro, int ndir);
 int gfx_micro_sweeps(int n, int ndir);
 /* SA temperature of step i (simulated_annealing.cu:265-299) */
 double gfx_sa_temperature(int i, double Tmin, double Tmax, double expo, int N);
+/* n draws of the SU(2) subgroup sampler of the SA sweep (sampler 0: the
+ * reference rule of simulated_annealing.cu:5-73, 1: Kennedy-Pendleton) for
+ * w = (wnorm, 0, 0, 0) at inverse temperature beta, from the current twister
+ * fields, which are refilled afterwards; a0[i] = alpha_0 (NaN where the
+ * Kennedy-Pendleton trials all failed).  For distribution tests. */
+int gfx_su2_sample(int sampler, double beta, double wnorm, int n, double *a0);
 /* nsweep link-heatbath sweeps of the Wilson action at beta (Kennedy-Pendleton
  * SU(2) subgroups, random numbers from the twisters) */
 int gfx_heatbath_sweeps(int nsweep, double beta);

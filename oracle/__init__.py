@@ -232,3 +232,20 @@ def mt_generate(params, state, nper, initialized, want_words=False):
     lib.orc_mt_generate(params, state, count, nper, initialized, out.ctypes.data,
                         words.ctypes.data if words is not None else None)
     return (out, words) if want_words else out
+
+
+# ---------------------------------------------------------------- SA sampler
+def sa_sample_ref(gauss, unif, beta, wnorm, n):
+    """alpha_0 of dev_heatbath_su2 for w = (wnorm, 0, 0, 0), draw i using
+    gauss[i] and unif[4i..4i+3] (simulated_annealing.cu:1-73): dev_get_a0 in
+    float arithmetic (y = (|g|^2 - log(1 - u0)) / (|w|^2 beta), accepted when
+    2 u1^2 <= 2 - y, then a0 = 1 - y), else alpha = w.  With v = w/|w| =
+    (1, 0, 0, 0) the product a v is a itself, so alpha_0 = a0."""
+    f32 = np.float32
+    g = np.abs(np.asarray(gauss[:n], dtype=f32))
+    u = np.asarray(unif[:4 * n], dtype=f32).reshape(n, 4)
+    b = -np.log(f32(1.0) - u[:, 0])
+    y = g * g + b
+    y = (y.astype(np.float64) / (wnorm * wnorm * beta)).astype(f32)
+    acc = f32(2.0) * (u[:, 1] * u[:, 1]) <= f32(2.0) - y
+    return np.where(acc, (f32(1.0) - y).astype(np.float64), wnorm)

@@ -180,3 +180,48 @@ def test_micro_steps_preserve_functional(gfx, orc):
     gfx.finalize_gaugefixing()
     assert abs(F1 - F0) < 1e-12 * abs(F0)
     assert np.abs(g1 - g0).max() > 1e-2
+
+
+def _a0_cdf(a, x):
+    """CDF on [-1, 1] of the density sqrt(1 - a0^2) exp(a a0)."""
+    from scipy import integrate
+    f = lambda t: np.sqrt(max(0.0, 1.0 - t * t)) * np.exp(a * (t - 1.0))
+    grid = np.linspace(-1.0, 1.0, 4001)
+    pdf = np.array([f(t) for t in grid])
+    cdf = integrate.cumulative_trapezoid(pdf, grid, initial=0.0)
+    cdf /= cdf[-1]
+    return np.interp(x, grid, cdf)
+
+
+@pytest.mark.parametrize("a", [0.7, 3.0, 12.0])
+def test_kennedy_pendleton_sampler_density(gfx, a):
+    """alpha_0 of the Kennedy-Pendleton sampler follows
+    sqrt(1 - a0^2) exp(a a0), a = beta |w| (Kolmogorov-Smirnov)."""
+    from scipy import stats
+    gfx.set_mt_params(synthetic_params(91).ravel())
+    gfx.set_mt_seed(91)
+    a0 = gfx.su2_sample(1, a / 2.0, 2.0, 200000)
+    gfx.finalize_MT()
+    acc = a0[~np.isnan(a0)]
+    # four trials; at small a the proposals often fall outside [-1, 1] and all
+    # four can fail (the subgroup then stays as it is)
+    assert acc.size > (0.8 if a < 1 else 0.95) * a0.size
+    assert np.all(np.abs(acc) <= 1.0)
+    res = stats.kstest(acc, lambda x: _a0_cdf(a, x))
+    assert res.pvalue > 1e-3, res
+
+
+def test_reference_sampler_follows_its_rule(gfx, orc):
+    """The reference rule (simulated_annealing.cu:1-73) on the same twister
+    numbers as the oracle's restatement."""
+    gfx.set_mt_params(synthetic_params(92).ravel())
+    gfx.set_mt_seed(92)
+    n, beta, wnorm = 50000, 0.8, 1.7
+    gfx.su2_sample(0, beta, wnorm, n)  # sizes the fields, then refills them
+    gauss, unif = gfx.get_mt_fields()[:2]
+    a0 = gfx.su2_sample(0, beta, wnorm, n)
+    gfx.finalize_MT()
+    want = orc.sa_sample_ref(gauss, unif, beta, wnorm, n)
+    same = np.abs(a0 - want) < 1e-5
+    assert same.mean() > 0.9999, same.mean()
+    assert 0.05 < np.mean(a0 != wnorm) < 1.0

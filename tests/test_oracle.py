@@ -310,3 +310,36 @@ def test_coulomb_slices_are_independent(orc):
     assert rc == 0 and np.all(iters > 0)
     F, D, M = orc.functional_slices(u, g, dims, 3)
     assert np.all(M < 1e-12)
+
+
+def test_reference_sa_rule_density(orc):
+    """Pin of the oracle's restatement of the reference SA sampler
+    (simulated_annealing.cu:1-73): y = (g^2 - ln(1-u0))/kb with g normal and
+    u0 uniform, accepted when 2 u1^2 <= 2 - y, a0 = 1 - y.  S = g^2 + E
+    (chi-square(1) plus Exp(1)) has density exp(-s) erfi(sqrt(s/2)), so the
+    accepted a0 follow exp(-kb y) erfi(sqrt(kb y/2)) sqrt(1 - y/2), y = 1-a0
+    (not the Kennedy-Pendleton density: the reference squares g where KP
+    halves it); rejected draws return w."""
+    from scipy import integrate, special, stats
+    rng = np.random.default_rng(5)
+    n, beta, wnorm = 200000, 0.9, 2.0
+    gauss = rng.standard_normal(n).astype(np.float32)
+    unif = (1.0 - rng.random(4 * n)).astype(np.float32)  # (0, 1] like the twisters
+    a0 = orc.sa_sample_ref(gauss, unif, beta, wnorm, n)
+    acc = a0[a0 != wnorm]
+    kb = wnorm * wnorm * beta
+    grid = np.linspace(-1.0, 1.0, 8001)
+    y = 1.0 - grid
+    pdf = np.exp(-kb * y) * special.erfi(np.sqrt(kb * y / 2.0)) * np.sqrt(np.clip(1 - y / 2, 0, None))
+
+    def cdf_of(p):
+        c = integrate.cumulative_trapezoid(p, grid, initial=0.0)
+        return lambda x: np.interp(x, grid, c / c[-1])
+
+    assert stats.kstest(acc, cdf_of(pdf)).pvalue > 1e-3
+    # the acceptance rate is the integral of the same density over y in [0, 2]
+    rate = integrate.quad(lambda t: np.exp(-t) * special.erfi(np.sqrt(t / 2)) * np.sqrt(max(0.0, 1 - t / (2 * kb))), 0, 2 * kb)[0]
+    assert abs(acc.size / n - rate) < 0.005
+    # a planted mistake (the Kennedy-Pendleton density with a = kb/2) is caught
+    kp = np.sqrt(np.clip(1 - grid ** 2, 0, None)) * np.exp(kb / 2 * (grid - 1))
+    assert stats.kstest(acc, cdf_of(kp)).pvalue < 1e-6
