@@ -29,62 +29,58 @@ __device__ __forceinline__ uint32_t mt_temper(uint32_t x, uint32_t b, uint32_t c
   return x;
 }
 
-// One step at compile-time buffer position K; mt[] stays in registers.
-template <int K>
-__device__ __forceinline__ uint32_t mt_step(uint32_t (&mt)[MT_N], uint32_t& m1, uint32_t a) {
-  constexpr int I1 = (K + 1) % MT_N;
-  constexpr int IM = (K + MT_M) % MT_N;
-  const uint32_t m0 = m1;
-  m1 = mt[I1];
-  uint32_t x = (m0 & 0xFFFFFFFEu) | (m1 & 0x1u);
-  x = mt[IM] ^ (x >> 1) ^ ((x & 1u) ? a : 0u);
-  mt[K] = x;
-  return x;
-}
-
-template <int K>
-__device__ __forceinline__ void mt_emit(uint32_t (&mt)[MT_N], uint32_t& m1, uint32_t a, uint32_t b,
-                                        uint32_t c, float* __restrict__ out, int r, int k) {
-  const uint32_t x = mt_temper(mt_step<K>(mt, m1, a), b, c);
-  out[r + (size_t)k * MT_COUNT] = ((float)x + 1.0f) / 4294967296.0f;
-}
-
-template <int K>
-__device__ __forceinline__ void mt_round(uint32_t (&mt)[MT_N], uint32_t& m1, uint32_t a, uint32_t b,
-                                         uint32_t c, float* __restrict__ out, int r, int k0, int n) {
-  if constexpr (K < MT_N) {
-    if (k0 + K < n) {
-      mt_emit<K>(mt, m1, a, b, c, out, r, k0 + K);
-      mt_round<K + 1>(mt, m1, a, b, c, out, r, k0, n);
-    }
-  }
-}
+// A call continues the linear sequence y with y_0..y_18 = the stored buffer
+// (position K holds y_K; "restart at index 0") and
+//   y_{t+19} = y_{t+9} ^ A(y_t & upper | y_{t+1} & lower),
+// emitting temper(y_{19+k}) for k < nper; afterwards position K of the
+// buffer holds the newest y_t with t = K (mod 19).  y_{19+k} .. y_{28+k}
+// depend only on y_k .. y_{k+18}, so MT_L = 10 lanes per twister produce ten
+// outputs per step from a 32-entry ring in LDS (one barrier per step; the
+// slots written in a step are never read in it).  A block holds MT_TPB
+// twisters; lane (j, r) = thread j * MT_TPB + r, so a wavefront writes
+// MT_TPB consecutive floats of two output rows.
+constexpr int MT_TPB = 32;
+constexpr int MT_L = 10;
 
 // params: MT_COUNT x {matrix_a, mask_b, mask_c, seed}; state: MT_COUNT x 19
-__global__ __launch_bounds__(256) void k_mt_generate(uint32_t* __restrict__ params,
-                                                     uint32_t* __restrict__ state,
-                                                     float* __restrict__ out, int nper,
-                                                     int initialized) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= MT_COUNT) return;
+__global__ __launch_bounds__(MT_TPB * MT_L) void k_mt_generate(uint32_t* __restrict__ params,
+                                                               uint32_t* __restrict__ state,
+                                                               float* __restrict__ out, int nper,
+                                                               int initialized) {
+  __shared__ uint32_t ring[MT_TPB][33];  // 33: the twisters of a wavefront hit different banks
+  const int rl = threadIdx.x % MT_TPB, j = threadIdx.x / MT_TPB;
+  const int r = blockIdx.x * MT_TPB + rl;
   const uint32_t a = params[4 * r + 0], b = params[4 * r + 1], c = params[4 * r + 2];
-  uint32_t mt[MT_N];
   if (!initialized) {
-    mt[0] = params[4 * r + 3];
-#pragma unroll
-    for (int j = 1; j < MT_N; j++) mt[j] = 1812433253u * (mt[j - 1] ^ (mt[j - 1] >> 30)) + (uint32_t)j;
+    if (j == 0) {
+      uint32_t y = params[4 * r + 3];
+      ring[rl][0] = y;
+      for (int t = 1; t < MT_N; t++) {
+        y = 1812433253u * (y ^ (y >> 30)) + (uint32_t)t;
+        ring[rl][t] = y;
+      }
+    }
   } else {
-#pragma unroll
-    for (int j = 0; j < MT_N; j++) mt[j] = state[(size_t)r * MT_N + j];
+    for (int t = j; t < MT_N; t += MT_L) ring[rl][t] = state[(size_t)r * MT_N + t];
   }
-  uint32_t m1 = mt[0];
-  // a full round of 19 outputs returns the circular index to 0
-  int k0 = 0;
-  for (; k0 + MT_N <= nper; k0 += MT_N) mt_round<0>(mt, m1, a, b, c, out, r, k0, nper);
-  if (k0 < nper) mt_round<0>(mt, m1, a, b, c, out, r, k0, nper);
-  params[4 * r + 3] = mt[0];
-#pragma unroll
-  for (int j = 0; j < MT_N; j++) state[(size_t)r * MT_N + j] = mt[j];
+  __syncthreads();
+  for (int k0 = 0; k0 < nper; k0 += MT_L) {
+    const int k = k0 + j;
+    if (k < nper) {
+      const uint32_t y0 = ring[rl][k & 31], y1 = ring[rl][(k + 1) & 31];
+      uint32_t x = (y0 & 0xFFFFFFFEu) | (y1 & 0x1u);
+      x = ring[rl][(k + MT_M) & 31] ^ (x >> 1) ^ ((x & 1u) ? a : 0u);
+      ring[rl][(k + MT_N) & 31] = x;
+      out[r + (size_t)k * MT_COUNT] = ((float)mt_temper(x, b, c) + 1.0f) / 4294967296.0f;
+    }
+    __syncthreads();
+  }
+  const int T = MT_N - 1 + nper;  // index of the last y produced
+  for (int K = j; K < MT_N; K += MT_L) {
+    const uint32_t v = ring[rl][(T - (T - K) % MT_N) & 31];
+    state[(size_t)r * MT_N + K] = v;
+    if (K == 0) params[4 * r + 3] = v;
+  }
 }
 
 // Cartesian Box-Muller on pairs (out[r + 2k*4096], out[r + (2k+1)*4096])
@@ -109,8 +105,8 @@ __global__ __launch_bounds__(256) void k_box_muller(float* __restrict__ d, int n
 
 void launch_mt_generate(uint32_t* params, uint32_t* state, float* out, int nper, int initialized,
                         hipStream_t st) {
-  hipLaunchKernelGGL(k_mt_generate, dim3(MT_COUNT / 256), dim3(256), 0, st, params, state, out,
-                     nper, initialized);
+  hipLaunchKernelGGL(k_mt_generate, dim3(MT_COUNT / MT_TPB), dim3(MT_TPB * MT_L), 0, st, params,
+                     state, out, nper, initialized);
 }
 
 void launch_box_muller(float* d, int nper, hipStream_t st) {
