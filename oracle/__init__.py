@@ -1,7 +1,7 @@
 """CPU oracle for the cudagfx gauge-fixing path -- TEST INFRASTRUCTURE ONLY.
 
 This package is imported only by ``tests/``, by ``__graft_entry__.smoke()``
-(as the checker) and by ``bench.py``'s ``cpu_baseline`` leg.  The shipped
+(to check the GPU result) and by ``bench.py``'s ``cpu_baseline`` leg.  The shipped
 library (``cudagfx_amd``) never imports it.
 
 It wraps ``oracle/liboracle.so`` (built from ``oracle/oracle.c`` by

@@ -2,7 +2,7 @@
  * oracle.h -- CPU restatement of the cudagfx gauge-fixing path.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links this code;
- * it is loaded by tests/, by __graft_entry__.smoke() as the checker and by
+ * it is loaded by tests/, by __graft_entry__.smoke() to check the GPU result and by
  * bench.py as the CPU baseline.  Every function restates the arithmetic of
  * the reference (files cited per function, paths relative to the reference
  * checkout) in the same operation order, on the same 2-row (12-real) storage
@@ -134,7 +134,9 @@ void orc_mt_generate(orc_mt_params *params, uint32_t *state, int count,
 uint32_t orc_mt19937_nth(uint32_t seed, int n);
 
 /* MersenneTwister.cu:210-229 restated in double precision: the exact
- * Box-Muller pair for (u1, u2), used to bound the GPU float results. */
+ * Box-Muller pair for (u1, u2), used to bound the GPU float results.  The
+ * reference's float path (logf, __cosf, __sinf) is not reproduced bit for
+ * bit: parity unpinned at the last bit, bounded to 2e-5 instead. */
 void orc_box_muller_exact(double u1, double u2, double *z1, double *z2);
 
 /* iAlignUp(iDivUp(n, count), 2) (MersenneTwister.cu:246,252) */
