@@ -33,23 +33,40 @@ struct BProbHost {
   double2* g64 = nullptr;
   float4* g32 = nullptr;
   int ushare = 0;  // problem whose links this one uses (itself if it owns them)
-  int stop_iter[2] = {-1, -1};  // per phase (FP32 bulk, FP64)
+  int stop_iter = -1;  // iteration of the stop check (-1: not converged)
+  int switch_iter = -1;  // mixed: iteration of the check that left FP32
+  int idle = 0;  // iterations it waited for the FP64 sweeps to be launched
 };
 
+// Index 0 = FP64, 1 = FP32 for every per-precision array.
 struct Batch {
   std::vector<BProbHost> probs;
-  BatchProblem* d_probs = nullptr;
-  int2* d_bmap[2][TILINGS] = {};  // per [block 256/64][tiling]
+  BatchProblem* d_probs[2] = {nullptr, nullptr};  // field pointers in each precision
+  int2* d_bmap[2][TILINGS] = {};  // per [block 256/64][tiling], every lattice
   int nbmap[2] = {0, 0};
-  int* d_poff[2] = {nullptr, nullptr};  // check-partial offsets per problem
-  double* d_podd = nullptr;   // check partials of the odd sweeps
-  double* d_peven = nullptr;  // and of the even sweeps
+  std::vector<int2> h_order[2][TILINGS];  // the same lists before dealing to XCDs
+  // per [precision][block][tiling]: the lattices sweeping in that precision
+  // only, rebuilt when the census version moves on
+  struct CMap {
+    int2* d = nullptr;
+    int n = 0;
+    long version = -1;
+    std::vector<int2> h;
+  } cmap[2][2][TILINGS];
+  long census_version = 0;
+  int* d_poff[2] = {nullptr, nullptr};  // check-partial offsets per problem, per block size
+  double* d_pod[2] = {nullptr, nullptr};  // check partials of the odd sweeps
+  double* d_pev[2] = {nullptr, nullptr};  // and of the even sweeps
+  int* d_act[2] = {nullptr, nullptr};     // lattice is sweeping in this precision
+  int* h_act = nullptr;                   // pinned, 2 x n
+  std::vector<int> h_act_prev;            // the census before
+  int* d_conv = nullptr;                  // FP32 -> FP64 conversions of a check
+  int* d_sw = nullptr;                    // iteration of each lattice's switch
+  int* h_sw = nullptr;
   int* d_roff = nullptr;      // per problem: first site of its random numbers
   int nh_total = 0;           // sites per parity over all problems
-  double* d_best = nullptr;   // per problem: best max dAdA of the phase
+  double* d_best = nullptr;   // per problem: best max dAdA of the FP32 phase
   int* d_stall = nullptr;     // checks since it last halved
-  int* d_active = nullptr;
-  int* h_active = nullptr;
   double* d_results = nullptr;
   double* h_results = nullptr;
   int* d_iter = nullptr;
@@ -73,31 +90,41 @@ static void batch_free_maps(Batch& b) {
     for (int t = 0; t < TILINGS; t++) {
       hipFree(b.d_bmap[k][t]);
       b.d_bmap[k][t] = nullptr;
+      b.h_order[k][t].clear();
+      for (int pr = 0; pr < 2; pr++) {
+        hipFree(b.cmap[pr][k][t].d);
+        b.cmap[pr][k][t].d = nullptr;
+        b.cmap[pr][k][t].n = 0;
+        b.cmap[pr][k][t].version = -1;
+      }
     }
     hipFree(b.d_poff[k]);
+    hipFree(b.d_probs[k]);
+    hipFree(b.d_pod[k]);
+    hipFree(b.d_pev[k]);
+    hipFree(b.d_act[k]);
     b.d_poff[k] = nullptr;
+    b.d_probs[k] = nullptr;
+    b.d_pod[k] = b.d_pev[k] = nullptr;
+    b.d_act[k] = nullptr;
   }
-  hipFree(b.d_probs);
-  hipFree(b.d_podd);
-  hipFree(b.d_peven);
+  hipFree(b.d_conv);
+  hipFree(b.d_sw);
   hipFree(b.d_best);
   hipFree(b.d_roff);
   hipFree(b.d_stall);
-  hipFree(b.d_active);
   hipFree(b.d_results);
   hipFree(b.d_iter);
-  if (b.h_active) hipHostFree(b.h_active);
+  if (b.h_act) hipHostFree(b.h_act);
+  if (b.h_sw) hipHostFree(b.h_sw);
   if (b.h_results) hipHostFree(b.h_results);
-  b.d_probs = nullptr;
-  b.d_podd = nullptr;
-  b.d_peven = nullptr;
+  b.d_conv = b.d_sw = nullptr;
   b.d_best = nullptr;
   b.d_roff = nullptr;
   b.d_stall = nullptr;
-  b.d_active = nullptr;
   b.d_results = nullptr;
   b.d_iter = nullptr;
-  b.h_active = nullptr;
+  b.h_act = b.h_sw = nullptr;
   b.h_results = nullptr;
   b.built = false;
 }
@@ -147,6 +174,9 @@ static int batch_build(Batch& b) {
       b.nbmap[k] = (int)map.size();
       if (dev_alloc((void**)&b.d_bmap[k][t], sizeof(int2) * map.size(), "batch map")) return GFX_ERR_ALLOC;
       hipMemcpy(b.d_bmap[k][t], map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice);
+      b.h_order[k][t] = list;
+      for (int pr = 0; pr < 2; pr++)
+        if (dev_alloc((void**)&b.cmap[pr][k][t].d, sizeof(int2) * map.size(), "batch map")) return GFX_ERR_ALLOC;
     }
     std::vector<int> poff(n + 1, 0);
     for (int q = 0; q < n; q++) poff[q + 1] = poff[q] + (b.probs[q].geo.nh + BLOCKS[k] - 1) / BLOCKS[k];
@@ -161,16 +191,21 @@ static int batch_build(Batch& b) {
   // partial slots: as many as blocks of the smallest workgroup
   size_t npart = 0;
   for (int q = 0; q < n; q++) npart += (b.probs[q].geo.nh + 63) / 64;
-  if (dev_alloc((void**)&b.d_podd, sizeof(double) * 3 * npart, "batch partials") ||
-      dev_alloc((void**)&b.d_peven, sizeof(double) * 3 * npart, "batch partials") ||
-      dev_alloc((void**)&b.d_probs, sizeof(BatchProblem) * n, "batch problems") ||
-      dev_alloc((void**)&b.d_best, sizeof(double) * n, "batch best") ||
+  for (int k = 0; k < 2; k++)
+    if (dev_alloc((void**)&b.d_pod[k], sizeof(double) * 3 * npart, "batch partials") ||
+        dev_alloc((void**)&b.d_pev[k], sizeof(double) * 3 * npart, "batch partials") ||
+        dev_alloc((void**)&b.d_probs[k], sizeof(BatchProblem) * n, "batch problems") ||
+        dev_alloc((void**)&b.d_act[k], sizeof(int) * n, "batch flags"))
+      return GFX_ERR_ALLOC;
+  if (dev_alloc((void**)&b.d_best, sizeof(double) * n, "batch best") ||
       dev_alloc((void**)&b.d_stall, sizeof(int) * n, "batch stall") ||
-      dev_alloc((void**)&b.d_active, sizeof(int) * n, "batch active") ||
+      dev_alloc((void**)&b.d_conv, sizeof(int) * n, "batch conversions") ||
+      dev_alloc((void**)&b.d_sw, sizeof(int) * n, "batch switches") ||
       dev_alloc((void**)&b.d_results, sizeof(double) * 4 * n, "batch results") ||
       dev_alloc((void**)&b.d_iter, sizeof(int), "batch iter"))
     return GFX_ERR_ALLOC;
-  if (hipHostMalloc((void**)&b.h_active, sizeof(int) * n, 0) != hipSuccess ||
+  if (hipHostMalloc((void**)&b.h_act, sizeof(int) * 2 * n, 0) != hipSuccess ||
+      hipHostMalloc((void**)&b.h_sw, sizeof(int) * n, 0) != hipSuccess ||
       hipHostMalloc((void**)&b.h_results, sizeof(double) * 4 * n, 0) != hipSuccess)
     return GFX_ERR_ALLOC;
   b.built = true;
@@ -182,36 +217,40 @@ static V* bfield(V* g, const Geom& geo, int k) {
   return g + (size_t)k * Store<V>::NPLANE * geo.stride;
 }
 
-// canonical field pointers of the phase's precision: even = field 0, odd =
-// field 1, spare = field 2
-static void set_phase_pointers(Batch& b, bool fp32) {
-  std::vector<BatchProblem> h(b.probs.size());
-  for (size_t q = 0; q < b.probs.size(); q++) {
+// canonical field pointers in both precisions: even = field 0, odd =
+// field 1, spare = field 2 (FP32 pointers only where FP32 fields exist)
+static void set_pointers(Batch& b) {
+  const size_t n = b.probs.size();
+  std::vector<BatchProblem> h64(n), h32(n);
+  for (size_t q = 0; q < n; q++) {
     const BProbHost& p = b.probs[q];
     const BProbHost& o = b.probs[p.ushare];
-    h[q].geo = p.geo;
-    h[q].U = fp32 ? (const void*)o.U32 : (const void*)o.U64;
-    for (int k = 0; k < 2; k++)
-      h[q].g[k] = fp32 ? (void*)bfield(p.g32, p.geo, k) : (void*)bfield(p.g64, p.geo, k);
-    h[q].galt = fp32 ? (void*)bfield(p.g32, p.geo, 2) : (void*)bfield(p.g64, p.geo, 2);
+    h64[q].geo = h32[q].geo = p.geo;
+    h64[q].U = o.U64;
+    h32[q].U = o.U32;
+    for (int k = 0; k < 2; k++) {
+      h64[q].g[k] = bfield(p.g64, p.geo, k);
+      h32[q].g[k] = p.g32 ? (void*)bfield(p.g32, p.geo, k) : nullptr;
+    }
+    h64[q].galt = bfield(p.g64, p.geo, 2);
+    h32[q].galt = p.g32 ? (void*)bfield(p.g32, p.geo, 2) : nullptr;
   }
   hipStreamSynchronize(g_st.stream);
-  hipMemcpy(b.d_probs, h.data(), sizeof(BatchProblem) * h.size(), hipMemcpyHostToDevice);
+  hipMemcpy(b.d_probs[0], h64.data(), sizeof(BatchProblem) * n, hipMemcpyHostToDevice);
+  hipMemcpy(b.d_probs[1], h32.data(), sizeof(BatchProblem) * n, hipMemcpyHostToDevice);
 }
 
-// after a phase: copy each even field the checks left in the spare slot back
-// to field 0
-static void batch_canonical(Batch& b, bool fp32) {
+// after a run: copy each FP64 even field the checks left in the spare slot
+// back to field 0
+static void batch_canonical(Batch& b) {
   State& s = g_st;
   std::vector<BatchProblem> h(b.probs.size());
   hipStreamSynchronize(s.stream);
-  hipMemcpy(h.data(), b.d_probs, sizeof(BatchProblem) * h.size(), hipMemcpyDeviceToHost);
+  hipMemcpy(h.data(), b.d_probs[0], sizeof(BatchProblem) * h.size(), hipMemcpyDeviceToHost);
   for (size_t q = 0; q < b.probs.size(); q++) {
     const BProbHost& p = b.probs[q];
-    void* f0 = fp32 ? (void*)p.g32 : (void*)p.g64;
-    if (h[q].g[0] != f0)
-      hipMemcpyAsync(f0, h[q].g[0], fp32 ? field_bytes32(p.geo) : field_bytes64(p.geo),
-                     hipMemcpyDeviceToDevice, s.stream);
+    if (h[q].g[0] != (void*)p.g64)
+      hipMemcpyAsync(p.g64, h[q].g[0], field_bytes64(p.geo), hipMemcpyDeviceToDevice, s.stream);
   }
   hipStreamSynchronize(s.stream);
 }
@@ -230,119 +269,210 @@ static int ensure_fp32(Batch& b) {
   return GFX_OK;
 }
 
-// one half sweep of every active lattice; chk as in launch_batch_sweep
-static void batch_half(Batch& b, bool fp32, int parity, int variant, int chk) {
+// the workgroup map of precision pr for variant v: the blocks of the
+// lattices sweeping in pr at the last census, in the variant's order, dealt
+// to the XCDs (stopped or other-precision lattices cost no workgroups)
+static const Batch::CMap& batch_map(Batch& b, int pr, int v) {
   State& s = g_st;
-  const int k = vblock_index(variant);
-  launch_batch_sweep(fp32, b.d_probs, b.d_bmap[k][sweep_variant_tiling(variant)], b.nbmap[k],
-                     b.d_active, b.d_poff[k], parity, s.omega, variant, chk,
-                     chk == 2 ? b.d_peven : b.d_podd, s.stream);
+  const int k = vblock_index(v), t = sweep_variant_tiling(v);
+  Batch::CMap& m = b.cmap[pr][k][t];
+  if (m.version != b.census_version) {
+    const size_t n = b.probs.size();
+    std::vector<int2> list;
+    list.reserve(b.h_order[k][t].size());
+    for (const int2& e : b.h_order[k][t])
+      if (b.h_act[pr * n + e.x]) list.push_back(e);
+    deal_to_xcds(list, m.h);
+    m.n = (int)m.h.size();
+    if (m.n > 0) hipMemcpyAsync(m.d, m.h.data(), sizeof(int2) * m.n, hipMemcpyHostToDevice, s.stream);
+    m.version = b.census_version;
+  }
+  return m;
 }
 
-// the end of a check at iteration *d_iter: the even sweep of the next
-// iteration carrying the check, then the per-lattice grid pass
-static void batch_check_tail(Batch& b, bool fp32, int variant, double thr, int maxit,
-                             int stall_checks) {
+// one half sweep in precision pr (0 FP64, 1 FP32) of the lattices sweeping
+// in it; chk as in launch_batch_sweep
+static void batch_half(Batch& b, int pr, int parity, int chk) {
   State& s = g_st;
-  batch_half(b, fp32, 0, variant, 2);
-  launch_batch_finalize(b.d_podd, b.d_peven, b.d_poff[vblock_index(variant)], b.d_probs,
-                        (int)b.probs.size(), 4, thr, b.d_active, b.d_results, b.d_iter, maxit,
-                        b.d_best, b.d_stall, stall_checks, s.stall_gate, s.stream);
+  const int v = b.variant[pr];
+  const Batch::CMap& m = batch_map(b, pr, v);
+  if (m.n == 0) return;
+  launch_batch_sweep(pr == 1, b.d_probs[pr], m.d, m.n, b.d_act[pr], b.d_poff[vblock_index(v)],
+                     parity, s.omega, v, chk, chk == 2 ? b.d_pev[pr] : b.d_pod[pr], s.stream);
 }
 
-static int count_active(Batch& b) {
+struct RunParams {
+  double eps, switch_theta;
+  int ck, maxit, periods, stall_checks;
+};
+
+// the end of the check of iteration *d_iter: the even sweeps of the next
+// iteration carrying the check, the per-lattice grid pass, and the
+// conversions of the lattices that leave FP32
+static void batch_check_tail(Batch& b, const bool use[2], const RunParams& rp) {
   State& s = g_st;
-  hipMemcpyAsync(b.h_active, b.d_active, sizeof(int) * b.probs.size(), hipMemcpyDeviceToHost, s.stream);
-  if (hip_check(hipStreamSynchronize(s.stream), "batch flags")) return -1;
-  int n = 0;
-  for (size_t q = 0; q < b.probs.size(); q++) n += b.h_active[q] ? 1 : 0;
-  return n;
+  for (int pr = 0; pr < 2; pr++)
+    if (use[pr]) batch_half(b, pr, 0, 2);
+  BatchCheck c;
+  for (int pr = 0; pr < 2; pr++) {
+    c.pod[pr] = b.d_pod[pr];
+    c.pev[pr] = b.d_pev[pr];
+    c.poff[pr] = b.d_poff[vblock_index(b.variant[pr])];
+    c.probs[pr] = b.d_probs[pr];
+    c.act[pr] = b.d_act[pr];
+    c.ran[pr] = use[pr];
+  }
+  c.conv = b.d_conv;
+  c.sw = b.d_sw;
+  c.results = b.d_results;
+  c.iter = b.d_iter;
+  c.best = b.d_best;
+  c.stall = b.d_stall;
+  const int n = (int)b.probs.size();
+  launch_batch_finalize(c, n, 4, rp.eps, rp.switch_theta, rp.maxit, rp.stall_checks, s.stall_gate,
+                        s.stream);
+  if (use[1]) {
+    const int k = block_index(batch_convert_block());
+    launch_batch_convert(b.d_probs[1], b.d_probs[0], b.d_bmap[k][0], b.nbmap[k], b.d_conv,
+                         s.stream);
+    launch_fill_i32(b.d_conv, n, 0, s.stream);
+  }
 }
 
-// Runs one phase (FP32 bulk or FP64) from global iteration *it until every
-// lattice has stopped (max dAdA < thr, or an FP32 stall) or maxit is reached.
-// The first iterations run eagerly (the sweep variants are timed on them when
-// not tuned yet) up to the iteration after a check; then graphs of `periods`
-// check periods are replayed while they end before maxit, and the rest runs
-// eagerly again.
-static int batch_phase(Batch& b, bool fp32, double thr, int ck, int maxit, int periods, int* it,
-                       int stall_checks) {
+// lattices sweeping in each precision (reads the device flags)
+static int census(Batch& b, int cnt[2]) {
+  State& s = g_st;
+  const size_t n = b.probs.size();
+  hipMemcpyAsync(b.h_act, b.d_act[0], sizeof(int) * n, hipMemcpyDeviceToHost, s.stream);
+  hipMemcpyAsync(b.h_act + n, b.d_act[1], sizeof(int) * n, hipMemcpyDeviceToHost, s.stream);
+  hipMemcpyAsync(b.h_sw, b.d_sw, sizeof(int) * n, hipMemcpyDeviceToHost, s.stream);
+  if (hip_check(hipStreamSynchronize(s.stream), "batch flags")) return GFX_ERR_DEVICE;
+  cnt[0] = cnt[1] = 0;
+  bool changed = false;
+  for (size_t q = 0; q < n; q++) {
+    cnt[0] += b.h_act[q] ? 1 : 0;
+    cnt[1] += b.h_act[n + q] ? 1 : 0;
+    changed |= (b.h_act[q] != b.h_act_prev[q]) || (b.h_act[n + q] != b.h_act_prev[n + q]);
+    b.h_act_prev[q] = b.h_act[q];
+    b.h_act_prev[n + q] = b.h_act[n + q];
+  }
+  if (changed) b.census_version++;
+  return GFX_OK;
+}
+
+// The run: every lattice sweeps in FP32 (mixed) or FP64 from iteration 0;
+// a check every ck iterations may stop it or move it from FP32 to FP64.
+// Eager iterations (which also time the sweep variants of a precision the
+// first time it has lattices) run up to the iteration after a check; then a
+// graph of `periods` check periods, with the sweeps of the precisions that
+// have lattices, is captured and replayed while it ends before maxit and no
+// lattice changed precision; then back to eager iterations.  A lattice that
+// leaves FP32 inside a replay whose graph has no FP64 sweeps waits for the
+// replay to end (its `idle` count; its stop iteration is reported without
+// the wait).
+static int batch_loop(Batch& b, bool mixed, const RunParams& rp, int* it_end) {
   State& s = g_st;
   const int n = (int)b.probs.size();
-  const int pi = fp32 ? 1 : 0;
-  set_phase_pointers(b, fp32);
-  launch_fill_i32(b.d_active, n, 1, s.stream);
+  const int ck = rp.ck;
+  set_pointers(b);
+  launch_fill_i32(b.d_act[0], n, mixed ? 0 : 1, s.stream);
+  launch_fill_i32(b.d_act[1], n, mixed ? 1 : 0, s.stream);
+  launch_fill_i32(b.d_conv, n, 0, s.stream);
+  launch_fill_i32(b.d_sw, n, -1, s.stream);
   launch_fill_i32(b.d_stall, n, 0, s.stream);
   launch_fill_f64(b.d_best, n, 1e300, s.stream);
-  int i = *it;
+  launch_fill_f64(b.d_results, 4 * n, -1.0, s.stream);
+  int cnt[2] = {mixed ? 0 : n, mixed ? n : 0};
+  b.h_act_prev.assign(2 * n, 0);
+  for (int q = 0; q < n; q++) {
+    b.h_act[q] = b.h_act_prev[q] = mixed ? 0 : 1;
+    b.h_act[n + q] = b.h_act_prev[n + q] = mixed ? 1 : 0;
+  }
+  b.census_version++;
+  int i = 0, rc = GFX_OK;
   bool even_done = false;  // the even half of iteration i ran with the last check
-  int tune_v = b.tuned[pi] ? -1 : 0;
-  int rc = GFX_OK;
-  auto count_iter = [&]() {
-    if (fp32) b.iters32++; else b.iters64++;
+  int tune_v[2] = {b.tuned[0] ? -1 : 0, b.tuned[1] ? -1 : 0};
+  std::vector<int> sw_before(n, -1);
+  // after a check (census just read): a lattice that left FP32 while the
+  // sweeps launched had no FP64 part waited until iteration `resume`
+  auto after_check = [&](int resume, bool fp64_ran) {
+    for (int q = 0; q < n; q++) {
+      const int sw = b.h_sw[q];
+      if (sw >= 0 && sw_before[q] < 0 && !fp64_ran && b.h_act[q] && resume > sw + 1)
+        b.probs[q].idle += resume - (sw + 1);
+      sw_before[q] = sw;
+    }
   };
-  // one eager iteration; false when every lattice has stopped
-  auto eager = [&](int v, bool timed) {
+  // one eager iteration
+  auto eager = [&]() {
     const bool chk = (i % ck) == 0;
     const bool both = !even_done;
-    if (timed && both) hipEventRecord(s.ev0, s.stream);
-    if (both) batch_half(b, fp32, 0, v, 0);
-    if (timed && !both) hipEventRecord(s.ev0, s.stream);
-    batch_half(b, fp32, 1, v, chk ? 1 : 0);
-    if (timed) {
-      hipEventRecord(s.ev1, s.stream);
-      hipEventSynchronize(s.ev1);
-      float t = 0.f;
-      hipEventElapsedTime(&t, s.ev0, s.ev1);
-      b.tune_ms[pi][tune_v] = both ? 0.5f * t : t;
+    const bool use[2] = {cnt[0] > 0, cnt[1] > 0};
+    int keep[2] = {b.variant[0], b.variant[1]};
+    bool timed[2] = {false, false};
+    for (int pr = 0; pr < 2; pr++) {
+      if (!use[pr]) continue;
+      // time full non-check iterations when there are any (ck >= 3), else
+      // every iteration (the check sweeps then use the variant being timed)
+      timed[pr] = tune_v[pr] >= 0 && ((ck >= 3) ? (both && !chk) : true);
+      if (timed[pr]) b.variant[pr] = tune_v[pr];
+      if (timed[pr]) hipEventRecord(s.ev0, s.stream);
+      if (both) batch_half(b, pr, 0, 0);
+      batch_half(b, pr, 1, chk ? 1 : 0);
+      if (timed[pr]) {
+        hipEventRecord(s.ev1, s.stream);
+        hipEventSynchronize(s.ev1);
+        float t = 0.f;
+        hipEventElapsedTime(&t, s.ev0, s.ev1);
+        b.tune_ms[pr][tune_v[pr]] = both ? 0.5f * t : t;
+      }
+      if (pr == 1) b.iters32++; else b.iters64++;
     }
     even_done = false;
-    count_iter();
-    bool go_on = true;
     if (chk) {
       launch_fill_i32(b.d_iter, 1, i, s.stream);
-      batch_check_tail(b, fp32, v, thr, maxit, stall_checks);
+      batch_check_tail(b, use, rp);
       even_done = true;
-      go_on = count_active(b) > 0;
+      if (census(b, cnt)) rc = GFX_ERR_DEVICE;
+      after_check(i + 1, true);
+    }
+    for (int pr = 0; pr < 2; pr++) {
+      if (!timed[pr]) continue;
+      b.variant[pr] = keep[pr];
+      if (++tune_v[pr] == SWEEP_VARIANTS) {
+        int best = 0;
+        for (int k = 1; k < SWEEP_VARIANTS; k++)
+          if (b.tune_ms[pr][k] < b.tune_ms[pr][best]) best = k;
+        b.variant[pr] = best;
+        b.tuned[pr] = true;
+        tune_v[pr] = -1;
+      }
     }
     i++;
-    return go_on;
   };
-  bool go_on = true;
-  while (go_on && i < maxit) {
-    int v = b.variant[pi];
-    bool timed = false;
-    if (tune_v >= 0) {
-      // time full non-check iterations when there are any (ck >= 3), else
-      // the odd half sweeps
-      const bool chk = (i % ck) == 0;
-      timed = (ck >= 3) ? (!even_done && !chk) : true;
-      if (timed) v = tune_v;
-    }
-    go_on = eager(v, timed);
-    if (timed && ++tune_v == SWEEP_VARIANTS) {
-      int best = 0;
-      for (int k = 1; k < SWEEP_VARIANTS; k++)
-        if (b.tune_ms[pi][k] < b.tune_ms[pi][best]) best = k;
-      b.variant[pi] = best;
-      b.tuned[pi] = true;
-      tune_v = -1;
-    }
-    if (tune_v < 0 && even_done) break;  // just after a check: periods start here
-  }
-  if (go_on && i + periods * ck <= maxit) {
-    // graph of `periods` check periods over iterations i .. i + ck - 1, ...:
-    // the even half of the first iteration ran with the previous check
+  while (rc == GFX_OK && i < rp.maxit && cnt[0] + cnt[1] > 0) {
+    // eager until tuned (for the precisions in use) and just after a check
+    do {
+      eager();
+    } while (rc == GFX_OK && i < rp.maxit && cnt[0] + cnt[1] > 0 &&
+             !(even_done && (cnt[0] == 0 || tune_v[0] < 0) && (cnt[1] == 0 || tune_v[1] < 0)));
+    if (rc != GFX_OK || cnt[0] + cnt[1] == 0 || i + rp.periods * ck > rp.maxit) continue;
+    // graph of `periods` check periods for the precisions in use
+    const bool use[2] = {cnt[0] > 0, cnt[1] > 0};
+    // maps are uploaded before the capture (no copy may enter the graph)
+    for (int pr = 0; pr < 2; pr++)
+      if (use[pr]) batch_map(b, pr, b.variant[pr]);
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    const int v = b.variant[pi];
     if (hip_check(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal), "capture")) return GFX_ERR_DEVICE;
-    for (int per = 0; per < periods; per++) {
+    for (int per = 0; per < rp.periods; per++) {
       for (int j = 0; j < ck; j++) {
-        if (j > 0) batch_half(b, fp32, 0, v, 0);
-        batch_half(b, fp32, 1, v, j == ck - 1 ? 1 : 0);
+        for (int pr = 0; pr < 2; pr++)
+          if (use[pr] && j > 0) batch_half(b, pr, 0, 0);
+        for (int pr = 0; pr < 2; pr++)
+          if (use[pr]) batch_half(b, pr, 1, j == ck - 1 ? 1 : 0);
       }
-      batch_check_tail(b, fp32, v, thr, maxit, stall_checks);
+      batch_check_tail(b, use, rp);
       launch_batch_advance(b.d_iter, ck, s.stream);
     }
     if (hip_check(hipStreamEndCapture(s.stream, &graph), "end capture")) return GFX_ERR_DEVICE;
@@ -351,25 +481,45 @@ static int batch_phase(Batch& b, bool fp32, double thr, int ck, int maxit, int p
       return GFX_ERR_DEVICE;
     }
     launch_fill_i32(b.d_iter, 1, i + ck - 1, s.stream);
-    while (go_on && i + periods * ck <= maxit) {
+    const long version_at_capture = b.census_version;
+    while (rc == GFX_OK && i + rp.periods * ck <= rp.maxit) {
       if (hip_check(hipGraphLaunch(exec, s.stream), "graph launch")) {
         rc = GFX_ERR_DEVICE;
         break;
       }
       b.graph_launches++;
-      i += periods * ck;
-      if (fp32) b.iters32 += periods * ck; else b.iters64 += periods * ck;
-      const int na = count_active(b);
-      if (na < 0) rc = GFX_ERR_DEVICE;
-      go_on = na > 0;
+      i += rp.periods * ck;
+      if (use[1]) b.iters32 += rp.periods * ck;
+      if (use[0]) b.iters64 += rp.periods * ck;
+      int c2[2];
+      if (census(b, c2)) {
+        rc = GFX_ERR_DEVICE;
+        break;
+      }
+      after_check(i, use[0]);
+      cnt[0] = c2[0];
+      cnt[1] = c2[1];
+      if (cnt[0] + cnt[1] == 0) break;
+      if (b.census_version != version_at_capture) break;  // a lattice stopped or switched
     }
     hipGraphExecDestroy(exec);
     hipGraphDestroy(graph);
   }
-  // tail: the iterations left before maxit
-  while (rc == GFX_OK && go_on && i < maxit) go_on = eager(b.variant[pi], false);
-  batch_canonical(b, fp32);
-  *it = i;
+  if (mixed && rc == GFX_OK) {
+    // lattices still in FP32 at maxit: their checked state to FP64
+    int c2[2];
+    if (census(b, c2) == GFX_OK && c2[1] > 0) {
+      std::vector<int> cv(n);
+      for (int q = 0; q < n; q++) cv[q] = b.h_act[n + q] ? 2 : 0;
+      hipMemcpy(b.d_conv, cv.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+      const int k = block_index(batch_convert_block());
+      launch_batch_convert(b.d_probs[1], b.d_probs[0], b.d_bmap[k][0], b.nbmap[k], b.d_conv,
+                           s.stream);
+      launch_fill_i32(b.d_conv, n, 0, s.stream);
+    }
+  }
+  batch_canonical(b);
+  *it_end = i;
   return rc;
 }
 
@@ -454,40 +604,36 @@ int gfx_batch_run(int maxit, double eps, int checkinterval, int precision, doubl
   const bool mixed = (precision == GFX_PREC_MIXED);
   if (mixed && ensure_fp32(b)) return GFX_ERR_ALLOC;
   b.graph_launches = b.iters32 = b.iters64 = 0;
-  for (auto& p : b.probs) p.stop_iter[0] = p.stop_iter[1] = -1;
+  for (auto& p : b.probs) {
+    p.stop_iter = p.switch_iter = -1;
+    p.idle = 0;
+  }
+  RunParams rp;
+  rp.eps = eps;
+  rp.switch_theta = mixed ? std::max(switch_theta > 0 ? switch_theta : s.switch_theta, eps) : 0.0;
+  rp.ck = checkinterval;
+  rp.maxit = maxit;
+  rp.periods = periods_per_graph;
+  rp.stall_checks = mixed ? (s.stall_iters + checkinterval - 1) / checkinterval : 0;
   hipStreamSynchronize(s.stream);
   const auto t0 = std::chrono::steady_clock::now();
-  int it = 0, rc = GFX_OK;
-  if (mixed) {
-    launch_fill_f64(b.d_results, 4 * (int)b.probs.size(), -1.0, s.stream);
-    rc = batch_phase(b, true, std::max(switch_theta > 0 ? switch_theta : s.switch_theta, eps),
-                     checkinterval, maxit, periods_per_graph, &it,
-                     (s.stall_iters + checkinterval - 1) / checkinterval);
-    hipMemcpyAsync(b.h_results, b.d_results, sizeof(double) * 4 * b.probs.size(), hipMemcpyDeviceToHost, s.stream);
-    hipStreamSynchronize(s.stream);
-    for (size_t q = 0; q < b.probs.size(); q++) {
-      b.probs[q].stop_iter[0] = (int)b.h_results[4 * q + 3];
-      launch_g32_to_g64(b.probs[q].g32, b.probs[q].g64, b.probs[q].geo, s.stream);
-    }
-  }
-  const int it_fp64_start = it;
-  if (rc == GFX_OK) {
-    launch_fill_f64(b.d_results, 4 * (int)b.probs.size(), -1.0, s.stream);
-    rc = batch_phase(b, false, eps, checkinterval, maxit, periods_per_graph, &it, 0);
-  }
+  int it = 0;
+  int rc = batch_loop(b, mixed, rp, &it);
   hipMemcpyAsync(b.h_results, b.d_results, sizeof(double) * 4 * b.probs.size(), hipMemcpyDeviceToHost, s.stream);
+  hipMemcpyAsync(b.h_sw, b.d_sw, sizeof(int) * b.probs.size(), hipMemcpyDeviceToHost, s.stream);
   hipStreamSynchronize(s.stream);
   b.run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   int nconv = 0;
   b.site_updates = 0.0;
   for (size_t q = 0; q < b.probs.size(); q++) {
     BProbHost& p = b.probs[q];
-    p.stop_iter[1] = (int)b.h_results[4 * q + 3];
-    if (p.stop_iter[1] >= 0) nconv++;
-    // iterations this lattice was updated in each phase
-    const int n32 = mixed ? (p.stop_iter[0] >= 0 ? p.stop_iter[0] + 1 : it_fp64_start) : 0;
-    const int n64 = (p.stop_iter[1] >= 0 ? p.stop_iter[1] + 1 : it) - it_fp64_start;
-    b.site_updates += 2.0 * p.geo.nh * (double)(n32 + n64);
+    const int stop = (int)b.h_results[4 * q + 3];
+    p.switch_iter = b.h_sw[q];
+    // iterations this lattice ran: up to its stop check, without waits
+    p.stop_iter = stop >= 0 ? stop - p.idle : -1;
+    if (stop >= 0) nconv++;
+    const int ran = (stop >= 0 ? stop + 1 : it) - p.idle;
+    b.site_updates += 2.0 * p.geo.nh * (double)ran;
   }
   if (rc != GFX_OK) return rc;
   return nconv == (int)b.probs.size() ? GFX_OK : GFX_ERR_NOCONV;
@@ -518,7 +664,7 @@ int gfx_batch_anneal(int nsteps, double Tmin, double Tmax, double expo, int nmic
   if (nu > (1L << 31) - 4096 * 2) return GFX_ERR_GEOMETRY;
   int rc = mt_prepare(s, (int)ng, (int)nu);
   if (rc) return rc;
-  set_phase_pointers(b, false);
+  set_pointers(b);
   std::vector<double> betas(nsteps > 0 ? nsteps : 1);
   for (int i = 0; i < nsteps; i++) betas[i] = 1.0 / gfx_sa_temperature(i, Tmin, Tmax, expo, nsteps);
   double* d_betas = nullptr;
@@ -535,12 +681,12 @@ int gfx_batch_anneal(int nsteps, double Tmin, double Tmax, double expo, int nmic
   const int nblk = b.nbmap[k];
   auto one_step = [&]() {
     for (int p = 0; p < 2; p++) {
-      launch_batch_sa_sweep(b.d_probs, map, nblk, b.d_roff, p, d_betas, d_step, sampler,
+      launch_batch_sa_sweep(b.d_probs[0], map, nblk, b.d_roff, p, d_betas, d_step, sampler,
                             s.rnd_gauss, s.rnd_unif, s.stream);
       mt_refill(s, 1);
     }
     for (int m = 0; m < nmicro; m++)
-      for (int p = 0; p < 2; p++) launch_batch_micro_sweep(b.d_probs, map, nblk, p, s.stream);
+      for (int p = 0; p < 2; p++) launch_batch_micro_sweep(b.d_probs[0], map, nblk, p, s.stream);
     launch_batch_advance(d_step, 1, s.stream);
   };
   const auto t0 = std::chrono::steady_clock::now();
@@ -596,7 +742,7 @@ int gfx_batch_result(int q, double* F, double* dAdA, double* maxdAdA, int* iter)
   if (F) *F = g_b.h_results[4 * q];
   if (dAdA) *dAdA = g_b.h_results[4 * q + 1];
   if (maxdAdA) *maxdAdA = g_b.h_results[4 * q + 2];
-  if (iter) *iter = g_b.probs[q].stop_iter[1];
+  if (iter) *iter = g_b.probs[q].stop_iter;
   return GFX_OK;
 }
 

@@ -75,10 +75,27 @@ void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t st
 void launch_batch_sweep(bool fp32, const BatchProblem* probs, const int2* bmap, int nblocks,
                         const int* active, const int* poff, int parity, double omega, int variant,
                         int chk, double* partials, hipStream_t st);
-void launch_batch_finalize(const double* podd, const double* peven, const int* poff,
-                           BatchProblem* probs, int nprob, int ndir, double thr, int* active,
-                           double* results, const int* iter, int maxit, double* best, int* stall,
-                           int stall_checks, double stall_gate, hipStream_t st);
+// device arrays of a batch check, index 0 = FP64 phase, 1 = FP32 phase
+struct BatchCheck {
+  const double* pod[2];  // block partials of the odd check sweeps
+  const double* pev[2];  // and of the even ones
+  const int* poff[2];    // partial offsets per lattice (block size of the phase's variant)
+  bool ran[2];           // the precision's sweeps ran in this check period
+  BatchProblem* probs[2];
+  int* act[2];
+  int* conv;
+  int* sw;
+  double* results;
+  const int* iter;
+  double* best;
+  int* stall;
+};
+void launch_batch_finalize(const BatchCheck& c, int nprob, int ndir, double eps,
+                           double switch_theta, int maxit, int stall_checks, double stall_gate,
+                           hipStream_t st);
+void launch_batch_convert(const BatchProblem* probs32, const BatchProblem* probs64,
+                          const int2* bmap, int nblocks, const int* conv, hipStream_t st);
+int batch_convert_block();
 void launch_batch_advance(int* iter, int step, hipStream_t st);
 // batched annealing (gfx_heatbath.hip); bmap of sa_block_size() blocks
 void launch_batch_sa_sweep(const BatchProblem* probs, const int2* bmap, int nblocks,

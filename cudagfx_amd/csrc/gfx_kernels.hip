@@ -399,53 +399,110 @@ __global__ __launch_bounds__(BLK) void k_functional(const V* __restrict__ U,
   }
 }
 
-// Grid-level pass of a batch's fused check, one block per problem over its
-// block partials [poff[q], poff[q+1]) of the odd (podd) and even (peven)
-// check sweeps: results[4q..] = F/(ndir V), dAdA/V, max dAdA.  A problem
-// stops (active = 0, results[4q+3] = *iter) when max < thr, or, with
-// stall_checks > 0, when its best max dAdA has stayed below stall_gate
-// without halving for stall_checks checks (the FP32 phase has reached its
-// rounding floor).  A problem that goes on takes the even field of the
-// check sweep (swap g[0] and galt) unless *iter + 1 == maxit, where the run
-// ends with the state of the check.
+// Grid-level pass of a batch's fused check, one block per lattice.  A
+// lattice is in the FP64 phase (act64), the FP32 phase (act32) or stopped;
+// its block partials of the odd (pod) and even (pev) check sweeps of its
+// phase lie at [poff[q], poff[q+1]) of that phase's arrays; ran64 / ran32
+// tell which precisions' sweeps ran in the period of this check.
+// results[4q..] = F/(ndir V), dAdA/V, max dAdA, stop iteration.
+// FP64: stop when max < eps, else take the check sweep's even field (swap
+// g[0] and galt) unless *iter + 1 == maxit (the run ends on the checked
+// state).  FP32: leave the phase when max < switch_theta or when the best
+// max has stayed below stall_gate without halving for stall_checks checks;
+// the lattice then continues in FP64 from the FP32 state after the check
+// sweep (conv = 1: even field from galt), or, when it already reached eps or
+// the run ends, stops with the checked state converted (conv = 2).  sw[q]
+// records the iteration of the switch.
 __global__ __launch_bounds__(BLK) void k_batch_finalize(
-    const double* __restrict__ podd, const double* __restrict__ peven,
-    const int* __restrict__ poff, BatchProblem* __restrict__ probs, int ndir, double thr,
-    int* __restrict__ active, double* __restrict__ results, const int* __restrict__ iter,
-    int maxit, double* __restrict__ best, int* __restrict__ stall, int stall_checks,
-    double stall_gate) {
+    const double* __restrict__ pod64, const double* __restrict__ pev64,
+    const double* __restrict__ pod32, const double* __restrict__ pev32,
+    const int* __restrict__ poff64, const int* __restrict__ poff32, int ran64, int ran32,
+    BatchProblem* __restrict__ probs64, BatchProblem* __restrict__ probs32, int ndir, double eps,
+    double switch_theta, int* __restrict__ act64, int* __restrict__ act32,
+    int* __restrict__ conv, int* __restrict__ sw, double* __restrict__ results,
+    const int* __restrict__ iter, int maxit, double* __restrict__ best, int* __restrict__ stall,
+    int stall_checks, double stall_gate) {
   const int q = blockIdx.x;
-  if (!active[q]) return;
+  const bool f64 = act64[q] != 0;
+  if (!f64 && !act32[q]) return;
+  // a lattice whose precision had no sweeps at this check (it just left
+  // FP32 inside a graph without FP64 sweeps) waits
+  if (f64 ? !ran64 : !ran32) return;
+  const double* pod = f64 ? pod64 : pod32;
+  const double* pev = f64 ? pev64 : pev32;
+  const int* poff = f64 ? poff64 : poff32;
   double F = 0.0, d = 0.0, mx = 0.0;
   for (int b = poff[q] + threadIdx.x; b < poff[q + 1]; b += BLK) {
-    F += podd[3 * (size_t)b];
-    d += podd[3 * (size_t)b + 1] + peven[3 * (size_t)b + 1];
-    mx = fmax(mx, fmax(podd[3 * (size_t)b + 2], peven[3 * (size_t)b + 2]));
+    F += pod[3 * (size_t)b];
+    d += pod[3 * (size_t)b + 1] + pev[3 * (size_t)b + 1];
+    mx = fmax(mx, fmax(pod[3 * (size_t)b + 2], pev[3 * (size_t)b + 2]));
   }
   block_reduce3(F, d, mx);
-  if (threadIdx.x == 0) {
-    const double V = 2.0 * probs[q].geo.nh;
-    results[4 * q + 0] = F / (ndir * V);
-    results[4 * q + 1] = d / V;
-    results[4 * q + 2] = mx;
-    bool stop = mx < thr;
-    if (!stop && stall_checks > 0) {
-      if (mx < 0.5 * best[q]) {
-        best[q] = mx;
-        stall[q] = 0;
-      } else if (best[q] < stall_gate && ++stall[q] >= stall_checks) {
-        stop = true;
-      }
+  if (threadIdx.x != 0) return;
+  const double V = 2.0 * probs64[q].geo.nh;
+  const int it = *iter;
+  const bool last = it + 1 >= maxit;
+  results[4 * q + 0] = F / (ndir * V);
+  results[4 * q + 1] = d / V;
+  results[4 * q + 2] = mx;
+  if (f64) {
+    if (mx < eps) {
+      results[4 * q + 3] = (double)it;
+      act64[q] = 0;
+    } else if (!last) {
+      void* t = probs64[q].g[0];
+      probs64[q].g[0] = probs64[q].galt;
+      probs64[q].galt = t;
     }
-    if (stop) {
-      results[4 * q + 3] = (double)(*iter);
-      active[q] = 0;
-    } else if (*iter + 1 < maxit) {
-      void* t = probs[q].g[0];
-      probs[q].g[0] = probs[q].galt;
-      probs[q].galt = t;
+    return;
+  }
+  const bool stop = mx < eps;
+  bool leave = stop || mx < switch_theta;
+  if (!leave && stall_checks > 0) {
+    if (mx < 0.5 * best[q]) {
+      best[q] = mx;
+      stall[q] = 0;
+    } else if (best[q] < stall_gate && ++stall[q] >= stall_checks) {
+      leave = true;
     }
   }
+  if (leave) {
+    act32[q] = 0;
+    sw[q] = it;
+    if (stop || last) {
+      conv[q] = 2;
+      if (stop) results[4 * q + 3] = (double)it;
+    } else {
+      conv[q] = 1;
+      act64[q] = 1;
+    }
+  } else if (!last) {
+    void* t = probs32[q].g[0];
+    probs32[q].g[0] = probs32[q].galt;
+    probs32[q].galt = t;
+  }
+}
+
+// FP32 -> FP64 g of the lattices with conv[q] != 0 (FP64 Gram-Schmidt
+// reunitarisation, as k_g32_to_g64): bmap of BLK-site blocks, y = parity.
+// conv 1 takes the even field from galt (the check sweep's update), conv 2
+// from g[0] (the checked state).
+__global__ __launch_bounds__(BLK) void k_batch_convert(const BatchProblem* __restrict__ probs32,
+                                                       const BatchProblem* __restrict__ probs64,
+                                                       const int2* __restrict__ bmap,
+                                                       const int* __restrict__ conv) {
+  const int2 m = bmap[blockIdx.x];
+  const int cv = conv[m.x];
+  if (!cv) return;
+  const int parity = blockIdx.y;
+  const BatchProblem& P = probs32[m.x];
+  const int i = m.y * BLK + threadIdx.x;
+  if (i >= P.geo.nh) return;
+  const float4* src = static_cast<const float4*>(parity ? P.g[1] : (cv == 1 ? P.galt : P.g[0]));
+  double2* dst = static_cast<double2*>(probs64[m.x].g[parity]);
+  M3<double> M = load_m<float4, double>(src, P.geo.stride, i);
+  m_normalize(M);
+  Store<double2>::store(dst, P.geo.stride, i, M);
 }
 
 __global__ void k_batch_advance(int* iter, int step) {
@@ -791,14 +848,23 @@ void launch_batch_sweep(bool fp32, const BatchProblem* probs, const int2* bmap, 
     launch_bsweep_t<double2, double>(a, omega, variant, chk, st);
 }
 
-void launch_batch_finalize(const double* podd, const double* peven, const int* poff,
-                           BatchProblem* probs, int nprob, int ndir, double thr, int* active,
-                           double* results, const int* iter, int maxit, double* best, int* stall,
-                           int stall_checks, double stall_gate, hipStream_t st) {
-  hipLaunchKernelGGL(k_batch_finalize, dim3(nprob), dim3(BLK), 0, st, podd, peven, poff, probs,
-                     ndir, thr, active, results, iter, maxit, best, stall, stall_checks,
-                     stall_gate);
+void launch_batch_finalize(const BatchCheck& c, int nprob, int ndir, double eps,
+                           double switch_theta, int maxit, int stall_checks, double stall_gate,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_finalize, dim3(nprob), dim3(BLK), 0, st, c.pod[0], c.pev[0], c.pod[1],
+                     c.pev[1], c.poff[0], c.poff[1], c.ran[0] ? 1 : 0, c.ran[1] ? 1 : 0,
+                     c.probs[0], c.probs[1], ndir, eps,
+                     switch_theta, c.act[0], c.act[1], c.conv, c.sw, c.results, c.iter, maxit,
+                     c.best, c.stall, stall_checks, stall_gate);
 }
+
+void launch_batch_convert(const BatchProblem* probs32, const BatchProblem* probs64,
+                          const int2* bmap, int nblocks, const int* conv, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_convert, dim3(nblocks, 2), dim3(BLK), 0, st, probs32, probs64, bmap,
+                     conv);
+}
+
+int batch_convert_block() { return BLK; }
 
 void launch_batch_advance(int* iter, int step, hipStream_t st) {
   hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, st, iter, step);

@@ -250,3 +250,39 @@ def test_gribov_batch_full_shape(gfx, orc):
     assert Fb == max(r[0] for r in res)
     # the selected copy's fixed links have the reported F
     assert abs(_host_F(orc, u, g_best, dims) - Fb) < 1e-10
+
+
+def test_mixed_batch_lattices_switch_on_their_own(gfx, orc):
+    """In a mixed batch every lattice leaves FP32 at its own check and goes
+    on in FP64 while the others are still in FP32; each reaches the bound
+    with the F of its FP64 oracle run, and its reported stop iteration is
+    the number of iterations it ran (as the single-lattice mixed run)."""
+    gfx.set_omega(1.7)
+    shapes = [(4, 4, 4, 4), (8, 8, 8, 8), (6, 6, 6, 8), (8, 8, 8, 16)]
+    cases = []
+    b = gfx.Batch()
+    for k, dims in enumerate(shapes):
+        u = smooth_config(orc, dims, 950 + k, eps=0.3 + 0.05 * k)
+        g = orc.random_init_trafo(orc.volume(dims), 960 + k)
+        b.add(dims, u, g)
+        cases.append((dims, u, g))
+    rc = b.run(20000, EPS12, 10, gfx.PREC_MIXED, 1e-6, 4)
+    res = [b.result(q) for q in range(len(shapes))]
+    gs = [b.trafo(q) for q in range(len(shapes))]
+    st = b.stats()
+    b.close()
+    assert rc == 0 and st["iters_fp32"] > 0 and st["iters_fp64"] > 0
+    for q, (dims, u, g) in enumerate(cases):
+        u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+        it, hist = orc.overrelax_gauge(u2, g2, dims, 20000, EPS12, 10, 1.7)
+        assert res[q][2] < EPS12
+        assert abs(res[q][0] - hist[-1, 0]) / hist[-1, 0] < 1e-10
+        # the same lattice alone, mixed, single-lattice path
+        gfx.set_lattice(*dims)
+        gfx.set_verbose(0)
+        gfx.init_gaugefixing(u, g)
+        gfx.set_precision(gfx.PREC_MIXED, 1e-6)
+        it1 = gfx.overrelax_gauge(20000, EPS12, 10)
+        gfx.set_precision(gfx.PREC_FP64)
+        gfx.finalize_gaugefixing()
+        assert abs(res[q][3] - it1) <= 20, (q, res[q][3], it1)
