@@ -143,6 +143,10 @@ int alloc_reduction(State& s) {
   if (dev_alloc((void**)&s.active, (size_t)s.geo.L[3], "active")) return GFX_ERR_ALLOC;
   s.ncheck = (s.geo.nh + 63) / 64;  // blocks of the smallest sweep workgroup
   if (dev_alloc((void**)&s.cpart, sizeof(double) * 6 * s.ncheck, "check partials")) return GFX_ERR_ALLOC;
+  for (int pr = 0; pr < 2; pr++) {
+    if (dev_alloc((void**)&s.smap[pr], sizeof(int) * s.ncheck, "slice map")) return GFX_ERR_ALLOC;
+    s.smap_n[pr] = 0;
+  }
   for (int bi = 0; bi < 2; bi++)
     for (int t = 0; t < TILINGS; t++) {
       std::vector<int> order, map;
@@ -219,6 +223,11 @@ void free_all(State& s) {
       hipFree(s.bmap[bi][t]);
       s.bmap[bi][t] = nullptr;
     }
+  for (int pr = 0; pr < 2; pr++) {
+    hipFree(s.smap[pr]);
+    s.smap[pr] = nullptr;
+    s.smap_n[pr] = 0;
+  }
   hipFree(s.results);
   hipFree(s.active);
   hipFree(s.staging);
@@ -282,12 +291,17 @@ void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* ac
   const int v = s.variant[fp32 ? 1 : 0][ndir == 4 ? 1 : 0];
   double* part = (chk == 2) ? s.cpart + 3 * (size_t)s.ncheck : s.cpart;
   const int* map = sweep_map(s, v);
+  int nb = 0;
+  if (ndir == 3 && s.smap_n[fp32 ? 1 : 0] > 0) {
+    map = s.smap[fp32 ? 1 : 0];
+    nb = s.smap_n[fp32 ? 1 : 0];
+  }
   if (fp32)
     launch_sweep_f32(s.U32, field32(s, even), field32(s, 1), s.geo, parity, ndir, s.omega, active,
-                     v, map, s.stream, chk, field32(s, 2 - even), part);
+                     v, map, s.stream, chk, field32(s, 2 - even), part, nb);
   else
     launch_sweep_f64(s.U64, field64(s, even), field64(s, 1), s.geo, parity, ndir, s.omega, active,
-                     v, map, s.stream, chk, field64(s, 2 - even), part);
+                     v, map, s.stream, chk, field64(s, 2 - even), part, nb);
 }
 void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* active) {
   do_sweep(s, fp32, parity, ndir, active, 0, 0);
@@ -296,6 +310,29 @@ void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* ac
 static void stop_clock_ms(State& s, std::chrono::steady_clock::time_point t0) {
   hipStreamSynchronize(s.stream);
   s.run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Coulomb: the map of precision fp32's variant restricted to the blocks
+// that hold a site of a slice still being fixed (act), dealt to the XCDs
+static void build_slice_map(State& s, bool fp32, const std::vector<unsigned char>& act, int nsh) {
+  const int pr = fp32 ? 1 : 0;
+  const int v = s.variant[pr][0];
+  const int blk = sweep_variant_block(v);
+  std::vector<int> order, list;
+  block_order(s.geo, blk, sweep_variant_tiling(v), order);
+  list.reserve(order.size());
+  for (int b : order) {
+    const int t0 = (b * blk) / nsh;
+    const int t1 = (std::min((b + 1) * blk, s.geo.nh) - 1) / nsh;
+    bool any = false;
+    for (int t = t0; t <= t1 && !any; t++) any = act[t] != 0;
+    if (any) list.push_back(b);
+  }
+  deal_to_xcds(list, s.smap_h[pr]);
+  s.smap_n[pr] = (int)s.smap_h[pr].size();
+  if (s.smap_n[pr] > 0)
+    hipMemcpyAsync(s.smap[pr], s.smap_h[pr].data(), sizeof(int) * s.smap_n[pr],
+                   hipMemcpyHostToDevice, s.stream);
 }
 
 // bring the even field back to slot 0
@@ -591,6 +628,11 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   tune_sweep(s, false, 3);
   hipMemcpyAsync(s.active, act.data(), nt, hipMemcpyHostToDevice, s.stream);
   s.run_iters32 = s.run_iters64 = 0;
+  auto maps = [&]() {
+    build_slice_map(s, false, act, nsh);
+    if (s.U32) build_slice_map(s, true, act, nsh);
+  };
+  maps();
   auto fused_ok = [&](bool f32) {
     const int blk = sweep_variant_block(s.variant[f32 ? 1 : 0][0]);
     return nsh % blk == 0;
@@ -653,6 +695,7 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
         stalled = true;
       }
     }
+    if (!stopped.empty() && left > 0) maps();
     if (fp32 && left > 0 && (all_below_switch || stalled)) {
       // leave FP32 at the checked state (the check sweep's values are dropped)
       if (!stopped.empty()) hipMemcpyAsync(s.active, act.data(), nt, hipMemcpyHostToDevice, s.stream);
@@ -688,6 +731,7 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   if (fp32) to_fp64();
   canonical_even(s, false, even);
   hipStreamSynchronize(s.stream);
+  s.smap_n[0] = s.smap_n[1] = 0;
   s.run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return left == 0 ? GFX_OK : GFX_ERR_NOCONV;
 }

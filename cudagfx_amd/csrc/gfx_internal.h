@@ -49,15 +49,16 @@ void deal_to_xcds(const std::vector<T>& in, std::vector<T>& out) {
 // check iteration); 2 check before the update, new field written to galt
 // (even sweep that follows it).  Checks write 3 doubles per block to
 // partials (sweep_blocks(nh, variant) blocks).
-// bmap: launch position -> local block (block_order dealt to the XCDs)
+// bmap: launch position -> local block (block_order dealt to the XCDs);
+// nblocks > 0 launches only the first nblocks entries (a map of a subset)
 void launch_sweep_f64(const double2* U, double2* ge, double2* go, const Geom& geo, int parity,
                       int ndir, double omega, const unsigned char* active, int variant,
                       const int* bmap, hipStream_t st, int chk = 0, double2* galt = nullptr,
-                      double* partials = nullptr);
+                      double* partials = nullptr, int nblocks = 0);
 void launch_sweep_f32(const float4* U, float4* ge, float4* go, const Geom& geo, int parity,
                       int ndir, double omega, const unsigned char* active, int variant,
                       const int* bmap, hipStream_t st, int chk = 0, float4* galt = nullptr,
-                      double* partials = nullptr);
+                      double* partials = nullptr, int nblocks = 0);
 int sweep_blocks(int nh, int variant);
 // per-slice grid pass of a fused Coulomb check (bps blocks per slice; only
 // active slices are written)

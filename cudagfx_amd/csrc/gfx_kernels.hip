@@ -711,12 +711,13 @@ struct SweepArgs {
   int parity;
   const unsigned char* active;
   const int* bmap;
+  int nblocks;  // entries of bmap to launch (0: every block of the lattice)
   double* partials;
 };
 
 template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK>
 static void launch_sweep_v(const SweepArgs<V>& a, R c1, R c2, hipStream_t st) {
-  dim3 grid((a.geo.nh + BLOCK - 1) / BLOCK);
+  dim3 grid(a.nblocks > 0 ? a.nblocks : (a.geo.nh + BLOCK - 1) / BLOCK);
   hipLaunchKernelGGL((k_or_sweep<V, R, NDIR, BLOCK, ROLLED, NT, CHK>), grid, dim3(BLOCK), 0, st,
                      a.U, a.ge, a.go, a.galt, a.geo, a.parity, c1, c2, a.active, a.bmap,
                      a.partials);
@@ -764,15 +765,17 @@ static void launch_sweep_t(const SweepArgs<V>& a, int ndir, double omega, int va
 
 void launch_sweep_f64(const double2* U, double2* ge, double2* go, const Geom& geo, int parity,
                       int ndir, double omega, const unsigned char* active, int variant,
-                      const int* bmap, hipStream_t st, int chk, double2* galt, double* partials) {
-  launch_sweep_t<double2, double>({U, ge, go, galt, geo, parity, active, bmap, partials}, ndir,
-                                  omega, variant, chk, st);
+                      const int* bmap, hipStream_t st, int chk, double2* galt, double* partials,
+                      int nblocks) {
+  launch_sweep_t<double2, double>({U, ge, go, galt, geo, parity, active, bmap, nblocks, partials},
+                                  ndir, omega, variant, chk, st);
 }
 void launch_sweep_f32(const float4* U, float4* ge, float4* go, const Geom& geo, int parity,
                       int ndir, double omega, const unsigned char* active, int variant,
-                      const int* bmap, hipStream_t st, int chk, float4* galt, double* partials) {
-  launch_sweep_t<float4, float>({U, ge, go, galt, geo, parity, active, bmap, partials}, ndir,
-                                omega, variant, chk, st);
+                      const int* bmap, hipStream_t st, int chk, float4* galt, double* partials,
+                      int nblocks) {
+  launch_sweep_t<float4, float>({U, ge, go, galt, geo, parity, active, bmap, nblocks, partials},
+                                ndir, omega, variant, chk, st);
 }
 
 int sweep_blocks(int nh, int variant) {

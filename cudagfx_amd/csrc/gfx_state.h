@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "gfx_lattice.h"
 
 namespace gfx {
@@ -66,6 +68,11 @@ struct State {
   // tiling applies to this lattice
   int* bmap[2][4] = {};
   bool tiling_ok[2][4] = {};
+  // Coulomb: per precision, a map of the blocks of the slices still being
+  // fixed (nblocks entries; 0 = use the full map)
+  int* smap[2] = {nullptr, nullptr};
+  int smap_n[2] = {0, 0};
+  std::vector<int> smap_h[2];
 
   // last run statistics
   double run_ms = 0.0;

@@ -129,7 +129,8 @@ def test_coulomb_48x96_mixed_vs_fp64(gfx, orc):
         out[name] = (rc, iters, F, M, wall, gfx.get_trafo() if name == "mixed" else None)
     gfx.set_precision(gfx.PREC_FP64)
     gfx.finalize_gaugefixing()
-    print({k: (v[0], int(np.max(v[1])), v[4]) for k, v in out.items()})
+    print({k: (v[0], int(np.min(v[1])), int(np.mean(v[1])), int(np.max(v[1])), v[4])
+           for k, v in out.items()})
     for name in out:
         rc, iters, F, M, _, _ = out[name]
         assert rc == 0 and np.all(iters >= 0) and np.all(M < EPS12), name
