@@ -140,9 +140,11 @@ int alloc_reduction(State& s) {
   if (dev_alloc((void**)&s.partials, sizeof(double) * 3 * s.npartials, "partials")) return GFX_ERR_ALLOC;
   if (dev_alloc((void**)&s.results, sizeof(double) * 4 * (s.geo.L[3] + 1), "results")) return GFX_ERR_ALLOC;
   if (hipHostMalloc((void**)&s.h_results, sizeof(double) * 4 * (s.geo.L[3] + 1), 0) != hipSuccess) return GFX_ERR_ALLOC;
-  if (dev_alloc((void**)&s.active, (size_t)s.geo.L[3], "active")) return GFX_ERR_ALLOC;
+  // per time slice: sweeping in FP64 [0, T), sweeping in FP32 [T, 2T)
+  if (dev_alloc((void**)&s.active, 2 * (size_t)s.geo.L[3], "active")) return GFX_ERR_ALLOC;
   s.ncheck = (s.geo.nh + 63) / 64;  // blocks of the smallest sweep workgroup
-  if (dev_alloc((void**)&s.cpart, sizeof(double) * 6 * s.ncheck, "check partials")) return GFX_ERR_ALLOC;
+  // check partials: odd + even sweeps, for each precision (Coulomb runs both)
+  if (dev_alloc((void**)&s.cpart, sizeof(double) * 12 * s.ncheck, "check partials")) return GFX_ERR_ALLOC;
   for (int pr = 0; pr < 2; pr++) {
     if (dev_alloc((void**)&s.smap[pr], sizeof(int) * s.ncheck, "slice map")) return GFX_ERR_ALLOC;
     s.smap_n[pr] = 0;
@@ -275,6 +277,10 @@ void reset_tuning(State& s) {
     }
 }
 
+// block partials of the fused checks of one precision: odd sweep at +0,
+// even sweep at +3 ncheck
+double* check_partials(State& s, bool fp32) { return s.cpart + (fp32 ? 6 * (size_t)s.ncheck : 0); }
+
 const int* sweep_map(State& s, int v) {
   return s.bmap[block_index(sweep_variant_block(v))][sweep_variant_tiling(v)];
 }
@@ -289,7 +295,7 @@ float4* field32(State& s, int k) { return s.g32 + (size_t)k * 3 * s.geo.stride; 
 void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* active, int even,
               int chk) {
   const int v = s.variant[fp32 ? 1 : 0][ndir == 4 ? 1 : 0];
-  double* part = (chk == 2) ? s.cpart + 3 * (size_t)s.ncheck : s.cpart;
+  double* part = check_partials(s, fp32) + (chk == 2 ? 3 * (size_t)s.ncheck : 0);
   const int* map = sweep_map(s, v);
   int nb = 0;
   if (ndir == 3 && s.smap_n[fp32 ? 1 : 0] > 0) {
@@ -553,8 +559,9 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
     // the check of state i rides on the even sweep of iteration i+1
     do_sweep(s, fp32, 0, 4, nullptr, even, 2);
     const int v = s.variant[fp32 ? 1 : 0][1];
-    launch_check_finalize(s.cpart, s.cpart + 3 * (size_t)s.ncheck, sweep_blocks(s.geo.nh, v),
-                          4.0 * V, V, eps, s.results, s.stream);
+    const double* pc = check_partials(s, fp32);
+    launch_check_finalize(pc, pc + 3 * (size_t)s.ncheck, sweep_blocks(s.geo.nh, v), 4.0 * V, V,
+                          eps, s.results, s.stream);
     if (hip_check(hipMemcpyAsync(s.h_results, s.results, sizeof(double) * 4, hipMemcpyDeviceToHost, s.stream), "D2H results") ||
         hip_check(hipStreamSynchronize(s.stream), "check"))
       return -1;
@@ -606,130 +613,194 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
 
 // Coulomb gauge: every time slice is fixed with the 3 spatial directions and
 // stops on its own (per-slice max dAdA < eps, checked every checkinterval
-// iterations); a stopped slice is no longer updated.  When the slices hold
-// a whole number of sweep blocks the checks ride on the sweeps as for
-// Landau (per-slice partials); a slice that stops keeps the state of its
-// check: its even sites are copied back over the check sweep's new values
-// before the fields swap.
+// iterations); a stopped slice is no longer updated, and the sweeps launch
+// only the blocks of the slices still being fixed.  In the mixed form every
+// slice starts in FP32 and moves to FP64 on its own (per-slice max dAdA <
+// switch, or its FP32 stall), so FP32 and FP64 slices are swept side by
+// side.  When the slices hold a whole number of sweep blocks the checks ride
+// on the sweeps as for Landau (per-slice partials); a slice that stops keeps
+// the state of its check: its even sites are copied back over the check
+// sweep's new values before the fields swap.
 int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   State& s = g_st;
   if (!s.init) return GFX_ERR_STATE;
   if (checkinterval < 1) checkinterval = 1;
   const int nt = s.geo.L[3];
   const int nsh = s.geo.nh / nt;  // sites of one parity in a slice
-  std::vector<unsigned char> act(nt, 1);
+  const bool mixed = (s.precision == GFX_PREC_MIXED);
   for (int t = 0; t < nt; t++) iters[t] = -1;
-  bool fp32 = (s.precision == GFX_PREC_MIXED);
-  if (fp32) {
+  // phase per slice: 1 sweeping in FP32, 0 sweeping in FP64, -1 stopped
+  std::vector<int> phase(nt, mixed ? 1 : 0);
+  std::vector<unsigned char> act[2] = {std::vector<unsigned char>(nt, mixed ? 0 : 1),
+                                       std::vector<unsigned char>(nt, mixed ? 1 : 0)};
+  unsigned char* dact[2] = {s.active, s.active + nt};
+  if (mixed) {
     if (ensure_f32_links(s)) return GFX_ERR_ALLOC;
     launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
     tune_sweep(s, true, 3);
   }
   tune_sweep(s, false, 3);
-  hipMemcpyAsync(s.active, act.data(), nt, hipMemcpyHostToDevice, s.stream);
+  auto upload = [&]() {
+    for (int pr = 0; pr < 2; pr++)
+      hipMemcpyAsync(dact[pr], act[pr].data(), nt, hipMemcpyHostToDevice, s.stream);
+  };
+  upload();
   s.run_iters32 = s.run_iters64 = 0;
+  auto count = [&](int pr) {
+    int c = 0;
+    for (int t = 0; t < nt; t++) c += act[pr][t] ? 1 : 0;
+    return c;
+  };
   auto maps = [&]() {
-    build_slice_map(s, false, act, nsh);
-    if (s.U32) build_slice_map(s, true, act, nsh);
+    build_slice_map(s, false, act[0], nsh);
+    if (mixed) build_slice_map(s, true, act[1], nsh);
   };
   maps();
-  auto fused_ok = [&](bool f32) {
-    const int blk = sweep_variant_block(s.variant[f32 ? 1 : 0][0]);
-    return nsh % blk == 0;
-  };
+  // fused checks need whole blocks per slice, in both precisions when mixed
+  // (a slice moving to FP64 takes the FP32 check sweep's update with it)
+  const bool fusable = nsh % sweep_variant_block(s.variant[0][0]) == 0 &&
+                       (!mixed || nsh % sweep_variant_block(s.variant[1][0]) == 0);
+  auto fused_ok = [&](int) { return fusable; };
   const int stall_checks = (s.stall_iters + checkinterval - 1) / checkinterval;
-  double best = 1e300;
-  int stall = 0;
+  std::vector<double> best(nt, 1e300), res(4 * nt, 0.0);
+  std::vector<int> stall(nt, 0);
   int left = nt;
-  int even = 0;
-  bool even_done = false;
+  int even[2] = {0, 0};
+  bool even_done[2] = {false, false};
   hipStreamSynchronize(s.stream);
   const auto t0 = std::chrono::steady_clock::now();
-  auto to_fp64 = [&]() {
-    canonical_even(s, true, even);
-    launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
-    even = 0;
-    fp32 = false;
+  // copy the even sites of slice t from field `from` to field `to` (planes)
+  auto copy_slice = [&](bool f32, int from, int to, int t) {
+    const int np = f32 ? 3 : 6;
+    const size_t vb = f32 ? sizeof(float4) : sizeof(double2);
+    char* src = f32 ? (char*)field32(s, from) : (char*)field64(s, from);
+    char* dst = f32 ? (char*)field32(s, to) : (char*)field64(s, to);
+    for (int c = 0; c < np; c++) {
+      const size_t off = ((size_t)c * s.geo.stride + (size_t)t * nsh) * vb;
+      hipMemcpyAsync(dst + off, src + off, (size_t)nsh * vb, hipMemcpyDeviceToDevice, s.stream);
+    }
   };
-  for (int i = 0; i < maxit && left > 0; i++) {
+  int i;
+  for (i = 0; i < maxit && left > 0; i++) {
     const bool chk = (i % checkinterval) == 0;
-    const bool fused = chk && fused_ok(fp32);
-    if (!even_done) do_sweep(s, fp32, 0, 3, s.active, even, 0);
-    do_sweep(s, fp32, 1, 3, s.active, even, fused ? 1 : 0);
-    even_done = false;
-    if (fp32) s.run_iters32++; else s.run_iters64++;
+    int n[2] = {count(0), count(1)};
+    bool fused[2] = {false, false};
+    for (int pr = 0; pr < 2; pr++) {
+      if (!n[pr]) continue;
+      fused[pr] = chk && fused_ok(pr);
+      if (!even_done[pr]) do_sweep(s, pr == 1, 0, 3, dact[pr], even[pr], 0);
+      do_sweep(s, pr == 1, 1, 3, dact[pr], even[pr], fused[pr] ? 1 : 0);
+      even_done[pr] = false;
+      if (pr == 1) s.run_iters32++; else s.run_iters64++;
+    }
     if (!chk) continue;
-    if (fused) {
-      do_sweep(s, fp32, 0, 3, s.active, even, 2);
-      const int bps = nsh / sweep_variant_block(s.variant[fp32 ? 1 : 0][0]);
-      launch_slice_finalize(s.cpart, s.cpart + 3 * (size_t)s.ncheck, bps, nt, s.active,
-                            3.0 * 2 * nsh, 2.0 * nsh, eps, s.results, s.stream);
-      if (hip_check(hipMemcpyAsync(s.h_results, s.results, sizeof(double) * 4 * nt, hipMemcpyDeviceToHost, s.stream), "D2H results") ||
-          hip_check(hipStreamSynchronize(s.stream), "check"))
+    // the checks of state i, per precision
+    for (int pr = 0; pr < 2; pr++) {
+      if (!n[pr]) continue;
+      if (fused[pr]) {
+        do_sweep(s, pr == 1, 0, 3, dact[pr], even[pr], 2);
+        const double* pc = check_partials(s, pr == 1);
+        const int bps = nsh / sweep_variant_block(s.variant[pr][0]);
+        launch_slice_finalize(pc, pc + 3 * (size_t)s.ncheck, bps, nt, dact[pr], 3.0 * 2 * nsh,
+                              2.0 * nsh, eps, s.results, s.stream);
+        if (hip_check(hipMemcpyAsync(s.h_results, s.results, sizeof(double) * 4 * nt, hipMemcpyDeviceToHost, s.stream), "D2H results") ||
+            hip_check(hipStreamSynchronize(s.stream), "check"))
+          return GFX_ERR_DEVICE;
+      } else if (functional_now(s, pr == 1, 3, nt, eps)) {
         return GFX_ERR_DEVICE;
-    } else if (functional_now(s, fp32, 3, nt, eps)) {
-      return GFX_ERR_DEVICE;
+      }
+      for (int t = 0; t < nt; t++)
+        if (act[pr][t])
+          for (int k = 0; k < 4; k++) res[4 * t + k] = s.h_results[4 * t + k];
     }
-    std::vector<int> stopped;
-    bool all_below_switch = true;
-    double mx = 0.0;
+    // decisions
+    std::vector<int> stop64, stop32, sw32;
     for (int t = 0; t < nt; t++) {
-      if (!act[t]) continue;
-      const double m = s.h_results[4 * t + 2];
-      mx = fmax(mx, m);
-      if (m < eps) {
-        act[t] = 0;
-        iters[t] = i;
-        left--;
-        stopped.push_back(t);
-      } else if (m >= s.switch_theta) {
-        all_below_switch = false;
-      }
-    }
-    bool stalled = false;
-    if (fp32 && left > 0) {
-      if (mx < 0.5 * best) {
-        best = mx;
-        stall = 0;
-      } else if (best < s.stall_gate && ++stall >= stall_checks) {
-        stalled = true;
-      }
-    }
-    if (!stopped.empty() && left > 0) maps();
-    if (fp32 && left > 0 && (all_below_switch || stalled)) {
-      // leave FP32 at the checked state (the check sweep's values are dropped)
-      if (!stopped.empty()) hipMemcpyAsync(s.active, act.data(), nt, hipMemcpyHostToDevice, s.stream);
-      to_fp64();
-    } else if (fused && i + 1 < maxit && left > 0) {
-      // slices that stop keep their checked even sites; the others take the
-      // check sweep's update
-      const int np = fp32 ? 3 : 6;
-      const size_t vb = fp32 ? sizeof(float4) : sizeof(double2);
-      char* cur = fp32 ? (char*)field32(s, even) : (char*)field64(s, even);
-      char* nxt = fp32 ? (char*)field32(s, 2 - even) : (char*)field64(s, 2 - even);
-      for (int t : stopped)
-        for (int c = 0; c < np; c++) {
-          const size_t off = ((size_t)c * s.geo.stride + (size_t)t * nsh) * vb;
-          hipMemcpyAsync(nxt + off, cur + off, (size_t)nsh * vb, hipMemcpyDeviceToDevice, s.stream);
+      const double m = res[4 * t + 2];
+      if (act[0][t]) {
+        if (m < eps) stop64.push_back(t);
+      } else if (act[1][t]) {
+        bool leave = m < s.switch_theta;
+        if (!leave) {
+          if (m < 0.5 * best[t]) {
+            best[t] = m;
+            stall[t] = 0;
+          } else if (best[t] < s.stall_gate && ++stall[t] >= stall_checks) {
+            leave = true;
+          }
         }
-      if (!stopped.empty()) hipMemcpyAsync(s.active, act.data(), nt, hipMemcpyHostToDevice, s.stream);
-      even = 2 - even;
-      even_done = true;
-    } else if (!stopped.empty()) {
-      hipMemcpyAsync(s.active, act.data(), nt, hipMemcpyHostToDevice, s.stream);
+        if (m < eps) stop32.push_back(t);
+        else if (leave) sw32.push_back(t);
+      }
+    }
+    const bool last = i + 1 >= maxit;
+    // FP64: stopped slices keep their checked state, the others take the
+    // check sweep's update
+    const bool swap64 = fused[0] && n[0] > 0 && !last;
+    if (swap64) {
+      for (int t : stop64) copy_slice(false, even[0], 2 - even[0], t);
+      even[0] = 2 - even[0];
+      even_done[0] = true;
+    }
+    // FP32 slices that stop or leave: to FP64.  A slice going on in FP64
+    // continues from the FP32 check sweep's update when the FP64 slices also
+    // took theirs (their next sweep is the odd half), else from the checked
+    // state.
+    const bool swap32 = fused[1] && n[1] > 0 && !last;
+    for (int t : stop32) {
+      // a stopped slice's even sites must be in both FP64 even fields, which
+      // keep swapping roles
+      launch_g32_to_g64_range(field32(s, even[1]), field32(s, 1), field64(s, even[0]),
+                              field64(s, 1), s.geo, t * nsh, nsh, s.stream);
+      copy_slice(false, even[0], 2 - even[0], t);
+    }
+    for (int t : sw32) {
+      const bool from_update = swap32 && (even_done[0] || n[0] == 0);
+      launch_g32_to_g64_range(field32(s, from_update ? 2 - even[1] : even[1]), field32(s, 1),
+                              field64(s, even[0]), field64(s, 1), s.geo, t * nsh, nsh, s.stream);
+    }
+    if (!sw32.empty() && n[0] == 0 && swap32) even_done[0] = true;
+    if (swap32) {
+      even[1] = 2 - even[1];
+      even_done[1] = true;
+    }
+    for (int t : stop64) {
+      act[0][t] = 0;
+      phase[t] = -1;
+      iters[t] = i;
+      left--;
+    }
+    for (int t : stop32) {
+      act[1][t] = 0;
+      phase[t] = -1;
+      iters[t] = i;
+      left--;
+    }
+    for (int t : sw32) {
+      act[1][t] = 0;
+      act[0][t] = 1;
+      phase[t] = 0;
+    }
+    if (!stop64.empty() || !stop32.empty() || !sw32.empty()) {
+      upload();
+      maps();
     }
     if (s.verbose) {
       double F = 0, D = 0, M = 0;
       for (int t = 0; t < nt; t++) {
-        F += s.h_results[4 * t];
-        D += s.h_results[4 * t + 1];
-        M = fmax(M, s.h_results[4 * t + 2]);
+        F += res[4 * t];
+        D += res[4 * t + 1];
+        M = fmax(M, res[4 * t + 2]);
       }
       printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e\t slices left = %d\n", i, F / nt, D / nt, M, left);
     }
   }
-  if (fp32) to_fp64();
-  canonical_even(s, false, even);
+  // slices still in FP32 at maxit: their state to FP64
+  for (int t = 0; t < nt; t++)
+    if (act[1][t])
+      launch_g32_to_g64_range(field32(s, even[1]), field32(s, 1), field64(s, even[0]),
+                              field64(s, 1), s.geo, t * nsh, nsh, s.stream);
+  canonical_even(s, false, even[0]);
   hipStreamSynchronize(s.stream);
   s.smap_n[0] = s.smap_n[1] = 0;
   s.run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

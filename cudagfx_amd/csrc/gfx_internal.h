@@ -124,5 +124,8 @@ void launch_apply_trafo_aos(double* aos, const double2* g, const Geom& geo, int 
 void launch_f64_to_f32(const double2* src, float4* dst, const Geom& geo, int nfield,
                        hipStream_t st);
 void launch_g32_to_g64(const float4* src, double2* dst, const Geom& geo, hipStream_t st);
+// sites [i0, i0 + n) of both parities, fields given separately
+void launch_g32_to_g64_range(const float4* se, const float4* so, double2* de, double2* dd,
+                             const Geom& geo, int i0, int n, hipStream_t st);
 
 }  // namespace gfx

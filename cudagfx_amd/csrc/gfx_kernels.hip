@@ -692,6 +692,23 @@ __global__ __launch_bounds__(BLK) void k_g32_to_g64(const float4* __restrict__ s
   Store<double2>::store(field_ptr(dst, f, stride), stride, i, M);
 }
 
+// the same for the sites [i0, i0 + n) of each parity, with the even and odd
+// fields given separately (Coulomb: one time slice leaving FP32)
+__global__ __launch_bounds__(BLK) void k_g32_to_g64_range(const float4* __restrict__ se,
+                                                          const float4* __restrict__ so,
+                                                          double2* __restrict__ de,
+                                                          double2* __restrict__ dd, int i0, int n,
+                                                          int stride) {
+  const int k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= n) return;
+  const int i = i0 + k;
+  const float4* src = blockIdx.y ? so : se;
+  double2* dst = blockIdx.y ? dd : de;
+  M3<double> M = load_m<float4, double>(src, stride, i);
+  m_normalize(M);
+  Store<double2>::store(dst, stride, i, M);
+}
+
 // ------------------------------------------------------------ launchers
 static inline dim3 grid1(long n) { return dim3((unsigned)((n + BLK - 1) / BLK)); }
 
@@ -955,6 +972,12 @@ void launch_f64_to_f32(const double2* src, float4* dst, const Geom& geo, int nfi
 void launch_g32_to_g64(const float4* src, double2* dst, const Geom& geo, hipStream_t st) {
   dim3 grid((geo.nh + BLK - 1) / BLK, 2);
   hipLaunchKernelGGL(k_g32_to_g64, grid, dim3(BLK), 0, st, src, dst, geo.nh, geo.stride);
+}
+
+void launch_g32_to_g64_range(const float4* se, const float4* so, double2* de, double2* dd,
+                             const Geom& geo, int i0, int n, hipStream_t st) {
+  dim3 grid((n + BLK - 1) / BLK, 2);
+  hipLaunchKernelGGL(k_g32_to_g64_range, grid, dim3(BLK), 0, st, se, so, de, dd, i0, n, geo.stride);
 }
 
 }  // namespace gfx

@@ -330,3 +330,24 @@ def test_fp32_halfsweep_close_to_oracle(gfx, orc, rolled):
     orc.halfsweep(u2, g2, dims, 1, 4, 1.7)
     err = np.abs(g_gpu - orc.from_2v(g2)).max()
     assert 1e-9 < err < 2e-5, err
+
+
+def test_coulomb_mixed_slices_switch_on_their_own(gfx, orc):
+    """Mixed Coulomb: each slice leaves FP32 at its own check and is polished
+    in FP64 while others are still in FP32; every slice ends below the bound
+    with the F of the oracle's FP64 fixing of that slice."""
+    dims = (8, 8, 8, 6)
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, 61, eps=0.4)
+    g = orc.random_init_trafo(V, 62)
+    _setup(gfx, dims, u, g, 1.7, prec=gfx.PREC_MIXED)
+    rc, it_gpu = gfx.coulomb_gauge(8000, EPS12, 5)
+    F, D, M = gfx.functional_slices()
+    gfx.set_precision(gfx.PREC_FP64)
+    gfx.finalize_gaugefixing()
+    u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+    rc_o, it_cpu = orc.coulomb_gauge(u2, g2, dims, 8000, EPS12, 5, 1.7)
+    Fo, Do, Mo = orc.functional_slices(u2, g2, dims, 3)
+    assert rc == 0 and rc_o == 0 and np.all(M < EPS12)
+    assert np.max(np.abs(F - Fo) / Fo) < 1e-10
+    assert len(set(it_gpu.tolist())) > 1
