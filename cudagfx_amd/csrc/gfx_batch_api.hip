@@ -58,6 +58,7 @@ struct Batch {
   double* d_pod[2] = {nullptr, nullptr};  // check partials of the odd sweeps
   double* d_pev[2] = {nullptr, nullptr};  // and of the even sweeps
   int* d_act[2] = {nullptr, nullptr};     // lattice is sweeping in this precision
+  int* d_inmap[2] = {nullptr, nullptr};   // lattice is in the precision's maps (last census)
   int* h_act = nullptr;                   // pinned, 2 x n
   std::vector<int> h_act_prev;            // the census before
   int* d_conv = nullptr;                  // FP32 -> FP64 conversions of a check
@@ -103,6 +104,8 @@ static void batch_free_maps(Batch& b) {
     hipFree(b.d_pod[k]);
     hipFree(b.d_pev[k]);
     hipFree(b.d_act[k]);
+    hipFree(b.d_inmap[k]);
+    b.d_inmap[k] = nullptr;
     b.d_poff[k] = nullptr;
     b.d_probs[k] = nullptr;
     b.d_pod[k] = b.d_pev[k] = nullptr;
@@ -195,7 +198,8 @@ static int batch_build(Batch& b) {
     if (dev_alloc((void**)&b.d_pod[k], sizeof(double) * 3 * npart, "batch partials") ||
         dev_alloc((void**)&b.d_pev[k], sizeof(double) * 3 * npart, "batch partials") ||
         dev_alloc((void**)&b.d_probs[k], sizeof(BatchProblem) * n, "batch problems") ||
-        dev_alloc((void**)&b.d_act[k], sizeof(int) * n, "batch flags"))
+        dev_alloc((void**)&b.d_act[k], sizeof(int) * n, "batch flags") ||
+        dev_alloc((void**)&b.d_inmap[k], sizeof(int) * n, "batch flags"))
       return GFX_ERR_ALLOC;
   if (dev_alloc((void**)&b.d_best, sizeof(double) * n, "batch best") ||
       dev_alloc((void**)&b.d_stall, sizeof(int) * n, "batch stall") ||
@@ -320,8 +324,9 @@ static void batch_check_tail(Batch& b, const bool use[2], const RunParams& rp) {
     c.poff[pr] = b.d_poff[vblock_index(b.variant[pr])];
     c.probs[pr] = b.d_probs[pr];
     c.act[pr] = b.d_act[pr];
-    c.ran[pr] = use[pr];
+    c.inmap[pr] = b.d_inmap[pr];
   }
+  (void)use;
   c.conv = b.d_conv;
   c.sw = b.d_sw;
   c.results = b.d_results;
@@ -356,7 +361,12 @@ static int census(Batch& b, int cnt[2]) {
     b.h_act_prev[q] = b.h_act[q];
     b.h_act_prev[n + q] = b.h_act[n + q];
   }
-  if (changed) b.census_version++;
+  if (changed) {
+    b.census_version++;
+    // the maps rebuilt from this census hold exactly these lattices
+    for (int pr = 0; pr < 2; pr++)
+      hipMemcpyAsync(b.d_inmap[pr], b.h_act + pr * n, sizeof(int) * n, hipMemcpyHostToDevice, s.stream);
+  }
   return GFX_OK;
 }
 
@@ -389,16 +399,18 @@ static int batch_loop(Batch& b, bool mixed, const RunParams& rp, int* it_end) {
     b.h_act[n + q] = b.h_act_prev[n + q] = mixed ? 1 : 0;
   }
   b.census_version++;
+  for (int pr = 0; pr < 2; pr++)
+    hipMemcpyAsync(b.d_inmap[pr], b.h_act + pr * n, sizeof(int) * n, hipMemcpyHostToDevice, s.stream);
   int i = 0, rc = GFX_OK;
   bool even_done = false;  // the even half of iteration i ran with the last check
   int tune_v[2] = {b.tuned[0] ? -1 : 0, b.tuned[1] ? -1 : 0};
   std::vector<int> sw_before(n, -1);
-  // after a check (census just read): a lattice that left FP32 while the
-  // sweeps launched had no FP64 part waited until iteration `resume`
-  auto after_check = [&](int resume, bool fp64_ran) {
+  // after a check (census just read): a lattice that left FP32 inside a
+  // graph replay (whose FP64 map did not hold it) waited until `resume`
+  auto after_check = [&](int resume, bool in_graph) {
     for (int q = 0; q < n; q++) {
       const int sw = b.h_sw[q];
-      if (sw >= 0 && sw_before[q] < 0 && !fp64_ran && b.h_act[q] && resume > sw + 1)
+      if (sw >= 0 && sw_before[q] < 0 && in_graph && b.h_act[q] && resume > sw + 1)
         b.probs[q].idle += resume - (sw + 1);
       sw_before[q] = sw;
     }
@@ -434,7 +446,7 @@ static int batch_loop(Batch& b, bool mixed, const RunParams& rp, int* it_end) {
       batch_check_tail(b, use, rp);
       even_done = true;
       if (census(b, cnt)) rc = GFX_ERR_DEVICE;
-      after_check(i + 1, true);
+      after_check(i + 1, false);
     }
     for (int pr = 0; pr < 2; pr++) {
       if (!timed[pr]) continue;
@@ -496,7 +508,7 @@ static int batch_loop(Batch& b, bool mixed, const RunParams& rp, int* it_end) {
         rc = GFX_ERR_DEVICE;
         break;
       }
-      after_check(i, use[0]);
+      after_check(i, true);
       cnt[0] = c2[0];
       cnt[1] = c2[1];
       if (cnt[0] + cnt[1] == 0) break;

@@ -81,7 +81,7 @@ struct BatchCheck {
   const double* pod[2];  // block partials of the odd check sweeps
   const double* pev[2];  // and of the even ones
   const int* poff[2];    // partial offsets per lattice (block size of the phase's variant)
-  bool ran[2];           // the precision's sweeps ran in this check period
+  const int* inmap[2];   // per lattice: its blocks are in the precision's current map
   BatchProblem* probs[2];
   int* act[2];
   int* conv;

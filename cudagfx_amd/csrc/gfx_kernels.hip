@@ -402,8 +402,8 @@ __global__ __launch_bounds__(BLK) void k_functional(const V* __restrict__ U,
 // Grid-level pass of a batch's fused check, one block per lattice.  A
 // lattice is in the FP64 phase (act64), the FP32 phase (act32) or stopped;
 // its block partials of the odd (pod) and even (pev) check sweeps of its
-// phase lie at [poff[q], poff[q+1]) of that phase's arrays; ran64 / ran32
-// tell which precisions' sweeps ran in the period of this check.
+// phase lie at [poff[q], poff[q+1]) of that phase's arrays; inmap64 /
+// inmap32 say whose blocks the sweeps of this period launched.
 // results[4q..] = F/(ndir V), dAdA/V, max dAdA, stop iteration.
 // FP64: stop when max < eps, else take the check sweep's even field (swap
 // g[0] and galt) unless *iter + 1 == maxit (the run ends on the checked
@@ -416,7 +416,8 @@ __global__ __launch_bounds__(BLK) void k_functional(const V* __restrict__ U,
 __global__ __launch_bounds__(BLK) void k_batch_finalize(
     const double* __restrict__ pod64, const double* __restrict__ pev64,
     const double* __restrict__ pod32, const double* __restrict__ pev32,
-    const int* __restrict__ poff64, const int* __restrict__ poff32, int ran64, int ran32,
+    const int* __restrict__ poff64, const int* __restrict__ poff32,
+    const int* __restrict__ inmap64, const int* __restrict__ inmap32,
     BatchProblem* __restrict__ probs64, BatchProblem* __restrict__ probs32, int ndir, double eps,
     double switch_theta, int* __restrict__ act64, int* __restrict__ act32,
     int* __restrict__ conv, int* __restrict__ sw, double* __restrict__ results,
@@ -425,9 +426,9 @@ __global__ __launch_bounds__(BLK) void k_batch_finalize(
   const int q = blockIdx.x;
   const bool f64 = act64[q] != 0;
   if (!f64 && !act32[q]) return;
-  // a lattice whose precision had no sweeps at this check (it just left
-  // FP32 inside a graph without FP64 sweeps) waits
-  if (f64 ? !ran64 : !ran32) return;
+  // a lattice whose blocks are not in its precision's workgroup map (it
+  // left FP32 inside a graph captured before) was not swept: it waits
+  if (!(f64 ? inmap64[q] : inmap32[q])) return;
   const double* pod = f64 ? pod64 : pod32;
   const double* pev = f64 ? pev64 : pev32;
   const int* poff = f64 ? poff64 : poff32;
@@ -872,7 +873,7 @@ void launch_batch_finalize(const BatchCheck& c, int nprob, int ndir, double eps,
                            double switch_theta, int maxit, int stall_checks, double stall_gate,
                            hipStream_t st) {
   hipLaunchKernelGGL(k_batch_finalize, dim3(nprob), dim3(BLK), 0, st, c.pod[0], c.pev[0], c.pod[1],
-                     c.pev[1], c.poff[0], c.poff[1], c.ran[0] ? 1 : 0, c.ran[1] ? 1 : 0,
+                     c.pev[1], c.poff[0], c.poff[1], c.inmap[0], c.inmap[1],
                      c.probs[0], c.probs[1], ndir, eps,
                      switch_theta, c.act[0], c.act[1], c.conv, c.sw, c.results, c.iter, maxit,
                      c.best, c.stall, stall_checks, stall_gate);
