@@ -155,7 +155,8 @@ int alloc_reduction(State& s) {
       s.tiling_ok[bi][t] = block_order(s.geo, bi == 0 ? 256 : 64, t, order);
       deal_to_xcds(order, map);
       if (dev_alloc((void**)&s.bmap[bi][t], sizeof(int) * map.size(), "workgroup map")) return GFX_ERR_ALLOC;
-      hipMemcpy(s.bmap[bi][t], map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice);
+      if (hip_check(hipMemcpy(s.bmap[bi][t], map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice), "workgroup map"))
+        return GFX_ERR_DEVICE;
     }
   return GFX_OK;
 }

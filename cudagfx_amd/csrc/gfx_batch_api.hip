@@ -176,7 +176,8 @@ static int batch_build(Batch& b) {
       deal_to_xcds(list, map);
       b.nbmap[k] = (int)map.size();
       if (dev_alloc((void**)&b.d_bmap[k][t], sizeof(int2) * map.size(), "batch map")) return GFX_ERR_ALLOC;
-      hipMemcpy(b.d_bmap[k][t], map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice);
+      if (hip_check(hipMemcpy(b.d_bmap[k][t], map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice), "batch map"))
+        return GFX_ERR_DEVICE;
       b.h_order[k][t] = list;
       for (int pr = 0; pr < 2; pr++)
         if (dev_alloc((void**)&b.cmap[pr][k][t].d, sizeof(int2) * map.size(), "batch map")) return GFX_ERR_ALLOC;
@@ -184,13 +185,15 @@ static int batch_build(Batch& b) {
     std::vector<int> poff(n + 1, 0);
     for (int q = 0; q < n; q++) poff[q + 1] = poff[q] + (b.probs[q].geo.nh + BLOCKS[k] - 1) / BLOCKS[k];
     if (dev_alloc((void**)&b.d_poff[k], sizeof(int) * (n + 1), "batch offsets")) return GFX_ERR_ALLOC;
-    hipMemcpy(b.d_poff[k], poff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice);
+    if (hip_check(hipMemcpy(b.d_poff[k], poff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice), "batch offsets"))
+      return GFX_ERR_DEVICE;
   }
   std::vector<int> roff(n + 1, 0);
   for (int q = 0; q < n; q++) roff[q + 1] = roff[q] + b.probs[q].geo.nh;
   b.nh_total = roff[n];
   if (dev_alloc((void**)&b.d_roff, sizeof(int) * (n + 1), "batch random offsets")) return GFX_ERR_ALLOC;
-  hipMemcpy(b.d_roff, roff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice);
+  if (hip_check(hipMemcpy(b.d_roff, roff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice), "batch offsets"))
+    return GFX_ERR_DEVICE;
   // partial slots: as many as blocks of the smallest workgroup
   size_t npart = 0;
   for (int q = 0; q < n; q++) npart += (b.probs[q].geo.nh + 63) / 64;
@@ -686,7 +689,11 @@ int gfx_batch_anneal(int nsteps, double Tmin, double Tmax, double expo, int nmic
     hipFree(d_betas);
     return GFX_ERR_ALLOC;
   }
-  hipMemcpy(d_betas, betas.data(), sizeof(double) * betas.size(), hipMemcpyHostToDevice);
+  if (hip_check(hipMemcpy(d_betas, betas.data(), sizeof(double) * betas.size(), hipMemcpyHostToDevice), "SA schedule")) {
+    hipFree(d_betas);
+    hipFree(d_step);
+    return GFX_ERR_DEVICE;
+  }
   launch_fill_i32(d_step, 1, 0, s.stream);
   const int k = block_index(sa_block_size());
   const int2* map = b.d_bmap[k][0];

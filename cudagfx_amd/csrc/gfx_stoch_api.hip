@@ -198,10 +198,13 @@ void finalize_MT(void) { mt_free(g_st); }
 int gfx_get_mt_fields(float* gauss, float* unif, uint32_t* state, uint32_t* params) {
   State& s = g_st;
   if (!s.mt_state) return GFX_ERR_STATE;
-  if (gauss) hipMemcpy(gauss, s.rnd_gauss, sizeof(float) * MT_COUNT * s.n_per_gauss, hipMemcpyDeviceToHost);
-  if (unif) hipMemcpy(unif, s.rnd_unif, sizeof(float) * MT_COUNT * s.n_per_unif, hipMemcpyDeviceToHost);
-  if (state) hipMemcpy(state, s.mt_state, sizeof(uint32_t) * 19 * MT_COUNT, hipMemcpyDeviceToHost);
-  if (params) hipMemcpy(params, s.mt_params, sizeof(uint32_t) * 4 * MT_COUNT, hipMemcpyDeviceToHost);
+  // the library stream does not synchronise with the null stream
+  if (hip_check(hipStreamSynchronize(s.stream), "MT fields")) return GFX_ERR_DEVICE;
+  if ((gauss && hip_check(hipMemcpy(gauss, s.rnd_gauss, sizeof(float) * MT_COUNT * s.n_per_gauss, hipMemcpyDeviceToHost), "D2H")) ||
+      (unif && hip_check(hipMemcpy(unif, s.rnd_unif, sizeof(float) * MT_COUNT * s.n_per_unif, hipMemcpyDeviceToHost), "D2H")) ||
+      (state && hip_check(hipMemcpy(state, s.mt_state, sizeof(uint32_t) * 19 * MT_COUNT, hipMemcpyDeviceToHost), "D2H")) ||
+      (params && hip_check(hipMemcpy(params, s.mt_params, sizeof(uint32_t) * 4 * MT_COUNT, hipMemcpyDeviceToHost), "D2H")))
+    return GFX_ERR_DEVICE;
   return GFX_OK;
 }
 
