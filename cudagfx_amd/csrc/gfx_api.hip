@@ -109,7 +109,24 @@ bool block_order(const Geom& geo, int block, int tile, std::vector<int>& order) 
   order.resize(nblk);
   for (int b = 0; b < nblk; b++) order[b] = b;
   if (tile <= 0) return true;
-  const int nrow = geo.h0 * geo.L[1];
+  const int nrow = geo.h0 * geo.L[1];  // sites of one parity in an (x2, x3) plane
+  const int l2 = geo.L[2], l3 = geo.L[3];
+  int k = 0;
+  if (tile == 3) {
+    // chunks in x1 as well: pieces of C1 blocks of a plane x C2 planes in
+    // x2, x3 outermost inside a chunk (layers of about 4096 sites)
+    if (nrow % block != 0) return false;
+    const int bpp = nrow / block;  // blocks per plane
+    const int C1 = std::min(bpp, std::max(1, 512 / block));
+    const int C2 = std::min(l2, 8);
+    if (C1 == bpp && C2 == l2) return false;  // one chunk: lexicographic
+    for (int c1 = 0; c1 < bpp; c1 += C1)
+      for (int c2 = 0; c2 < l2; c2 += C2)
+        for (int x3 = 0; x3 < l3; x3++)
+          for (int x2 = c2; x2 < std::min(c2 + C2, l2); x2++)
+            for (int b = c1; b < std::min(c1 + C1, bpp); b++) order[k++] = b + bpp * (x2 + l2 * x3);
+    return k == nblk;
+  }
   int a = nrow, c = block;
   while (c) {
     const int t = a % c;
@@ -117,12 +134,11 @@ bool block_order(const Geom& geo, int block, int tile, std::vector<int>& order) 
     c = t;
   }
   const int G = block / a;  // rows per superrow: G * nrow is a multiple of block
-  if (geo.L[2] % G != 0) return false;
-  const int ns2 = geo.L[2] / G, l3 = geo.L[3];
+  if (l2 % G != 0) return false;
+  const int ns2 = l2 / G;
   const int bps = G * nrow / block;  // blocks per superrow
-  const int C = 1 << tile;
+  const int C = 2 << tile;           // tile 1: 4 superrows, tile 2: 8
   if (C >= ns2) return false;  // one chunk: the lexicographic order
-  int k = 0;
   for (int c0 = 0; c0 < ns2; c0 += C) {
     const int cw = std::min(C, ns2 - c0);
     for (int x3 = 0; x3 < l3; x3++)

@@ -22,12 +22,14 @@ inline int sweep_variant_tiling(int v) { return v >> 3; }
 inline int block_index(int block) { return block == 256 ? 0 : 1; }
 
 // Workgroup order of a sweep over one lattice (local block indices, block
-// sites each).  tile 0: lexicographic.  tile t > 0: a superrow is the G
-// consecutive (x2, x3) rows (h0*L1 sites each) that make a whole number of
-// blocks; superrows are taken in chunks of 2^t consecutive x2-superrows,
-// x3 outermost inside a chunk, so that the x3 neighbours of a block were
-// read 2^t superrows earlier and are still in L2.  Returns false (and the
-// lexicographic order) when the lattice does not split into superrows.
+// sites each).  tile 0: lexicographic.  tiles 1, 2: a superrow is the G
+// consecutive (x2, x3) planes (h0*L1 sites each) that make a whole number of
+// blocks; superrows are taken in chunks of 4 (tile 1) or 8 (tile 2)
+// consecutive x2-superrows, x3 outermost inside a chunk, so that the x3
+// neighbours of a block were read a few superrows earlier and are still in
+// L2.  tile 3: the planes are also cut in x1, into pieces of about 512 sites,
+// for large lattices whose planes are too big for that.  Returns false (and
+// the lexicographic order) when the order does not apply to the lattice.
 bool block_order(const Geom& geo, int block, int tile, std::vector<int>& order);
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
