@@ -207,11 +207,13 @@ int gfx_heatbath_sweeps(int nsweep, double beta);
 /* ================= batching form (no counterpart in the reference) =================
  * Many lattices gauge fixed together: Gribov copies of one configuration
  * (sharing its links) and configurations of different sizes.  Every half sweep
- * is one launch over all lattices; converged lattices drop out through device
- * flags; after a short eager start (which also times the sweep variants and
- * keeps the fastest), `periods_per_graph` check periods of `checkinterval`
- * iterations each are captured in a HIP graph and replayed.  Each lattice
- * follows overrelax_gauge's stop rule and records its own stop iteration. */
+ * is one launch per precision over the lattices sweeping in it; a lattice
+ * stops, or (mixed) moves from FP32 to FP64, at its own check; after a short
+ * eager start (which also times the sweep variants and keeps the fastest),
+ * `periods_per_graph` check periods of `checkinterval` iterations each are
+ * captured in a HIP graph and replayed until some lattice changes state.
+ * Each lattice follows overrelax_gauge's stop rule and records its own stop
+ * iteration (the iterations it ran). */
 int gfx_batch_create(void);
 void gfx_batch_destroy(void);
 int gfx_batch_size(void);
@@ -223,8 +225,9 @@ int gfx_batch_add_copy(int base, gfx_su3 *trafo);
 /* a copy of `base` with a Haar-random initial g drawn from the twisters
  * (needs MersenneTwister.dat or gfx_set_mt_params) */
 int gfx_batch_add_random_copy(int base);
-/* precision GFX_PREC_FP64 or GFX_PREC_MIXED (FP32 bulk to switch_theta, then
- * FP64); returns GFX_OK when every lattice reached eps, else GFX_ERR_NOCONV */
+/* precision GFX_PREC_FP64 or GFX_PREC_MIXED (each lattice: FP32 until its max
+ * dAdA < switch_theta or its FP32 stall, then FP64); returns GFX_OK when every
+ * lattice reached eps, else GFX_ERR_NOCONV */
 int gfx_batch_run(int maxit, double eps, int checkinterval, int precision, double switch_theta,
                   int periods_per_graph);
 int gfx_batch_result(int q, double *F, double *dAdA, double *maxdAdA, int *iter);
