@@ -8,13 +8,16 @@ checked every 10 iterations.
 
 A *step* is one complete gauge-fixing run from that fixed start to the stop
 decision in the mixed path (FP32 12-real sweeps until max dAdA < 1e-6,
-then FP64 reunitarisation and FP64 polish; F/theta reductions in FP64).
+then FP64 reunitarisation and FP64 polish sweeps, which read the links as
+FP32 + int16 parts within 2^-40 of the FP64 links; F/theta reductions in
+FP64; the stop confirmed by a functional on the FP64 links).
 value = site-updates/s of those steps (iterations x V / wall time).  The
 plain FP64 run to the same bound is timed beside it (time-to-theta,
 speedup), with the error of F and of the links of the mixed result against
 it.  The roofline entry is the sweep kernel that dominates the mixed run,
 timed with HIP events on the library's stream; algorithmic bytes per
-site-update are 8 links + 3 g (FP32 12-real: 528 B, FP64: 1056 B).
+site-update are 8 links + 3 g (FP32 12-real: 528 B, FP64: 1056 B, FP64 g
+on split links: 864 B).
 
 cpu_baseline: the oracle's OpenMP half-sweep (a C restatement of the
 reference's sweep on 2-row AoS data) timed on the host cores on a bounded
@@ -46,6 +49,7 @@ MTSEED = 4357
 HBM_PEAK_GBS = 8000.0
 BYTES_FP32 = 8 * 48 + 3 * 48
 BYTES_FP64 = 8 * 96 + 3 * 96
+BYTES_SPLIT = 8 * 72 + 3 * 96
 
 
 def synthetic_mt_params():
@@ -150,9 +154,9 @@ def align_global(g_a, g_b):
 
 def profile_run(gfx):
     """The kernels of the timed region in isolation, for counter collection:
-    the autotuner's variant timing for both precisions, then 20 FP64 and 20
-    FP32 sweeps of the picked variants (the last 40 launches of each picked
-    kernel), 5 standalone FP64 checks, a 31-iteration FP64 run with fused
+    the autotuner's variant timing for the three sweep forms, then 20 FP64,
+    20 FP32 and 20 split-link FP64 sweeps of the picked variants (the last 40
+    launches of each picked kernel), 5 standalone FP64 checks, a 31-iteration FP64 run with fused
     checks every 10 iterations, and a plain copy of 512 MiB whose bytes are
     known."""
     global NTHERM
@@ -165,43 +169,32 @@ def profile_run(gfx):
     gfx.set_omega(OMEGA)
     gfx.time_sweeps(1, gfx.PREC_FP64)
     gfx.time_sweeps(1, gfx.PREC_MIXED)
+    gfx.time_sweeps(1, gfx.SWEEP_SPLIT)
     t64 = gfx.time_sweeps(20, gfx.PREC_FP64)
     t32 = gfx.time_sweeps(20, gfx.PREC_MIXED)
+    tsp = gfx.time_sweeps(20, gfx.SWEEP_SPLIT)
     for _ in range(5):
         gfx.calc_functional()
     gfx.set_trafo(g0)
     gfx.set_precision(gfx.PREC_FP64)
     gfx.overrelax_gauge(31, EPS, CK)
     tc = gfx.stream_copy(512 << 20, 5)
-    out = {"profile": True, "fp64_sweep_ms": t64, "fp32_sweep_ms": t32, "copy_512MiB_ms": tc,
+    out = {"profile": True, "fp64_sweep_ms": t64, "fp32_sweep_ms": t32, "split_sweep_ms": tsp,
+           "copy_512MiB_ms": tc,
            "variant_fp64": list(gfx.sweep_variant(gfx.PREC_FP64, 4)[:3]),
-           "variant_fp32": list(gfx.sweep_variant(gfx.PREC_MIXED, 4)[:3])}
+           "variant_fp32": list(gfx.sweep_variant(gfx.PREC_MIXED, 4)[:3]),
+           "variant_split": list(gfx.sweep_variant(gfx.SWEEP_SPLIT, 4)[:3])}
     gfx.finalize_gaugefixing()
-    print(json.dumps(out))
+    return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile", action="store_true",
-                    help="short run for rocprofv3: autotuning, 20 sweeps of each precision, "
-                         "checks, and a 512 MiB streaming copy as the byte-counter reference")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on one GPU; the path would shard by "
-                          "time-slab domain decomposition across GPUs, which is not built"}))
-        sys.exit(2)
-
+def measure(args):
     import cudagfx_amd as gfx
 
     gfx.load()
     gfx.set_verbose(0)
     if args.profile:
-        profile_run(gfx)
-        return
+        return profile_run(gfx)
     V = int(np.prod(DIMS))
     t_setup = time.perf_counter()
     u, plaq = thermalized_config(gfx)
@@ -215,11 +208,13 @@ def main():
     # pick the sweep variants (autotuner) before anything is timed
     gfx.time_sweeps(1, gfx.PREC_FP64)
     gfx.time_sweeps(1, gfx.PREC_MIXED)
+    gfx.time_sweeps(1, gfx.SWEEP_SPLIT)
     setup_s = time.perf_counter() - t_setup
 
-    def run(prec, switch=SWITCH):
+    def run(prec, switch=SWITCH, split=1):
         gfx.set_trafo(g0)
         gfx.set_precision(prec, switch)
+        gfx.set_polish_links(split)
         gfx.sync()
         t0 = time.perf_counter()
         it = gfx.overrelax_gauge(20000, EPS, CK)
@@ -246,6 +241,9 @@ def main():
     # the same run with the switch at 1e-10 (FP32 rounding floor of this
     # lattice ~5e-11; the stall guard covers lattices whose floor is higher)
     tuned = run(gfx.PREC_MIXED, SWITCH_LOW)
+    # the mixed step with its polish on the FP64 links, for comparison
+    plain_polish = run(gfx.PREC_MIXED, SWITCH, split=0)
+    gfx.set_polish_links(1)
     iters = last["it32"] + last["it64"]
     value = iters * V / (ms_step / 1000.0)
 
@@ -253,21 +251,25 @@ def main():
     gfx.set_trafo(g0)
     t32 = gfx.time_sweeps(50, gfx.PREC_MIXED)
     t64 = gfx.time_sweeps(50, gfx.PREC_FP64)
+    tsp = gfx.time_sweeps(50, gfx.SWEEP_SPLIT)
     gb32 = (V // 2) * BYTES_FP32 / (t32 * 1e-3) / 1e9
     gb64 = (V // 2) * BYTES_FP64 / (t64 * 1e-3) / 1e9
+    gbsp = (V // 2) * BYTES_SPLIT / (tsp * 1e-3) / 1e9
+    # the sweep kernel with the larger share of the mixed step
     share32 = last["it32"] * 2 * t32
-    share64 = last["it64"] * 2 * t64
+    share64 = last["it64"] * 2 * tsp
     dom32 = share32 >= share64
-    kernel = "k_or_sweep<float4,float,4>" if dom32 else "k_or_sweep<double2,double,4>"
-    ach = gb32 if dom32 else gb64
+    kernel = "k_or_sweep<float4,float,4>" if dom32 else "k_or_sweep_split"
+    ach = gb32 if dom32 else gbsp
     roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "traffic": read_traffic(kernel), "kernel": kernel,
-            "ms_per_launch": t32 if dom32 else t64,
-            "algorithmic_bytes_per_launch": (V // 2) * (BYTES_FP32 if dom32 else BYTES_FP64)}
+            "ms_per_launch": t32 if dom32 else tsp,
+            "algorithmic_bytes_per_launch": (V // 2) * (BYTES_FP32 if dom32 else BYTES_SPLIT)}
 
     link_err = align_global(g_mixed, g_fp64)
     variants = {}
-    for name, prec in (("fp64", gfx.PREC_FP64), ("fp32", gfx.PREC_MIXED)):
+    for name, prec in (("fp64", gfx.PREC_FP64), ("fp32", gfx.PREC_MIXED),
+                       ("split", gfx.SWEEP_SPLIT)):
         sv = gfx.sweep_variant(prec, 4)
         if sv is not None:
             variants[name] = {"variant": sv[0], "block": sv[1], "rolled": sv[2],
@@ -303,10 +305,15 @@ def main():
                                "iters_fp32": tuned["it32"], "iters_fp64": tuned["it64"],
                                "stop": tuned["iter"], "theta": tuned["max"],
                                "F_rel": abs(tuned["F"] - r64["F"]) / r64["F"]},
+        "mixed_polish_on_fp64_links": {"time_s": plain_polish["wall_s"],
+                                       "speedup": r64["wall_s"] / plain_polish["wall_s"],
+                                       "stop": plain_polish["iter"],
+                                       "F_rel": abs(plain_polish["F"] - r64["F"]) / r64["F"]},
         "iterations": {"mixed_fp32": last["it32"], "mixed_fp64": last["it64"],
                        "fp64": r64["it64"], "stop_mixed": last["iter"], "stop_fp64": r64["iter"]},
         "sweep_kernels": {"fp32_ms": t32, "fp32_GBs": gb32, "fp32_site_updates_per_s": (V // 2) / (t32 * 1e-3),
-                          "fp64_ms": t64, "fp64_GBs": gb64, "fp64_site_updates_per_s": (V // 2) / (t64 * 1e-3)},
+                          "fp64_ms": t64, "fp64_GBs": gb64, "fp64_site_updates_per_s": (V // 2) / (t64 * 1e-3),
+                          "split_ms": tsp, "split_GBs": gbsp, "split_site_updates_per_s": (V // 2) / (tsp * 1e-3)},
         "accuracy_mixed_vs_fp64": {"F_rel": abs(last["F"] - r64["F"]) / r64["F"],
                                    "theta_mixed": last["max"], "theta_fp64": r64["max"],
                                    "g_maxabs_after_global_rotation": link_err},
@@ -319,6 +326,46 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    return out
+
+
+class _CStdoutToStderr:
+    """The library's own messages (C printf, as the reference prints them) go
+    to stderr while this is active, so that stdout carries the JSON line only."""
+
+    def __enter__(self):
+        import ctypes
+        self.libc = ctypes.CDLL(None)
+        sys.stdout.flush()
+        self.libc.fflush(None)
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        self.libc.fflush(None)
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3: autotuning, 20 sweeps of each form, "
+                         "checks, and a 512 MiB streaming copy as the byte-counter reference")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; the path would shard by "
+                          "time-slab domain decomposition across GPUs, which is not built"}))
+        sys.exit(2)
+    with _CStdoutToStderr():
+        out = measure(args)
     print(json.dumps(out))
 
 

@@ -226,6 +226,21 @@ int ensure_f32_links(State& s) {
   }
   launch_f64_to_f32(s.U64, s.U32, s.geo, 8, s.stream);
   s.U32_valid = true;
+  s.lo_valid = false;
+  return GFX_OK;
+}
+
+// the split links of the mixed path's FP64 polish: the FP32 links and their
+// int16 planes
+int ensure_split_links(State& s) {
+  int rc = ensure_f32_links(s);
+  if (rc) return rc;
+  if (s.lo_valid) return GFX_OK;
+  if (!s.U32lo &&
+      dev_alloc((void**)&s.U32lo, (size_t)8 * 3 * s.geo.stride * sizeof(uint2), "split links"))
+    return GFX_ERR_ALLOC;
+  launch_split_lo(s.U64, s.U32lo, s.geo, 8, s.stream);
+  s.lo_valid = true;
   return GFX_OK;
 }
 
@@ -235,6 +250,9 @@ void free_all(State& s) {
   hipFree(s.g64);
   hipFree(s.U32);
   hipFree(s.g32);
+  hipFree(s.U32lo);
+  s.U32lo = nullptr;
+  s.lo_valid = false;
   hipFree(s.partials);
   hipFree(s.cpart);
   for (int bi = 0; bi < 2; bi++)
@@ -286,7 +304,7 @@ int functional_now(State& s, bool fp32, int ndir, int nseg, double eps) {
 }
 
 void reset_tuning(State& s) {
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < NFORM; a++)
     for (int b = 0; b < 2; b++) {
       s.tuned[a][b] = false;
       s.variant[a][b] = 0;
@@ -305,13 +323,21 @@ const int* sweep_map(State& s, int v) {
 double2* field64(State& s, int k) { return s.g64 + (size_t)k * 6 * s.geo.stride; }
 float4* field32(State& s, int k) { return s.g32 + (size_t)k * 3 * s.geo.stride; }
 
-// one half sweep with the tuned variant for (precision, ndir): in place on
-// the even field in slot `even` (0 or 2) and the odd field (slot 1); chk as
-// in launch_sweep_f64, writing the new even field to slot 2 - even and the
+// the form of the next sweep: FP32, or FP64 on the FP64 links, or (polish of
+// a mixed Landau run) FP64 on the split links
+int sweep_form(const State& s, bool fp32, int ndir) {
+  if (fp32) return FORM_F32;
+  return (s.polish_on && ndir == 4) ? FORM_SPLIT : FORM_F64;
+}
+
+// one half sweep with the tuned variant for (form, ndir): in place on the
+// even field in slot `even` (0 or 2) and the odd field (slot 1); chk as in
+// launch_sweep_f64, writing the new even field to slot 2 - even and the
 // block partials to cpart (odd sweep first half, even sweep second half)
 void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* active, int even,
               int chk) {
-  const int v = s.variant[fp32 ? 1 : 0][ndir == 4 ? 1 : 0];
+  const int form = sweep_form(s, fp32, ndir);
+  const int v = s.variant[form][ndir == 4 ? 1 : 0];
   double* part = check_partials(s, fp32) + (chk == 2 ? 3 * (size_t)s.ncheck : 0);
   const int* map = sweep_map(s, v);
   int nb = 0;
@@ -319,9 +345,12 @@ void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* ac
     map = s.smap[fp32 ? 1 : 0];
     nb = s.smap_n[fp32 ? 1 : 0];
   }
-  if (fp32)
+  if (form == FORM_F32)
     launch_sweep_f32(s.U32, field32(s, even), field32(s, 1), s.geo, parity, ndir, s.omega, active,
                      v, map, s.stream, chk, field32(s, 2 - even), part, nb);
+  else if (form == FORM_SPLIT)
+    launch_sweep_split(s.U32, s.U32lo, field64(s, even), field64(s, 1), s.geo, parity, s.omega, v,
+                       map, s.stream, chk, field64(s, 2 - even), part, nb);
   else
     launch_sweep_f64(s.U64, field64(s, even), field64(s, 1), s.geo, parity, ndir, s.omega, active,
                      v, map, s.stream, chk, field64(s, 2 - even), part, nb);
@@ -368,10 +397,12 @@ static void canonical_even(State& s, bool fp32, int even) {
 }
 
 // Times every sweep variant on a scratch copy of g (the state is not
-// touched) and keeps the fastest for this lattice, precision and ndir.
-int tune_sweep(State& s, bool fp32, int ndir) {
-  const int pi = fp32 ? 1 : 0, di = ndir == 4 ? 1 : 0;
+// touched) and keeps the fastest for this lattice, form and ndir.  The
+// split form (Landau only) has rolled variants only.
+int tune_sweep(State& s, int form, int ndir) {
+  const int pi = form, di = ndir == 4 ? 1 : 0;
   if (!s.autotune || s.tuned[pi][di]) return GFX_OK;
+  const bool fp32 = form == FORM_F32;
   const size_t bytes = 2 * (fp32 ? field_bytes32(s.geo) : field_bytes64(s.geo));
   void* scratch = nullptr;
   if (hipMalloc(&scratch, bytes) != hipSuccess) {
@@ -381,26 +412,25 @@ int tune_sweep(State& s, bool fp32, int ndir) {
   hipMemcpyAsync(scratch, fp32 ? (void*)s.g32 : (void*)s.g64, bytes, hipMemcpyDeviceToDevice, s.stream);
   float4* sc32 = static_cast<float4*>(scratch);
   double2* sc64 = static_cast<double2*>(scratch);
+  auto sweep = [&](int v, const int* map, int p) {
+    if (form == FORM_F32)
+      launch_sweep_f32(s.U32, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
+    else if (form == FORM_SPLIT)
+      launch_sweep_split(s.U32, s.U32lo, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, s.omega, v, map, s.stream);
+    else
+      launch_sweep_f64(s.U64, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
+  };
   float best = 1e30f;
   int bestv = s.variant[pi][di];
   for (int v = 0; v < SWEEP_VARIANTS; v++) {
     if (!s.tiling_ok[block_index(sweep_variant_block(v))][sweep_variant_tiling(v)]) continue;
+    if (form == FORM_SPLIT && !sweep_variant_rolled(v)) continue;
     const int* map = sweep_map(s, v);
     for (int rep = 0; rep < 2; rep++)
-      for (int p = 0; p < 2; p++) {
-        if (fp32)
-          launch_sweep_f32(s.U32, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
-        else
-          launch_sweep_f64(s.U64, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
-      }
+      for (int p = 0; p < 2; p++) sweep(v, map, p);
     hipEventRecord(s.ev0, s.stream);
     for (int rep = 0; rep < 3; rep++)
-      for (int p = 0; p < 2; p++) {
-        if (fp32)
-          launch_sweep_f32(s.U32, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
-        else
-          launch_sweep_f64(s.U64, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
-      }
+      for (int p = 0; p < 2; p++) sweep(v, map, p);
     hipEventRecord(s.ev1, s.stream);
     if (hip_check(hipEventSynchronize(s.ev1), "tune")) break;
     float t = 0.f;
@@ -476,6 +506,11 @@ int gfx_set_precision(int mode, double switch_theta) {
   if (switch_theta > 0) g_st.switch_theta = switch_theta;
   return GFX_OK;
 }
+int gfx_set_polish_links(int split) {
+  if (split != 0 && split != 1) return GFX_ERR_STATE;
+  g_st.polish_split = split;
+  return GFX_OK;
+}
 int gfx_set_fp32_stall(int iters, double gate) {
   if (iters < 0 || !(gate > 0)) return GFX_ERR_STATE;
   g_st.stall_iters = iters;
@@ -541,12 +576,18 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
   if (!s.init) return -1;
   if (checkinterval < 1) checkinterval = 1;
   bool fp32 = (s.precision == GFX_PREC_MIXED);
+  const bool split = fp32 && s.polish_split;
   if (fp32) {
     if (ensure_f32_links(s)) return -1;
     launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
-    tune_sweep(s, true, 4);
+    tune_sweep(s, FORM_F32, 4);
   }
-  tune_sweep(s, false, 4);
+  if (split) {
+    if (ensure_split_links(s)) return -1;
+    tune_sweep(s, FORM_SPLIT, 4);
+  }
+  tune_sweep(s, FORM_F64, 4);
+  s.polish_on = false;
   s.run_iters32 = 0;
   s.run_iters64 = 0;
   const double V = (double)VOLUME;
@@ -565,6 +606,7 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
     launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
     even = 0;
     fp32 = false;
+    s.polish_on = split;
   };
   for (i = 0; i < maxit; i++) {
     const bool chk = (i % checkinterval) == 0;
@@ -575,7 +617,7 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
     if (!chk) continue;
     // the check of state i rides on the even sweep of iteration i+1
     do_sweep(s, fp32, 0, 4, nullptr, even, 2);
-    const int v = s.variant[fp32 ? 1 : 0][1];
+    const int v = s.variant[sweep_form(s, fp32, 4)][1];
     const double* pc = check_partials(s, fp32);
     launch_check_finalize(pc, pc + 3 * (size_t)s.ncheck, sweep_blocks(s.geo.nh, v), 4.0 * V, V,
                           eps, s.results, s.stream);
@@ -588,6 +630,24 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
     if (s.verbose)
       printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e\n", i, FUNC, DADA, maxdada);
     if (maxdada < eps) {
+      if (s.polish_on) {
+        // a stop found on the split links is decided on the FP64 links:
+        // state i (the even field of the check) with the per-link functional
+        canonical_even(s, false, even);
+        even = 0;
+        if (functional_now(s, false, 4, 1, eps)) return -1;
+        FUNC = s.h_results[0];
+        DADA = s.h_results[1];
+        maxDADA = maxdada = s.h_results[2];
+        if (s.verbose)
+          printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e (FP64 links)\n", i,
+                 FUNC, DADA, maxdada);
+        if (!(maxdada < eps)) {
+          // not there on the FP64 links: iteration i+1 onwards in plain FP64
+          s.polish_on = false;
+          continue;
+        }
+      }
       converged = true;
       break;
     }
@@ -610,6 +670,7 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
     }
   }
   if (fp32) to_fp64();
+  s.polish_on = false;
   canonical_even(s, false, even);
   stop_clock_ms(s, t0);
   if (converged) {
@@ -654,9 +715,9 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   if (mixed) {
     if (ensure_f32_links(s)) return GFX_ERR_ALLOC;
     launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
-    tune_sweep(s, true, 3);
+    tune_sweep(s, FORM_F32, 3);
   }
-  tune_sweep(s, false, 3);
+  tune_sweep(s, FORM_F64, 3);
   auto upload = [&]() {
     for (int pr = 0; pr < 2; pr++)
       hipMemcpyAsync(dact[pr], act[pr].data(), nt, hipMemcpyHostToDevice, s.stream);
@@ -857,25 +918,32 @@ int apply_trafo(gfx_su3* gf) {
   return GFX_OK;
 }
 
+static int form_of_precision(int precision) {
+  if (precision == GFX_PREC_MIXED) return FORM_F32;
+  if (precision == GFX_SWEEP_SPLIT) return FORM_SPLIT;
+  return FORM_F64;
+}
+
 int gfx_time_sweeps(int nsweeps, int precision, double* ms) {
   State& s = g_st;
   if (!s.init) return GFX_ERR_STATE;
-  const bool fp32 = (precision == GFX_PREC_MIXED);
+  const int form = form_of_precision(precision);
+  const bool fp32 = form == FORM_F32;
   if (fp32) {
     int rc = ensure_f32_links(s);
     if (rc) return rc;
     launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
+  } else if (form == FORM_SPLIT) {
+    int rc = ensure_split_links(s);
+    if (rc) return rc;
   }
-  tune_sweep(s, fp32, 4);
+  tune_sweep(s, form, 4);
+  s.polish_on = form == FORM_SPLIT;
   hipEventRecord(s.ev0, s.stream);
   for (int k = 0; k < nsweeps; k++)
-    for (int p = 0; p < 2; p++) {
-      if (fp32)
-        do_sweep(s, true, p, 4, nullptr);
-      else
-        do_sweep(s, false, p, 4, nullptr);
-    }
+    for (int p = 0; p < 2; p++) do_sweep(s, fp32, p, 4, nullptr);
   hipEventRecord(s.ev1, s.stream);
+  s.polish_on = false;
   if (hip_check(hipEventSynchronize(s.ev1), "timing")) return GFX_ERR_DEVICE;
   float t = 0.f;
   hipEventElapsedTime(&t, s.ev0, s.ev1);
@@ -929,7 +997,7 @@ int gfx_set_autotune(int on) {
 
 int gfx_get_sweep_variant(int precision, int ndir, int* variant, int* block, int* rolled,
                           double* ms_per_variant) {
-  const int pi = precision == GFX_PREC_MIXED ? 1 : 0, di = ndir == 4 ? 1 : 0;
+  const int pi = form_of_precision(precision), di = ndir == 4 ? 1 : 0;
   const int v = g_st.variant[pi][di];
   if (variant) *variant = v;
   if (block) *block = sweep_variant_block(v);
@@ -941,7 +1009,7 @@ int gfx_get_sweep_variant(int precision, int ndir, int* variant, int* block, int
 
 int gfx_set_sweep_variant(int precision, int ndir, int variant) {
   if (variant < 0 || variant >= SWEEP_VARIANTS) return GFX_ERR_STATE;
-  const int pi = precision == GFX_PREC_MIXED ? 1 : 0, di = ndir == 4 ? 1 : 0;
+  const int pi = form_of_precision(precision), di = ndir == 4 ? 1 : 0;
   g_st.variant[pi][di] = variant;
   g_st.tuned[pi][di] = true;
   return GFX_OK;

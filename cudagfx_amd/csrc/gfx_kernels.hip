@@ -106,16 +106,16 @@ __device__ __forceinline__ void site_check(const M3<R>& G, const M3<R>& K, doubl
   d = (double)acc;
 }
 
-// the two star terms of direction mu in the FP32 form (see sweep_site)
-template <typename V, typename R, bool NT>
-__device__ __forceinline__ void star_terms_short(M3<R>& K, const V* __restrict__ U,
-                                                 const V* __restrict__ gop, const Geom& geo,
-                                                 const Site& s, int i, int parity, int mu) {
+// the two star terms of direction mu in the short form (see sweep_site)
+template <typename V, typename R, bool NT, typename LK>
+__device__ __forceinline__ void star_terms_short(M3<R>& K, LK U, const V* __restrict__ gop,
+                                                 const Geom& geo, const Site& s, int i,
+                                                 int parity, int mu) {
   const int S = geo.stride;
   const int jf = nb_fwd(geo, s, i, mu);
   const int jb = nb_bwd(geo, s, i, mu);
   Cx<R> a[2][3], b[3][3], p[3][3];
-  Store<V>::template load<R, NT>(field_ptr(U, parity * 4 + mu, S), S, i, a[0], a[1]);
+  link_rows<R, NT>(U, parity * 4 + mu, S, i, a[0], a[1]);
   Store<V>::template load<R, false>(gop, S, jf, b[0], b[1]);
   row2_of(b[2], b[0], b[1]);
   rows_mul_dagger(p, a, b);
@@ -125,7 +125,7 @@ __device__ __forceinline__ void star_terms_short(M3<R>& K, const V* __restrict__
 #pragma unroll
     for (int c = 0; c < 3; c++) K.e[r][c] = cadd(K.e[r][c], p[r][c]);
   Store<V>::template load<R, false>(gop, S, jb, a[0], a[1]);
-  Store<V>::template load<R, NT>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb, b[0], b[1]);
+  link_rows<R, NT>(U, (1 - parity) * 4 + mu, S, jb, b[0], b[1]);
   row2_of(b[2], b[0], b[1]);
   rows_mul(p, a, b);
   row2_of(p[2], p[0], p[1]);
@@ -135,18 +135,23 @@ __device__ __forceinline__ void star_terms_short(M3<R>& K, const V* __restrict__
     for (int c = 0; c < 3; c++) K.e[r][c] = cadd(K.e[r][c], cconj(p[c][r]));
 }
 
-template <typename V, typename R, int NDIR, bool ROLLED, bool NT, int CHK>
-__device__ __forceinline__ void sweep_site(const V* __restrict__ U, const V* __restrict__ gop,
-                                           const V* gin, V* gout, const Geom& geo, const Site& s,
-                                           int i, int parity, R c1, R c2, double& F, double& d) {
+// LK: the links, as planes of V (const V*) or, for the FP64 polish of the
+// mixed path, as SplitLinks.  FP64 (on either link form) keeps the
+// reference's operation order; FP32 uses the short form.  (The short form in
+// FP64 on split links is 2% faster only, and as those links are unitary to
+// 1e-12 only, it no longer matches the reference order to rounding.)
+template <typename V, typename R, int NDIR, bool ROLLED, bool NT, int CHK, typename LK>
+__device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, const V* gin, V* gout,
+                                           const Geom& geo, const Site& s, int i, int parity, R c1,
+                                           R c2, double& F, double& d) {
   const int S = geo.stride;
   // star K = sum_mu U_mu(x) g^+(x+mu) + U_mu^+(x-mu) g^+(x-mu)
   M3<R> K;
   m_zero(K);
   constexpr bool SHORT = std::is_same<R, float>::value;
-  if (SHORT) {
-    // FP32: each term is a product of two SU(3) matrices; form its rows 0,1
-    // and rebuild row 2 (U g^+ directly, U^+(x-mu) g^+(x-mu) as (g U)^+)
+  if constexpr (SHORT) {
+    // each term is a product of two SU(3) matrices; form its rows 0,1 and
+    // rebuild row 2 (U g^+ directly, U^+(x-mu) g^+(x-mu) as (g U)^+)
     if (ROLLED) {
 #pragma unroll 1
       for (int mu = 0; mu < NDIR; mu++) star_terms_short<V, R, NT>(K, U, gop, geo, s, i, parity, mu);
@@ -163,10 +168,10 @@ __device__ __forceinline__ void sweep_site(const V* __restrict__ U, const V* __r
     }
 #pragma unroll 1
     for (int mu = 0; mu < NDIR; mu++) {
-      M3<R> A = load_m<V, R, NT>(field_ptr(U, parity * 4 + mu, S), S, i);
+      M3<R> A = link_m<R, NT>(U, parity * 4 + mu, S, i);
       M3<R> B = load_m_dagger<V, R>(gop, S, jf[mu]);
       m_addmul(K, A, B);
-      A = load_m_dagger<V, R, NT>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb[mu]);
+      A = link_m_dagger<R, NT>(U, (1 - parity) * 4 + mu, S, jb[mu]);
       B = load_m_dagger<V, R>(gop, S, jb[mu]);
       m_addmul(K, A, B);
     }
@@ -175,10 +180,10 @@ __device__ __forceinline__ void sweep_site(const V* __restrict__ U, const V* __r
     for (int mu = 0; mu < NDIR; mu++) {
       const int jf = nb_fwd(geo, s, i, mu);
       const int jb = nb_bwd(geo, s, i, mu);
-      M3<R> A = load_m<V, R, NT>(field_ptr(U, parity * 4 + mu, S), S, i);
+      M3<R> A = link_m<R, NT>(U, parity * 4 + mu, S, i);
       M3<R> B = load_m_dagger<V, R>(gop, S, jf);
       m_addmul(K, A, B);
-      A = load_m_dagger<V, R, NT>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb);
+      A = link_m_dagger<R, NT>(U, (1 - parity) * 4 + mu, S, jb);
       B = load_m_dagger<V, R>(gop, S, jb);
       m_addmul(K, A, B);
     }
@@ -188,7 +193,7 @@ __device__ __forceinline__ void sweep_site(const V* __restrict__ U, const V* __r
   M3<R> h = G;
   cm_relax(h, K);
   m_normalize(h);
-  if (SHORT) {
+  if constexpr (SHORT) {
     overrelax_apply_short(G, h, c1, c2);
   } else {
     overrelax_apply(G, h, c1, c2);
@@ -213,12 +218,12 @@ __device__ __forceinline__ void store_check_partials(double F, double d, double*
 // ge / go: even and odd fields of g.  CHK = 2 writes the new own field to
 // galt; otherwise the sweep is in place.
 // bmap[launch position] = block of the lattice (workgroup order)
-template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK>
-__global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* ge, V* go, V* galt,
-                                                     Geom geo, int parity, R c1, R c2,
-                                                     const unsigned char* __restrict__ active,
-                                                     const int* __restrict__ bmap,
-                                                     double* __restrict__ partials) {
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK, typename LK>
+__device__ __forceinline__ void or_sweep_block(LK U, V* ge, V* go, V* galt, const Geom& geo,
+                                               int parity, R c1, R c2,
+                                               const unsigned char* __restrict__ active,
+                                               const int* __restrict__ bmap,
+                                               double* __restrict__ partials) {
   const int b = bmap[blockIdx.x];
   const int i = b * BLOCK + threadIdx.x;
   V* gme = parity ? go : ge;
@@ -240,6 +245,31 @@ __global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* 
     }
     store_check_partials<CHK>(F, d, partials, b);
   }
+}
+
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK>
+__global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* ge, V* go, V* galt,
+                                                     Geom geo, int parity, R c1, R c2,
+                                                     const unsigned char* __restrict__ active,
+                                                     const int* __restrict__ bmap,
+                                                     double* __restrict__ partials) {
+  or_sweep_block<V, R, NDIR, BLOCK, ROLLED, NT, CHK>(U, ge, go, galt, geo, parity, c1, c2, active,
+                                                     bmap, partials);
+}
+
+// The FP64 Landau half sweep of the mixed path's polish: FP64 g and FP64
+// arithmetic (the reference's operation order), links read as split planes
+// (FP32 + int16, within 2^-40 of the FP64 links): 864 instead of 1056 bytes
+// per site update.  Star loop rolled.
+template <int BLOCK, bool NT, int CHK>
+__global__ __launch_bounds__(BLOCK) void k_or_sweep_split(const float4* __restrict__ Uhi,
+                                                           const uint2* __restrict__ Ulo,
+                                                           double2* ge, double2* go, double2* galt,
+                                                           Geom geo, int parity, double c1,
+                                                           double c2, const int* __restrict__ bmap,
+                                                           double* __restrict__ partials) {
+  or_sweep_block<double2, double, 4, BLOCK, true, NT, CHK>(SplitLinks{Uhi, Ulo}, ge, go, galt, geo,
+                                                           parity, c1, c2, nullptr, bmap, partials);
 }
 
 // ------------------------------------------------------------ batched sweep
@@ -682,6 +712,24 @@ __global__ __launch_bounds__(BLK) void k_f64_to_f32(const double2* __restrict__ 
   Store<float4>::store(field_ptr(dst, f, stride), stride, i, M);
 }
 
+// int16 planes of the split links (SplitLinks): FP32 plane p holds the reals
+// of FP64 planes 2p and 2p+1, so word p takes its two int16 pairs from them
+__global__ __launch_bounds__(BLK) void k_split_lo(const double2* __restrict__ src,
+                                                  uint2* __restrict__ lo, int nh, int stride,
+                                                  int nfield) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  const int f = blockIdx.y;
+  if (i >= nh || f >= nfield) return;
+  const double2* u = field_ptr(src, f, stride);
+  uint2* q = lo + (size_t)f * 3 * stride;
+#pragma unroll
+  for (int p = 0; p < 3; p++) {
+    const double2 a = u[(size_t)(2 * p) * stride + i];
+    const double2 b = u[(size_t)(2 * p + 1) * stride + i];
+    q[(size_t)p * stride + i] = make_uint2(split_lo_pair(a.x, a.y), split_lo_pair(b.x, b.y));
+  }
+}
+
 // FP32 g -> FP64 g with FP64 reunitarisation (Gram-Schmidt, dev_su3.cu:1569-1634)
 __global__ __launch_bounds__(BLK) void k_g32_to_g64(const float4* __restrict__ src,
                                                     double2* __restrict__ dst, int nh, int stride) {
@@ -794,6 +842,39 @@ void launch_sweep_f32(const float4* U, float4* ge, float4* go, const Geom& geo, 
                       int nblocks) {
   launch_sweep_t<float4, float>({U, ge, go, galt, geo, parity, active, bmap, nblocks, partials},
                                 ndir, omega, variant, chk, st);
+}
+
+template <int BLOCK, bool NT, int CHK>
+static void launch_split_v(const float4* Uhi, const uint2* Ulo, const SweepArgs<double2>& a,
+                           double c1, double c2, hipStream_t st) {
+  dim3 grid(a.nblocks > 0 ? a.nblocks : (a.geo.nh + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL((k_or_sweep_split<BLOCK, NT, CHK>), grid, dim3(BLOCK), 0, st, Uhi, Ulo, a.ge,
+                     a.go, a.galt, a.geo, a.parity, c1, c2, a.bmap, a.partials);
+}
+
+template <int CHK>
+static void launch_split_chk(const float4* Uhi, const uint2* Ulo, const SweepArgs<double2>& a,
+                             double c1, double c2, int variant, hipStream_t st) {
+  const bool b64 = (variant & 1) != 0, nt = sweep_variant_nt(variant);
+  if (b64)
+    nt ? launch_split_v<64, true, CHK>(Uhi, Ulo, a, c1, c2, st)
+       : launch_split_v<64, false, CHK>(Uhi, Ulo, a, c1, c2, st);
+  else
+    nt ? launch_split_v<256, true, CHK>(Uhi, Ulo, a, c1, c2, st)
+       : launch_split_v<256, false, CHK>(Uhi, Ulo, a, c1, c2, st);
+}
+
+void launch_sweep_split(const float4* Uhi, const uint2* Ulo, double2* ge, double2* go,
+                        const Geom& geo, int parity, double omega, int variant, const int* bmap,
+                        hipStream_t st, int chk, double2* galt, double* partials, int nblocks) {
+  const SweepArgs<double2> a{nullptr, ge, go, galt, geo, parity, nullptr, bmap, nblocks, partials};
+  const double c1 = (1.0 + omega - 1) / 1, c2 = (1.0 + omega - 2) / 2;
+  if (chk == 1)
+    launch_split_chk<1>(Uhi, Ulo, a, c1, c2, variant, st);
+  else if (chk == 2)
+    launch_split_chk<2>(Uhi, Ulo, a, c1, c2, variant, st);
+  else
+    launch_split_chk<0>(Uhi, Ulo, a, c1, c2, variant, st);
 }
 
 int sweep_blocks(int nh, int variant) {
@@ -969,6 +1050,10 @@ void launch_f64_to_f32(const double2* src, float4* dst, const Geom& geo, int nfi
                        hipStream_t st) {
   dim3 grid((geo.nh + BLK - 1) / BLK, nfield);
   hipLaunchKernelGGL(k_f64_to_f32, grid, dim3(BLK), 0, st, src, dst, geo.nh, geo.stride, nfield);
+}
+void launch_split_lo(const double2* src, uint2* lo, const Geom& geo, int nfield, hipStream_t st) {
+  dim3 grid((geo.nh + BLK - 1) / BLK, nfield);
+  hipLaunchKernelGGL(k_split_lo, grid, dim3(BLK), 0, st, src, lo, geo.nh, geo.stride, nfield);
 }
 void launch_g32_to_g64(const float4* src, double2* dst, const Geom& geo, hipStream_t st) {
   dim3 grid((geo.nh + BLK - 1) / BLK, 2);

@@ -202,6 +202,77 @@ __device__ __forceinline__ V* field_ptr(V* base, int field, int stride) {
   return base + (size_t)field * Store<V>::NPLANE * stride;
 }
 
+// ---- split links of the mixed path's FP64 polish ----
+// Each FP64 link element u (|u| <= 1) is held as hi = (float)u in the FP32
+// link planes plus q = rint((u - hi) 2^39) as an int16 in a parallel set of
+// planes, four per 8-byte word in the order of the FP32 plane's four reals.
+// |u - hi| <= 2^-25 gives |q| <= 2^14, and hi + q 2^-39 is within 2^-40 of
+// u: 72 instead of 96 bytes per link.
+constexpr double SPLIT_SCALE = 0x1p39;
+struct SplitLinks {
+  const float4* hi;
+  const uint2* lo;
+};
+
+typedef unsigned int nat_u2 __attribute__((ext_vector_type(2)));
+template <bool NT>
+__device__ __forceinline__ uint2 ld8(const uint2* p) {
+  if (NT) {
+    const nat_u2 v = __builtin_nontemporal_load(reinterpret_cast<const nat_u2*>(p));
+    return make_uint2(v.x, v.y);
+  }
+  return *p;
+}
+__device__ __forceinline__ double split_join(float hi, unsigned w, int upper) {
+  const int q = upper ? ((int)w >> 16) : (int)(short)(w & 0xffffu);
+  return fma((double)q, 1.0 / SPLIT_SCALE, (double)hi);
+}
+__device__ __forceinline__ int split_q(double u) {
+  // clamped so that a non-unitary input cannot wrap around
+  const double q = rint((u - (double)(float)u) * SPLIT_SCALE);
+  return (int)fmin(fmax(q, -32767.0), 32767.0);
+}
+__device__ __forceinline__ unsigned split_lo_pair(double u0, double u1) {
+  return ((unsigned)split_q(u0) & 0xffffu) | ((unsigned)split_q(u1) << 16);
+}
+
+// rows 0, 1 of link field `field` at site index i, from plain planes or
+// from split planes
+template <typename R, bool NT, typename V>
+__device__ __forceinline__ void link_rows(const V* U, int field, int stride, int i, Cx<R> r0[3],
+                                          Cx<R> r1[3]) {
+  Store<V>::template load<R, NT>(field_ptr(U, field, stride), stride, i, r0, r1);
+}
+template <typename R, bool NT>
+__device__ __forceinline__ void link_rows(const SplitLinks& U, int field, int stride, int i,
+                                          Cx<R> r0[3], Cx<R> r1[3]) {
+  const float4* h = U.hi + (size_t)field * 3 * stride + (unsigned)i;
+  const uint2* l = U.lo + (size_t)field * 3 * stride + (unsigned)i;
+  const float4 a = ld16<NT>(h), b = ld16<NT>(h + stride), c = ld16<NT>(h + 2 * (size_t)stride);
+  const uint2 la = ld8<NT>(l), lb = ld8<NT>(l + stride), lc = ld8<NT>(l + 2 * (size_t)stride);
+  r0[0] = cx<R>((R)split_join(a.x, la.x, 0), (R)split_join(a.y, la.x, 1));
+  r0[1] = cx<R>((R)split_join(a.z, la.y, 0), (R)split_join(a.w, la.y, 1));
+  r0[2] = cx<R>((R)split_join(b.x, lb.x, 0), (R)split_join(b.y, lb.x, 1));
+  r1[0] = cx<R>((R)split_join(b.z, lb.y, 0), (R)split_join(b.w, lb.y, 1));
+  r1[1] = cx<R>((R)split_join(c.x, lc.x, 0), (R)split_join(c.y, lc.x, 1));
+  r1[2] = cx<R>((R)split_join(c.z, lc.y, 0), (R)split_join(c.w, lc.y, 1));
+}
+template <typename R, bool NT, typename LK>
+__device__ __forceinline__ M3<R> link_m(const LK& U, int field, int stride, int i) {
+  M3<R> m;
+  link_rows<R, NT>(U, field, stride, i, m.e[0], m.e[1]);
+  m_row2(m);
+  return m;
+}
+template <typename R, bool NT, typename LK>
+__device__ __forceinline__ M3<R> link_m_dagger(const LK& U, int field, int stride, int i) {
+  Cx<R> r0[3], r1[3];
+  link_rows<R, NT>(U, field, stride, i, r0, r1);
+  M3<R> d;
+  m_dagger_from_rows(d, r0, r1);
+  return d;
+}
+
 // One lattice of a batch: its geometry and its fields (FP64 double2 or FP32
 // float4 storage, by the phase of the run).  Gribov copies share U.  g[0] and
 // g[1] are the even and odd fields of the gauge transformation; galt is a

@@ -11,6 +11,9 @@
 
 namespace gfx {
 
+// sweep forms: FP64 links and g, FP32 links and g, FP64 g on split links
+enum { FORM_F64 = 0, FORM_F32 = 1, FORM_SPLIT = 2, NFORM = 3 };
+
 struct State {
   int init = 0;  // 1 gauge fixing, 2 thermalization
   Geom geo{};
@@ -26,6 +29,10 @@ struct State {
   float4* U32 = nullptr;
   float4* g32 = nullptr;
   bool U32_valid = false;
+  // int16 planes that with U32 form the split links of the FP64 polish
+  // (SplitLinks); valid while U32 is
+  uint2* U32lo = nullptr;
+  bool lo_valid = false;
 
   // fused-check partials: odd sweep [0, ncheck), even sweep [ncheck, 2 ncheck)
   double* cpart = nullptr;
@@ -48,6 +55,10 @@ struct State {
   // stall_gate without halving for stall_iters iterations (rounding floor)
   int stall_iters = 300;
   double stall_gate = 1e-8;
+  // the FP64 polish of a mixed Landau run reads the split links (1) or the
+  // FP64 links (0); polish_on: the FP64 sweeps in progress do
+  int polish_split = 1;
+  bool polish_on = false;
   int verbose = 1;
   long mt_seed = -1;
 
@@ -58,11 +69,11 @@ struct State {
   float* rnd_unif = nullptr;
   int n_per_gauss = 0, n_per_unif = 0;
 
-  // sweep kernel variant per [precision fp64/fp32][ndir 3/4], picked by the
-  // autotuner on first use for the current lattice
-  int variant[2][2] = {{0, 0}, {0, 0}};
-  bool tuned[2][2] = {{false, false}, {false, false}};
-  float tune_ms[2][2][32] = {};
+  // sweep kernel variant per [form: FP64, FP32, FP64 on split links][ndir
+  // 3/4], picked by the autotuner on first use for the current lattice
+  int variant[NFORM][2] = {};
+  bool tuned[NFORM][2] = {};
+  float tune_ms[NFORM][2][32] = {};
   int autotune = 1;
   // workgroup maps per [block 256/64][tiling] (device), and whether the
   // tiling applies to this lattice

@@ -116,6 +116,10 @@ int apply_trafo(gfx_su3 *gf);
 
 #define GFX_PREC_FP64 0   /* plain FP64 path                                    */
 #define GFX_PREC_MIXED 1  /* FP32 12-real bulk sweeps, FP64 reductions + polish */
+/* not a precision mode: the sweep form of the mixed path's Landau polish (FP64
+ * g and arithmetic on split links), as the `precision` argument of
+ * gfx_time_sweeps / gfx_get_sweep_variant / gfx_set_sweep_variant */
+#define GFX_SWEEP_SPLIT 2
 
 #define GFX_GAUGE_LANDAU 0
 #define GFX_GAUGE_COULOMB 1
@@ -129,6 +133,12 @@ int gfx_set_precision(int mode, double switch_theta);
  * and has not halved for `iters` iterations (the FP32 rounding floor, near
  * 5e-11 on 32^4); iters = 0 turns this off.  Default 300, 1e-8. */
 int gfx_set_fp32_stall(int iters, double gate);
+/* links read by the FP64 polish of a mixed Landau run: 1 (default) split
+ * links, each element as its FP32 value plus an int16 multiple of 2^-39
+ * (within 2^-40 of the FP64 link, 72 instead of 96 bytes per link), with
+ * the stop confirmed by a functional on the FP64 links (plain FP64 sweeps go
+ * on if it does not hold); 0 the FP64 links. */
+int gfx_set_polish_links(int split);
 /* 0 silences the per-check log lines */
 int gfx_set_verbose(int verbose);
 /* seed for srand() before the twister seeds are drawn (negative: time(NULL)) */
@@ -164,9 +174,11 @@ int gfx_stream_copy(size_t bytes, int nrep, double *ms);
  * gauge, every variant is timed on a scratch copy of g and the fastest is
  * kept.  Variant v (0..31): workgroup size 64 if bit 0 else 256, star loop
  * unrolled if bit 1 else rolled, non-temporal link loads if bit 2, and bits
- * 3-4 the workgroup order: 0 lexicographic, t > 0 chunks of 2^t x2-rows
- * with x3 outermost (variants whose order does not fit the lattice are not
- * timed).  on = 0 keeps variant 0 or the one set by gfx_set_sweep_variant. */
+ * 3-4 the workgroup order: 0 lexicographic, 1 and 2 chunks of 4 and 8
+ * superrows (x2 rows holding whole workgroups) with x3 outermost, 3 chunks
+ * in x1 as well (variants whose order does not fit the lattice are not
+ * timed; the split form has rolled variants only).  on = 0 keeps variant 0
+ * or the one set by gfx_set_sweep_variant. */
 int gfx_set_autotune(int on);
 /* the chosen variant and the measured ms per half sweep of each of the 32
  * variants (0 when not timed); GFX_ERR_STATE if not tuned yet */

@@ -7,6 +7,8 @@ stop at the same iteration (+-2) with checkinterval 1.
 import numpy as np
 import pytest
 
+from test_oracle import split_round
+
 pytestmark = pytest.mark.gpu
 
 EPS12 = float(np.float32(1e-12))  # (double)(float)1e-12, read_input.l:189-192
@@ -351,3 +353,59 @@ def test_coulomb_mixed_slices_switch_on_their_own(gfx, orc):
     assert rc == 0 and rc_o == 0 and np.all(M < EPS12)
     assert np.max(np.abs(F - Fo) / Fo) < 1e-10
     assert len(set(it_gpu.tolist())) > 1
+
+
+def test_split_sweeps_match_oracle(gfx, orc):
+    """The sweep of the mixed run's FP64 polish (FP64 g and arithmetic, links
+    as FP32 + int16 parts), each of its 16 variants, against the oracle's FP64
+    sweep on the same split-rounded links: equal to FP64 rounding.  Against
+    the oracle on the exact links the difference is the effect of the 2^-40
+    link rounding (below 1e-10 on these random links)."""
+    dims = (16, 16, 16, 16)
+    V = orc.volume(dims)
+    u = orc.random_init_gauge(V, 4381)
+    g = orc.random_init_trafo(V, 4382)
+    u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+    g2s = g2.copy()
+    u2s = split_round(u2)
+    for parity in (0, 1):
+        orc.halfsweep(u2, g2, dims, parity, 4, 1.68)
+        orc.halfsweep(u2s, g2s, dims, parity, 4, 1.68)
+    g_cpu, g_cpu_split = orc.from_2v(g2), orc.from_2v(g2s)
+    _setup(gfx, dims, u, g)
+    for v in [v for v in range(32) if not v & 2]:
+        gfx.set_sweep_variant(gfx.SWEEP_SPLIT, 4, v)
+        gfx.set_trafo(g)
+        gfx.time_sweeps(1, gfx.SWEEP_SPLIT)
+        g_gpu = gfx.get_trafo()
+        assert np.abs(g_gpu - g_cpu_split).max() < 1e-13, v
+        assert np.abs(g_gpu - g_cpu).max() < 1e-10, v
+    gfx.finalize_gaugefixing()
+    assert np.abs(g_cpu_split - g_cpu).max() > 1e-14  # the rounding is visible
+
+
+@pytest.mark.parametrize("dims,seed", [((8, 8, 8, 16), 90), ((8, 8, 8, 8), 911)])
+def test_split_polish_matches_fp64_polish(gfx, orc, dims, seed):
+    """The mixed run with its FP64 polish on split links and on the FP64
+    links: the same FP32 phase, the same stop, g and F equal to rounding, and
+    the stop bound holds on the FP64 links (host functional of the result)."""
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, seed, eps=0.35)
+    g = orc.random_init_trafo(V, seed + 1)
+    _setup(gfx, dims, u, g, 1.7, prec=gfx.PREC_MIXED)
+    runs = []
+    for split in (0, 1):
+        gfx.set_polish_links(split)
+        gfx.set_trafo(g)
+        it = gfx.overrelax_gauge(8000, EPS12, 1)
+        runs.append((it, gfx.results(), gfx.last_run_stats(), gfx.get_trafo()))
+    gfx.set_polish_links(1)
+    gfx.set_precision(gfx.PREC_FP64)
+    gfx.finalize_gaugefixing()
+    (it0, r0, s0, g0), (it1, r1, s1, g1) = runs
+    assert it0 > 0 and it1 > 0 and abs(it0 - it1) <= 2, (it0, it1)
+    assert s0[1] == s1[1] and s1[2] > 0  # same FP32 phase, a polish ran
+    assert r1[2] < EPS12 and abs(r1[0] - r0[0]) / r0[0] < 1e-12
+    assert np.abs(g1 - g0).max() < 1e-9
+    _, mx = orc.dAdA(orc.g_trafo(u, g1, dims), dims)
+    assert mx < EPS12 * (1 + 1e-9)

@@ -343,3 +343,21 @@ def test_reference_sa_rule_density(orc):
     # a planted mistake (the Kennedy-Pendleton density with a = kb/2) is caught
     kp = np.sqrt(np.clip(1 - grid ** 2, 0, None)) * np.exp(kb / 2 * (grid - 1))
     assert stats.kstest(acc, cdf_of(kp)).pvalue < 1e-6
+
+
+def split_round(u2):
+    """The split links of the mixed polish restated in numpy: each real of the
+    2-row links as its FP32 value plus an int16 multiple of 2^-39."""
+    hi = u2.astype(np.float32).astype(np.float64)
+    q = np.clip(np.rint((u2 - hi) * 2.0 ** 39), -32767, 32767)
+    return hi + q * 2.0 ** -39
+
+
+def test_split_round_bound(orc):
+    """|split(u) - u| <= 2^-40 for SU(3) links and the int16 part never
+    clamps (|u - (float)u| <= 2^-25 for |u| <= 1)."""
+    u2 = orc.su3_to_2v(orc.random_init_gauge(4096, 4391))
+    hi = u2.astype(np.float32).astype(np.float64)
+    assert np.abs(np.rint((u2 - hi) * 2.0 ** 39)).max() <= 2 ** 14
+    err = np.abs(split_round(u2) - u2)
+    assert err.max() <= 2.0 ** -40 and err.max() > 2.0 ** -44
