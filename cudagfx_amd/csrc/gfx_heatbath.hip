@@ -150,15 +150,21 @@ __device__ __forceinline__ M3<double> star_of(const double2* __restrict__ U,
   const int S = geo.stride;
   M3<double> K;
   m_zero(K);
+  // direction loop kept rolled (one direction's operands live at a time):
+  // unrolled, the heatbath kernels spilled to scratch at 1 wave/SIMD
+  int jf[NDIR], jb[NDIR];
 #pragma unroll
   for (int mu = 0; mu < NDIR; mu++) {
-    const int jf = nb_fwd(geo, s, i, mu);
-    const int jb = nb_bwd(geo, s, i, mu);
+    jf[mu] = nb_fwd(geo, s, i, mu);
+    jb[mu] = nb_bwd(geo, s, i, mu);
+  }
+#pragma unroll 1
+  for (int mu = 0; mu < NDIR; mu++) {
     M3<double> A = load_m<double2, double>(field_ptr(U, parity * 4 + mu, S), S, i);
-    M3<double> B = load_m_dagger<double2, double>(gop, S, jf);
+    M3<double> B = load_m_dagger<double2, double>(gop, S, jf[mu]);
     m_addmul(K, A, B);
-    A = load_m_dagger<double2, double>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb);
-    B = load_m_dagger<double2, double>(gop, S, jb);
+    A = load_m_dagger<double2, double>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb[mu]);
+    B = load_m_dagger<double2, double>(gop, S, jb[mu]);
     m_addmul(K, A, B);
   }
   return K;
