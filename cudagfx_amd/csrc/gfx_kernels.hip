@@ -287,7 +287,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_sweep(const BatchProblem* __res
   const int2 m = bmap[blockIdx.x];
   if (!active[m.x]) return;
   const BatchProblem& P = probs[m.x];
-  const Geom geo = P.geo;
+  const Geom& geo = P.geo;
   const V* U = static_cast<const V*>(P.U);
   V* gme = static_cast<V*>(P.g[parity]);
   const V* gop = static_cast<const V*>(P.g[1 - parity]);
@@ -583,10 +583,11 @@ __global__ __launch_bounds__(BLK) void k_plaquette(const V* __restrict__ U, Geom
     const int i = k - p * geo.nh;
     const int S = geo.stride;
     const Site s = site_coords(geo, i, p);
-#pragma unroll
+    // loops kept rolled: unrolled, the six plaquettes' loads spill
+#pragma unroll 1
     for (int nu = 0; nu < 3; nu++) {
       const int jnu = nb_fwd(geo, s, i, nu);
-#pragma unroll
+#pragma unroll 1
       for (int mu = nu + 1; mu < 4; mu++) {
         const int jmu = nb_fwd(geo, s, i, mu);
         M3<double> M1 = load_m<V, double>(field_ptr(U, p * 4 + mu, S), S, i);
