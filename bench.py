@@ -316,6 +316,9 @@ def measure(args):
                           "split_ms": tsp, "split_GBs": gbsp, "split_site_updates_per_s": (V // 2) / (tsp * 1e-3)},
         "accuracy_mixed_vs_fp64": {"F_rel": abs(last["F"] - r64["F"]) / r64["F"],
                                    "theta_mixed": last["max"], "theta_fp64": r64["max"],
+                                   "dAdA_mixed": last["dAdA"], "dAdA_fp64": r64["dAdA"],
+                                   "dAdA_over_3_fp64": r64["dAdA"] / 3.0,
+                                   "F_mixed": last["F"], "F_fp64": r64["F"],
                                    "g_maxabs_after_global_rotation": link_err},
         "autotuned_sweep": variants,
         "host_boundary": {"upload_links_and_g_s": t_up, "download_g_s": t_down,
