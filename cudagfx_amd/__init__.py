@@ -39,7 +39,7 @@ ABI_FUNCTIONS = [
     "get_gauge", "apply_trafo", "gfx_set_omega", "gfx_set_precision", "gfx_set_fp32_stall", "gfx_set_polish_links", "gfx_set_verbose",
     "gfx_set_mt_seed", "gfx_coulomb_gauge", "gfx_functional_slices", "gfx_halfsweep",
     "gfx_time_sweeps", "gfx_last_run_stats", "gfx_error_string",
-    "read_input", "read_gf_ildg", "write_gf_ildg", "Random", "PlantSeeds", "PutSeed", "GetSeed",
+    "read_input", "read_gf_ildg", "read_ildg_dims", "write_gf_ildg", "Random", "PlantSeeds", "PutSeed", "GetSeed",
     "SelectStream", "unit_init_trafo", "random_init_trafo", "unit_init_gauge",
     "random_init_gauge", "g_trafo", "gauge_functional", "dAdA", "mean_plaq",
 ]
@@ -140,6 +140,7 @@ def load():
         "gfx_error_string": (C.c_char_p, [C.c_int]),
         "read_input": (C.c_int, [C.c_char_p]),
         "read_gf_ildg": (C.c_int, [vp, C.c_char_p]),
+        "read_ildg_dims": (C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),
         "write_gf_ildg": (C.c_int, [vp, C.c_char_p, C.c_int]),
         "Random": (C.c_double, []),
         "PlantSeeds": (None, [C.c_long]),
@@ -532,6 +533,13 @@ def read_gf_ildg(path):
     u = np.zeros(18 * 4 * V)
     rc = load().read_gf_ildg(_ptr(u), str(path).encode())
     return rc, u
+
+
+def ildg_dims(path):
+    """(lx, ly, lz, lt) from an ILDG file's ildg-format record, or None."""
+    d = (C.c_int * 4)()
+    rc = load().read_ildg_dims(str(path).encode(), d)
+    return tuple(d) if rc == 0 else None
 
 
 def write_gf_ildg(gf, path, prec=64):

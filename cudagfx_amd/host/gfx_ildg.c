@@ -110,6 +110,40 @@ static int lime_write_record(FILE *f, const char *type, const void *data, uint64
   return 0;
 }
 
+int read_ildg_dims(char *filename, int dims[4]) {
+  FILE *f = fopen(filename, "rb");
+  if (!f) return -1;
+  unsigned char h[LIME_HDR];
+  int rc = -1;
+  while (rc != 0 && fread(h, 1, LIME_HDR, f) == LIME_HDR && get_be32(h) == LIME_MAGIC) {
+    const uint64_t len = get_be64(h + 8);
+    char type[129];
+    memcpy(type, h + 16, 128);
+    type[128] = 0;
+    const uint64_t padded = len + (8 - len % 8) % 8;
+    if (!strcmp(type, "ildg-binary-data")) break;
+    if (strcmp(type, "ildg-format") != 0 || len > (1u << 20)) {
+      if (fseek(f, (long)padded, SEEK_CUR) != 0) break;
+      continue;
+    }
+    char *msg = (char *)calloc(len + 1, 1);
+    if (msg && (!len || fread(msg, 1, len, f) == len)) {
+      const char *tags[4] = {"lx", "ly", "lz", "lt"};
+      char v[64];
+      rc = 0;
+      for (int k = 0; k < 4; k++) {
+        dims[k] = 0;
+        if (!xml_entry(msg, tags[k], v, sizeof v) || sscanf(v, "%d", &dims[k]) != 1 || dims[k] <= 0)
+          rc = -1;
+      }
+    }
+    free(msg);
+    break;
+  }
+  fclose(f);
+  return rc;
+}
+
 int read_gf_ildg(gfx_su3 *gf, char *filename) {
   FILE *f = fopen(filename, "rb");
   if (!f) {

@@ -17,6 +17,10 @@
  *   top level:  MTSeed = <int>
  *   ORX block:  omega = <flt>  precision = fp64|mixed  switch = <flt>
  *               gauge = landau|coulomb
+ *               copies = <int>   Gribov copies of every configuration
+ *               gaugefiles = <file> <file> ...   (rest of the line, blanks
+ *                   or commas between names) configurations fixed in one
+ *                   batch together with the -f file
  *   SA block:   micro = <int>  sampler = reference|kp
  */
 #include <ctype.h>
@@ -40,6 +44,9 @@ int orxgauge = 0;
 long mtseed = -1;
 int samicro = 0;
 int sasampler = 0;
+int orxcopies = 1;
+int norxfiles = 0;
+char orxfiles[GFX_MAX_GAUGEFILES][GFX_MAX_PATH];
 
 enum { ST_TOP, ST_ORX, ST_SA, ST_THERM };
 
@@ -82,6 +89,27 @@ static int key_eq(const char *p, const char *key) {
   q++;
   while (is_blank(*q)) q++;
   return (int)(q - p);
+}
+
+/* file names from p to the end of the line or a '#', separated by blanks or
+ * commas, appended to orxfiles; returns chars consumed, 0 if there is no name
+ * or one does not fit */
+static int files_len(const char *p) {
+  const char *q = p;
+  int found = 0;
+  while (*q && *q != '#') {
+    while (is_blank(*q) || *q == ',') q++;
+    if (!*q || *q == '#') break;
+    const char *s = q;
+    while (*q && *q != '#' && *q != ',' && !is_blank(*q)) q++;
+    const size_t len = (size_t)(q - s);
+    if (norxfiles >= GFX_MAX_GAUGEFILES || len >= GFX_MAX_PATH) return 0;
+    memcpy(orxfiles[norxfiles], s, len);
+    orxfiles[norxfiles][len] = '\0';
+    norxfiles++;
+    found = 1;
+  }
+  return found ? (int)(q - p) : 0;
 }
 
 static float as_float(const char *p, int n) {
@@ -163,6 +191,10 @@ static int parse_block_line(int *state, const char *p, int line) {
       } else if ((n = key_eq(q, "gauge")) && (!strncmp(q + n, "landau", 6) || !strncmp(q + n, "coulomb", 7))) {
         orxgauge = (q[n] == 'c') ? 1 : 0;
         m = (q[n] == 'c') ? 7 : 6;
+      } else if ((n = key_eq(q, "copies")) && (m = int_len(q + n))) {
+        orxcopies = atoi(q + n);
+      } else if ((n = key_eq(q, "gaugefiles")) && (m = files_len(q + n))) {
+        /* the names run to the end of the line */
       } else {
         return unexpected(line, q, state);
       }
@@ -306,6 +338,7 @@ int read_input(char *conf_file) {
   if (!f) return 2;
   char buf[4096];
   int line = 1, state = ST_TOP, rc = 0;
+  norxfiles = 0; /* the list is the one of this file */
   while (fgets(buf, sizeof buf, f)) {
     size_t n = strlen(buf);
     while (n > 0 && (buf[n - 1] == '\n' || buf[n - 1] == '\r')) buf[--n] = '\0';

@@ -44,6 +44,12 @@ extern int orxgauge;          /* "gauge = landau | coulomb"                     
 extern long mtseed;           /* "MTSeed = <int>"    srand seed of the twisters     */
 extern int samicro;           /* "micro = <int>"     in BeginSAInit                 */
 extern int sasampler;         /* "sampler = reference | kp" in BeginSAInit          */
+/* batch keys of BeginORXInit (no counterpart in the reference) */
+#define GFX_MAX_GAUGEFILES 64
+#define GFX_MAX_PATH 1024
+extern int orxcopies;         /* "copies = <int>"    Gribov copies per configuration */
+extern int norxfiles;         /* "gaugefiles = a b ..." configurations fixed with    */
+extern char orxfiles[GFX_MAX_GAUGEFILES][GFX_MAX_PATH]; /* the -f file in one batch  */
 
 /* read_input.l:265: returns 0, 2 if the file cannot be opened; a line the
  * grammar rejects prints "Parsing error in line %d" and returns 1 (the
@@ -55,6 +61,11 @@ int read_input(char *conf_file);
  * record errors, 120 on a dimension mismatch with LX..T, 500 if the
  * precision entry does not match the data size (reference: exit(120|500)). */
 int read_gf_ildg(gfx_su3 *gf, char *filename);
+/* the lattice extents (lx, ly, lz, lt) of an ILDG file from its ildg-format
+ * record, without reading the field (no counterpart in the reference: for
+ * batches of configurations of different sizes).  0, or -1 if the file or
+ * the record cannot be read. */
+int read_ildg_dims(char *filename, int dims[4]);
 /* write 4V links with prec = 32 or 64; 0 or -1 (open) / 500 (bad prec). */
 int write_gf_ildg(gfx_su3 *gf, char *filename, const int prec);
 

@@ -156,3 +156,52 @@ def test_sa_schedule(gfx):
     pw = [gfx.sa_temperature(i, Tmin, Tmax, 2.0, N) for i in range(N)]
     assert abs(pw[0] - Tmax) < 1e-12 and abs(pw[-1] - Tmin) < 1e-12
     assert gfx.sa_temperature(3, 0.2, 0.2, -1.0, N) == 0.2
+
+
+def test_ildg_dims_without_reading_the_field(gfx, tmp_path):
+    """read_ildg_dims takes the extents from the ildg-format record, whatever
+    LX..T are set to (the batch driver reads files of different sizes)."""
+    gfx.set_lattice(6, 4, 2, 8)
+    path = tmp_path / "d.ildg"
+    assert gfx.write_gf_ildg(gfx.unit_gauge(), path, 32) == 0
+    gfx.set_lattice(4, 4, 4, 4)
+    assert gfx.ildg_dims(path) == (6, 4, 2, 8)
+    assert gfx.ildg_dims(tmp_path / "missing.ildg") is None
+    (tmp_path / "junk.ildg").write_bytes(b"\0" * 300)
+    assert gfx.ildg_dims(tmp_path / "junk.ildg") is None
+
+
+_BATCH_KEYS_CHILD = r"""
+import sys, json
+sys.path.insert(0, sys.argv[1])
+import cudagfx_amd as g, ctypes as C
+lib = g.load()
+rc = g.read_input(sys.argv[2])
+n = C.c_int.in_dll(lib, "norxfiles").value
+files = (C.c_char * 1024 * 64).in_dll(lib, "orxfiles")
+print("JSON" + json.dumps(dict(rc=rc, copies=C.c_int.in_dll(lib, "orxcopies").value,
+      files=[bytes(files[k]).split(b"\0")[0].decode() for k in range(n)],
+      orxmaxit=C.c_int.in_dll(lib, "orxmaxit").value, orxflag=C.c_int.in_dll(lib, "orxflag").value)))
+"""
+
+
+@pytest.mark.parametrize("text,want", [
+    ("BeginORXInit\n  maxit = 100\n  copies = 8\n  gaugefiles = a.ildg b.ildg,c.ildg  # three\nEndORXInit\n",
+     dict(rc=0, copies=8, files=["a.ildg", "b.ildg", "c.ildg"], orxmaxit=100, orxflag=1)),
+    ("BeginORXInit\n  gaugefiles = one.ildg\n  maxit = 7\nEndORXInit\n",
+     dict(rc=0, copies=1, files=["one.ildg"], orxmaxit=7, orxflag=1)),
+    ("BeginORXInit\n  gaugefiles =\nEndORXInit\n", dict(rc=1)),
+    ("BeginORXInit\n  copies = x\nEndORXInit\n", dict(rc=1)),
+])
+def test_batch_keys_of_the_grammar(tmp_path, text, want):
+    """The batch keys of BeginORXInit: copies = <int>, gaugefiles = names to
+    the end of the line (blanks or commas between, '#' ends the list); an
+    empty list or a non-integer count is a parsing error."""
+    path = tmp_path / "batch.in"
+    path.write_text("T=8\nL=8\n\n" + text)
+    r = subprocess.run([sys.executable, "-c", _BATCH_KEYS_CHILD, ROOT, str(path)],
+                       capture_output=True, text=True, timeout=120)
+    got = next(json.loads(l[4:]) for l in r.stdout.splitlines() if l.startswith("JSON"))
+    assert got["rc"] == want["rc"]
+    if want["rc"] == 0:
+        assert {k: got[k] for k in want} == want
