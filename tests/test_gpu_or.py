@@ -409,3 +409,29 @@ def test_split_polish_matches_fp64_polish(gfx, orc, dims, seed):
     assert np.abs(g1 - g0).max() < 1e-9
     _, mx = orc.dAdA(orc.g_trafo(u, g1, dims), dims)
     assert mx < EPS12 * (1 + 1e-9)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_coulomb_slices_without_whole_blocks(gfx, orc, prec):
+    """6^3 x 4: a slice holds 108 sites of a parity, not a whole number of
+    sweep blocks, so the checks run as standalone per-slice functionals (the
+    fused form needs whole blocks); FP64 matches the oracle's stops and g, the
+    mixed form reaches the bound with the oracle's per-slice F."""
+    dims = (6, 6, 6, 4)
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, 71, eps=0.35)
+    g = orc.random_init_trafo(V, 72)
+    _setup(gfx, dims, u, g, 1.7, prec=prec)
+    rc, it_gpu = gfx.coulomb_gauge(6000, EPS12, 3)
+    g_gpu = gfx.get_trafo()
+    F, D, M = gfx.functional_slices()
+    gfx.set_precision(gfx.PREC_FP64)
+    gfx.finalize_gaugefixing()
+    u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+    rc_o, it_cpu = orc.coulomb_gauge(u2, g2, dims, 6000, EPS12, 3, 1.7)
+    Fo, Do, Mo = orc.functional_slices(u2, g2, dims, 3)
+    assert rc == 0 and rc_o == 0 and np.all(M < EPS12)
+    assert np.max(np.abs(F - Fo) / Fo) < 1e-10
+    if prec == 0:
+        assert np.all(np.abs(it_gpu - it_cpu) <= 2), (it_gpu, it_cpu)
+        assert np.abs(g_gpu - orc.from_2v(g2)).max() < 1e-10
