@@ -147,26 +147,14 @@ template <int NDIR>
 __device__ __forceinline__ M3<double> star_of(const double2* __restrict__ U,
                                               const double2* __restrict__ gop, const Geom& geo,
                                               const Site& s, int i, int parity) {
-  const int S = geo.stride;
   M3<double> K;
   m_zero(K);
-  // direction loop kept rolled (one direction's operands live at a time):
-  // unrolled, the heatbath kernels spilled to scratch at 1 wave/SIMD
-  int jf[NDIR], jb[NDIR];
-#pragma unroll
-  for (int mu = 0; mu < NDIR; mu++) {
-    jf[mu] = nb_fwd(geo, s, i, mu);
-    jb[mu] = nb_bwd(geo, s, i, mu);
-  }
+  // direction loop kept rolled (one direction's operands live at a time:
+  // unrolled, these kernels spilled to scratch at 1 wave/SIMD), products in
+  // the short form (the stochastic updates have no bitwise target)
 #pragma unroll 1
-  for (int mu = 0; mu < NDIR; mu++) {
-    M3<double> A = load_m<double2, double>(field_ptr(U, parity * 4 + mu, S), S, i);
-    M3<double> B = load_m_dagger<double2, double>(gop, S, jf[mu]);
-    m_addmul(K, A, B);
-    A = load_m_dagger<double2, double>(field_ptr(U, (1 - parity) * 4 + mu, S), S, jb[mu]);
-    B = load_m_dagger<double2, double>(gop, S, jb[mu]);
-    m_addmul(K, A, B);
-  }
+  for (int mu = 0; mu < NDIR; mu++)
+    star_terms_short<double2, double, false>(K, U, gop, geo, s, i, parity, mu);
   return K;
 }
 
@@ -237,7 +225,7 @@ __global__ __launch_bounds__(HB_BLK) void k_batch_sa_sweep(const BatchProblem* _
                                                            const float* __restrict__ unif) {
   const int2 m = bmap[blockIdx.x];
   const BatchProblem& P = probs[m.x];
-  const Geom geo = P.geo;
+  const Geom& geo = P.geo;
   const int i = m.y * HB_BLK + threadIdx.x;
   if (i >= geo.nh) return;
   const Site s = site_coords(geo, i, parity);
@@ -255,7 +243,7 @@ __global__ __launch_bounds__(HB_BLK) void k_batch_micro_sweep(const BatchProblem
                                                               int parity) {
   const int2 m = bmap[blockIdx.x];
   const BatchProblem& P = probs[m.x];
-  const Geom geo = P.geo;
+  const Geom& geo = P.geo;
   const int i = m.y * HB_BLK + threadIdx.x;
   if (i >= geo.nh) return;
   const Site s = site_coords(geo, i, parity);

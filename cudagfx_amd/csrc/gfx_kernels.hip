@@ -106,41 +106,16 @@ __device__ __forceinline__ void site_check(const M3<R>& G, const M3<R>& K, doubl
   d = (double)acc;
 }
 
-// the two star terms of direction mu in the short form (see sweep_site)
-template <typename V, typename R, bool NT, typename LK>
-__device__ __forceinline__ void star_terms_short(M3<R>& K, LK U, const V* __restrict__ gop,
-                                                 const Geom& geo, const Site& s, int i,
-                                                 int parity, int mu) {
-  const int S = geo.stride;
-  const int jf = nb_fwd(geo, s, i, mu);
-  const int jb = nb_bwd(geo, s, i, mu);
-  Cx<R> a[2][3], b[3][3], p[3][3];
-  link_rows<R, NT>(U, parity * 4 + mu, S, i, a[0], a[1]);
-  Store<V>::template load<R, false>(gop, S, jf, b[0], b[1]);
-  row2_of(b[2], b[0], b[1]);
-  rows_mul_dagger(p, a, b);
-  row2_of(p[2], p[0], p[1]);
-#pragma unroll
-  for (int r = 0; r < 3; r++)
-#pragma unroll
-    for (int c = 0; c < 3; c++) K.e[r][c] = cadd(K.e[r][c], p[r][c]);
-  Store<V>::template load<R, false>(gop, S, jb, a[0], a[1]);
-  link_rows<R, NT>(U, (1 - parity) * 4 + mu, S, jb, b[0], b[1]);
-  row2_of(b[2], b[0], b[1]);
-  rows_mul(p, a, b);
-  row2_of(p[2], p[0], p[1]);
-#pragma unroll
-  for (int r = 0; r < 3; r++)
-#pragma unroll
-    for (int c = 0; c < 3; c++) K.e[r][c] = cadd(K.e[r][c], cconj(p[c][r]));
-}
-
 // LK: the links, as planes of V (const V*) or, for the FP64 polish of the
-// mixed path, as SplitLinks.  FP64 (on either link form) keeps the
-// reference's operation order; FP32 uses the short form.  (The short form in
-// FP64 on split links is 2% faster only, and as those links are unitary to
-// 1e-12 only, it no longer matches the reference order to rounding.)
-template <typename V, typename R, int NDIR, bool ROLLED, bool NT, int CHK, typename LK>
+// mixed path, as SplitLinks.  The single-lattice FP64 sweep (on either link
+// form) keeps the reference's operation order; FP32 and, with FSHORT, the
+// batching form's FP64 sweep use the short form (star_terms_short,
+// overrelax_apply_short).  In FP64 on FP64 links the two agree to rounding;
+// where many Gribov copies share their links the sweep is bound by its
+// arithmetic, and the short form is 10% faster there.  (On split links,
+// unitary to 1e-12 only, it no longer matches the reference order to
+// rounding, and it is 2% faster only: the polish keeps the reference order.)
+template <typename V, typename R, int NDIR, bool ROLLED, bool NT, int CHK, typename LK, bool FSHORT = false>
 __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, const V* gin, V* gout,
                                            const Geom& geo, const Site& s, int i, int parity, R c1,
                                            R c2, double& F, double& d) {
@@ -148,7 +123,7 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
   // star K = sum_mu U_mu(x) g^+(x+mu) + U_mu^+(x-mu) g^+(x-mu)
   M3<R> K;
   m_zero(K);
-  constexpr bool SHORT = std::is_same<R, float>::value;
+  constexpr bool SHORT = std::is_same<R, float>::value || FSHORT;
   if constexpr (SHORT) {
     // each term is a product of two SU(3) matrices; form its rows 0,1 and
     // rebuild row 2 (U g^+ directly, U^+(x-mu) g^+(x-mu) as (g U)^+)
@@ -277,7 +252,8 @@ __global__ __launch_bounds__(BLOCK) void k_or_sweep_split(const float4* __restri
 // configurations of different sizes) in one launch.  bmap[block] = (problem,
 // block inside the problem); a problem whose active flag is 0 has converged
 // and its blocks return at once.  Check partials of block (q, k) go to
-// partials[3 * (poff[q] + k)].
+// partials[3 * (poff[q] + k)].  Both precisions use the short form of the
+// products (see sweep_site).
 template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK>
 __global__ __launch_bounds__(BLOCK) void k_batch_sweep(const BatchProblem* __restrict__ probs,
                                                         const int2* __restrict__ bmap,
@@ -297,11 +273,11 @@ __global__ __launch_bounds__(BLOCK) void k_batch_sweep(const BatchProblem* __res
   if (CHK == 0) {
     if (i >= geo.nh) return;
     const Site s = site_coords(geo, i, parity);
-    sweep_site<V, R, NDIR, ROLLED, NT, 0>(U, gop, gme, gout, geo, s, i, parity, c1, c2, F, d);
+    sweep_site<V, R, NDIR, ROLLED, NT, 0, const V*, true>(U, gop, gme, gout, geo, s, i, parity, c1, c2, F, d);
   } else {
     if (i < geo.nh) {
       const Site s = site_coords(geo, i, parity);
-      sweep_site<V, R, NDIR, ROLLED, NT, CHK>(U, gop, gme, gout, geo, s, i, parity, c1, c2, F, d);
+      sweep_site<V, R, NDIR, ROLLED, NT, CHK, const V*, true>(U, gop, gme, gout, geo, s, i, parity, c1, c2, F, d);
     }
     store_check_partials<CHK>(F, d, partials, poff[m.x] + m.y);
   }

@@ -285,4 +285,37 @@ struct BatchProblem {
   void* galt;
 };
 
+// The two star terms of direction mu, U_mu(x) g^+(x+mu) and
+// U_mu^+(x-mu) g^+(x-mu), added to K in the short form: each is a product of
+// two SU(3) matrices, so rows 0, 1 are formed and row 2 is rebuilt as
+// conj(row0 x row1) (the second as (g(x-mu) U_mu(x-mu))^+).  Equal to the
+// full products for unitary operands; about 25% fewer complex multiplies.
+template <typename V, typename R, bool NT, typename LK>
+__device__ __forceinline__ void star_terms_short(M3<R>& K, LK U, const V* __restrict__ gop,
+                                                 const Geom& geo, const Site& s, int i,
+                                                 int parity, int mu) {
+  const int S = geo.stride;
+  const int jf = nb_fwd(geo, s, i, mu);
+  const int jb = nb_bwd(geo, s, i, mu);
+  Cx<R> a[2][3], b[3][3], p[3][3];
+  link_rows<R, NT>(U, parity * 4 + mu, S, i, a[0], a[1]);
+  Store<V>::template load<R, false>(gop, S, jf, b[0], b[1]);
+  row2_of(b[2], b[0], b[1]);
+  rows_mul_dagger(p, a, b);
+  row2_of(p[2], p[0], p[1]);
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) K.e[r][c] = cadd(K.e[r][c], p[r][c]);
+  Store<V>::template load<R, false>(gop, S, jb, a[0], a[1]);
+  link_rows<R, NT>(U, (1 - parity) * 4 + mu, S, jb, b[0], b[1]);
+  row2_of(b[2], b[0], b[1]);
+  rows_mul(p, a, b);
+  row2_of(p[2], p[0], p[1]);
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) K.e[r][c] = cadd(K.e[r][c], cconj(p[c][r]));
+}
+
 }  // namespace gfx
