@@ -156,9 +156,11 @@ int gfx_functional_slices(double *F, double *dAdA, double *maxdAdA);
  * precision mode; ndir 4 (Landau) or 3 (Coulomb, all slices) */
 int gfx_halfsweep(int parity, int ndir);
 
-/* Timing of the plain sweep: runs `nsweeps` full sweeps (even + odd) in
- * `precision` and returns the mean kernel time of one half-sweep launch in
- * ms, measured with HIP events on the library's stream. */
+/* Timing of the plain sweep: runs `nsweeps` full Landau sweeps (even + odd)
+ * on the current g in `precision` (GFX_PREC_FP64, GFX_PREC_MIXED for the FP32
+ * sweep, or GFX_SWEEP_SPLIT for the polish form) and returns the mean kernel
+ * time of one half-sweep launch in ms, measured with HIP events on the
+ * library's stream. */
 int gfx_time_sweeps(int nsweeps, int precision, double *ms_per_halfsweep);
 /* wall-clock of the last overrelax_gauge / gfx_coulomb_gauge call in ms
  * (first sweep to the stop decision), and the number of FP32 and FP64
@@ -181,7 +183,8 @@ int gfx_stream_copy(size_t bytes, int nrep, double *ms);
  * or the one set by gfx_set_sweep_variant. */
 int gfx_set_autotune(int on);
 /* the chosen variant and the measured ms per half sweep of each of the 32
- * variants (0 when not timed); GFX_ERR_STATE if not tuned yet */
+ * variants (0 when not timed) for precision GFX_PREC_FP64, GFX_PREC_MIXED
+ * (the FP32 sweep) or GFX_SWEEP_SPLIT; GFX_ERR_STATE if not tuned yet */
 int gfx_get_sweep_variant(int precision, int ndir, int *variant, int *block, int *rolled,
                           double *ms_per_variant);
 int gfx_set_sweep_variant(int precision, int ndir, int variant);
