@@ -74,7 +74,8 @@ def test_uneven_batch_full_shapes(gfx, orc):
     """The uneven batch at its real shapes: 4 x 16^4, 4 x 24^4, 4 x 32^3x64,
     4 x 48^4 (31.2 M sites), 200 heatbath sweeps per configuration from a
     cold start, mixed OR to theta < 1e-12 in one batch with the autotuned
-    sweep."""
+    sweep; every fixed configuration is checked with the oracle's host
+    functional (F to 1e-12, max dAdA below the bound)."""
     shapes = ([(16, 16, 16, 16)] * 4 + [(24, 24, 24, 24)] * 4 + [(32, 32, 32, 64)] * 4
               + [(48, 48, 48, 48)] * 4)
     gfx.set_omega(1.7)
@@ -92,16 +93,16 @@ def test_uneven_batch_full_shapes(gfx, orc):
     res = [b.result(q) for q in range(len(cases))]
     st = b.stats()
     v = b.variant(gfx.PREC_MIXED)
-    g0 = b.trafo(0)
+    gs = [b.trafo(q) for q in range(len(cases))]
     b.close()
     print(f"uneven batch: {wall:.1f} s, {st}, FP32 variant {v[0] if v else None}")
     assert rc == 0 and st["graph_launches"] > 0
     assert all(r[2] < EPS12 and r[3] >= 0 for r in res)
-    # the first 16^4 configuration: host check of its fixed links
-    dims, u, _ = cases[0]
-    uf = orc.g_trafo(u, g0, dims)
-    assert abs(orc.gauge_functional(uf, dims) - res[0][0]) < 1e-12
-    assert orc.dAdA(uf, dims)[1] < 2 * EPS12
+    # every configuration: the oracle's host functional of its fixed links
+    for q, (dims, u, _) in enumerate(cases):
+        uf = orc.g_trafo(u, gs[q], dims)
+        assert abs(orc.gauge_functional(uf, dims) - res[q][0]) < 1e-12, (q, dims)
+        assert orc.dAdA(uf, dims)[1] < 2 * EPS12, (q, dims)
 
 
 @pytest.mark.timeout(900)
