@@ -8,16 +8,18 @@ checked every 10 iterations.
 
 A *step* is one complete gauge-fixing run from that fixed start to the stop
 decision in the mixed path (FP32 12-real sweeps until max dAdA < 1e-6,
-then FP64 reunitarisation and FP64 polish sweeps, which read the links as
-FP32 + int16 parts within 2^-40 of the FP64 links; F/theta reductions in
-FP64; the stop confirmed by a functional on the FP64 links).
+then FP64 reunitarisation and FP64 polish sweeps; both phases read packed
+links, each real as a 24-bit fixed-point high part (the FP32 phase, within
+2^-24) plus a 16-bit low part (the polish, within 2^-38 of the FP64 links);
+F/theta reductions in FP64; the stop confirmed by a functional on the FP64
+links).
 value = site-updates/s of those steps (iterations x V / wall time).  The
 plain FP64 run to the same bound is timed beside it (time-to-theta,
 speedup), with the error of F and of the links of the mixed result against
 it.  The roofline entry is the sweep kernel that dominates the mixed run,
 timed with HIP events on the library's stream; algorithmic bytes per
-site-update are 8 links + 3 g (FP32 12-real: 528 B, FP64: 1056 B, FP64 g
-on split links: 864 B).
+site-update are 8 links + 3 g (FP32 g on packed high parts: 432 B, FP64 g
+on packed links: 768 B, FP32 on FP32 links: 528 B, FP64: 1056 B).
 
 cpu_baseline: the oracle's OpenMP half-sweep (a C restatement of the
 reference's sweep on 2-row AoS data) timed on the host cores on a bounded
@@ -49,7 +51,15 @@ MTSEED = 4357
 HBM_PEAK_GBS = 8000.0
 BYTES_FP32 = 8 * 48 + 3 * 48
 BYTES_FP64 = 8 * 96 + 3 * 96
-BYTES_SPLIT = 8 * 72 + 3 * 96
+BYTES_P32 = 8 * 36 + 3 * 48
+BYTES_P64 = 8 * 60 + 3 * 96
+# the roofline kernels by sweep form
+KERNEL_P32 = "k_or_sweep_packed<float4,float>"
+KERNEL_P64 = "k_or_sweep_packed<double2,double>"
+# sweep forms timed: name -> precision argument of time_sweeps (the values of
+# PREC_FP64, PREC_MIXED, SWEEP_PACKED64, SWEEP_PACKED32 in cudagfx_amd)
+FORMS = {"fp64": 0, "fp32": 1, "packed64": 2, "packed32": 3}
+FORM_BYTES = {"fp64": BYTES_FP64, "fp32": BYTES_FP32, "packed64": BYTES_P64, "packed32": BYTES_P32}
 
 
 def synthetic_mt_params():
@@ -154,9 +164,9 @@ def align_global(g_a, g_b):
 
 def profile_run(gfx):
     """The kernels of the timed region in isolation, for counter collection:
-    the autotuner's variant timing for the three sweep forms, then 20 FP64,
-    20 FP32 and 20 split-link FP64 sweeps of the picked variants (the last 40
-    launches of each picked kernel), 5 standalone FP64 checks, a 31-iteration FP64 run with fused
+    the autotuner's variant timing for the four sweep forms, then 20 FP64,
+    20 FP32, 20 packed FP64 and 20 packed FP32 sweeps of the picked variants
+    (the last 40 launches of each picked kernel), 5 standalone FP64 checks, a 31-iteration FP64 run with fused
     checks every 10 iterations, and a plain copy of 512 MiB whose bytes are
     known."""
     global NTHERM
@@ -167,23 +177,17 @@ def profile_run(gfx):
     g0 = gfx.random_init_trafo_host(V)
     gfx.init_gaugefixing(u, g0)
     gfx.set_omega(OMEGA)
-    gfx.time_sweeps(1, gfx.PREC_FP64)
-    gfx.time_sweeps(1, gfx.PREC_MIXED)
-    gfx.time_sweeps(1, gfx.SWEEP_SPLIT)
-    t64 = gfx.time_sweeps(20, gfx.PREC_FP64)
-    t32 = gfx.time_sweeps(20, gfx.PREC_MIXED)
-    tsp = gfx.time_sweeps(20, gfx.SWEEP_SPLIT)
+    for form in FORMS.values():
+        gfx.time_sweeps(1, form)
+    t = {name: gfx.time_sweeps(20, form) for name, form in FORMS.items()}
     for _ in range(5):
         gfx.calc_functional()
     gfx.set_trafo(g0)
     gfx.set_precision(gfx.PREC_FP64)
     gfx.overrelax_gauge(31, EPS, CK)
     tc = gfx.stream_copy(512 << 20, 5)
-    out = {"profile": True, "fp64_sweep_ms": t64, "fp32_sweep_ms": t32, "split_sweep_ms": tsp,
-           "copy_512MiB_ms": tc,
-           "variant_fp64": list(gfx.sweep_variant(gfx.PREC_FP64, 4)[:3]),
-           "variant_fp32": list(gfx.sweep_variant(gfx.PREC_MIXED, 4)[:3]),
-           "variant_split": list(gfx.sweep_variant(gfx.SWEEP_SPLIT, 4)[:3])}
+    out = {"profile": True, "sweep_ms": t, "copy_512MiB_ms": tc,
+           "variants": {name: list(gfx.sweep_variant(form, 4)[:3]) for name, form in FORMS.items()}}
     gfx.finalize_gaugefixing()
     return out
 
@@ -193,6 +197,8 @@ def measure(args):
 
     gfx.load()
     gfx.set_verbose(0)
+    assert FORMS == {"fp64": gfx.PREC_FP64, "fp32": gfx.PREC_MIXED,
+                     "packed64": gfx.SWEEP_PACKED64, "packed32": gfx.SWEEP_PACKED32}
     if args.profile:
         return profile_run(gfx)
     V = int(np.prod(DIMS))
@@ -206,15 +212,14 @@ def measure(args):
     t_up = time.perf_counter() - t_up
     gfx.set_omega(OMEGA)
     # pick the sweep variants (autotuner) before anything is timed
-    gfx.time_sweeps(1, gfx.PREC_FP64)
-    gfx.time_sweeps(1, gfx.PREC_MIXED)
-    gfx.time_sweeps(1, gfx.SWEEP_SPLIT)
+    for form in FORMS.values():
+        gfx.time_sweeps(1, form)
     setup_s = time.perf_counter() - t_setup
 
-    def run(prec, switch=SWITCH, split=1):
+    def run(prec, switch=SWITCH, packed=1):
         gfx.set_trafo(g0)
         gfx.set_precision(prec, switch)
-        gfx.set_polish_links(split)
+        gfx.set_packed_links(packed)
         gfx.sync()
         t0 = time.perf_counter()
         it = gfx.overrelax_gauge(20000, EPS, CK)
@@ -241,35 +246,28 @@ def measure(args):
     # the same run with the switch at 1e-10 (FP32 rounding floor of this
     # lattice ~5e-11; the stall guard covers lattices whose floor is higher)
     tuned = run(gfx.PREC_MIXED, SWITCH_LOW)
-    # the mixed step with its polish on the FP64 links, for comparison
-    plain_polish = run(gfx.PREC_MIXED, SWITCH, split=0)
-    gfx.set_polish_links(1)
+    # the mixed step on the FP32 and FP64 links, for comparison
+    plain_links = run(gfx.PREC_MIXED, SWITCH, packed=0)
+    gfx.set_packed_links(1)
     iters = last["it32"] + last["it64"]
     value = iters * V / (ms_step / 1000.0)
 
-    # kernel timing of the two sweeps (HIP events on the library stream)
+    # kernel timing of the sweeps (HIP events on the library stream)
     gfx.set_trafo(g0)
-    t32 = gfx.time_sweeps(50, gfx.PREC_MIXED)
-    t64 = gfx.time_sweeps(50, gfx.PREC_FP64)
-    tsp = gfx.time_sweeps(50, gfx.SWEEP_SPLIT)
-    gb32 = (V // 2) * BYTES_FP32 / (t32 * 1e-3) / 1e9
-    gb64 = (V // 2) * BYTES_FP64 / (t64 * 1e-3) / 1e9
-    gbsp = (V // 2) * BYTES_SPLIT / (tsp * 1e-3) / 1e9
+    tk = {name: gfx.time_sweeps(50, form) for name, form in FORMS.items()}
+    gbs = {name: (V // 2) * FORM_BYTES[name] / (tk[name] * 1e-3) / 1e9 for name in FORMS}
     # the sweep kernel with the larger share of the mixed step
-    share32 = last["it32"] * 2 * t32
-    share64 = last["it64"] * 2 * tsp
-    dom32 = share32 >= share64
-    kernel = "k_or_sweep<float4,float,4>" if dom32 else "k_or_sweep_split"
-    ach = gb32 if dom32 else gbsp
-    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": ach / HBM_PEAK_GBS, "traffic": read_traffic(kernel), "kernel": kernel,
-            "ms_per_launch": t32 if dom32 else tsp,
-            "algorithmic_bytes_per_launch": (V // 2) * (BYTES_FP32 if dom32 else BYTES_SPLIT)}
+    dom32 = last["it32"] * tk["packed32"] >= last["it64"] * tk["packed64"]
+    name = "packed32" if dom32 else "packed64"
+    kernel = KERNEL_P32 if dom32 else KERNEL_P64
+    roof = {"bound": "hbm", "achieved": gbs[name], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs[name] / HBM_PEAK_GBS, "traffic": read_traffic(kernel), "kernel": kernel,
+            "ms_per_launch": tk[name],
+            "algorithmic_bytes_per_launch": (V // 2) * FORM_BYTES[name]}
 
     link_err = align_global(g_mixed, g_fp64)
     variants = {}
-    for name, prec in (("fp64", gfx.PREC_FP64), ("fp32", gfx.PREC_MIXED),
-                       ("split", gfx.SWEEP_SPLIT)):
+    for name, prec in FORMS.items():
         sv = gfx.sweep_variant(prec, 4)
         if sv is not None:
             variants[name] = {"variant": sv[0], "block": sv[1], "rolled": sv[2],
@@ -305,15 +303,15 @@ def measure(args):
                                "iters_fp32": tuned["it32"], "iters_fp64": tuned["it64"],
                                "stop": tuned["iter"], "theta": tuned["max"],
                                "F_rel": abs(tuned["F"] - r64["F"]) / r64["F"]},
-        "mixed_polish_on_fp64_links": {"time_s": plain_polish["wall_s"],
-                                       "speedup": r64["wall_s"] / plain_polish["wall_s"],
-                                       "stop": plain_polish["iter"],
-                                       "F_rel": abs(plain_polish["F"] - r64["F"]) / r64["F"]},
+        "mixed_on_fp32_fp64_links": {"time_s": plain_links["wall_s"],
+                                     "speedup": r64["wall_s"] / plain_links["wall_s"],
+                                     "iters_fp32": plain_links["it32"], "iters_fp64": plain_links["it64"],
+                                     "stop": plain_links["iter"],
+                                     "F_rel": abs(plain_links["F"] - r64["F"]) / r64["F"]},
         "iterations": {"mixed_fp32": last["it32"], "mixed_fp64": last["it64"],
                        "fp64": r64["it64"], "stop_mixed": last["iter"], "stop_fp64": r64["iter"]},
-        "sweep_kernels": {"fp32_ms": t32, "fp32_GBs": gb32, "fp32_site_updates_per_s": (V // 2) / (t32 * 1e-3),
-                          "fp64_ms": t64, "fp64_GBs": gb64, "fp64_site_updates_per_s": (V // 2) / (t64 * 1e-3),
-                          "split_ms": tsp, "split_GBs": gbsp, "split_site_updates_per_s": (V // 2) / (tsp * 1e-3)},
+        "sweep_kernels": {name: {"ms": tk[name], "GBs": gbs[name], "bytes_per_site_update": FORM_BYTES[name],
+                                 "site_updates_per_s": (V // 2) / (tk[name] * 1e-3)} for name in FORMS},
         "accuracy_mixed_vs_fp64": {"F_rel": abs(last["F"] - r64["F"]) / r64["F"],
                                    "theta_mixed": last["max"], "theta_fp64": r64["max"],
                                    "dAdA_mixed": last["dAdA"], "dAdA_fp64": r64["dAdA"],

@@ -25,9 +25,11 @@ DRIVER_PATH = os.path.join(_HERE, "bin", "cudagaugefix")
 
 PREC_FP64 = 0
 PREC_MIXED = 1
-# the sweep form of the mixed Landau polish (FP64 on split links), for
-# time_sweeps / sweep_variant / set_sweep_variant
-SWEEP_SPLIT = 2
+# the sweep forms of a mixed Landau run on packed links, for time_sweeps /
+# sweep_variant / set_sweep_variant: FP64 polish on high + low parts, FP32
+# phase on high parts
+SWEEP_PACKED64 = 2
+SWEEP_PACKED32 = 3
 GAUGE_LANDAU = 0
 GAUGE_COULOMB = 1
 
@@ -36,7 +38,7 @@ ABI_FUNCTIONS = [
     "init_gaugefixing", "finalize_gaugefixing", "init_thermalization", "finalize_thermalization",
     "overrelax_gauge", "calc_functional", "calc_plaquette", "simannealing_gauge",
     "thermalize_gauge", "init_MT", "update_MT", "finalize_MT", "get_trafo", "set_trafo",
-    "get_gauge", "apply_trafo", "gfx_set_omega", "gfx_set_precision", "gfx_set_fp32_stall", "gfx_set_polish_links", "gfx_set_verbose",
+    "get_gauge", "apply_trafo", "gfx_set_omega", "gfx_set_precision", "gfx_set_fp32_stall", "gfx_set_packed_links", "gfx_set_verbose",
     "gfx_set_mt_seed", "gfx_coulomb_gauge", "gfx_functional_slices", "gfx_halfsweep",
     "gfx_time_sweeps", "gfx_last_run_stats", "gfx_error_string",
     "read_input", "read_gf_ildg", "read_ildg_dims", "write_gf_ildg", "Random", "PlantSeeds", "PutSeed", "GetSeed",
@@ -96,7 +98,7 @@ def load():
         "gfx_set_omega": (C.c_int, [C.c_double]),
         "gfx_set_precision": (C.c_int, [C.c_int, C.c_double]),
         "gfx_set_fp32_stall": (C.c_int, [C.c_int, C.c_double]),
-        "gfx_set_polish_links": (C.c_int, [C.c_int]),
+        "gfx_set_packed_links": (C.c_int, [C.c_int]),
         "gfx_set_verbose": (C.c_int, [C.c_int]),
         "gfx_set_mt_seed": (C.c_int, [C.c_long]),
         "gfx_set_mt_file": (C.c_int, [C.c_char_p]),
@@ -298,10 +300,11 @@ def set_fp32_stall(iters=300, gate=1e-8):
     _check(load().gfx_set_fp32_stall(int(iters), float(gate)), "set_fp32_stall")
 
 
-def set_polish_links(split=1):
-    """Links of the mixed Landau run's FP64 polish: 1 split (FP32 + int16
-    parts, within 2^-40 of the FP64 links), 0 the FP64 links."""
-    _check(load().gfx_set_polish_links(int(split)), "set_polish_links")
+def set_packed_links(on=1):
+    """Links of a mixed Landau run: 1 packed (FP32 phase on 24-bit high
+    parts within 2^-24, FP64 polish on high + 16-bit low parts within 2^-38
+    of the FP64 links), 0 the FP32 and FP64 links."""
+    _check(load().gfx_set_packed_links(int(on)), "set_packed_links")
 
 
 def set_verbose(v):

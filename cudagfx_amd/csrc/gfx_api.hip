@@ -226,21 +226,28 @@ int ensure_f32_links(State& s) {
   }
   launch_f64_to_f32(s.U64, s.U32, s.geo, 8, s.stream);
   s.U32_valid = true;
-  s.lo_valid = false;
+  s.packed_valid = false;
   return GFX_OK;
 }
 
-// the split links of the mixed path's FP64 polish: the FP32 links and their
-// int16 planes
-int ensure_split_links(State& s) {
+// the packed links of the mixed Landau run (60 bytes per link in one
+// allocation: a, b, c planes of the high parts, then the low parts); also
+// makes the FP32 fields, whose validity they share
+int ensure_packed_links(State& s) {
   int rc = ensure_f32_links(s);
   if (rc) return rc;
-  if (s.lo_valid) return GFX_OK;
-  if (!s.U32lo &&
-      dev_alloc((void**)&s.U32lo, (size_t)8 * 3 * s.geo.stride * sizeof(uint2), "split links"))
-    return GFX_ERR_ALLOC;
-  launch_split_lo(s.U64, s.U32lo, s.geo, 8, s.stream);
-  s.lo_valid = true;
+  if (s.packed_valid) return GFX_OK;
+  if (!s.pk.a) {
+    const size_t n = (size_t)8 * s.geo.stride;
+    char* base = nullptr;
+    if (dev_alloc((void**)&base, n * (16 + 16 + 4 + 24), "packed links")) return GFX_ERR_ALLOC;
+    s.pk.a = reinterpret_cast<uint4*>(base);
+    s.pk.b = reinterpret_cast<uint4*>(base + n * 16);
+    s.pk.c = reinterpret_cast<uint32_t*>(base + n * 32);
+    s.pk.lo = reinterpret_cast<uint2*>(base + n * 36);
+  }
+  launch_pack_links(s.U64, s.pk, s.geo, 8, s.stream);
+  s.packed_valid = true;
   return GFX_OK;
 }
 
@@ -250,9 +257,9 @@ void free_all(State& s) {
   hipFree(s.g64);
   hipFree(s.U32);
   hipFree(s.g32);
-  hipFree(s.U32lo);
-  s.U32lo = nullptr;
-  s.lo_valid = false;
+  hipFree(s.pk.a);
+  s.pk = PackedArgs{};
+  s.packed_valid = false;
   hipFree(s.partials);
   hipFree(s.cpart);
   for (int bi = 0; bi < 2; bi++)
@@ -323,11 +330,11 @@ const int* sweep_map(State& s, int v) {
 double2* field64(State& s, int k) { return s.g64 + (size_t)k * 6 * s.geo.stride; }
 float4* field32(State& s, int k) { return s.g32 + (size_t)k * 3 * s.geo.stride; }
 
-// the form of the next sweep: FP32, or FP64 on the FP64 links, or (polish of
-// a mixed Landau run) FP64 on the split links
+// the form of the next sweep: FP32 or FP64, on the packed links while a
+// mixed Landau run reads them
 int sweep_form(const State& s, bool fp32, int ndir) {
-  if (fp32) return FORM_F32;
-  return (s.polish_on && ndir == 4) ? FORM_SPLIT : FORM_F64;
+  if (s.packed_on && ndir == 4) return fp32 ? FORM_P32 : FORM_P64;
+  return fp32 ? FORM_F32 : FORM_F64;
 }
 
 // one half sweep with the tuned variant for (form, ndir): in place on the
@@ -348,9 +355,12 @@ void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* ac
   if (form == FORM_F32)
     launch_sweep_f32(s.U32, field32(s, even), field32(s, 1), s.geo, parity, ndir, s.omega, active,
                      v, map, s.stream, chk, field32(s, 2 - even), part, nb);
-  else if (form == FORM_SPLIT)
-    launch_sweep_split(s.U32, s.U32lo, field64(s, even), field64(s, 1), s.geo, parity, s.omega, v,
-                       map, s.stream, chk, field64(s, 2 - even), part, nb);
+  else if (form == FORM_P32)
+    launch_sweep_packed_f32(s.pk, field32(s, even), field32(s, 1), s.geo, parity, s.omega, v, map,
+                            s.stream, chk, field32(s, 2 - even), part, nb);
+  else if (form == FORM_P64)
+    launch_sweep_packed_f64(s.pk, field64(s, even), field64(s, 1), s.geo, parity, s.omega, v, map,
+                            s.stream, chk, field64(s, 2 - even), part, nb);
   else
     launch_sweep_f64(s.U64, field64(s, even), field64(s, 1), s.geo, parity, ndir, s.omega, active,
                      v, map, s.stream, chk, field64(s, 2 - even), part, nb);
@@ -398,11 +408,11 @@ static void canonical_even(State& s, bool fp32, int even) {
 
 // Times every sweep variant on a scratch copy of g (the state is not
 // touched) and keeps the fastest for this lattice, form and ndir.  The
-// split form (Landau only) has rolled variants only.
+// packed FP64 form (Landau only) has rolled variants only.
 int tune_sweep(State& s, int form, int ndir) {
   const int pi = form, di = ndir == 4 ? 1 : 0;
   if (!s.autotune || s.tuned[pi][di]) return GFX_OK;
-  const bool fp32 = form == FORM_F32;
+  const bool fp32 = form == FORM_F32 || form == FORM_P32;
   const size_t bytes = 2 * (fp32 ? field_bytes32(s.geo) : field_bytes64(s.geo));
   void* scratch = nullptr;
   if (hipMalloc(&scratch, bytes) != hipSuccess) {
@@ -415,8 +425,10 @@ int tune_sweep(State& s, int form, int ndir) {
   auto sweep = [&](int v, const int* map, int p) {
     if (form == FORM_F32)
       launch_sweep_f32(s.U32, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
-    else if (form == FORM_SPLIT)
-      launch_sweep_split(s.U32, s.U32lo, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, s.omega, v, map, s.stream);
+    else if (form == FORM_P32)
+      launch_sweep_packed_f32(s.pk, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, s.omega, v, map, s.stream);
+    else if (form == FORM_P64)
+      launch_sweep_packed_f64(s.pk, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, s.omega, v, map, s.stream);
     else
       launch_sweep_f64(s.U64, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
   };
@@ -424,7 +436,7 @@ int tune_sweep(State& s, int form, int ndir) {
   int bestv = s.variant[pi][di];
   for (int v = 0; v < SWEEP_VARIANTS; v++) {
     if (!s.tiling_ok[block_index(sweep_variant_block(v))][sweep_variant_tiling(v)]) continue;
-    if (form == FORM_SPLIT && !sweep_variant_rolled(v)) continue;
+    if (form == FORM_P64 && !sweep_variant_rolled(v)) continue;
     const int* map = sweep_map(s, v);
     for (int rep = 0; rep < 2; rep++)
       for (int p = 0; p < 2; p++) sweep(v, map, p);
@@ -506,9 +518,9 @@ int gfx_set_precision(int mode, double switch_theta) {
   if (switch_theta > 0) g_st.switch_theta = switch_theta;
   return GFX_OK;
 }
-int gfx_set_polish_links(int split) {
-  if (split != 0 && split != 1) return GFX_ERR_STATE;
-  g_st.polish_split = split;
+int gfx_set_packed_links(int on) {
+  if (on != 0 && on != 1) return GFX_ERR_STATE;
+  g_st.packed_links = on;
   return GFX_OK;
 }
 int gfx_set_fp32_stall(int iters, double gate) {
@@ -576,18 +588,27 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
   if (!s.init) return -1;
   if (checkinterval < 1) checkinterval = 1;
   bool fp32 = (s.precision == GFX_PREC_MIXED);
-  const bool split = fp32 && s.polish_split;
+  const bool packed = fp32 && s.packed_links;
+  // every exit leaves the packed forms off (gfx_halfsweep and the other
+  // entry points sweep on the FP32 and FP64 links)
+  struct PackedOff {
+    State& s;
+    ~PackedOff() { s.packed_on = false; }
+  } packed_off{s};
+  s.packed_on = false;
   if (fp32) {
     if (ensure_f32_links(s)) return -1;
     launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
+  }
+  if (packed) {
+    if (ensure_packed_links(s)) return -1;
+    tune_sweep(s, FORM_P32, 4);
+    tune_sweep(s, FORM_P64, 4);
+  } else if (fp32) {
     tune_sweep(s, FORM_F32, 4);
   }
-  if (split) {
-    if (ensure_split_links(s)) return -1;
-    tune_sweep(s, FORM_SPLIT, 4);
-  }
   tune_sweep(s, FORM_F64, 4);
-  s.polish_on = false;
+  s.packed_on = packed;
   s.run_iters32 = 0;
   s.run_iters64 = 0;
   const double V = (double)VOLUME;
@@ -606,7 +627,6 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
     launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
     even = 0;
     fp32 = false;
-    s.polish_on = split;
   };
   for (i = 0; i < maxit; i++) {
     const bool chk = (i % checkinterval) == 0;
@@ -630,8 +650,8 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
     if (s.verbose)
       printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e\n", i, FUNC, DADA, maxdada);
     if (maxdada < eps) {
-      if (s.polish_on) {
-        // a stop found on the split links is decided on the FP64 links:
+      if (s.packed_on) {
+        // a stop found on the packed links is decided on the FP64 links:
         // state i (the even field of the check) with the per-link functional
         canonical_even(s, false, even);
         even = 0;
@@ -644,7 +664,7 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
                  FUNC, DADA, maxdada);
         if (!(maxdada < eps)) {
           // not there on the FP64 links: iteration i+1 onwards in plain FP64
-          s.polish_on = false;
+          s.packed_on = false;
           continue;
         }
       }
@@ -670,7 +690,7 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
     }
   }
   if (fp32) to_fp64();
-  s.polish_on = false;
+  s.packed_on = false;
   canonical_even(s, false, even);
   stop_clock_ms(s, t0);
   if (converged) {
@@ -920,7 +940,8 @@ int apply_trafo(gfx_su3* gf) {
 
 static int form_of_precision(int precision) {
   if (precision == GFX_PREC_MIXED) return FORM_F32;
-  if (precision == GFX_SWEEP_SPLIT) return FORM_SPLIT;
+  if (precision == GFX_SWEEP_PACKED64) return FORM_P64;
+  if (precision == GFX_SWEEP_PACKED32) return FORM_P32;
   return FORM_F64;
 }
 
@@ -928,22 +949,18 @@ int gfx_time_sweeps(int nsweeps, int precision, double* ms) {
   State& s = g_st;
   if (!s.init) return GFX_ERR_STATE;
   const int form = form_of_precision(precision);
-  const bool fp32 = form == FORM_F32;
-  if (fp32) {
-    int rc = ensure_f32_links(s);
-    if (rc) return rc;
-    launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
-  } else if (form == FORM_SPLIT) {
-    int rc = ensure_split_links(s);
-    if (rc) return rc;
-  }
+  const bool fp32 = form == FORM_F32 || form == FORM_P32;
+  const bool packed = form == FORM_P32 || form == FORM_P64;
+  int rc = packed ? ensure_packed_links(s) : fp32 ? ensure_f32_links(s) : GFX_OK;
+  if (rc) return rc;
+  if (fp32) launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
   tune_sweep(s, form, 4);
-  s.polish_on = form == FORM_SPLIT;
+  s.packed_on = packed;
   hipEventRecord(s.ev0, s.stream);
   for (int k = 0; k < nsweeps; k++)
     for (int p = 0; p < 2; p++) do_sweep(s, fp32, p, 4, nullptr);
   hipEventRecord(s.ev1, s.stream);
-  s.polish_on = false;
+  s.packed_on = false;
   if (hip_check(hipEventSynchronize(s.ev1), "timing")) return GFX_ERR_DEVICE;
   float t = 0.f;
   hipEventElapsedTime(&t, s.ev0, s.ev1);

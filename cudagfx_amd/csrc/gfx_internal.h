@@ -61,13 +61,17 @@ void launch_sweep_f32(const float4* U, float4* ge, float4* go, const Geom& geo, 
                       int ndir, double omega, const unsigned char* active, int variant,
                       const int* bmap, hipStream_t st, int chk = 0, float4* galt = nullptr,
                       double* partials = nullptr, int nblocks = 0);
-// FP64 Landau half sweep on split links (hi: the FP32 link planes, lo: their
-// int16 planes from launch_split_lo); variant as above, the star loop is
-// always rolled (bit 1 ignored)
-void launch_sweep_split(const float4* Uhi, const uint2* Ulo, double2* ge, double2* go,
-                        const Geom& geo, int parity, double omega, int variant, const int* bmap,
-                        hipStream_t st, int chk = 0, double2* galt = nullptr,
-                        double* partials = nullptr, int nblocks = 0);
+// Landau half sweeps on packed links: FP32 g on the high parts (every
+// variant), FP64 g on high + low parts (rolled variants only, bit 1 of the
+// variant ignored)
+void launch_sweep_packed_f32(const PackedArgs& p, float4* ge, float4* go, const Geom& geo,
+                             int parity, double omega, int variant, const int* bmap,
+                             hipStream_t st, int chk = 0, float4* galt = nullptr,
+                             double* partials = nullptr, int nblocks = 0);
+void launch_sweep_packed_f64(const PackedArgs& p, double2* ge, double2* go, const Geom& geo,
+                             int parity, double omega, int variant, const int* bmap,
+                             hipStream_t st, int chk = 0, double2* galt = nullptr,
+                             double* partials = nullptr, int nblocks = 0);
 int sweep_blocks(int nh, int variant);
 // per-slice grid pass of a fused Coulomb check (bps blocks per slice; only
 // active slices are written)
@@ -132,8 +136,9 @@ void launch_apply_trafo_aos(double* aos, const double2* g, const Geom& geo, int 
                             hipStream_t st);
 void launch_f64_to_f32(const double2* src, float4* dst, const Geom& geo, int nfield,
                        hipStream_t st);
-// int16 planes of the split form of nfield FP64 link fields
-void launch_split_lo(const double2* src, uint2* lo, const Geom& geo, int nfield, hipStream_t st);
+// packed form of nfield FP64 link fields
+void launch_pack_links(const double2* src, const PackedArgs& p, const Geom& geo, int nfield,
+                       hipStream_t st);
 void launch_g32_to_g64(const float4* src, double2* dst, const Geom& geo, hipStream_t st);
 // sites [i0, i0 + n) of both parities, fields given separately
 void launch_g32_to_g64_range(const float4* se, const float4* so, double2* de, double2* dd,

@@ -106,15 +106,15 @@ __device__ __forceinline__ void site_check(const M3<R>& G, const M3<R>& K, doubl
   d = (double)acc;
 }
 
-// LK: the links, as planes of V (const V*) or, for the FP64 polish of the
-// mixed path, as SplitLinks.  The single-lattice FP64 sweep (on either link
-// form) keeps the reference's operation order; FP32 and, with FSHORT, the
-// batching form's FP64 sweep use the short form (star_terms_short,
+// LK: the links, as planes of V (const V*) or, for the mixed Landau run, as
+// PackedLinks.  The single-lattice FP64 sweep (on either link form) keeps
+// the reference's operation order; FP32 and, with FSHORT, the batching
+// form's FP64 sweep use the short form (star_terms_short,
 // overrelax_apply_short).  In FP64 on FP64 links the two agree to rounding;
 // where many Gribov copies share their links the sweep is bound by its
-// arithmetic, and the short form is 10% faster there.  (On split links,
-// unitary to 1e-12 only, it no longer matches the reference order to
-// rounding, and it is 2% faster only: the polish keeps the reference order.)
+// arithmetic, and the short form is 10% faster there.  (On packed links,
+// unitary to 1e-11 only, it would no longer match the reference order to
+// rounding: the FP64 polish keeps the reference order.)
 template <typename V, typename R, int NDIR, bool ROLLED, bool NT, int CHK, typename LK, bool FSHORT = false>
 __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, const V* gin, V* gout,
                                            const Geom& geo, const Site& s, int i, int parity, R c1,
@@ -232,19 +232,22 @@ __global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* 
                                                      bmap, partials);
 }
 
-// The FP64 Landau half sweep of the mixed path's polish: FP64 g and FP64
-// arithmetic (the reference's operation order), links read as split planes
-// (FP32 + int16, within 2^-40 of the FP64 links): 864 instead of 1056 bytes
-// per site update.  Star loop rolled.
-template <int BLOCK, bool NT, int CHK>
-__global__ __launch_bounds__(BLOCK) void k_or_sweep_split(const float4* __restrict__ Uhi,
-                                                           const uint2* __restrict__ Ulo,
-                                                           double2* ge, double2* go, double2* galt,
-                                                           Geom geo, int parity, double c1,
-                                                           double c2, const int* __restrict__ bmap,
-                                                           double* __restrict__ partials) {
-  or_sweep_block<double2, double, 4, BLOCK, true, NT, CHK>(SplitLinks{Uhi, Ulo}, ge, go, galt, geo,
-                                                           parity, c1, c2, nullptr, bmap, partials);
+// The Landau half sweeps of the mixed path on packed links (PackedLinks):
+// R = float, g in FP32 planes, links read as 24-bit fixed point (432 instead
+// of 528 bytes per site update); R = double, g in FP64 planes and the
+// reference's FP64 operation order, links as 24 + 16-bit fixed point within
+// 2^-38 of the FP64 links (768 instead of 1056 bytes per site update).
+template <typename V, typename R, int BLOCK, bool ROLLED, bool NT, int CHK>
+__global__ __launch_bounds__(BLOCK) void k_or_sweep_packed(const uint4* __restrict__ Ua,
+                                                            const uint4* __restrict__ Ub,
+                                                            const uint32_t* __restrict__ Uc,
+                                                            const uint2* __restrict__ Ulo, V* ge,
+                                                            V* go, V* galt, Geom geo, int parity,
+                                                            R c1, R c2,
+                                                            const int* __restrict__ bmap,
+                                                            double* __restrict__ partials) {
+  or_sweep_block<V, R, 4, BLOCK, ROLLED, NT, CHK>(PackedLinks{Ua, Ub, Uc, Ulo}, ge, go, galt, geo,
+                                                  parity, c1, c2, nullptr, bmap, partials);
 }
 
 // ------------------------------------------------------------ batched sweep
@@ -689,22 +692,44 @@ __global__ __launch_bounds__(BLK) void k_f64_to_f32(const double2* __restrict__ 
   Store<float4>::store(field_ptr(dst, f, stride), stride, i, M);
 }
 
-// int16 planes of the split links (SplitLinks): FP32 plane p holds the reals
-// of FP64 planes 2p and 2p+1, so word p takes its two int16 pairs from them
-__global__ __launch_bounds__(BLK) void k_split_lo(const double2* __restrict__ src,
-                                                  uint2* __restrict__ lo, int nh, int stride,
-                                                  int nfield) {
+// nfield FP64 link fields -> packed links (PackedLinks): 24-bit hi parts in
+// the a/b/c planes, 16-bit lo parts in the lo planes
+__global__ __launch_bounds__(BLK) void k_pack_links(const double2* __restrict__ src,
+                                                    uint4* __restrict__ pa, uint4* __restrict__ pb,
+                                                    uint32_t* __restrict__ pc,
+                                                    uint2* __restrict__ lo, int nh, int stride,
+                                                    int nfield) {
   const int i = blockIdx.x * BLK + threadIdx.x;
   const int f = blockIdx.y;
   if (i >= nh || f >= nfield) return;
   const double2* u = field_ptr(src, f, stride);
-  uint2* q = lo + (size_t)f * 3 * stride;
+  double r[12];
 #pragma unroll
-  for (int p = 0; p < 3; p++) {
-    const double2 a = u[(size_t)(2 * p) * stride + i];
-    const double2 b = u[(size_t)(2 * p + 1) * stride + i];
-    q[(size_t)p * stride + i] = make_uint2(split_lo_pair(a.x, a.y), split_lo_pair(b.x, b.y));
+  for (int c = 0; c < 6; c++) {
+    const double2 v = u[(size_t)c * stride + i];
+    r[2 * c] = v.x;
+    r[2 * c + 1] = v.y;
   }
+  uint32_t w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int q[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    const int h = pack_hi_of(r[k]);
+    q[k] = pack_lo_of(r[k], h);
+    const uint32_t hv = (uint32_t)h & 0xffffffu;
+    const int o = 24 * k, d = o / 32, sh = o % 32;
+    w[d] |= hv << sh;
+    if (sh > 8) w[d + 1] |= hv >> (32 - sh);
+  }
+  const size_t fi = (size_t)f * stride + i;
+  pa[fi] = make_uint4(w[0], w[1], w[2], w[3]);
+  pb[fi] = make_uint4(w[4], w[5], w[6], w[7]);
+  pc[fi] = w[8];
+  uint2* l = lo + (size_t)f * 3 * stride + i;
+#pragma unroll
+  for (int p = 0; p < 3; p++)
+    l[(size_t)p * stride] = make_uint2(((uint32_t)q[4 * p] & 0xffffu) | ((uint32_t)q[4 * p + 1] << 16),
+                                       ((uint32_t)q[4 * p + 2] & 0xffffu) | ((uint32_t)q[4 * p + 3] << 16));
 }
 
 // FP32 g -> FP64 g with FP64 reunitarisation (Gram-Schmidt, dev_su3.cu:1569-1634)
@@ -821,37 +846,64 @@ void launch_sweep_f32(const float4* U, float4* ge, float4* go, const Geom& geo, 
                                 ndir, omega, variant, chk, st);
 }
 
-template <int BLOCK, bool NT, int CHK>
-static void launch_split_v(const float4* Uhi, const uint2* Ulo, const SweepArgs<double2>& a,
-                           double c1, double c2, hipStream_t st) {
+template <typename V, typename R, int BLOCK, bool ROLLED, bool NT, int CHK>
+static void launch_packed_v(const PackedArgs& p, const SweepArgs<V>& a, R c1, R c2,
+                            hipStream_t st) {
   dim3 grid(a.nblocks > 0 ? a.nblocks : (a.geo.nh + BLOCK - 1) / BLOCK);
-  hipLaunchKernelGGL((k_or_sweep_split<BLOCK, NT, CHK>), grid, dim3(BLOCK), 0, st, Uhi, Ulo, a.ge,
-                     a.go, a.galt, a.geo, a.parity, c1, c2, a.bmap, a.partials);
+  hipLaunchKernelGGL((k_or_sweep_packed<V, R, BLOCK, ROLLED, NT, CHK>), grid, dim3(BLOCK), 0, st,
+                     p.a, p.b, p.c, p.lo, a.ge, a.go, a.galt, a.geo, a.parity, c1, c2, a.bmap,
+                     a.partials);
 }
 
-template <int CHK>
-static void launch_split_chk(const float4* Uhi, const uint2* Ulo, const SweepArgs<double2>& a,
-                             double c1, double c2, int variant, hipStream_t st) {
-  const bool b64 = (variant & 1) != 0, nt = sweep_variant_nt(variant);
-  if (b64)
-    nt ? launch_split_v<64, true, CHK>(Uhi, Ulo, a, c1, c2, st)
-       : launch_split_v<64, false, CHK>(Uhi, Ulo, a, c1, c2, st);
-  else
-    nt ? launch_split_v<256, true, CHK>(Uhi, Ulo, a, c1, c2, st)
-       : launch_split_v<256, false, CHK>(Uhi, Ulo, a, c1, c2, st);
+// FP64 on packed links: rolled variants only (bit 1 of the variant ignored)
+template <typename V, typename R, int CHK>
+static void launch_packed_chk(const PackedArgs& p, const SweepArgs<V>& a, R c1, R c2, int variant,
+                              hipStream_t st) {
+  const int v = std::is_same<R, double>::value ? (variant & ~2) : variant;
+  switch (v & 7) {
+    case 0: launch_packed_v<V, R, 256, true, false, CHK>(p, a, c1, c2, st); break;
+    case 1: launch_packed_v<V, R, 64, true, false, CHK>(p, a, c1, c2, st); break;
+    case 4: launch_packed_v<V, R, 256, true, true, CHK>(p, a, c1, c2, st); break;
+    case 5: launch_packed_v<V, R, 64, true, true, CHK>(p, a, c1, c2, st); break;
+    default:
+      if constexpr (std::is_same<R, float>::value) {
+        switch (v & 7) {
+          case 2: launch_packed_v<V, R, 256, false, false, CHK>(p, a, c1, c2, st); break;
+          case 3: launch_packed_v<V, R, 64, false, false, CHK>(p, a, c1, c2, st); break;
+          case 6: launch_packed_v<V, R, 256, false, true, CHK>(p, a, c1, c2, st); break;
+          default: launch_packed_v<V, R, 64, false, true, CHK>(p, a, c1, c2, st); break;
+        }
+      }
+      break;
+  }
 }
 
-void launch_sweep_split(const float4* Uhi, const uint2* Ulo, double2* ge, double2* go,
-                        const Geom& geo, int parity, double omega, int variant, const int* bmap,
-                        hipStream_t st, int chk, double2* galt, double* partials, int nblocks) {
-  const SweepArgs<double2> a{nullptr, ge, go, galt, geo, parity, nullptr, bmap, nblocks, partials};
-  const double c1 = (1.0 + omega - 1) / 1, c2 = (1.0 + omega - 2) / 2;
+template <typename V, typename R>
+static void launch_packed_t(const PackedArgs& p, const SweepArgs<V>& a, double omega, int variant,
+                            int chk, hipStream_t st) {
+  const R c1 = (R)((1.0 + omega - 1) / 1);
+  const R c2 = (R)((1.0 + omega - 2) / 2);
   if (chk == 1)
-    launch_split_chk<1>(Uhi, Ulo, a, c1, c2, variant, st);
+    launch_packed_chk<V, R, 1>(p, a, c1, c2, variant, st);
   else if (chk == 2)
-    launch_split_chk<2>(Uhi, Ulo, a, c1, c2, variant, st);
+    launch_packed_chk<V, R, 2>(p, a, c1, c2, variant, st);
   else
-    launch_split_chk<0>(Uhi, Ulo, a, c1, c2, variant, st);
+    launch_packed_chk<V, R, 0>(p, a, c1, c2, variant, st);
+}
+
+void launch_sweep_packed_f32(const PackedArgs& p, float4* ge, float4* go, const Geom& geo,
+                             int parity, double omega, int variant, const int* bmap,
+                             hipStream_t st, int chk, float4* galt, double* partials,
+                             int nblocks) {
+  launch_packed_t<float4, float>(p, {nullptr, ge, go, galt, geo, parity, nullptr, bmap, nblocks, partials},
+                                 omega, variant, chk, st);
+}
+void launch_sweep_packed_f64(const PackedArgs& p, double2* ge, double2* go, const Geom& geo,
+                             int parity, double omega, int variant, const int* bmap,
+                             hipStream_t st, int chk, double2* galt, double* partials,
+                             int nblocks) {
+  launch_packed_t<double2, double>(p, {nullptr, ge, go, galt, geo, parity, nullptr, bmap, nblocks, partials},
+                                   omega, variant, chk, st);
 }
 
 int sweep_blocks(int nh, int variant) {
@@ -1028,9 +1080,11 @@ void launch_f64_to_f32(const double2* src, float4* dst, const Geom& geo, int nfi
   dim3 grid((geo.nh + BLK - 1) / BLK, nfield);
   hipLaunchKernelGGL(k_f64_to_f32, grid, dim3(BLK), 0, st, src, dst, geo.nh, geo.stride, nfield);
 }
-void launch_split_lo(const double2* src, uint2* lo, const Geom& geo, int nfield, hipStream_t st) {
+void launch_pack_links(const double2* src, const PackedArgs& p, const Geom& geo, int nfield,
+                       hipStream_t st) {
   dim3 grid((geo.nh + BLK - 1) / BLK, nfield);
-  hipLaunchKernelGGL(k_split_lo, grid, dim3(BLK), 0, st, src, lo, geo.nh, geo.stride, nfield);
+  hipLaunchKernelGGL(k_pack_links, grid, dim3(BLK), 0, st, src, (uint4*)p.a, (uint4*)p.b,
+                     (uint32_t*)p.c, (uint2*)p.lo, geo.nh, geo.stride, nfield);
 }
 void launch_g32_to_g64(const float4* src, double2* dst, const Geom& geo, hipStream_t st) {
   dim3 grid((geo.nh + BLK - 1) / BLK, 2);

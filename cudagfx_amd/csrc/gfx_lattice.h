@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "gfx_su3.h"
 
 namespace gfx {
@@ -202,19 +204,35 @@ __device__ __forceinline__ V* field_ptr(V* base, int field, int stride) {
   return base + (size_t)field * Store<V>::NPLANE * stride;
 }
 
-// ---- split links of the mixed path's FP64 polish ----
-// Each FP64 link element u (|u| <= 1) is held as hi = (float)u in the FP32
-// link planes plus q = rint((u - hi) 2^39) as an int16 in a parallel set of
-// planes, four per 8-byte word in the order of the FP32 plane's four reals.
-// |u - hi| <= 2^-25 gives |q| <= 2^14, and hi + q 2^-39 is within 2^-40 of
-// u: 72 instead of 96 bytes per link.
-constexpr double SPLIT_SCALE = 0x1p39;
-struct SplitLinks {
-  const float4* hi;
-  const uint2* lo;
+// ---- packed links of the mixed Landau path ----
+// The elements u of a unitary link lie in [-1, 1], so they are held in fixed
+// point.  hi = rint(u 2^23), clamped to +-(2^23 - 1), is a signed 24-bit
+// integer; the 12 reals of a link take 288 bits = 9 dwords, stored as two
+// uint4 planes (dwords 0-3, 4-7) and one uint plane (dword 8), real k at bits
+// [24k, 24k + 24).  The FP32 bulk sweeps read hi 2^-23 (within 2^-24 of u, 36
+// instead of 48 bytes per link).  The FP64 polish adds lo = rint((u - hi
+// 2^-23) 2^38), an int16 (|u - hi 2^-23| <= 2^-23 even where hi is clamped, so
+// |lo| <= 2^15, clamped to 32767): hi 2^-23 + lo 2^-38 is within 2^-38 of u,
+// 60 instead of 96 bytes per link.  lo is held in three uint2 planes per
+// field, four int16 per 8-byte word in the order of the reals.
+constexpr double PACK_HI = 0x1p23;
+constexpr double PACK_LO = 0x1p38;
+struct PackedLinks {
+  const uint4* a;       // field f, site i at [f * stride + i]
+  const uint4* b;
+  const uint32_t* c;
+  const uint2* lo;      // field f, plane p at [(3f + p) * stride + i]; FP64 only
+};
+// the same planes as the packing kernel writes them (host side)
+struct PackedArgs {
+  uint4* a = nullptr;
+  uint4* b = nullptr;
+  uint32_t* c = nullptr;
+  uint2* lo = nullptr;
 };
 
 typedef unsigned int nat_u2 __attribute__((ext_vector_type(2)));
+typedef unsigned int nat_u4 __attribute__((ext_vector_type(4)));
 template <bool NT>
 __device__ __forceinline__ uint2 ld8(const uint2* p) {
   if (NT) {
@@ -223,39 +241,75 @@ __device__ __forceinline__ uint2 ld8(const uint2* p) {
   }
   return *p;
 }
-__device__ __forceinline__ double split_join(float hi, unsigned w, int upper) {
-  const int q = upper ? ((int)w >> 16) : (int)(short)(w & 0xffffu);
-  return fma((double)q, 1.0 / SPLIT_SCALE, (double)hi);
+template <bool NT>
+__device__ __forceinline__ uint4 ld16(const uint4* p) {
+  if (NT) {
+    const nat_u4 v = __builtin_nontemporal_load(reinterpret_cast<const nat_u4*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+  }
+  return *p;
 }
-__device__ __forceinline__ int split_q(double u) {
-  // clamped so that a non-unitary input cannot wrap around
-  const double q = rint((u - (double)(float)u) * SPLIT_SCALE);
-  return (int)fmin(fmax(q, -32767.0), 32767.0);
+template <bool NT>
+__device__ __forceinline__ uint32_t ld4(const uint32_t* p) {
+  if (NT) return __builtin_nontemporal_load(p);
+  return *p;
 }
-__device__ __forceinline__ unsigned split_lo_pair(double u0, double u1) {
-  return ((unsigned)split_q(u0) & 0xffffu) | ((unsigned)split_q(u1) << 16);
+// signed 24-bit value K of the 288-bit stream w[0..8]
+template <int K>
+__device__ __forceinline__ int pack_s24(const uint32_t w[9]) {
+  constexpr int o = 24 * K, d = o / 32, sh = o % 32;
+  uint32_t v;
+  if (sh <= 8)
+    v = w[d] >> sh;
+  else
+    v = __builtin_amdgcn_alignbit(w[d + 1], w[d], sh);
+  return ((int)(v << 8)) >> 8;
+}
+__device__ __forceinline__ int pack_lo16(const uint32_t q[6], int k) {
+  const uint32_t v = q[k >> 1];
+  return (k & 1) ? ((int)v >> 16) : (int)(short)(v & 0xffffu);
+}
+// hi and lo of u, as stored by k_pack_links
+__device__ __forceinline__ int pack_hi_of(double u) {
+  return (int)fmin(fmax(rint(u * PACK_HI), -8388607.0), 8388607.0);
+}
+__device__ __forceinline__ int pack_lo_of(double u, int hi) {
+  return (int)fmin(fmax(rint((u - (double)hi / PACK_HI) * PACK_LO), -32767.0), 32767.0);
 }
 
 // rows 0, 1 of link field `field` at site index i, from plain planes or
-// from split planes
+// from packed planes (R = float: hi only; R = double: hi + lo)
 template <typename R, bool NT, typename V>
 __device__ __forceinline__ void link_rows(const V* U, int field, int stride, int i, Cx<R> r0[3],
                                           Cx<R> r1[3]) {
   Store<V>::template load<R, NT>(field_ptr(U, field, stride), stride, i, r0, r1);
 }
 template <typename R, bool NT>
-__device__ __forceinline__ void link_rows(const SplitLinks& U, int field, int stride, int i,
+__device__ __forceinline__ void link_rows(const PackedLinks& U, int field, int stride, int i,
                                           Cx<R> r0[3], Cx<R> r1[3]) {
-  const float4* h = U.hi + (size_t)field * 3 * stride + (unsigned)i;
-  const uint2* l = U.lo + (size_t)field * 3 * stride + (unsigned)i;
-  const float4 a = ld16<NT>(h), b = ld16<NT>(h + stride), c = ld16<NT>(h + 2 * (size_t)stride);
-  const uint2 la = ld8<NT>(l), lb = ld8<NT>(l + stride), lc = ld8<NT>(l + 2 * (size_t)stride);
-  r0[0] = cx<R>((R)split_join(a.x, la.x, 0), (R)split_join(a.y, la.x, 1));
-  r0[1] = cx<R>((R)split_join(a.z, la.y, 0), (R)split_join(a.w, la.y, 1));
-  r0[2] = cx<R>((R)split_join(b.x, lb.x, 0), (R)split_join(b.y, lb.x, 1));
-  r1[0] = cx<R>((R)split_join(b.z, lb.y, 0), (R)split_join(b.w, lb.y, 1));
-  r1[1] = cx<R>((R)split_join(c.x, lc.x, 0), (R)split_join(c.y, lc.x, 1));
-  r1[2] = cx<R>((R)split_join(c.z, lc.y, 0), (R)split_join(c.w, lc.y, 1));
+  const size_t f = (size_t)field * stride + (unsigned)i;
+  const uint4 a = ld16<NT>(U.a + f), b = ld16<NT>(U.b + f);
+  const uint32_t c = ld4<NT>(U.c + f);
+  const uint32_t w[9] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c};
+  const int h[12] = {pack_s24<0>(w), pack_s24<1>(w), pack_s24<2>(w),  pack_s24<3>(w),
+                     pack_s24<4>(w), pack_s24<5>(w), pack_s24<6>(w),  pack_s24<7>(w),
+                     pack_s24<8>(w), pack_s24<9>(w), pack_s24<10>(w), pack_s24<11>(w)};
+  R v[12];
+  if constexpr (std::is_same<R, double>::value) {
+    const uint2* l = U.lo + (size_t)field * 3 * stride + (unsigned)i;
+    const uint2 la = ld8<NT>(l), lb = ld8<NT>(l + stride), lc = ld8<NT>(l + 2 * (size_t)stride);
+    const uint32_t q[6] = {la.x, la.y, lb.x, lb.y, lc.x, lc.y};
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = fma((double)pack_lo16(q, k), 1.0 / PACK_LO, (double)h[k] / PACK_HI);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = (R)h[k] * R(1.0 / PACK_HI);
+  }
+#pragma unroll
+  for (int c3 = 0; c3 < 3; c3++) {
+    r0[c3] = cx<R>(v[2 * c3], v[2 * c3 + 1]);
+    r1[c3] = cx<R>(v[6 + 2 * c3], v[6 + 2 * c3 + 1]);
+  }
 }
 template <typename R, bool NT, typename LK>
 __device__ __forceinline__ M3<R> link_m(const LK& U, int field, int stride, int i) {

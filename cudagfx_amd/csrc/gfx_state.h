@@ -11,8 +11,10 @@
 
 namespace gfx {
 
-// sweep forms: FP64 links and g, FP32 links and g, FP64 g on split links
-enum { FORM_F64 = 0, FORM_F32 = 1, FORM_SPLIT = 2, NFORM = 3 };
+// sweep forms: FP64 links and g, FP32 links and g, and the two forms of the
+// mixed Landau run on packed links: FP64 g on high + low parts, FP32 g on
+// high parts
+enum { FORM_F64 = 0, FORM_F32 = 1, FORM_P64 = 2, FORM_P32 = 3, NFORM = 4 };
 
 struct State {
   int init = 0;  // 1 gauge fixing, 2 thermalization
@@ -29,10 +31,10 @@ struct State {
   float4* U32 = nullptr;
   float4* g32 = nullptr;
   bool U32_valid = false;
-  // int16 planes that with U32 form the split links of the FP64 polish
-  // (SplitLinks); valid while U32 is
-  uint2* U32lo = nullptr;
-  bool lo_valid = false;
+  // packed links (PackedLinks) of the mixed Landau run, made from U64;
+  // valid while U32 is (both are invalidated together)
+  PackedArgs pk{};
+  bool packed_valid = false;
 
   // fused-check partials: odd sweep [0, ncheck), even sweep [ncheck, 2 ncheck)
   double* cpart = nullptr;
@@ -55,10 +57,11 @@ struct State {
   // stall_gate without halving for stall_iters iterations (rounding floor)
   int stall_iters = 300;
   double stall_gate = 1e-8;
-  // the FP64 polish of a mixed Landau run reads the split links (1) or the
-  // FP64 links (0); polish_on: the FP64 sweeps in progress do
-  int polish_split = 1;
-  bool polish_on = false;
+  // a mixed Landau run sweeps on packed links (1: FP32 phase on their high
+  // parts, FP64 polish on high + low parts) or on the FP32 and FP64 links
+  // (0); packed_on: the Landau sweeps in progress read the packed links
+  int packed_links = 1;
+  bool packed_on = false;
   int verbose = 1;
   long mt_seed = -1;
 
@@ -69,7 +72,7 @@ struct State {
   float* rnd_unif = nullptr;
   int n_per_gauss = 0, n_per_unif = 0;
 
-  // sweep kernel variant per [form: FP64, FP32, FP64 on split links][ndir
+  // sweep kernel variant per [form: FORM_*][ndir
   // 3/4], picked by the autotuner on first use for the current lattice
   int variant[NFORM][2] = {};
   bool tuned[NFORM][2] = {};

@@ -116,10 +116,13 @@ int apply_trafo(gfx_su3 *gf);
 
 #define GFX_PREC_FP64 0   /* plain FP64 path                                    */
 #define GFX_PREC_MIXED 1  /* FP32 12-real bulk sweeps, FP64 reductions + polish */
-/* not a precision mode: the sweep form of the mixed path's Landau polish (FP64
- * g and arithmetic on split links), as the `precision` argument of
- * gfx_time_sweeps / gfx_get_sweep_variant / gfx_set_sweep_variant */
-#define GFX_SWEEP_SPLIT 2
+/* not precision modes: the two sweep forms of a mixed Landau run on packed
+ * links (gfx_set_packed_links), as the `precision` argument of
+ * gfx_time_sweeps / gfx_get_sweep_variant / gfx_set_sweep_variant:
+ * PACKED64 the FP64 polish (FP64 g on high + low parts), PACKED32 the FP32
+ * phase (FP32 g on the high parts) */
+#define GFX_SWEEP_PACKED64 2
+#define GFX_SWEEP_PACKED32 3
 
 #define GFX_GAUGE_LANDAU 0
 #define GFX_GAUGE_COULOMB 1
@@ -133,12 +136,14 @@ int gfx_set_precision(int mode, double switch_theta);
  * and has not halved for `iters` iterations (the FP32 rounding floor, near
  * 5e-11 on 32^4); iters = 0 turns this off.  Default 300, 1e-8. */
 int gfx_set_fp32_stall(int iters, double gate);
-/* links read by the FP64 polish of a mixed Landau run: 1 (default) split
- * links, each element as its FP32 value plus an int16 multiple of 2^-39
- * (within 2^-40 of the FP64 link, 72 instead of 96 bytes per link), with
- * the stop confirmed by a functional on the FP64 links (plain FP64 sweeps go
- * on if it does not hold); 0 the FP64 links. */
-int gfx_set_polish_links(int split);
+/* links read by the sweeps of a mixed Landau run: 1 (default) packed links,
+ * each real u of a link as a 24-bit integer hi = rint(u 2^23) plus an int16
+ * lo = rint((u - hi 2^-23) 2^38).  The FP32 phase reads hi 2^-23 (within
+ * 2^-24 of u, 36 instead of 48 bytes per link); the FP64 polish reads
+ * hi 2^-23 + lo 2^-38 (within 2^-38 of u, 60 instead of 96 bytes per link),
+ * with the stop confirmed by a functional on the FP64 links (plain FP64
+ * sweeps go on if it does not hold).  0: the FP32 and FP64 links. */
+int gfx_set_packed_links(int on);
 /* 0 silences the per-check log lines */
 int gfx_set_verbose(int verbose);
 /* seed for srand() before the twister seeds are drawn (negative: time(NULL)) */
@@ -158,7 +163,8 @@ int gfx_halfsweep(int parity, int ndir);
 
 /* Timing of the plain sweep: runs `nsweeps` full Landau sweeps (even + odd)
  * on the current g in `precision` (GFX_PREC_FP64, GFX_PREC_MIXED for the FP32
- * sweep, or GFX_SWEEP_SPLIT for the polish form) and returns the mean kernel
+ * sweep on FP32 links, GFX_SWEEP_PACKED32 / PACKED64 for the packed forms)
+ * and returns the mean kernel
  * time of one half-sweep launch in ms, measured with HIP events on the
  * library's stream. */
 int gfx_time_sweeps(int nsweeps, int precision, double *ms_per_halfsweep);
@@ -179,12 +185,12 @@ int gfx_stream_copy(size_t bytes, int nrep, double *ms);
  * 3-4 the workgroup order: 0 lexicographic, 1 and 2 chunks of 4 and 8
  * superrows (x2 rows holding whole workgroups) with x3 outermost, 3 chunks
  * in x1 as well (variants whose order does not fit the lattice are not
- * timed; the split form has rolled variants only).  on = 0 keeps variant 0
+ * timed; the PACKED64 form has rolled variants only).  on = 0 keeps variant 0
  * or the one set by gfx_set_sweep_variant. */
 int gfx_set_autotune(int on);
 /* the chosen variant and the measured ms per half sweep of each of the 32
  * variants (0 when not timed) for precision GFX_PREC_FP64, GFX_PREC_MIXED
- * (the FP32 sweep) or GFX_SWEEP_SPLIT; GFX_ERR_STATE if not tuned yet */
+ * (the FP32 sweep), GFX_SWEEP_PACKED64 or GFX_SWEEP_PACKED32; GFX_ERR_STATE if not tuned yet */
 int gfx_get_sweep_variant(int precision, int ndir, int *variant, int *block, int *rolled,
                           double *ms_per_variant);
 int gfx_set_sweep_variant(int precision, int ndir, int variant);

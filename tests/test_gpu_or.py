@@ -7,11 +7,22 @@ stop at the same iteration (+-2) with checkinterval 1.
 import numpy as np
 import pytest
 
-from test_oracle import split_round
+from test_oracle import packed_round
 
 pytestmark = pytest.mark.gpu
 
 EPS12 = float(np.float32(1e-12))  # (double)(float)1e-12, read_input.l:189-192
+
+
+def rotation_distance(ga, gb):
+    """max |ga - h gb| over the sites for the constant unitary h that fits
+    best (F is invariant under g -> h g)."""
+    A = ga.reshape(-1, 3, 3, 2)
+    B = gb.reshape(-1, 3, 3, 2)
+    A = A[..., 0] + 1j * A[..., 1]
+    B = B[..., 0] + 1j * B[..., 1]
+    W, _, Vh = np.linalg.svd(np.einsum("nij,nkj->ik", A, B.conj()))
+    return np.abs(A - np.einsum("ij,njk->nik", W @ Vh, B)).max()
 
 
 def _setup(gfx, dims, u, g, omega=1.68, prec=0):
@@ -171,13 +182,7 @@ def test_mixed_precision_matches_fp64(gfx, orc):
     assert i32 > 0 and i64 > 0 and i32 + i64 == itm + 1
     assert abs(Fm - F64) / F64 < 1e-10
     # g agrees up to a constant rotation (F is invariant under g -> h g)
-    A = gm.reshape(-1, 3, 3, 2)
-    B = g64.reshape(-1, 3, 3, 2)
-    A = A[..., 0] + 1j * A[..., 1]
-    B = B[..., 0] + 1j * B[..., 1]
-    W, _, Vh = np.linalg.svd(np.einsum("nij,nkj->ik", A, B.conj()))
-    err = np.abs(A - np.einsum("ij,njk->nik", W @ Vh, B)).max()
-    assert err < 1e-5
+    assert rotation_distance(gm, g64) < 1e-5
 
 
 def test_coulomb_matches_oracle(gfx, orc):
@@ -355,58 +360,84 @@ def test_coulomb_mixed_slices_switch_on_their_own(gfx, orc):
     assert len(set(it_gpu.tolist())) > 1
 
 
-def test_split_sweeps_match_oracle(gfx, orc):
+def test_packed_sweeps_match_oracle(gfx, orc):
     """The sweep of the mixed run's FP64 polish (FP64 g and arithmetic, links
-    as FP32 + int16 parts), each of its 16 variants, against the oracle's FP64
-    sweep on the same split-rounded links: equal to FP64 rounding.  Against
-    the oracle on the exact links the difference is the effect of the 2^-40
-    link rounding (below 1e-10 on these random links)."""
+    as 24-bit high + 16-bit low parts), each of its 16 variants, against the
+    oracle's FP64 sweep on the same packed-rounded links: equal to FP64
+    rounding.  Against the oracle on the exact links the difference is the
+    effect of the 2^-39 link rounding (below 1e-10 on these random links)."""
     dims = (16, 16, 16, 16)
     V = orc.volume(dims)
     u = orc.random_init_gauge(V, 4381)
     g = orc.random_init_trafo(V, 4382)
     u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
     g2s = g2.copy()
-    u2s = split_round(u2)
+    u2s = packed_round(u2)
     for parity in (0, 1):
         orc.halfsweep(u2, g2, dims, parity, 4, 1.68)
         orc.halfsweep(u2s, g2s, dims, parity, 4, 1.68)
-    g_cpu, g_cpu_split = orc.from_2v(g2), orc.from_2v(g2s)
+    g_cpu, g_cpu_packed = orc.from_2v(g2), orc.from_2v(g2s)
     _setup(gfx, dims, u, g)
     for v in [v for v in range(32) if not v & 2]:
-        gfx.set_sweep_variant(gfx.SWEEP_SPLIT, 4, v)
+        gfx.set_sweep_variant(gfx.SWEEP_PACKED64, 4, v)
         gfx.set_trafo(g)
-        gfx.time_sweeps(1, gfx.SWEEP_SPLIT)
+        gfx.time_sweeps(1, gfx.SWEEP_PACKED64)
         g_gpu = gfx.get_trafo()
-        assert np.abs(g_gpu - g_cpu_split).max() < 1e-13, v
+        assert np.abs(g_gpu - g_cpu_packed).max() < 1e-13, v
         assert np.abs(g_gpu - g_cpu).max() < 1e-10, v
     gfx.finalize_gaugefixing()
-    assert np.abs(g_cpu_split - g_cpu).max() > 1e-14  # the rounding is visible
+    assert np.abs(g_cpu_packed - g_cpu).max() > 1e-14  # the rounding is visible
+
+
+def test_packed_fp32_sweeps_close_to_oracle(gfx, orc):
+    """The FP32 sweep of the mixed run's first phase on the 24-bit high parts
+    of the packed links, each of its 32 variants, against the oracle's FP64
+    sweep on the same high-part links: equal to FP32 rounding.  (The high
+    parts themselves are pinned to the bit by the FP64 packed sweep test.)"""
+    dims = (8, 8, 8, 16)
+    V = orc.volume(dims)
+    u = orc.random_init_gauge(V, 4391)
+    g = orc.random_init_trafo(V, 4392)
+    u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+    links = packed_round(u2, lo=False)
+    for parity in (0, 1):
+        orc.halfsweep(links, g2, dims, parity, 4, 1.7)
+    g_cpu = orc.from_2v(g2)
+    _setup(gfx, dims, u, g, 1.7)
+    for v in range(32):
+        gfx.set_sweep_variant(gfx.SWEEP_PACKED32, 4, v)
+        gfx.set_trafo(g)
+        gfx.time_sweeps(1, gfx.SWEEP_PACKED32)
+        err = np.abs(gfx.get_trafo() - g_cpu).max()
+        assert 1e-9 < err < 2e-5, (v, err)
+    gfx.finalize_gaugefixing()
 
 
 @pytest.mark.parametrize("dims,seed", [((8, 8, 8, 16), 90), ((8, 8, 8, 8), 911)])
-def test_split_polish_matches_fp64_polish(gfx, orc, dims, seed):
-    """The mixed run with its FP64 polish on split links and on the FP64
-    links: the same FP32 phase, the same stop, g and F equal to rounding, and
-    the stop bound holds on the FP64 links (host functional of the result)."""
+def test_packed_run_matches_plain_mixed_run(gfx, orc, dims, seed):
+    """The mixed run on packed links (FP32 phase on the high parts, FP64
+    polish on high + low parts) and on the FP32 and FP64 links: both stop
+    below the bound, at iterations within 10%, with F equal to 1e-12
+    relative and g equal up to a constant rotation; the bound holds on the
+    FP64 links (host functional of the packed run's result)."""
     V = orc.volume(dims)
     u = smooth_config(orc, dims, seed, eps=0.35)
     g = orc.random_init_trafo(V, seed + 1)
     _setup(gfx, dims, u, g, 1.7, prec=gfx.PREC_MIXED)
     runs = []
-    for split in (0, 1):
-        gfx.set_polish_links(split)
+    for packed in (0, 1):
+        gfx.set_packed_links(packed)
         gfx.set_trafo(g)
         it = gfx.overrelax_gauge(8000, EPS12, 1)
         runs.append((it, gfx.results(), gfx.last_run_stats(), gfx.get_trafo()))
-    gfx.set_polish_links(1)
+    gfx.set_packed_links(1)
     gfx.set_precision(gfx.PREC_FP64)
     gfx.finalize_gaugefixing()
     (it0, r0, s0, g0), (it1, r1, s1, g1) = runs
-    assert it0 > 0 and it1 > 0 and abs(it0 - it1) <= 2, (it0, it1)
-    assert s0[1] == s1[1] and s1[2] > 0  # same FP32 phase, a polish ran
+    assert it0 > 0 and it1 > 0 and abs(it0 - it1) <= 0.1 * it0, (it0, it1)
+    assert s1[1] > 0 and s1[2] > 0  # an FP32 phase and a polish ran
     assert r1[2] < EPS12 and abs(r1[0] - r0[0]) / r0[0] < 1e-12
-    assert np.abs(g1 - g0).max() < 1e-9
+    assert rotation_distance(g1, g0) < 1e-5
     _, mx = orc.dAdA(orc.g_trafo(u, g1, dims), dims)
     assert mx < EPS12 * (1 + 1e-9)
 

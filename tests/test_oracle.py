@@ -345,19 +345,31 @@ def test_reference_sa_rule_density(orc):
     assert stats.kstest(acc, cdf_of(kp)).pvalue < 1e-6
 
 
-def split_round(u2):
-    """The split links of the mixed polish restated in numpy: each real of the
-    2-row links as its FP32 value plus an int16 multiple of 2^-39."""
-    hi = u2.astype(np.float32).astype(np.float64)
-    q = np.clip(np.rint((u2 - hi) * 2.0 ** 39), -32767, 32767)
-    return hi + q * 2.0 ** -39
+def packed_round(u2, lo=True):
+    """The packed links of the mixed Landau run restated in numpy: each real
+    u of the 2-row links as hi = rint(u 2^23) clamped to 24 bits and, with
+    lo, lo = rint((u - hi 2^-23) 2^38) clamped to 16 bits; the value read
+    back is hi 2^-23 (+ lo 2^-38)."""
+    hi = np.clip(np.rint(u2 * 2.0 ** 23), -8388607, 8388607)
+    if not lo:
+        return hi * 2.0 ** -23
+    q = np.clip(np.rint((u2 - hi * 2.0 ** -23) * 2.0 ** 38), -32767, 32767)
+    return hi * 2.0 ** -23 + q * 2.0 ** -38
 
 
-def test_split_round_bound(orc):
-    """|split(u) - u| <= 2^-40 for SU(3) links and the int16 part never
-    clamps (|u - (float)u| <= 2^-25 for |u| <= 1)."""
+def test_packed_round_bound(orc):
+    """|packed(u) - u| <= 2^-39 for SU(3) links (2^-38 at |u| = 1, where hi
+    and lo both clamp), the high parts alone within 2^-24 (2^-23 at |u| = 1),
+    and the low part never clamps elsewhere."""
     u2 = orc.su3_to_2v(orc.random_init_gauge(4096, 4391))
-    hi = u2.astype(np.float32).astype(np.float64)
-    assert np.abs(np.rint((u2 - hi) * 2.0 ** 39)).max() <= 2 ** 14
-    err = np.abs(split_round(u2) - u2)
-    assert err.max() <= 2.0 ** -40 and err.max() > 2.0 ** -44
+    hi = np.rint(u2 * 2.0 ** 23)
+    assert np.abs(hi).max() <= 8388607
+    assert np.abs(np.rint((u2 - hi * 2.0 ** -23) * 2.0 ** 38)).max() <= 2 ** 14
+    err = np.abs(packed_round(u2) - u2)
+    assert err.max() <= 2.0 ** -39 and err.max() > 2.0 ** -43
+    err_hi = np.abs(packed_round(u2, lo=False) - u2)
+    assert err_hi.max() <= 2.0 ** -24 and err_hi.max() > 2.0 ** -28
+    # the unit link: 1 is not a 24-bit multiple of 2^-23, both parts clamp
+    one = np.array([1.0, -1.0, 0.0])
+    assert np.abs(packed_round(one) - one).max() == 2.0 ** -38
+    assert np.abs(packed_round(one, lo=False) - one).max() == 2.0 ** -23
