@@ -128,8 +128,13 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
     // each term is a product of two SU(3) matrices; form its rows 0,1 and
     // rebuild row 2 (U g^+ directly, U^+(x-mu) g^+(x-mu) as (g U)^+)
     if (ROLLED) {
+      // the FP32 rolled loop holds a direction's four operands in flight
+      // together (FP32 registers are cheap); the batching form's FP64 one
+      // does not
+      constexpr bool CL = std::is_same<R, float>::value;
 #pragma unroll 1
-      for (int mu = 0; mu < NDIR; mu++) star_terms_short<V, R, NT>(K, U, gop, geo, s, i, parity, mu);
+      for (int mu = 0; mu < NDIR; mu++)
+        star_terms_short<V, R, NT, LK, CL>(K, U, gop, geo, s, i, parity, mu);
     } else {
 #pragma unroll
       for (int mu = 0; mu < NDIR; mu++) star_terms_short<V, R, NT>(K, U, gop, geo, s, i, parity, mu);
@@ -143,11 +148,18 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
     }
 #pragma unroll 1
     for (int mu = 0; mu < NDIR; mu++) {
-      M3<R> A = link_m<R, NT>(U, parity * 4 + mu, S, i);
-      M3<R> B = load_m_dagger<V, R>(gop, S, jf[mu]);
+      Raw<LK, R> lf, lb;
+      Raw<const V*, R> gf, gb;
+      raw_load<R, NT>(U, parity * 4 + mu, S, i, lf);
+      raw_load<R, false>(gop, S, jf[mu], gf);
+      raw_load<R, NT>(U, (1 - parity) * 4 + mu, S, jb[mu], lb);
+      raw_load<R, false>(gop, S, jb[mu], gb);
+      __builtin_amdgcn_sched_barrier(0);
+      M3<R> A = raw_m<R>(lf);
+      M3<R> B = raw_m_dagger<R>(gf);
       m_addmul(K, A, B);
-      A = link_m_dagger<R, NT>(U, (1 - parity) * 4 + mu, S, jb[mu]);
-      B = load_m_dagger<V, R>(gop, S, jb[mu]);
+      A = raw_m_dagger<R>(lb);
+      B = raw_m_dagger<R>(gb);
       m_addmul(K, A, B);
     }
   } else {

@@ -327,6 +327,116 @@ __device__ __forceinline__ M3<R> link_m_dagger(const LK& U, int field, int strid
   return d;
 }
 
+// A matrix as loaded, before it is turned into rows: the sweeps issue the
+// loads of all four operands of a direction (two links, two neighbour g)
+// before decoding any of them, so that a wavefront has them in flight
+// together instead of one after another.
+template <typename LK, typename R>
+struct Raw;
+template <typename R>
+struct Raw<const double2*, R> {
+  double2 v[6];
+};
+template <typename R>
+struct Raw<const float4*, R> {
+  float4 v[3];
+};
+template <>
+struct Raw<PackedLinks, double> {
+  uint4 a, b;
+  uint32_t c;
+  uint2 lo[3];
+};
+template <>
+struct Raw<PackedLinks, float> {
+  uint4 a, b;
+  uint32_t c;
+};
+
+template <typename R, bool NT>
+__device__ __forceinline__ void raw_load(const double2* f, int stride, int i, Raw<const double2*, R>& r) {
+#pragma unroll
+  for (int c = 0; c < 6; c++) r.v[c] = ld16<NT>(f + (size_t)c * stride + (unsigned)i);
+}
+template <typename R, bool NT>
+__device__ __forceinline__ void raw_load(const float4* f, int stride, int i, Raw<const float4*, R>& r) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) r.v[c] = ld16<NT>(f + (size_t)c * stride + (unsigned)i);
+}
+template <typename R, bool NT>
+__device__ __forceinline__ void raw_load(const PackedLinks& U, int field, int stride, int i,
+                                         Raw<PackedLinks, R>& r) {
+  const size_t f = (size_t)field * stride + (unsigned)i;
+  r.a = ld16<NT>(U.a + f);
+  r.b = ld16<NT>(U.b + f);
+  r.c = ld4<NT>(U.c + f);
+  if constexpr (std::is_same<R, double>::value) {
+    const uint2* l = U.lo + (size_t)field * 3 * stride + (unsigned)i;
+#pragma unroll
+    for (int p = 0; p < 3; p++) r.lo[p] = ld8<NT>(l + (size_t)p * stride);
+  }
+}
+// links given as planes: the field's first plane
+template <typename R, bool NT, typename V>
+__device__ __forceinline__ void raw_load(const V* U, int field, int stride, int i, Raw<const V*, R>& r) {
+  raw_load<R, NT>(field_ptr(U, field, stride), stride, i, r);
+}
+
+template <typename R>
+__device__ __forceinline__ void raw_rows(const Raw<const double2*, R>& r, Cx<R> r0[3], Cx<R> r1[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    r0[c] = cx<R>((R)r.v[c].x, (R)r.v[c].y);
+    r1[c] = cx<R>((R)r.v[3 + c].x, (R)r.v[3 + c].y);
+  }
+}
+template <typename R>
+__device__ __forceinline__ void raw_rows(const Raw<const float4*, R>& r, Cx<R> r0[3], Cx<R> r1[3]) {
+  const float4 a = r.v[0], b = r.v[1], c = r.v[2];
+  r0[0] = cx<R>((R)a.x, (R)a.y);
+  r0[1] = cx<R>((R)a.z, (R)a.w);
+  r0[2] = cx<R>((R)b.x, (R)b.y);
+  r1[0] = cx<R>((R)b.z, (R)b.w);
+  r1[1] = cx<R>((R)c.x, (R)c.y);
+  r1[2] = cx<R>((R)c.z, (R)c.w);
+}
+template <typename R>
+__device__ __forceinline__ void raw_rows(const Raw<PackedLinks, R>& r, Cx<R> r0[3], Cx<R> r1[3]) {
+  const uint32_t w[9] = {r.a.x, r.a.y, r.a.z, r.a.w, r.b.x, r.b.y, r.b.z, r.b.w, r.c};
+  const int h[12] = {pack_s24<0>(w), pack_s24<1>(w), pack_s24<2>(w),  pack_s24<3>(w),
+                     pack_s24<4>(w), pack_s24<5>(w), pack_s24<6>(w),  pack_s24<7>(w),
+                     pack_s24<8>(w), pack_s24<9>(w), pack_s24<10>(w), pack_s24<11>(w)};
+  R v[12];
+  if constexpr (std::is_same<R, double>::value) {
+    const uint32_t q[6] = {r.lo[0].x, r.lo[0].y, r.lo[1].x, r.lo[1].y, r.lo[2].x, r.lo[2].y};
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = fma((double)pack_lo16(q, k), 1.0 / PACK_LO, (double)h[k] / PACK_HI);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = (R)h[k] * R(1.0 / PACK_HI);
+  }
+#pragma unroll
+  for (int c3 = 0; c3 < 3; c3++) {
+    r0[c3] = cx<R>(v[2 * c3], v[2 * c3 + 1]);
+    r1[c3] = cx<R>(v[6 + 2 * c3], v[6 + 2 * c3 + 1]);
+  }
+}
+template <typename R, typename LK>
+__device__ __forceinline__ M3<R> raw_m(const Raw<LK, R>& r) {
+  M3<R> m;
+  raw_rows<R>(r, m.e[0], m.e[1]);
+  m_row2(m);
+  return m;
+}
+template <typename R, typename LK>
+__device__ __forceinline__ M3<R> raw_m_dagger(const Raw<LK, R>& r) {
+  Cx<R> r0[3], r1[3];
+  raw_rows<R>(r, r0, r1);
+  M3<R> d;
+  m_dagger_from_rows(d, r0, r1);
+  return d;
+}
+
 // One lattice of a batch: its geometry and its fields (FP64 double2 or FP32
 // float4 storage, by the phase of the run).  Gribov copies share U.  g[0] and
 // g[1] are the even and odd fields of the gauge transformation; galt is a
@@ -344,7 +454,8 @@ struct BatchProblem {
 // two SU(3) matrices, so rows 0, 1 are formed and row 2 is rebuilt as
 // conj(row0 x row1) (the second as (g(x-mu) U_mu(x-mu))^+).  Equal to the
 // full products for unitary operands; about 25% fewer complex multiplies.
-template <typename V, typename R, bool NT, typename LK>
+// CLUSTER: the four operands are loaded (Raw) before either term is formed.
+template <typename V, typename R, bool NT, typename LK, bool CLUSTER = false>
 __device__ __forceinline__ void star_terms_short(M3<R>& K, LK U, const V* __restrict__ gop,
                                                  const Geom& geo, const Site& s, int i,
                                                  int parity, int mu) {
@@ -352,8 +463,20 @@ __device__ __forceinline__ void star_terms_short(M3<R>& K, LK U, const V* __rest
   const int jf = nb_fwd(geo, s, i, mu);
   const int jb = nb_bwd(geo, s, i, mu);
   Cx<R> a[2][3], b[3][3], p[3][3];
-  link_rows<R, NT>(U, parity * 4 + mu, S, i, a[0], a[1]);
-  Store<V>::template load<R, false>(gop, S, jf, b[0], b[1]);
+  Raw<LK, R> lf, lb;
+  Raw<const V*, R> gf, gb;
+  if constexpr (CLUSTER) {
+    raw_load<R, NT>(U, parity * 4 + mu, S, i, lf);
+    raw_load<R, false>(gop, S, jf, gf);
+    raw_load<R, false>(gop, S, jb, gb);
+    raw_load<R, NT>(U, (1 - parity) * 4 + mu, S, jb, lb);
+    __builtin_amdgcn_sched_barrier(0);
+    raw_rows<R>(lf, a[0], a[1]);
+    raw_rows<R>(gf, b[0], b[1]);
+  } else {
+    link_rows<R, NT>(U, parity * 4 + mu, S, i, a[0], a[1]);
+    Store<V>::template load<R, false>(gop, S, jf, b[0], b[1]);
+  }
   row2_of(b[2], b[0], b[1]);
   rows_mul_dagger(p, a, b);
   row2_of(p[2], p[0], p[1]);
@@ -361,8 +484,13 @@ __device__ __forceinline__ void star_terms_short(M3<R>& K, LK U, const V* __rest
   for (int r = 0; r < 3; r++)
 #pragma unroll
     for (int c = 0; c < 3; c++) K.e[r][c] = cadd(K.e[r][c], p[r][c]);
-  Store<V>::template load<R, false>(gop, S, jb, a[0], a[1]);
-  link_rows<R, NT>(U, (1 - parity) * 4 + mu, S, jb, b[0], b[1]);
+  if constexpr (CLUSTER) {
+    raw_rows<R>(gb, a[0], a[1]);
+    raw_rows<R>(lb, b[0], b[1]);
+  } else {
+    Store<V>::template load<R, false>(gop, S, jb, a[0], a[1]);
+    link_rows<R, NT>(U, (1 - parity) * 4 + mu, S, jb, b[0], b[1]);
+  }
   row2_of(b[2], b[0], b[1]);
   rows_mul(p, a, b);
   row2_of(p[2], p[0], p[1]);
