@@ -222,7 +222,7 @@ int ensure_f32_links(State& s) {
   if (s.U32 && s.U32_valid) return GFX_OK;
   if (!s.U32) {
     if (dev_alloc((void**)&s.U32, 8 * field_bytes32(s.geo), "FP32 links")) return GFX_ERR_ALLOC;
-    if (dev_alloc((void**)&s.g32, 3 * field_bytes32(s.geo), "FP32 trafo")) return GFX_ERR_ALLOC;
+    if (dev_alloc((void**)&s.g32, 4 * field_bytes32(s.geo), "FP32 trafo")) return GFX_ERR_ALLOC;
   }
   launch_f64_to_f32(s.U64, s.U32, s.geo, 8, s.stream);
   s.U32_valid = true;
@@ -337,12 +337,14 @@ int sweep_form(const State& s, bool fp32, int ndir) {
   return fp32 ? FORM_F32 : FORM_F64;
 }
 
-// one half sweep with the tuned variant for (form, ndir): in place on the
-// even field in slot `even` (0 or 2) and the odd field (slot 1); chk as in
-// launch_sweep_f64, writing the new even field to slot 2 - even and the
-// block partials to cpart (odd sweep first half, even sweep second half)
+// one half sweep with the tuned variant for (form, ndir) on the even field
+// in slot `even` (0 or 2) and the odd field in slot `odd` (1 or 3): in
+// place, or with chk = 2 (as in launch_sweep_f64) the new even field into
+// slot 2 - even, or with `spare` the new odd field into slot 4 - odd; the
+// block partials of a check go to cpart (odd sweep first half, even sweep
+// second half)
 void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* active, int even,
-              int chk) {
+              int chk, int odd = 1, bool spare = false) {
   const int form = sweep_form(s, fp32, ndir);
   const int v = s.variant[form][ndir == 4 ? 1 : 0];
   double* part = check_partials(s, fp32) + (chk == 2 ? 3 * (size_t)s.ncheck : 0);
@@ -352,18 +354,25 @@ void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* ac
     map = s.smap[fp32 ? 1 : 0];
     nb = s.smap_n[fp32 ? 1 : 0];
   }
-  if (form == FORM_F32)
-    launch_sweep_f32(s.U32, field32(s, even), field32(s, 1), s.geo, parity, ndir, s.omega, active,
-                     v, map, s.stream, chk, field32(s, 2 - even), part, nb);
-  else if (form == FORM_P32)
-    launch_sweep_packed_f32(s.pk, field32(s, even), field32(s, 1), s.geo, parity, s.omega, v, map,
-                            s.stream, chk, field32(s, 2 - even), part, nb);
-  else if (form == FORM_P64)
-    launch_sweep_packed_f64(s.pk, field64(s, even), field64(s, 1), s.geo, parity, s.omega, v, map,
-                            s.stream, chk, field64(s, 2 - even), part, nb);
-  else
-    launch_sweep_f64(s.U64, field64(s, even), field64(s, 1), s.geo, parity, ndir, s.omega, active,
-                     v, map, s.stream, chk, field64(s, 2 - even), part, nb);
+  // output slot of the updated parity (-1: in place)
+  const int out = (chk == 2) ? 2 - even : spare ? 4 - odd : -1;
+  if (form == FORM_F32 || form == FORM_P32) {
+    float4* alt = out < 0 ? nullptr : field32(s, out);
+    if (form == FORM_F32)
+      launch_sweep_f32(s.U32, field32(s, even), field32(s, odd), s.geo, parity, ndir, s.omega,
+                       active, v, map, s.stream, chk, alt, part, nb);
+    else
+      launch_sweep_packed_f32(s.pk, field32(s, even), field32(s, odd), s.geo, parity, s.omega, v,
+                              map, s.stream, chk, alt, part, nb);
+  } else {
+    double2* alt = out < 0 ? nullptr : field64(s, out);
+    if (form == FORM_P64)
+      launch_sweep_packed_f64(s.pk, field64(s, even), field64(s, odd), s.geo, parity, s.omega, v,
+                              map, s.stream, chk, alt, part, nb);
+    else
+      launch_sweep_f64(s.U64, field64(s, even), field64(s, odd), s.geo, parity, ndir, s.omega,
+                       active, v, map, s.stream, chk, alt, part, nb);
+  }
 }
 void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* active) {
   do_sweep(s, fp32, parity, ndir, active, 0, 0);
@@ -397,14 +406,15 @@ static void build_slice_map(State& s, bool fp32, const std::vector<unsigned char
                    hipMemcpyHostToDevice, s.stream);
 }
 
-// bring the even field back to slot 0
-static void canonical_even(State& s, bool fp32, int even) {
-  if (even == 0) return;
+// bring the even field back to slot 0 (the odd field to slot 1)
+static void canonical_slot(State& s, bool fp32, int slot, int home) {
+  if (slot == home) return;
   if (fp32)
-    hipMemcpyAsync(field32(s, 0), field32(s, 2), field_bytes32(s.geo), hipMemcpyDeviceToDevice, s.stream);
+    hipMemcpyAsync(field32(s, home), field32(s, slot), field_bytes32(s.geo), hipMemcpyDeviceToDevice, s.stream);
   else
-    hipMemcpyAsync(field64(s, 0), field64(s, 2), field_bytes64(s.geo), hipMemcpyDeviceToDevice, s.stream);
+    hipMemcpyAsync(field64(s, home), field64(s, slot), field_bytes64(s.geo), hipMemcpyDeviceToDevice, s.stream);
 }
+static void canonical_even(State& s, bool fp32, int even) { canonical_slot(s, fp32, even, 0); }
 
 // Times every sweep variant on a scratch copy of g (the state is not
 // touched) and keeps the fastest for this lattice, form and ndir.  The
@@ -476,7 +486,7 @@ int init_gaugefixing(gfx_su3* gf, gfx_su3* trafo) {
   if ((rc = ensure_common(0))) return rc;
   if (dev_alloc((void**)&s.U64, 8 * field_bytes64(s.geo), "gauge field")) return GFX_ERR_ALLOC;
   printf("Allocated gauge field on device\n");
-  if (dev_alloc((void**)&s.g64, 3 * field_bytes64(s.geo), "trafo field")) return GFX_ERR_ALLOC;
+  if (dev_alloc((void**)&s.g64, 4 * field_bytes64(s.geo), "trafo field")) return GFX_ERR_ALLOC;
   printf("Allocated trafo field 1 on device\n");
   if ((rc = alloc_reduction(s))) return rc;
   if ((rc = upload_aos(s, (const double*)gf, s.U64, 4))) return rc;
@@ -497,7 +507,7 @@ int init_thermalization(gfx_su3* gf) {
   if ((rc = ensure_common(0))) return rc;
   if (dev_alloc((void**)&s.U64, 8 * field_bytes64(s.geo), "gauge field")) return GFX_ERR_ALLOC;
   printf("Allocated gauge field on device\n");
-  if (dev_alloc((void**)&s.g64, 3 * field_bytes64(s.geo), "trafo field")) return GFX_ERR_ALLOC;
+  if (dev_alloc((void**)&s.g64, 4 * field_bytes64(s.geo), "trafo field")) return GFX_ERR_ALLOC;
   if ((rc = alloc_reduction(s))) return rc;
   if ((rc = upload_aos(s, (const double*)gf, s.U64, 4))) return rc;
   s.U32_valid = false;
@@ -616,34 +626,48 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
   double maxdada = 0.0, best = 1e300;
   int stall = 0;
   int even = 0;            // slot of the current even field (0 or 2)
+  int odd = 1;             // slot of the current odd field (1 or 3)
   bool even_done = false;  // the even half of iteration i ran with the last check
+  bool odd_done = false;   // and its odd half too (into the spare odd slot)
   bool converged = false;
   int i;
   hipStreamSynchronize(s.stream);
   const auto t0 = std::chrono::steady_clock::now();
+  // state i (the checked state) back into slots 0 and 1
+  auto canonical = [&](bool f32) {
+    canonical_even(s, f32, even);
+    canonical_slot(s, f32, odd, 1);
+    even = 0;
+    odd = 1;
+  };
   auto to_fp64 = [&]() {
     // leave FP32 at the checked state: reunitarise g in FP64
-    canonical_even(s, true, even);
+    canonical(true);
     launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
-    even = 0;
     fp32 = false;
   };
   for (i = 0; i < maxit; i++) {
     const bool chk = (i % checkinterval) == 0;
-    if (!even_done) do_sweep(s, fp32, 0, 4, nullptr, even, 0);
-    do_sweep(s, fp32, 1, 4, nullptr, even, chk ? 1 : 0);
-    even_done = false;
+    if (!even_done) do_sweep(s, fp32, 0, 4, nullptr, even, 0, odd);
+    if (!odd_done) do_sweep(s, fp32, 1, 4, nullptr, even, chk ? 1 : 0, odd);
+    even_done = odd_done = false;
     if (fp32) s.run_iters32++; else s.run_iters64++;
     if (!chk) continue;
     // the check of state i rides on the even sweep of iteration i+1
-    do_sweep(s, fp32, 0, 4, nullptr, even, 2);
+    do_sweep(s, fp32, 0, 4, nullptr, even, 2, odd);
     const int v = s.variant[sweep_form(s, fp32, 4)][1];
     const double* pc = check_partials(s, fp32);
     launch_check_finalize(pc, pc + 3 * (size_t)s.ncheck, sweep_blocks(s.geo.nh, v), 4.0 * V, V,
                           eps, s.results, s.stream);
     if (hip_check(hipMemcpyAsync(s.h_results, s.results, sizeof(double) * 4, hipMemcpyDeviceToHost, s.stream), "D2H results") ||
-        hip_check(hipStreamSynchronize(s.stream), "check"))
+        hip_check(hipEventRecord(s.ev1, s.stream), "check event"))
       return -1;
+    // while the result travels to the host, the GPU runs the odd sweep of
+    // iteration i+1 into the spare odd slot: kept if the run goes on as it
+    // is, dropped (state i is untouched) if it stops or changes form here
+    const bool ahead = i + 1 < maxit;
+    if (ahead) do_sweep(s, fp32, 1, 4, nullptr, 2 - even, ((i + 1) % checkinterval) == 0 ? 1 : 0, odd, true);
+    if (hip_check(hipEventSynchronize(s.ev1), "check")) return -1;
     FUNC = s.h_results[0];
     DADA = s.h_results[1];
     maxDADA = maxdada = s.h_results[2];
@@ -652,9 +676,8 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
     if (maxdada < eps) {
       if (s.packed_on) {
         // a stop found on the packed links is decided on the FP64 links:
-        // state i (the even field of the check) with the per-link functional
-        canonical_even(s, false, even);
-        even = 0;
+        // state i with the per-link functional
+        canonical(false);
         if (functional_now(s, false, 4, 1, eps)) return -1;
         FUNC = s.h_results[0];
         DADA = s.h_results[1];
@@ -684,14 +707,15 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
         continue;
       }
     }
-    if (i + 1 < maxit) {
+    if (ahead) {
       even = 2 - even;
-      even_done = true;
+      odd = 4 - odd;
+      even_done = odd_done = true;
     }
   }
   if (fp32) to_fp64();
   s.packed_on = false;
-  canonical_even(s, false, even);
+  canonical(false);
   stop_clock_ms(s, t0);
   if (converged) {
     if (s.verbose) {

@@ -47,10 +47,11 @@ void deal_to_xcds(const std::vector<T>& in, std::vector<T>& out) {
   }
 }
 // one half sweep over the fields ge (even) and go (odd) of g.  chk (Landau
-// only): 0 plain in-place sweep; 1 check after the update (odd sweep of a
-// check iteration); 2 check before the update, new field written to galt
-// (even sweep that follows it).  Checks write 3 doubles per block to
-// partials (sweep_blocks(nh, variant) blocks).
+// only): 0 plain sweep; 1 check after the update (odd sweep of a check
+// iteration); 2 check before the update (even sweep that follows it).  The
+// new field of the updated parity goes to galt when given (always with chk
+// = 2), otherwise the sweep is in place.  Checks write 3 doubles per block
+// to partials (sweep_blocks(nh, variant) blocks).
 // bmap: launch position -> local block (block_order dealt to the XCDs);
 // nblocks > 0 launches only the first nblocks entries (a map of a subset)
 void launch_sweep_f64(const double2* U, double2* ge, double2* go, const Geom& geo, int parity,

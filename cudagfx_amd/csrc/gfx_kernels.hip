@@ -128,13 +128,18 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
     // each term is a product of two SU(3) matrices; form its rows 0,1 and
     // rebuild row 2 (U g^+ directly, U^+(x-mu) g^+(x-mu) as (g U)^+)
     if (ROLLED) {
-      // the FP32 rolled loop holds a direction's four operands in flight
+      // the FP32 rolled loop holds two directions' eight operands in flight
       // together (FP32 registers are cheap); the batching form's FP64 one
-      // does not
-      constexpr bool CL = std::is_same<R, float>::value;
+      // loads operand by operand
+      if constexpr (std::is_same<R, float>::value) {
 #pragma unroll 1
-      for (int mu = 0; mu < NDIR; mu++)
-        star_terms_short<V, R, NT, LK, CL>(K, U, gop, geo, s, i, parity, mu);
+        for (int mu = 0; mu + 1 < NDIR; mu += 2)
+          star_terms_short<V, R, NT, LK, 2>(K, U, gop, geo, s, i, parity, mu);
+        if (NDIR & 1) star_terms_short<V, R, NT, LK, 1>(K, U, gop, geo, s, i, parity, NDIR - 1);
+      } else {
+#pragma unroll 1
+        for (int mu = 0; mu < NDIR; mu++) star_terms_short<V, R, NT>(K, U, gop, geo, s, i, parity, mu);
+      }
     } else {
 #pragma unroll
       for (int mu = 0; mu < NDIR; mu++) star_terms_short<V, R, NT>(K, U, gop, geo, s, i, parity, mu);
@@ -202,8 +207,8 @@ __device__ __forceinline__ void store_check_partials(double F, double d, double*
   }
 }
 
-// ge / go: even and odd fields of g.  CHK = 2 writes the new own field to
-// galt; otherwise the sweep is in place.
+// ge / go: even and odd fields of g.  The new own field goes to galt when
+// given (always with CHK = 2), otherwise the sweep is in place.
 // bmap[launch position] = block of the lattice (workgroup order)
 template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK, typename LK>
 __device__ __forceinline__ void or_sweep_block(LK U, V* ge, V* go, V* galt, const Geom& geo,
@@ -215,7 +220,7 @@ __device__ __forceinline__ void or_sweep_block(LK U, V* ge, V* go, V* galt, cons
   const int i = b * BLOCK + threadIdx.x;
   V* gme = parity ? go : ge;
   const V* gop = parity ? ge : go;
-  V* gout = (CHK == 2) ? galt : gme;
+  V* gout = (galt != nullptr) ? galt : gme;
   double F = 0.0, d = 0.0;
   if (CHK == 0) {
     if (i >= geo.nh) return;
@@ -333,6 +338,9 @@ __global__ __launch_bounds__(BLK) void k_check_finalize(const double* __restrict
                                                         double normF, double normD, double eps,
                                                         double* __restrict__ out) {
   double F = 0.0, d = 0.0, mx = 0.0;
+  // unrolled so that the loads of several blocks are in flight together (the
+  // sums keep their order)
+#pragma unroll 8
   for (int b = threadIdx.x; b < nb; b += BLK) {
     F += podd[3 * (size_t)b];
     d += podd[3 * (size_t)b + 1] + peven[3 * (size_t)b + 1];

@@ -454,29 +454,11 @@ struct BatchProblem {
 // two SU(3) matrices, so rows 0, 1 are formed and row 2 is rebuilt as
 // conj(row0 x row1) (the second as (g(x-mu) U_mu(x-mu))^+).  Equal to the
 // full products for unitary operands; about 25% fewer complex multiplies.
-// CLUSTER: the four operands are loaded (Raw) before either term is formed.
-template <typename V, typename R, bool NT, typename LK, bool CLUSTER = false>
-__device__ __forceinline__ void star_terms_short(M3<R>& K, LK U, const V* __restrict__ gop,
-                                                 const Geom& geo, const Site& s, int i,
-                                                 int parity, int mu) {
-  const int S = geo.stride;
-  const int jf = nb_fwd(geo, s, i, mu);
-  const int jb = nb_bwd(geo, s, i, mu);
-  Cx<R> a[2][3], b[3][3], p[3][3];
-  Raw<LK, R> lf, lb;
-  Raw<const V*, R> gf, gb;
-  if constexpr (CLUSTER) {
-    raw_load<R, NT>(U, parity * 4 + mu, S, i, lf);
-    raw_load<R, false>(gop, S, jf, gf);
-    raw_load<R, false>(gop, S, jb, gb);
-    raw_load<R, NT>(U, (1 - parity) * 4 + mu, S, jb, lb);
-    __builtin_amdgcn_sched_barrier(0);
-    raw_rows<R>(lf, a[0], a[1]);
-    raw_rows<R>(gf, b[0], b[1]);
-  } else {
-    link_rows<R, NT>(U, parity * 4 + mu, S, i, a[0], a[1]);
-    Store<V>::template load<R, false>(gop, S, jf, b[0], b[1]);
-  }
+// NC = 0: one direction, each operand decoded as it is loaded; NC = 1, 2:
+// NC directions from mu, every operand loaded (Raw) before any term is formed.
+template <typename R>
+__device__ __forceinline__ void short_term_fwd(M3<R>& K, const Cx<R> a[2][3], Cx<R> b[3][3]) {
+  Cx<R> p[3][3];
   row2_of(b[2], b[0], b[1]);
   rows_mul_dagger(p, a, b);
   row2_of(p[2], p[0], p[1]);
@@ -484,13 +466,10 @@ __device__ __forceinline__ void star_terms_short(M3<R>& K, LK U, const V* __rest
   for (int r = 0; r < 3; r++)
 #pragma unroll
     for (int c = 0; c < 3; c++) K.e[r][c] = cadd(K.e[r][c], p[r][c]);
-  if constexpr (CLUSTER) {
-    raw_rows<R>(gb, a[0], a[1]);
-    raw_rows<R>(lb, b[0], b[1]);
-  } else {
-    Store<V>::template load<R, false>(gop, S, jb, a[0], a[1]);
-    link_rows<R, NT>(U, (1 - parity) * 4 + mu, S, jb, b[0], b[1]);
-  }
+}
+template <typename R>
+__device__ __forceinline__ void short_term_bwd(M3<R>& K, const Cx<R> a[2][3], Cx<R> b[3][3]) {
+  Cx<R> p[3][3];
   row2_of(b[2], b[0], b[1]);
   rows_mul(p, a, b);
   row2_of(p[2], p[0], p[1]);
@@ -498,6 +477,45 @@ __device__ __forceinline__ void star_terms_short(M3<R>& K, LK U, const V* __rest
   for (int r = 0; r < 3; r++)
 #pragma unroll
     for (int c = 0; c < 3; c++) K.e[r][c] = cadd(K.e[r][c], cconj(p[c][r]));
+}
+template <typename V, typename R, bool NT, typename LK, int NC = 0>
+__device__ __forceinline__ void star_terms_short(M3<R>& K, LK U, const V* __restrict__ gop,
+                                                 const Geom& geo, const Site& s, int i,
+                                                 int parity, int mu) {
+  const int S = geo.stride;
+  Cx<R> a[2][3], b[3][3];
+  if constexpr (NC == 0) {
+    const int jf = nb_fwd(geo, s, i, mu);
+    const int jb = nb_bwd(geo, s, i, mu);
+    link_rows<R, NT>(U, parity * 4 + mu, S, i, a[0], a[1]);
+    Store<V>::template load<R, false>(gop, S, jf, b[0], b[1]);
+    short_term_fwd(K, a, b);
+    Store<V>::template load<R, false>(gop, S, jb, a[0], a[1]);
+    link_rows<R, NT>(U, (1 - parity) * 4 + mu, S, jb, b[0], b[1]);
+    short_term_bwd(K, a, b);
+  } else {
+    Raw<LK, R> lf[NC], lb[NC];
+    Raw<const V*, R> gf[NC], gb[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      const int jf = nb_fwd(geo, s, i, mu + k);
+      const int jb = nb_bwd(geo, s, i, mu + k);
+      raw_load<R, NT>(U, parity * 4 + mu + k, S, i, lf[k]);
+      raw_load<R, false>(gop, S, jf, gf[k]);
+      raw_load<R, false>(gop, S, jb, gb[k]);
+      raw_load<R, NT>(U, (1 - parity) * 4 + mu + k, S, jb, lb[k]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      raw_rows<R>(lf[k], a[0], a[1]);
+      raw_rows<R>(gf[k], b[0], b[1]);
+      short_term_fwd(K, a, b);
+      raw_rows<R>(gb[k], a[0], a[1]);
+      raw_rows<R>(lb[k], b[0], b[1]);
+      short_term_bwd(K, a, b);
+    }
+  }
 }
 
 }  // namespace gfx
