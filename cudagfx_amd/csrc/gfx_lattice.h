@@ -277,56 +277,6 @@ __device__ __forceinline__ int pack_lo_of(double u, int hi) {
   return (int)fmin(fmax(rint((u - (double)hi / PACK_HI) * PACK_LO), -32767.0), 32767.0);
 }
 
-// rows 0, 1 of link field `field` at site index i, from plain planes or
-// from packed planes (R = float: hi only; R = double: hi + lo)
-template <typename R, bool NT, typename V>
-__device__ __forceinline__ void link_rows(const V* U, int field, int stride, int i, Cx<R> r0[3],
-                                          Cx<R> r1[3]) {
-  Store<V>::template load<R, NT>(field_ptr(U, field, stride), stride, i, r0, r1);
-}
-template <typename R, bool NT>
-__device__ __forceinline__ void link_rows(const PackedLinks& U, int field, int stride, int i,
-                                          Cx<R> r0[3], Cx<R> r1[3]) {
-  const size_t f = (size_t)field * stride + (unsigned)i;
-  const uint4 a = ld16<NT>(U.a + f), b = ld16<NT>(U.b + f);
-  const uint32_t c = ld4<NT>(U.c + f);
-  const uint32_t w[9] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c};
-  const int h[12] = {pack_s24<0>(w), pack_s24<1>(w), pack_s24<2>(w),  pack_s24<3>(w),
-                     pack_s24<4>(w), pack_s24<5>(w), pack_s24<6>(w),  pack_s24<7>(w),
-                     pack_s24<8>(w), pack_s24<9>(w), pack_s24<10>(w), pack_s24<11>(w)};
-  R v[12];
-  if constexpr (std::is_same<R, double>::value) {
-    const uint2* l = U.lo + (size_t)field * 3 * stride + (unsigned)i;
-    const uint2 la = ld8<NT>(l), lb = ld8<NT>(l + stride), lc = ld8<NT>(l + 2 * (size_t)stride);
-    const uint32_t q[6] = {la.x, la.y, lb.x, lb.y, lc.x, lc.y};
-#pragma unroll
-    for (int k = 0; k < 12; k++) v[k] = fma((double)pack_lo16(q, k), 1.0 / PACK_LO, (double)h[k] / PACK_HI);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 12; k++) v[k] = (R)h[k] * R(1.0 / PACK_HI);
-  }
-#pragma unroll
-  for (int c3 = 0; c3 < 3; c3++) {
-    r0[c3] = cx<R>(v[2 * c3], v[2 * c3 + 1]);
-    r1[c3] = cx<R>(v[6 + 2 * c3], v[6 + 2 * c3 + 1]);
-  }
-}
-template <typename R, bool NT, typename LK>
-__device__ __forceinline__ M3<R> link_m(const LK& U, int field, int stride, int i) {
-  M3<R> m;
-  link_rows<R, NT>(U, field, stride, i, m.e[0], m.e[1]);
-  m_row2(m);
-  return m;
-}
-template <typename R, bool NT, typename LK>
-__device__ __forceinline__ M3<R> link_m_dagger(const LK& U, int field, int stride, int i) {
-  Cx<R> r0[3], r1[3];
-  link_rows<R, NT>(U, field, stride, i, r0, r1);
-  M3<R> d;
-  m_dagger_from_rows(d, r0, r1);
-  return d;
-}
-
 // A matrix as loaded, before it is turned into rows: the sweeps issue the
 // loads of all four operands of a direction (two links, two neighbour g)
 // before decoding any of them, so that a wavefront has them in flight
@@ -432,6 +382,36 @@ template <typename R, typename LK>
 __device__ __forceinline__ M3<R> raw_m_dagger(const Raw<LK, R>& r) {
   Cx<R> r0[3], r1[3];
   raw_rows<R>(r, r0, r1);
+  M3<R> d;
+  m_dagger_from_rows(d, r0, r1);
+  return d;
+}
+
+// rows 0, 1 of link field `field` at site index i, from plain planes or
+// from packed planes (R = float: hi only; R = double: hi + lo)
+template <typename R, bool NT, typename V>
+__device__ __forceinline__ void link_rows(const V* U, int field, int stride, int i, Cx<R> r0[3],
+                                          Cx<R> r1[3]) {
+  Store<V>::template load<R, NT>(field_ptr(U, field, stride), stride, i, r0, r1);
+}
+template <typename R, bool NT>
+__device__ __forceinline__ void link_rows(const PackedLinks& U, int field, int stride, int i,
+                                          Cx<R> r0[3], Cx<R> r1[3]) {
+  Raw<PackedLinks, R> r;
+  raw_load<R, NT>(U, field, stride, i, r);
+  raw_rows<R>(r, r0, r1);
+}
+template <typename R, bool NT, typename LK>
+__device__ __forceinline__ M3<R> link_m(const LK& U, int field, int stride, int i) {
+  M3<R> m;
+  link_rows<R, NT>(U, field, stride, i, m.e[0], m.e[1]);
+  m_row2(m);
+  return m;
+}
+template <typename R, bool NT, typename LK>
+__device__ __forceinline__ M3<R> link_m_dagger(const LK& U, int field, int stride, int i) {
+  Cx<R> r0[3], r1[3];
+  link_rows<R, NT>(U, field, stride, i, r0, r1);
   M3<R> d;
   m_dagger_from_rows(d, r0, r1);
   return d;
