@@ -301,11 +301,14 @@ int set_lattice_from_globals(State& s) {
 }
 
 // one functional evaluation into s.h_results[0..2] for nseg segments
-int functional_now(State& s, bool fp32, int ndir, int nseg, double eps) {
+int functional_now(State& s, bool fp32, int ndir, int nseg, double eps, int even, int odd) {
+  const size_t S = s.geo.stride;
   if (fp32)
-    launch_functional_f32(s.U32, s.g32, s.geo, ndir, nseg, s.partials, eps, s.results, s.stream);
+    launch_functional_f32(s.U32, s.g32 + (size_t)even * 3 * S, s.g32 + (size_t)odd * 3 * S, s.geo,
+                          ndir, nseg, s.partials, eps, s.results, s.stream);
   else
-    launch_functional_f64(s.U64, s.g64, s.geo, ndir, nseg, s.partials, eps, s.results, s.stream);
+    launch_functional_f64(s.U64, s.g64 + (size_t)even * 6 * S, s.g64 + (size_t)odd * 6 * S, s.geo,
+                          ndir, nseg, s.partials, eps, s.results, s.stream);
   if (hip_check(hipMemcpyAsync(s.h_results, s.results, sizeof(double) * 4 * nseg, hipMemcpyDeviceToHost, s.stream), "D2H results")) return GFX_ERR_DEVICE;
   return hip_check(hipStreamSynchronize(s.stream), "functional");
 }
@@ -331,9 +334,10 @@ double2* field64(State& s, int k) { return s.g64 + (size_t)k * 6 * s.geo.stride;
 float4* field32(State& s, int k) { return s.g32 + (size_t)k * 3 * s.geo.stride; }
 
 // the form of the next sweep: FP32 or FP64, on the packed links while a
-// mixed Landau run reads them
+// mixed run reads them
 int sweep_form(const State& s, bool fp32, int ndir) {
-  if (s.packed_on && ndir == 4) return fp32 ? FORM_P32 : FORM_P64;
+  (void)ndir;
+  if (s.packed_on) return fp32 ? FORM_P32 : FORM_P64;
   return fp32 ? FORM_F32 : FORM_F64;
 }
 
@@ -362,13 +366,13 @@ void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* ac
       launch_sweep_f32(s.U32, field32(s, even), field32(s, odd), s.geo, parity, ndir, s.omega,
                        active, v, map, s.stream, chk, alt, part, nb);
     else
-      launch_sweep_packed_f32(s.pk, field32(s, even), field32(s, odd), s.geo, parity, s.omega, v,
-                              map, s.stream, chk, alt, part, nb);
+      launch_sweep_packed_f32(s.pk, field32(s, even), field32(s, odd), s.geo, parity, ndir,
+                              s.omega, active, v, map, s.stream, chk, alt, part, nb);
   } else {
     double2* alt = out < 0 ? nullptr : field64(s, out);
     if (form == FORM_P64)
-      launch_sweep_packed_f64(s.pk, field64(s, even), field64(s, odd), s.geo, parity, s.omega, v,
-                              map, s.stream, chk, alt, part, nb);
+      launch_sweep_packed_f64(s.pk, field64(s, even), field64(s, odd), s.geo, parity, ndir,
+                              s.omega, active, v, map, s.stream, chk, alt, part, nb);
     else
       launch_sweep_f64(s.U64, field64(s, even), field64(s, odd), s.geo, parity, ndir, s.omega,
                        active, v, map, s.stream, chk, alt, part, nb);
@@ -387,7 +391,7 @@ static void stop_clock_ms(State& s, std::chrono::steady_clock::time_point t0) {
 // that hold a site of a slice still being fixed (act), dealt to the XCDs
 static void build_slice_map(State& s, bool fp32, const std::vector<unsigned char>& act, int nsh) {
   const int pr = fp32 ? 1 : 0;
-  const int v = s.variant[pr][0];
+  const int v = s.variant[sweep_form(s, fp32, 3)][0];
   const int blk = sweep_variant_block(v);
   std::vector<int> order, list;
   block_order(s.geo, blk, sweep_variant_tiling(v), order);
@@ -418,7 +422,7 @@ static void canonical_even(State& s, bool fp32, int even) { canonical_slot(s, fp
 
 // Times every sweep variant on a scratch copy of g (the state is not
 // touched) and keeps the fastest for this lattice, form and ndir.  The
-// packed FP64 form (Landau only) has rolled variants only.
+// packed FP64 form has rolled variants only.
 int tune_sweep(State& s, int form, int ndir) {
   const int pi = form, di = ndir == 4 ? 1 : 0;
   if (!s.autotune || s.tuned[pi][di]) return GFX_OK;
@@ -436,9 +440,9 @@ int tune_sweep(State& s, int form, int ndir) {
     if (form == FORM_F32)
       launch_sweep_f32(s.U32, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
     else if (form == FORM_P32)
-      launch_sweep_packed_f32(s.pk, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, s.omega, v, map, s.stream);
+      launch_sweep_packed_f32(s.pk, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
     else if (form == FORM_P64)
-      launch_sweep_packed_f64(s.pk, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, s.omega, v, map, s.stream);
+      launch_sweep_packed_f64(s.pk, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
     else
       launch_sweep_f64(s.U64, sc64, sc64 + 6 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
   };
@@ -756,12 +760,35 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   std::vector<unsigned char> act[2] = {std::vector<unsigned char>(nt, mixed ? 0 : 1),
                                        std::vector<unsigned char>(nt, mixed ? 1 : 0)};
   unsigned char* dact[2] = {s.active, s.active + nt};
+  const bool packed = mixed && s.packed_links;
+  // every exit leaves the packed forms off
+  struct PackedOff {
+    State& s;
+    ~PackedOff() { s.packed_on = false; }
+  } packed_off{s};
+  s.packed_on = false;
   if (mixed) {
     if (ensure_f32_links(s)) return GFX_ERR_ALLOC;
     launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
-    tune_sweep(s, FORM_F32, 3);
   }
-  tune_sweep(s, FORM_F64, 3);
+  if (packed) {
+    if (ensure_packed_links(s)) return GFX_ERR_ALLOC;
+    tune_sweep(s, FORM_P32, 3);
+    tune_sweep(s, FORM_P64, 3);
+  } else {
+    if (mixed) tune_sweep(s, FORM_F32, 3);
+    tune_sweep(s, FORM_F64, 3);
+  }
+  s.packed_on = packed;
+  // a slice stops when its max dAdA is below eps.  On packed links that is
+  // decided with the packing's worst effect on Delta added, so that the
+  // bound holds on the exact links too (a slice in the narrow band below
+  // eps stops one check later)
+  auto below = [&](double m, bool f32) {
+    if (!s.packed_on) return m < eps;
+    const double r = std::sqrt(m) + (f32 ? PACK_DDELTA32 : PACK_DDELTA64);
+    return r * r < eps;
+  };
   auto upload = [&]() {
     for (int pr = 0; pr < 2; pr++)
       hipMemcpyAsync(dact[pr], act[pr].data(), nt, hipMemcpyHostToDevice, s.stream);
@@ -780,8 +807,9 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   maps();
   // fused checks need whole blocks per slice, in both precisions when mixed
   // (a slice moving to FP64 takes the FP32 check sweep's update with it)
-  const bool fusable = nsh % sweep_variant_block(s.variant[0][0]) == 0 &&
-                       (!mixed || nsh % sweep_variant_block(s.variant[1][0]) == 0);
+  const bool fusable =
+      nsh % sweep_variant_block(s.variant[sweep_form(s, false, 3)][0]) == 0 &&
+      (!mixed || nsh % sweep_variant_block(s.variant[sweep_form(s, true, 3)][0]) == 0);
   auto fused_ok = [&](int) { return fusable; };
   const int stall_checks = (s.stall_iters + checkinterval - 1) / checkinterval;
   std::vector<double> best(nt, 1e300), res(4 * nt, 0.0);
@@ -822,7 +850,7 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
       if (fused[pr]) {
         do_sweep(s, pr == 1, 0, 3, dact[pr], even[pr], 2);
         const double* pc = check_partials(s, pr == 1);
-        const int bps = nsh / sweep_variant_block(s.variant[pr][0]);
+        const int bps = nsh / sweep_variant_block(s.variant[sweep_form(s, pr == 1, 3)][0]);
         launch_slice_finalize(pc, pc + 3 * (size_t)s.ncheck, bps, nt, dact[pr], 3.0 * 2 * nsh,
                               2.0 * nsh, eps, s.results, s.stream);
         if (hip_check(hipMemcpyAsync(s.h_results, s.results, sizeof(double) * 4 * nt, hipMemcpyDeviceToHost, s.stream), "D2H results") ||
@@ -840,7 +868,7 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
     for (int t = 0; t < nt; t++) {
       const double m = res[4 * t + 2];
       if (act[0][t]) {
-        if (m < eps) stop64.push_back(t);
+        if (below(m, false)) stop64.push_back(t);
       } else if (act[1][t]) {
         bool leave = m < s.switch_theta;
         if (!leave) {
@@ -851,7 +879,7 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
             leave = true;
           }
         }
-        if (m < eps) stop32.push_back(t);
+        if (below(m, true)) stop32.push_back(t);
         else if (leave) sw32.push_back(t);
       }
     }

@@ -62,17 +62,17 @@ void launch_sweep_f32(const float4* U, float4* ge, float4* go, const Geom& geo, 
                       int ndir, double omega, const unsigned char* active, int variant,
                       const int* bmap, hipStream_t st, int chk = 0, float4* galt = nullptr,
                       double* partials = nullptr, int nblocks = 0);
-// Landau half sweeps on packed links: FP32 g on the high parts (every
-// variant), FP64 g on high + low parts (rolled variants only, bit 1 of the
-// variant ignored)
+// half sweeps on packed links (Landau and Coulomb, arguments as above): FP32
+// g on the high parts (every variant), FP64 g on high + low parts (rolled
+// variants only, bit 1 of the variant ignored)
 void launch_sweep_packed_f32(const PackedArgs& p, float4* ge, float4* go, const Geom& geo,
-                             int parity, double omega, int variant, const int* bmap,
-                             hipStream_t st, int chk = 0, float4* galt = nullptr,
-                             double* partials = nullptr, int nblocks = 0);
+                             int parity, int ndir, double omega, const unsigned char* active,
+                             int variant, const int* bmap, hipStream_t st, int chk = 0,
+                             float4* galt = nullptr, double* partials = nullptr, int nblocks = 0);
 void launch_sweep_packed_f64(const PackedArgs& p, double2* ge, double2* go, const Geom& geo,
-                             int parity, double omega, int variant, const int* bmap,
-                             hipStream_t st, int chk = 0, double2* galt = nullptr,
-                             double* partials = nullptr, int nblocks = 0);
+                             int parity, int ndir, double omega, const unsigned char* active,
+                             int variant, const int* bmap, hipStream_t st, int chk = 0,
+                             double2* galt = nullptr, double* partials = nullptr, int nblocks = 0);
 int sweep_blocks(int nh, int variant);
 // per-slice grid pass of a fused Coulomb check (bps blocks per slice; only
 // active slices are written)
@@ -123,10 +123,12 @@ int kp_gauss_per_site();
 int kp_unif_per_site();
 void launch_fill_i32(int* p, int n, int v, hipStream_t st);
 void launch_fill_f64(double* p, int n, double v, hipStream_t st);
-void launch_functional_f64(const double2* U, const double2* g, const Geom& geo, int ndir, int nseg,
-                           double* partials, double eps, double* out, hipStream_t st);
-void launch_functional_f32(const float4* U, const float4* g, const Geom& geo, int ndir, int nseg,
-                           double* partials, double eps, double* out, hipStream_t st);
+void launch_functional_f64(const double2* U, const double2* ge, const double2* go, const Geom& geo,
+                           int ndir, int nseg, double* partials, double eps, double* out,
+                           hipStream_t st);
+void launch_functional_f32(const float4* U, const float4* ge, const float4* go, const Geom& geo,
+                           int ndir, int nseg, double* partials, double eps, double* out,
+                           hipStream_t st);
 void launch_plaquette_f64(const double2* U, const Geom& geo, double* partials, double* out,
                           hipStream_t st);
 void launch_pack_f64(const double* aos, double2* dst, const Geom& geo, int s0, int ns, int nmat,

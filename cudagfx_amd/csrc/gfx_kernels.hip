@@ -249,22 +249,24 @@ __global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* 
                                                      bmap, partials);
 }
 
-// The Landau half sweeps of the mixed path on packed links (PackedLinks):
+// The half sweeps of the mixed path on packed links (PackedLinks), Landau
+// (NDIR = 4) and Coulomb (NDIR = 3, active slices only):
 // R = float, g in FP32 planes, links read as 24-bit fixed point (432 instead
 // of 528 bytes per site update); R = double, g in FP64 planes and the
 // reference's FP64 operation order, links as 24 + 16-bit fixed point within
 // 2^-38 of the FP64 links (768 instead of 1056 bytes per site update).
-template <typename V, typename R, int BLOCK, bool ROLLED, bool NT, int CHK>
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK>
 __global__ __launch_bounds__(BLOCK) void k_or_sweep_packed(const uint4* __restrict__ Ua,
                                                             const uint4* __restrict__ Ub,
                                                             const uint32_t* __restrict__ Uc,
                                                             const uint2* __restrict__ Ulo, V* ge,
                                                             V* go, V* galt, Geom geo, int parity,
                                                             R c1, R c2,
+                                                            const unsigned char* __restrict__ active,
                                                             const int* __restrict__ bmap,
                                                             double* __restrict__ partials) {
-  or_sweep_block<V, R, 4, BLOCK, ROLLED, NT, CHK>(PackedLinks{Ua, Ub, Uc, Ulo}, ge, go, galt, geo,
-                                                  parity, c1, c2, nullptr, bmap, partials);
+  or_sweep_block<V, R, NDIR, BLOCK, ROLLED, NT, CHK>(PackedLinks{Ua, Ub, Uc, Ulo}, ge, go, galt,
+                                                     geo, parity, c1, c2, active, bmap, partials);
 }
 
 // ------------------------------------------------------------ batched sweep
@@ -363,13 +365,13 @@ __global__ __launch_bounds__(BLK) void k_check_finalize(const double* __restrict
 // Re tr(Delta Delta^+) of site i of parity p (overrelax.cu:498-586), in FP64
 // whatever the storage precision.  The direction loop is kept rolled.
 template <typename V, int NDIR>
-__device__ __forceinline__ void functional_site(const V* __restrict__ U, const V* __restrict__ g,
-                                                const Geom& geo, int p, int i, double& F,
-                                                double& d) {
+__device__ __forceinline__ void functional_site(const V* __restrict__ U, const V* __restrict__ ge,
+                                                const V* __restrict__ go, const Geom& geo, int p,
+                                                int i, double& F, double& d) {
   const int S = geo.stride;
   const Site s = site_coords(geo, i, p);
-  const V* gme = field_ptr(g, p, S);
-  const V* gop = field_ptr(g, 1 - p, S);
+  const V* gme = p ? go : ge;
+  const V* gop = p ? ge : go;
   int jf[NDIR], jb[NDIR];
 #pragma unroll
   for (int mu = 0; mu < NDIR; mu++) {
@@ -408,7 +410,8 @@ __device__ __forceinline__ void functional_site(const V* __restrict__ U, const V
 
 template <typename V, int NDIR>
 __global__ __launch_bounds__(BLK) void k_functional(const V* __restrict__ U,
-                                                     const V* __restrict__ g, Geom geo, int nsh,
+                                                     const V* __restrict__ ge,
+                                                     const V* __restrict__ go, Geom geo, int nsh,
                                                      double* __restrict__ partials) {
   const int seg = blockIdx.y;
   const int k = blockIdx.x * BLK + threadIdx.x;
@@ -416,7 +419,7 @@ __global__ __launch_bounds__(BLK) void k_functional(const V* __restrict__ U,
   if (k < 2 * nsh) {
     const int p = (k >= nsh) ? 1 : 0;
     const int i = seg * nsh + (k - p * nsh);
-    functional_site<V, NDIR>(U, g, geo, p, i, F, d);
+    functional_site<V, NDIR>(U, ge, go, geo, p, i, F, d);
     mx = d;
   }
   block_reduce3(F, d, mx);
@@ -866,32 +869,32 @@ void launch_sweep_f32(const float4* U, float4* ge, float4* go, const Geom& geo, 
                                 ndir, omega, variant, chk, st);
 }
 
-template <typename V, typename R, int BLOCK, bool ROLLED, bool NT, int CHK>
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK>
 static void launch_packed_v(const PackedArgs& p, const SweepArgs<V>& a, R c1, R c2,
                             hipStream_t st) {
   dim3 grid(a.nblocks > 0 ? a.nblocks : (a.geo.nh + BLOCK - 1) / BLOCK);
-  hipLaunchKernelGGL((k_or_sweep_packed<V, R, BLOCK, ROLLED, NT, CHK>), grid, dim3(BLOCK), 0, st,
-                     p.a, p.b, p.c, p.lo, a.ge, a.go, a.galt, a.geo, a.parity, c1, c2, a.bmap,
-                     a.partials);
+  hipLaunchKernelGGL((k_or_sweep_packed<V, R, NDIR, BLOCK, ROLLED, NT, CHK>), grid, dim3(BLOCK), 0,
+                     st, p.a, p.b, p.c, p.lo, a.ge, a.go, a.galt, a.geo, a.parity, c1, c2,
+                     a.active, a.bmap, a.partials);
 }
 
 // FP64 on packed links: rolled variants only (bit 1 of the variant ignored)
-template <typename V, typename R, int CHK>
+template <typename V, typename R, int NDIR, int CHK>
 static void launch_packed_chk(const PackedArgs& p, const SweepArgs<V>& a, R c1, R c2, int variant,
                               hipStream_t st) {
   const int v = std::is_same<R, double>::value ? (variant & ~2) : variant;
   switch (v & 7) {
-    case 0: launch_packed_v<V, R, 256, true, false, CHK>(p, a, c1, c2, st); break;
-    case 1: launch_packed_v<V, R, 64, true, false, CHK>(p, a, c1, c2, st); break;
-    case 4: launch_packed_v<V, R, 256, true, true, CHK>(p, a, c1, c2, st); break;
-    case 5: launch_packed_v<V, R, 64, true, true, CHK>(p, a, c1, c2, st); break;
+    case 0: launch_packed_v<V, R, NDIR, 256, true, false, CHK>(p, a, c1, c2, st); break;
+    case 1: launch_packed_v<V, R, NDIR, 64, true, false, CHK>(p, a, c1, c2, st); break;
+    case 4: launch_packed_v<V, R, NDIR, 256, true, true, CHK>(p, a, c1, c2, st); break;
+    case 5: launch_packed_v<V, R, NDIR, 64, true, true, CHK>(p, a, c1, c2, st); break;
     default:
       if constexpr (std::is_same<R, float>::value) {
         switch (v & 7) {
-          case 2: launch_packed_v<V, R, 256, false, false, CHK>(p, a, c1, c2, st); break;
-          case 3: launch_packed_v<V, R, 64, false, false, CHK>(p, a, c1, c2, st); break;
-          case 6: launch_packed_v<V, R, 256, false, true, CHK>(p, a, c1, c2, st); break;
-          default: launch_packed_v<V, R, 64, false, true, CHK>(p, a, c1, c2, st); break;
+          case 2: launch_packed_v<V, R, NDIR, 256, false, false, CHK>(p, a, c1, c2, st); break;
+          case 3: launch_packed_v<V, R, NDIR, 64, false, false, CHK>(p, a, c1, c2, st); break;
+          case 6: launch_packed_v<V, R, NDIR, 256, false, true, CHK>(p, a, c1, c2, st); break;
+          default: launch_packed_v<V, R, NDIR, 64, false, true, CHK>(p, a, c1, c2, st); break;
         }
       }
       break;
@@ -899,31 +902,42 @@ static void launch_packed_chk(const PackedArgs& p, const SweepArgs<V>& a, R c1, 
 }
 
 template <typename V, typename R>
-static void launch_packed_t(const PackedArgs& p, const SweepArgs<V>& a, double omega, int variant,
-                            int chk, hipStream_t st) {
+static void launch_packed_t(const PackedArgs& p, const SweepArgs<V>& a, int ndir, double omega,
+                            int variant, int chk, hipStream_t st) {
   const R c1 = (R)((1.0 + omega - 1) / 1);
   const R c2 = (R)((1.0 + omega - 2) / 2);
-  if (chk == 1)
-    launch_packed_chk<V, R, 1>(p, a, c1, c2, variant, st);
-  else if (chk == 2)
-    launch_packed_chk<V, R, 2>(p, a, c1, c2, variant, st);
-  else
-    launch_packed_chk<V, R, 0>(p, a, c1, c2, variant, st);
+  if (ndir == 4) {
+    SweepArgs<V> b = a;
+    b.active = nullptr;
+    if (chk == 1)
+      launch_packed_chk<V, R, 4, 1>(p, b, c1, c2, variant, st);
+    else if (chk == 2)
+      launch_packed_chk<V, R, 4, 2>(p, b, c1, c2, variant, st);
+    else
+      launch_packed_chk<V, R, 4, 0>(p, b, c1, c2, variant, st);
+  } else {
+    if (chk == 1)
+      launch_packed_chk<V, R, 3, 1>(p, a, c1, c2, variant, st);
+    else if (chk == 2)
+      launch_packed_chk<V, R, 3, 2>(p, a, c1, c2, variant, st);
+    else
+      launch_packed_chk<V, R, 3, 0>(p, a, c1, c2, variant, st);
+  }
 }
 
 void launch_sweep_packed_f32(const PackedArgs& p, float4* ge, float4* go, const Geom& geo,
-                             int parity, double omega, int variant, const int* bmap,
-                             hipStream_t st, int chk, float4* galt, double* partials,
-                             int nblocks) {
-  launch_packed_t<float4, float>(p, {nullptr, ge, go, galt, geo, parity, nullptr, bmap, nblocks, partials},
-                                 omega, variant, chk, st);
+                             int parity, int ndir, double omega, const unsigned char* active,
+                             int variant, const int* bmap, hipStream_t st, int chk, float4* galt,
+                             double* partials, int nblocks) {
+  launch_packed_t<float4, float>(p, {nullptr, ge, go, galt, geo, parity, active, bmap, nblocks, partials},
+                                 ndir, omega, variant, chk, st);
 }
 void launch_sweep_packed_f64(const PackedArgs& p, double2* ge, double2* go, const Geom& geo,
-                             int parity, double omega, int variant, const int* bmap,
-                             hipStream_t st, int chk, double2* galt, double* partials,
-                             int nblocks) {
-  launch_packed_t<double2, double>(p, {nullptr, ge, go, galt, geo, parity, nullptr, bmap, nblocks, partials},
-                                   omega, variant, chk, st);
+                             int parity, int ndir, double omega, const unsigned char* active,
+                             int variant, const int* bmap, hipStream_t st, int chk, double2* galt,
+                             double* partials, int nblocks) {
+  launch_packed_t<double2, double>(p, {nullptr, ge, go, galt, geo, parity, active, bmap, nblocks, partials},
+                                   ndir, omega, variant, chk, st);
 }
 
 int sweep_blocks(int nh, int variant) {
@@ -1046,27 +1060,30 @@ void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t st
 int functional_blocks(int nsh) { return (2 * nsh + BLK - 1) / BLK; }
 
 template <typename V>
-static void launch_functional_t(const V* U, const V* g, const Geom& geo, int ndir, int nseg,
-                                double* partials, double eps, double* out, hipStream_t st) {
+static void launch_functional_t(const V* U, const V* ge, const V* go, const Geom& geo, int ndir,
+                                int nseg, double* partials, double eps, double* out,
+                                hipStream_t st) {
   const int nsh = geo.nh / nseg;
   const int nb = functional_blocks(nsh);
   dim3 grid(nb, nseg);
   if (ndir == 4)
-    hipLaunchKernelGGL((k_functional<V, 4>), grid, dim3(BLK), 0, st, U, g, geo, nsh, partials);
+    hipLaunchKernelGGL((k_functional<V, 4>), grid, dim3(BLK), 0, st, U, ge, go, geo, nsh, partials);
   else
-    hipLaunchKernelGGL((k_functional<V, 3>), grid, dim3(BLK), 0, st, U, g, geo, nsh, partials);
+    hipLaunchKernelGGL((k_functional<V, 3>), grid, dim3(BLK), 0, st, U, ge, go, geo, nsh, partials);
   const double vseg = 2.0 * nsh;
   hipLaunchKernelGGL(k_finalize, dim3(nseg), dim3(BLK), 0, st, partials, nb, ndir * vseg, vseg,
                      eps, out);
 }
 
-void launch_functional_f64(const double2* U, const double2* g, const Geom& geo, int ndir, int nseg,
-                           double* partials, double eps, double* out, hipStream_t st) {
-  launch_functional_t<double2>(U, g, geo, ndir, nseg, partials, eps, out, st);
+void launch_functional_f64(const double2* U, const double2* ge, const double2* go, const Geom& geo,
+                           int ndir, int nseg, double* partials, double eps, double* out,
+                           hipStream_t st) {
+  launch_functional_t<double2>(U, ge, go, geo, ndir, nseg, partials, eps, out, st);
 }
-void launch_functional_f32(const float4* U, const float4* g, const Geom& geo, int ndir, int nseg,
-                           double* partials, double eps, double* out, hipStream_t st) {
-  launch_functional_t<float4>(U, g, geo, ndir, nseg, partials, eps, out, st);
+void launch_functional_f32(const float4* U, const float4* ge, const float4* go, const Geom& geo,
+                           int ndir, int nseg, double* partials, double eps, double* out,
+                           hipStream_t st) {
+  launch_functional_t<float4>(U, ge, go, geo, ndir, nseg, partials, eps, out, st);
 }
 
 void launch_plaquette_f64(const double2* U, const Geom& geo, double* partials, double* out,

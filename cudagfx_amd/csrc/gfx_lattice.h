@@ -217,6 +217,16 @@ __device__ __forceinline__ V* field_ptr(V* base, int field, int stride) {
 // field, four int16 per 8-byte word in the order of the reals.
 constexpr double PACK_HI = 0x1p23;
 constexpr double PACK_LO = 0x1p38;
+// Worst effect of the packing on a site's Delta = -A(g K) (A: traceless
+// antihermitian part, which does not increase the Frobenius norm).  A stored
+// real is off by at most e (2^-38 for high + low parts, 2^-23 for the high
+// parts alone, both where |u| rounds to 1); row 2 = conj(row0 x row1) then
+// by at most 4 sqrt(2) e per complex element, so |dU|_F <= sqrt(12 + 96) e
+// <= 10.4 e, and with 8 star terms of unitary g, |dDelta|_F <= 83.2 e.
+// theta(x) = |Delta(x)|_F^2, so theta on the exact links is below
+// (sqrt(theta on the packed links) + PACK_DDELTA)^2.
+constexpr double PACK_DDELTA64 = 83.2 * 0x1p-38;
+constexpr double PACK_DDELTA32 = 83.2 * 0x1p-23;
 struct PackedLinks {
   const uint4* a;       // field f, site i at [f * stride + i]
   const uint4* b;

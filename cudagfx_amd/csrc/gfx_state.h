@@ -96,7 +96,9 @@ struct State {
 extern State g_st;
 
 int hip_check(hipError_t e, const char* what);
-int functional_now(State& s, bool fp32, int ndir, int nseg, double eps);
+// one functional evaluation into s.h_results for nseg segments, the even
+// field in slot `even` and the odd field in slot `odd`
+int functional_now(State& s, bool fp32, int ndir, int nseg, double eps, int even = 0, int odd = 1);
 Geom make_geom(int l0, int l1, int l2, int l3);
 bool geometry_ok(int l0, int l1, int l2, int l3);
 int ensure_common(int need_links32);

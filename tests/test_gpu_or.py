@@ -442,6 +442,35 @@ def test_packed_run_matches_plain_mixed_run(gfx, orc, dims, seed):
     assert mx < EPS12 * (1 + 1e-9)
 
 
+@pytest.mark.parametrize("dims,ck", [((8, 8, 8, 6), 5), ((8, 8, 8, 4), 1)])
+def test_coulomb_packed_matches_plain_links(gfx, orc, dims, ck):
+    """Mixed Coulomb on packed links (FP32 slices on the high parts, FP64
+    slices on high + low parts) and on the FP32 and FP64 links: every slice
+    stops below the bound on the exact links (host functional of the packed
+    run's result), per-slice F equal to 1e-12 relative, stops within two
+    checks of each other."""
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, 63, eps=0.4)
+    g = orc.random_init_trafo(V, 64)
+    _setup(gfx, dims, u, g, 1.7, prec=gfx.PREC_MIXED)
+    runs = []
+    for packed in (0, 1):
+        gfx.set_packed_links(packed)
+        gfx.set_trafo(g)
+        rc, it = gfx.coulomb_gauge(8000, EPS12, ck)
+        runs.append((rc, it, gfx.functional_slices(), gfx.get_trafo(), gfx.last_run_stats()))
+    gfx.set_packed_links(1)
+    gfx.set_precision(gfx.PREC_FP64)
+    gfx.finalize_gaugefixing()
+    (rc0, it0, (F0, _, M0), _, _), (rc1, it1, (F1, _, M1), g1, st1) = runs
+    assert rc0 == 0 and rc1 == 0 and st1[1] > 0 and st1[2] > 0
+    assert np.all(M0 < EPS12) and np.all(M1 < EPS12)
+    assert np.max(np.abs(F1 - F0) / F0) < 1e-12
+    assert np.all(np.abs(it1 - it0) <= 2 * ck), (it0, it1)
+    Fo, Do, Mo = orc.functional_slices(orc.su3_to_2v(u), orc.su3_to_2v(g1), dims, 3)
+    assert np.all(Mo < EPS12 * (1 + 1e-9)) and np.max(np.abs(Fo - F1)) < 1e-12
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_coulomb_slices_without_whole_blocks(gfx, orc, prec):
     """6^3 x 4: a slice holds 108 sites of a parity, not a whole number of
