@@ -116,7 +116,7 @@ int apply_trafo(gfx_su3 *gf);
 
 #define GFX_PREC_FP64 0   /* plain FP64 path                                    */
 #define GFX_PREC_MIXED 1  /* FP32 12-real bulk sweeps, FP64 reductions + polish */
-/* not precision modes: the two sweep forms of a mixed Landau run on packed
+/* not precision modes: the two sweep forms of a mixed run on packed
  * links (gfx_set_packed_links), as the `precision` argument of
  * gfx_time_sweeps / gfx_get_sweep_variant / gfx_set_sweep_variant:
  * PACKED64 the FP64 polish (FP64 g on high + low parts), PACKED32 the FP32
@@ -136,13 +136,15 @@ int gfx_set_precision(int mode, double switch_theta);
  * and has not halved for `iters` iterations (the FP32 rounding floor, near
  * 5e-11 on 32^4); iters = 0 turns this off.  Default 300, 1e-8. */
 int gfx_set_fp32_stall(int iters, double gate);
-/* links read by the sweeps of a mixed Landau run: 1 (default) packed links,
- * each real u of a link as a 24-bit integer hi = rint(u 2^23) plus an int16
+/* links read by the sweeps of a mixed Landau or Coulomb run: 1 (default)
+ * packed links, each real u of a link as a 24-bit integer hi = rint(u 2^23) plus an int16
  * lo = rint((u - hi 2^-23) 2^38).  The FP32 phase reads hi 2^-23 (within
  * 2^-24 of u, 36 instead of 48 bytes per link); the FP64 polish reads
  * hi 2^-23 + lo 2^-38 (within 2^-38 of u, 60 instead of 96 bytes per link),
- * with the stop confirmed by a functional on the FP64 links (plain FP64
- * sweeps go on if it does not hold).  0: the FP32 and FP64 links. */
+ * with a Landau stop confirmed by a functional on the FP64 links (plain FP64
+ * sweeps go on if it does not hold) and a Coulomb slice stopped only when the
+ * packing's worst effect on Delta keeps its max dAdA below eps.  0: the FP32
+ * and FP64 links. */
 int gfx_set_packed_links(int on);
 /* 0 silences the per-check log lines */
 int gfx_set_verbose(int verbose);
