@@ -30,6 +30,7 @@ struct BProbHost {
   Geom geo{};
   double2* U64 = nullptr;
   float4* U32 = nullptr;
+  PackedArgs pk{};  // packed links (owner of the links only; mixed runs)
   double2* g64 = nullptr;
   float4* g32 = nullptr;
   int ushare = 0;  // problem whose links this one uses (itself if it owns them)
@@ -74,6 +75,8 @@ struct Batch {
   bool built = false;
   int variant[2] = {0, 0};
   bool tuned[2] = {false, false};
+  bool packed = false;         // the current run sweeps on packed links
+  bool tuned_packed = false;   // the link form the variants were timed on
   float tune_ms[2][SWEEP_VARIANTS] = {};
   // statistics of the last run
   double run_ms = 0.0;
@@ -140,6 +143,7 @@ static void batch_free(Batch& b) {
     if (&p - &b.probs[0] == p.ushare) {
       hipFree(p.U64);
       hipFree(p.U32);
+      hipFree(p.pk.a);
     }
   }
   b.probs.clear();
@@ -235,6 +239,8 @@ static void set_pointers(Batch& b) {
     h64[q].geo = h32[q].geo = p.geo;
     h64[q].U = o.U64;
     h32[q].U = o.U32;
+    const PackedLinks pk = b.packed ? PackedLinks{o.pk.a, o.pk.b, o.pk.c, o.pk.lo} : PackedLinks{};
+    h64[q].pk = h32[q].pk = pk;
     for (int k = 0; k < 2; k++) {
       h64[q].g[k] = bfield(p.g64, p.geo, k);
       h32[q].g[k] = p.g32 ? (void*)bfield(p.g32, p.geo, k) : nullptr;
@@ -276,6 +282,27 @@ static int ensure_fp32(Batch& b) {
   return GFX_OK;
 }
 
+// the packed links of every link set (as ensure_packed_links, 60 bytes per
+// link in one allocation), made from the FP64 links
+static int ensure_packed(Batch& b) {
+  State& s = g_st;
+  for (size_t q = 0; q < b.probs.size(); q++) {
+    BProbHost& p = b.probs[q];
+    if ((int)q != p.ushare) continue;
+    if (!p.pk.a) {
+      const size_t n = (size_t)8 * p.geo.stride;
+      char* base = nullptr;
+      if (dev_alloc((void**)&base, n * (16 + 16 + 4 + 24), "batch packed links")) return GFX_ERR_ALLOC;
+      p.pk.a = reinterpret_cast<uint4*>(base);
+      p.pk.b = reinterpret_cast<uint4*>(base + n * 16);
+      p.pk.c = reinterpret_cast<uint32_t*>(base + n * 32);
+      p.pk.lo = reinterpret_cast<uint2*>(base + n * 36);
+    }
+    launch_pack_links(p.U64, p.pk, p.geo, 8, s.stream);
+  }
+  return GFX_OK;
+}
+
 // the workgroup map of precision pr for variant v: the blocks of the
 // lattices sweeping in pr at the last census, in the variant's order, dealt
 // to the XCDs (stopped or other-precision lattices cost no workgroups)
@@ -305,7 +332,8 @@ static void batch_half(Batch& b, int pr, int parity, int chk) {
   const Batch::CMap& m = batch_map(b, pr, v);
   if (m.n == 0) return;
   launch_batch_sweep(pr == 1, b.d_probs[pr], m.d, m.n, b.d_act[pr], b.d_poff[vblock_index(v)],
-                     parity, s.omega, v, chk, chk == 2 ? b.d_pev[pr] : b.d_pod[pr], s.stream);
+                     parity, s.omega, v, chk, chk == 2 ? b.d_pev[pr] : b.d_pod[pr], s.stream,
+                     b.packed);
 }
 
 struct RunParams {
@@ -336,6 +364,7 @@ static void batch_check_tail(Batch& b, const bool use[2], const RunParams& rp) {
   c.iter = b.d_iter;
   c.best = b.d_best;
   c.stall = b.d_stall;
+  c.packed = b.packed;
   const int n = (int)b.probs.size();
   launch_batch_finalize(c, n, 4, rp.eps, rp.switch_theta, rp.maxit, rp.stall_checks, s.stall_gate,
                         s.stream);
@@ -618,6 +647,13 @@ int gfx_batch_run(int maxit, double eps, int checkinterval, int precision, doubl
   }
   const bool mixed = (precision == GFX_PREC_MIXED);
   if (mixed && ensure_fp32(b)) return GFX_ERR_ALLOC;
+  b.packed = mixed && s.packed_links;
+  if (b.packed && ensure_packed(b)) return GFX_ERR_ALLOC;
+  if (b.packed != b.tuned_packed) {
+    // the sweep variants are timed again on the other link form
+    b.tuned[0] = b.tuned[1] = false;
+    b.tuned_packed = b.packed;
+  }
   b.graph_launches = b.iters32 = b.iters64 = 0;
   for (auto& p : b.probs) {
     p.stop_iter = p.switch_iter = -1;

@@ -89,7 +89,7 @@ void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t st
 // batched lattices (gfx_batch_api.hip)
 void launch_batch_sweep(bool fp32, const BatchProblem* probs, const int2* bmap, int nblocks,
                         const int* active, const int* poff, int parity, double omega, int variant,
-                        int chk, double* partials, hipStream_t st);
+                        int chk, double* partials, hipStream_t st, bool packed);
 // device arrays of a batch check, index 0 = FP64 phase, 1 = FP32 phase
 struct BatchCheck {
   const double* pod[2];  // block partials of the odd check sweeps
@@ -104,6 +104,7 @@ struct BatchCheck {
   const int* iter;
   double* best;
   int* stall;
+  bool packed;  // the lattices sweep on packed links (stop rule with the packing bound)
 };
 void launch_batch_finalize(const BatchCheck& c, int nprob, int ndir, double eps,
                            double switch_theta, int maxit, int stall_checks, double stall_gate,

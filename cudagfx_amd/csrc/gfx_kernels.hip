@@ -275,8 +275,9 @@ __global__ __launch_bounds__(BLOCK) void k_or_sweep_packed(const uint4* __restri
 // block inside the problem); a problem whose active flag is 0 has converged
 // and its blocks return at once.  Check partials of block (q, k) go to
 // partials[3 * (poff[q] + k)].  Both precisions use the short form of the
-// products (see sweep_site).
-template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK>
+// products (see sweep_site).  PK: the links are read from the packed planes
+// P.pk (FP32: high parts, FP64: high + low parts).
+template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK, bool PK>
 __global__ __launch_bounds__(BLOCK) void k_batch_sweep(const BatchProblem* __restrict__ probs,
                                                         const int2* __restrict__ bmap,
                                                         const int* __restrict__ active,
@@ -292,15 +293,19 @@ __global__ __launch_bounds__(BLOCK) void k_batch_sweep(const BatchProblem* __res
   V* gout = (CHK == 2) ? static_cast<V*>(P.galt) : gme;
   const int i = m.y * BLOCK + threadIdx.x;
   double F = 0.0, d = 0.0;
+  auto site = [&](const Site& s) {
+    if constexpr (PK)
+      sweep_site<V, R, NDIR, ROLLED, NT, CHK, PackedLinks, true>(P.pk, gop, gme, gout, geo, s, i,
+                                                                 parity, c1, c2, F, d);
+    else
+      sweep_site<V, R, NDIR, ROLLED, NT, CHK, const V*, true>(U, gop, gme, gout, geo, s, i, parity,
+                                                              c1, c2, F, d);
+  };
   if (CHK == 0) {
     if (i >= geo.nh) return;
-    const Site s = site_coords(geo, i, parity);
-    sweep_site<V, R, NDIR, ROLLED, NT, 0, const V*, true>(U, gop, gme, gout, geo, s, i, parity, c1, c2, F, d);
+    site(site_coords(geo, i, parity));
   } else {
-    if (i < geo.nh) {
-      const Site s = site_coords(geo, i, parity);
-      sweep_site<V, R, NDIR, ROLLED, NT, CHK, const V*, true>(U, gop, gme, gout, geo, s, i, parity, c1, c2, F, d);
-    }
+    if (i < geo.nh) site(site_coords(geo, i, parity));
     store_check_partials<CHK>(F, d, partials, poff[m.x] + m.y);
   }
 }
@@ -454,7 +459,7 @@ __global__ __launch_bounds__(BLK) void k_batch_finalize(
     double switch_theta, int* __restrict__ act64, int* __restrict__ act32,
     int* __restrict__ conv, int* __restrict__ sw, double* __restrict__ results,
     const int* __restrict__ iter, int maxit, double* __restrict__ best, int* __restrict__ stall,
-    int stall_checks, double stall_gate) {
+    int stall_checks, double stall_gate, int packed) {
   const int q = blockIdx.x;
   const bool f64 = act64[q] != 0;
   if (!f64 && !act32[q]) return;
@@ -478,8 +483,12 @@ __global__ __launch_bounds__(BLK) void k_batch_finalize(
   results[4 * q + 0] = F / (ndir * V);
   results[4 * q + 1] = d / V;
   results[4 * q + 2] = mx;
+  // on packed links a lattice stops when the packing's worst effect on
+  // Delta keeps max dAdA below eps on the exact links as well
+  const double r = packed ? sqrt(mx) + (f64 ? PACK_DDELTA64 : PACK_DDELTA32) : 0.0;
+  const bool below = packed ? r * r < eps : mx < eps;
   if (f64) {
-    if (mx < eps) {
+    if (below) {
       results[4 * q + 3] = (double)it;
       act64[q] = 0;
     } else if (!last) {
@@ -489,7 +498,7 @@ __global__ __launch_bounds__(BLK) void k_batch_finalize(
     }
     return;
   }
-  const bool stop = mx < eps;
+  const bool stop = below;
   bool leave = stop || mx < switch_theta;
   if (!leave && stall_checks > 0) {
     if (mx < 0.5 * best[q]) {
@@ -968,12 +977,19 @@ struct BSweepArgs {
   const int* poff;
   int parity;
   double* partials;
+  bool packed;
 };
 
 template <typename V, typename R, int BLOCK, bool ROLLED, bool NT, int CHK>
 static void launch_bsweep_v(const BSweepArgs& a, R c1, R c2, hipStream_t st) {
-  hipLaunchKernelGGL((k_batch_sweep<V, R, 4, BLOCK, ROLLED, NT, CHK>), dim3(a.nblocks), dim3(BLOCK),
-                     0, st, a.probs, a.bmap, a.active, a.poff, a.parity, c1, c2, a.partials);
+  if (a.packed)
+    hipLaunchKernelGGL((k_batch_sweep<V, R, 4, BLOCK, ROLLED, NT, CHK, true>), dim3(a.nblocks),
+                       dim3(BLOCK), 0, st, a.probs, a.bmap, a.active, a.poff, a.parity, c1, c2,
+                       a.partials);
+  else
+    hipLaunchKernelGGL((k_batch_sweep<V, R, 4, BLOCK, ROLLED, NT, CHK, false>), dim3(a.nblocks),
+                       dim3(BLOCK), 0, st, a.probs, a.bmap, a.active, a.poff, a.parity, c1, c2,
+                       a.partials);
 }
 
 template <typename V, typename R, int CHK>
@@ -1005,8 +1021,8 @@ static void launch_bsweep_t(const BSweepArgs& a, double omega, int variant, int 
 
 void launch_batch_sweep(bool fp32, const BatchProblem* probs, const int2* bmap, int nblocks,
                         const int* active, const int* poff, int parity, double omega, int variant,
-                        int chk, double* partials, hipStream_t st) {
-  const BSweepArgs a{probs, bmap, nblocks, active, poff, parity, partials};
+                        int chk, double* partials, hipStream_t st, bool packed) {
+  const BSweepArgs a{probs, bmap, nblocks, active, poff, parity, partials, packed};
   if (fp32)
     launch_bsweep_t<float4, float>(a, omega, variant, chk, st);
   else
@@ -1020,7 +1036,7 @@ void launch_batch_finalize(const BatchCheck& c, int nprob, int ndir, double eps,
                      c.pev[1], c.poff[0], c.poff[1], c.inmap[0], c.inmap[1],
                      c.probs[0], c.probs[1], ndir, eps,
                      switch_theta, c.act[0], c.act[1], c.conv, c.sw, c.results, c.iter, maxit,
-                     c.best, c.stall, stall_checks, stall_gate);
+                     c.best, c.stall, stall_checks, stall_gate, c.packed ? 1 : 0);
 }
 
 void launch_batch_convert(const BatchProblem* probs32, const BatchProblem* probs64,

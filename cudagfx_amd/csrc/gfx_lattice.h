@@ -435,6 +435,7 @@ __device__ __forceinline__ M3<R> link_m_dagger(const LK& U, int field, int strid
 struct BatchProblem {
   Geom geo;
   const void* U;
+  PackedLinks pk;  // the packed form of U (mixed runs on packed links)
   void* g[2];
   void* galt;
 };
