@@ -106,15 +106,16 @@ __device__ __forceinline__ void site_check(const M3<R>& G, const M3<R>& K, doubl
   d = (double)acc;
 }
 
-// LK: the links, as planes of V (const V*) or, for the mixed Landau run, as
-// PackedLinks.  The single-lattice FP64 sweep (on either link form) keeps
-// the reference's operation order; FP32 and, with FSHORT, the batching
-// form's FP64 sweep use the short form (star_terms_short,
-// overrelax_apply_short).  In FP64 on FP64 links the two agree to rounding;
-// where many Gribov copies share their links the sweep is bound by its
-// arithmetic, and the short form is 10% faster there.  (On packed links,
-// unitary to 1e-11 only, it would no longer match the reference order to
-// rounding: the FP64 polish keeps the reference order.)
+// LK: the links, as planes of V (const V*) or, for the mixed runs, as
+// PackedLinks.  The single-lattice FP64 sweep keeps the reference's
+// operation order (on packed links: its star and relaxation, with the series
+// applied in the short form); FP32 and, with FSHORT, the batching form's FP64
+// sweep use the short form (star_terms_short, overrelax_apply_short).  In
+// FP64 on FP64 links the two agree to rounding; where many Gribov copies
+// share their links the sweep is bound by its arithmetic, and the short form
+// is 10% faster there.  (The short star rebuilds row 2 of each product from
+// rows 0 and 1, which packed links, unitary to 1e-11 only, would move by
+// more than rounding: the FP64 polish keeps the reference order there.)
 template <typename V, typename R, int NDIR, bool ROLLED, bool NT, int CHK, typename LK, bool FSHORT = false>
 __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, const V* gin, V* gout,
                                            const Geom& geo, const Site& s, int i, int parity, R c1,
@@ -185,7 +186,10 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
   M3<R> h = G;
   cm_relax(h, K);
   m_normalize(h);
-  if constexpr (SHORT) {
+  // the polish on packed links applies the series in the short form too: its
+  // star and relaxation keep the reference's order, which is what pins the
+  // packing to the oracle; the series differs from it by rounding only
+  if constexpr (SHORT || std::is_same<LK, PackedLinks>::value) {
     overrelax_apply_short(G, h, c1, c2);
   } else {
     overrelax_apply(G, h, c1, c2);
