@@ -760,7 +760,7 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   std::vector<unsigned char> act[2] = {std::vector<unsigned char>(nt, mixed ? 0 : 1),
                                        std::vector<unsigned char>(nt, mixed ? 1 : 0)};
   unsigned char* dact[2] = {s.active, s.active + nt};
-  const bool packed = mixed && s.packed_links;
+  const bool packed = mixed && s.packed_links && eps > PACK_EPS_MIN;
   // every exit leaves the packed forms off
   struct PackedOff {
     State& s;

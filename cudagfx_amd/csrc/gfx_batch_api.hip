@@ -647,7 +647,7 @@ int gfx_batch_run(int maxit, double eps, int checkinterval, int precision, doubl
   }
   const bool mixed = (precision == GFX_PREC_MIXED);
   if (mixed && ensure_fp32(b)) return GFX_ERR_ALLOC;
-  b.packed = mixed && s.packed_links;
+  b.packed = mixed && s.packed_links && eps > PACK_EPS_MIN;
   if (b.packed && ensure_packed(b)) return GFX_ERR_ALLOC;
   if (b.packed != b.tuned_packed) {
     // the sweep variants are timed again on the other link form

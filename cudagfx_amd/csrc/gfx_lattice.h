@@ -227,6 +227,9 @@ constexpr double PACK_LO = 0x1p38;
 // (sqrt(theta on the packed links) + PACK_DDELTA)^2.
 constexpr double PACK_DDELTA64 = 83.2 * 0x1p-38;
 constexpr double PACK_DDELTA32 = 83.2 * 0x1p-23;
+// runs whose stops are decided with that bound use packed links only for
+// eps above this (below it the bound would take most of eps)
+constexpr double PACK_EPS_MIN = 100.0 * PACK_DDELTA64 * PACK_DDELTA64;
 struct PackedLinks {
   const uint4* a;       // field f, site i at [f * stride + i]
   const uint4* b;

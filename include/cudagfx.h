@@ -142,8 +142,9 @@ int gfx_set_fp32_stall(int iters, double gate);
  * 2^-24 of u, 36 instead of 48 bytes per link); the FP64 polish reads
  * hi 2^-23 + lo 2^-38 (within 2^-38 of u, 60 instead of 96 bytes per link),
  * with a Landau stop confirmed by a functional on the FP64 links (plain FP64
- * sweeps go on if it does not hold) and a Coulomb slice stopped only when the
- * packing's worst effect on Delta keeps its max dAdA below eps.  0: the FP32
+ * sweeps go on if it does not hold) and a Coulomb slice or batch lattice
+ * stopped only when the packing's worst effect on Delta keeps its max dAdA
+ * below eps (these two use packed links for eps > 9e-18 only).  0: the FP32
  * and FP64 links. */
 int gfx_set_packed_links(int on);
 /* 0 silences the per-check log lines */
