@@ -442,6 +442,61 @@ def test_packed_run_matches_plain_mixed_run(gfx, orc, dims, seed):
     assert mx < EPS12 * (1 + 1e-9)
 
 
+def test_packed_polish_falls_back_below_its_floor(gfx, orc, capfd):
+    """With eps far below what the packed links resolve (their effect on
+    theta is ~1e-22), the polish reaches eps on the packed links, the check
+    on the FP64 links refuses the stop (its log line), and the run goes on
+    in plain FP64 until theta < eps holds on the FP64 links."""
+    import ctypes
+    dims = (8, 8, 8, 8)
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, 95, eps=0.35)
+    g = orc.random_init_trafo(V, 96)
+    eps = 1e-26
+    _setup(gfx, dims, u, g, 1.7, prec=gfx.PREC_MIXED)
+    capfd.readouterr()
+    gfx.set_verbose(1)
+    it = gfx.overrelax_gauge(20000, eps, 5)
+    gfx.set_verbose(0)
+    ctypes.CDLL(None).fflush(None)
+    log = capfd.readouterr().out
+    F, D, M = gfx.results()
+    _, i32, i64 = gfx.last_run_stats()
+    gq = gfx.get_trafo()
+    gfx.set_packed_links(0)
+    gfx.set_trafo(g)
+    it0 = gfx.overrelax_gauge(20000, eps, 5)
+    F0 = gfx.results()[0]
+    gfx.set_packed_links(1)
+    gfx.set_precision(gfx.PREC_FP64)
+    gfx.finalize_gaugefixing()
+    assert it > 0 and it0 > 0 and i32 > 0 and i64 > 0 and M < eps
+    assert abs(F - F0) / F0 < 1e-13
+    assert orc.dAdA(orc.g_trafo(u, gq, dims), dims)[1] < 10 * eps
+    confirm = [float(l.split("max(dAdA) =")[1].split()[0]) for l in log.splitlines()
+               if "(FP64 links)" in l]
+    assert len(confirm) >= 1 and confirm[0] >= eps, confirm
+
+
+def test_coulomb_tiny_eps_on_plain_links(gfx, orc):
+    """A Coulomb run whose eps is below the packing bound's reach runs on
+    the FP32 and FP64 link planes and stops every slice below eps."""
+    dims = (8, 8, 8, 4)
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, 97, eps=0.4)
+    g = orc.random_init_trafo(V, 98)
+    eps = 1e-20
+    _setup(gfx, dims, u, g, 1.7, prec=gfx.PREC_MIXED)
+    rc, it = gfx.coulomb_gauge(20000, eps, 5)
+    F, D, M = gfx.functional_slices()
+    g1 = gfx.get_trafo()
+    gfx.set_precision(gfx.PREC_FP64)
+    gfx.finalize_gaugefixing()
+    assert rc == 0 and np.all(it > 0) and np.all(M < eps)
+    Fo, Do, Mo = orc.functional_slices(orc.su3_to_2v(u), orc.su3_to_2v(g1), dims, 3)
+    assert np.all(Mo < 10 * eps)
+
+
 @pytest.mark.parametrize("dims,ck", [((8, 8, 8, 6), 5), ((8, 8, 8, 4), 1)])
 def test_coulomb_packed_matches_plain_links(gfx, orc, dims, ck):
     """Mixed Coulomb on packed links (FP32 slices on the high parts, FP64
