@@ -413,7 +413,8 @@ def test_packed_fp32_sweeps_close_to_oracle(gfx, orc):
     gfx.finalize_gaugefixing()
 
 
-@pytest.mark.parametrize("dims,seed", [((8, 8, 8, 16), 90), ((8, 8, 8, 8), 911)])
+@pytest.mark.parametrize("dims,seed", [((8, 8, 8, 16), 90), ((8, 8, 8, 8), 911),
+                                       ((6, 6, 6, 6), 912)])
 def test_packed_run_matches_plain_mixed_run(gfx, orc, dims, seed):
     """The mixed run on packed links (FP32 phase on the high parts, FP64
     polish on high + low parts) and on the FP32 and FP64 links: both stop
