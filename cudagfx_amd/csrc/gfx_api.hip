@@ -678,9 +678,10 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
     if (s.verbose)
       printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e\n", i, FUNC, DADA, maxdada);
     if (maxdada < eps) {
-      if (s.packed_on) {
-        // a stop found on the packed links is decided on the FP64 links:
-        // state i with the per-link functional
+      if (s.packed_on && !fp32) {
+        // a stop found by the polish on the packed links is decided on the
+        // FP64 links: state i with the per-link functional (a stop in the
+        // FP32 phase is taken as it is and converted after the loop)
         canonical(false);
         if (functional_now(s, false, 4, 1, eps)) return -1;
         FUNC = s.h_results[0];

@@ -479,6 +479,45 @@ def test_packed_polish_falls_back_below_its_floor(gfx, orc, capfd):
     assert len(confirm) >= 1 and confirm[0] >= eps, confirm
 
 
+@pytest.mark.parametrize("packed", [1, 0])
+def test_mixed_stop_inside_the_fp32_phase(gfx, orc, packed, capfd):
+    """eps above the FP32 floor and the switch below it: the run stops in
+    its FP32 phase at its first check below eps (no polish, no check on the
+    FP64 links), and the state it returns (converted to FP64) satisfies the
+    bound on the host, with F close to the FP64 run's."""
+    import ctypes
+    dims = (8, 8, 8, 8)
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, 93, eps=0.35)
+    g = orc.random_init_trafo(V, 94)
+    eps = 1e-8
+    _setup(gfx, dims, u, g, 1.7)
+    it64 = gfx.overrelax_gauge(20000, eps, 5)
+    F64 = gfx.results()[0]
+    gfx.set_trafo(g)
+    gfx.set_precision(gfx.PREC_MIXED, 1e-20)
+    gfx.set_packed_links(packed)
+    capfd.readouterr()
+    gfx.set_verbose(1)
+    it = gfx.overrelax_gauge(20000, eps, 5)
+    gfx.set_verbose(0)
+    ctypes.CDLL(None).fflush(None)
+    log = capfd.readouterr().out
+    F, D, M = gfx.results()
+    _, i32, i64 = gfx.last_run_stats()
+    gq = gfx.get_trafo()
+    gfx.set_packed_links(1)
+    gfx.set_precision(gfx.PREC_FP64)
+    gfx.finalize_gaugefixing()
+    assert it64 > 0 and it > 0 and i32 == it + 1 and i64 == 0 and M < eps
+    assert "(FP64 links)" not in log
+    checks = [float(l.split("max(dAdA) =")[1].split()[0]) for l in log.splitlines()
+              if l.startswith("iter")]
+    assert checks[-1] < eps and all(c >= eps for c in checks[:-1])
+    assert abs(F - F64) / F64 < 1e-6
+    assert orc.dAdA(orc.g_trafo(u, gq, dims), dims)[1] < 2 * eps
+
+
 def test_coulomb_tiny_eps_on_plain_links(gfx, orc):
     """A Coulomb run whose eps is below the packing bound's reach runs on
     the FP32 and FP64 link planes and stops every slice below eps."""
