@@ -781,12 +781,13 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
     tune_sweep(s, FORM_F64, 3);
   }
   s.packed_on = packed;
-  // a slice stops when its max dAdA is below eps.  On packed links that is
-  // decided with the packing's worst effect on Delta added, so that the
-  // bound holds on the exact links too (a slice in the narrow band below
-  // eps stops one check later)
-  auto below = [&](double m, bool f32) {
-    if (!s.packed_on) return m < eps;
+  // a slice stops when its max dAdA is below eps.  A check fused into the
+  // sweeps on packed links decides that with the packing's worst effect on
+  // Delta added, so that the bound holds on the exact links too (a slice in
+  // the narrow band below eps stops one check later); a standalone check
+  // reads the FP32 or FP64 link planes
+  auto below = [&](double m, bool f32, bool fused_check) {
+    if (!s.packed_on || !fused_check) return m < eps;
     const double r = std::sqrt(m) + (f32 ? PACK_DDELTA32 : PACK_DDELTA64);
     return r * r < eps;
   };
@@ -869,7 +870,7 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
     for (int t = 0; t < nt; t++) {
       const double m = res[4 * t + 2];
       if (act[0][t]) {
-        if (below(m, false)) stop64.push_back(t);
+        if (below(m, false, fused[0])) stop64.push_back(t);
       } else if (act[1][t]) {
         bool leave = m < s.switch_theta;
         if (!leave) {
@@ -880,7 +881,7 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
             leave = true;
           }
         }
-        if (below(m, true)) stop32.push_back(t);
+        if (below(m, true, fused[1])) stop32.push_back(t);
         else if (leave) sw32.push_back(t);
       }
     }
