@@ -326,6 +326,7 @@ __global__ __launch_bounds__(BLK) void k_slice_finalize(const double* __restrict
   const int t = blockIdx.x;
   if (!active[t]) return;
   double F = 0.0, d = 0.0, mx = 0.0;
+#pragma unroll 8
   for (int k = threadIdx.x; k < bps; k += BLK) {
     const size_t b = (size_t)t * bps + k;
     F += podd[3 * b];
@@ -474,6 +475,7 @@ __global__ __launch_bounds__(BLK) void k_batch_finalize(
   const double* pev = f64 ? pev64 : pev32;
   const int* poff = f64 ? poff64 : poff32;
   double F = 0.0, d = 0.0, mx = 0.0;
+#pragma unroll 8
   for (int b = poff[q] + threadIdx.x; b < poff[q + 1]; b += BLK) {
     F += pod[3 * (size_t)b];
     d += pod[3 * (size_t)b + 1] + pev[3 * (size_t)b + 1];

@@ -57,6 +57,16 @@ def test_compute_without_init_is_an_error(gfx):
     assert lib.overrelax_gauge(1, 1e-12, 1) == -1
 
 
+def test_setting_validation(gfx):
+    """Setters check their arguments on the host (no device work)."""
+    lib = gfx.load()
+    assert lib.gfx_set_packed_links(2) == 600 and lib.gfx_set_packed_links(-1) == 600
+    assert lib.gfx_set_packed_links(0) == 0 and lib.gfx_set_packed_links(1) == 0
+    assert lib.gfx_set_precision(7, 1e-6) == 600
+    assert lib.gfx_set_fp32_stall(-1, 1e-8) == 600 and lib.gfx_set_fp32_stall(300, 0.0) == 600
+    assert lib.gfx_set_fp32_stall(300, 1e-8) == 0
+
+
 def test_driver_binary_built():
     assert os.access(os.path.join(ROOT, "cudagfx_amd", "bin", "cudagaugefix"), os.X_OK)
 
