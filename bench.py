@@ -21,9 +21,10 @@ timed with HIP events on the library's stream; algorithmic bytes per
 site-update are 8 links + 3 g (FP32 g on packed high parts: 432 B, FP64 g
 on packed links: 768 B, FP32 on FP32 links: 528 B, FP64: 1056 B).
 
-cpu_baseline: the oracle's OpenMP half-sweep (a C restatement of the
-reference's sweep on 2-row AoS data) timed on the host cores on a bounded
-sample of the same 32^4 problem.
+cpu_baseline: an OpenMP CPU half sweep built from the reference's own host
+SU(3) routines (su3manip.c, complex.c) on its 18-real AoS layout
+(oracle/ref_sweep.c, compiled into oracle/_ref), timed on the host cores on
+a bounded sample of the same 32^4 problem.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -117,17 +118,28 @@ def _omp_threads(n):
 
 
 def cpu_baseline(u_su3, g_su3, budget_s=10.0, budget_1core_s=5.0):
+    """CPU half sweeps of the same 32^4 problem on the host cores: the sweep
+    built from the reference's own host SU(3) routines on its 18-real AoS
+    layout (oracle/ref_sweep.c in oracle/_ref, kind "reference"), or, where
+    that was not built, the oracle's 2-row restatement (kind "port")."""
     import oracle
 
-    u2 = oracle.su3_to_2v(u_su3)
-    g2 = oracle.su3_to_2v(g_su3)
     V = int(np.prod(DIMS))
+    ref = oracle.ref_lib() is not None
+    if ref:
+        u = np.ascontiguousarray(u_su3).copy()
+        g = np.ascontiguousarray(g_su3).copy()
+        sweep = lambda p: oracle.ref_halfsweep(u, g, DIMS, p, 4, OMEGA)
+    else:
+        u2 = oracle.su3_to_2v(u_su3)
+        g2 = oracle.su3_to_2v(g_su3)
+        sweep = lambda p: oracle.halfsweep(u2, g2, DIMS, p, 4, OMEGA)
 
     def timed(budget):
         n = 0
         t0 = time.perf_counter()
         while True:
-            oracle.halfsweep(u2, g2, DIMS, n % 2, 4, OMEGA)
+            sweep(n % 2)
             n += 1
             if time.perf_counter() - t0 > budget and n >= 2:
                 break
@@ -137,10 +149,14 @@ def cpu_baseline(u_su3, g_su3, budget_s=10.0, budget_1core_s=5.0):
     _omp_threads(cores)
     n, dt = timed(budget_s)
     rate = n * (V // 2) / dt
-    out = {"value": rate, "unit": "site-updates/s", "cores": cores, "kind": "port",
+    what = ("oracle/ref_sweep.c ref_or_halfsweep (the reference's host su3mult/su3dagger/su3add "
+            "routines on its 18-real AoS layout)" if ref else
+            "oracle/oracle.c orc_or_halfsweep (2-row restatement)")
+    out = {"value": rate, "unit": "site-updates/s", "cores": cores,
+           "kind": "reference" if ref else "port",
            "sample": f"{n} FP64 half-sweeps of the 32^4 problem ({n * (V // 2)} site-updates) "
-                     f"by oracle/oracle.c orc_or_halfsweep with OpenMP, {dt:.1f} s",
-           "GBs_at_1056B": rate * 1056 / 1e9}
+                     f"by {what} with OpenMP, {dt:.1f} s",
+           "GBs_at_1584B" if ref else "GBs_at_1056B": rate * (1584 if ref else 1056) / 1e9}
     if _omp_threads(1):
         n1, dt1 = timed(budget_1core_s)
         out["value_1core"] = n1 * (V // 2) / dt1

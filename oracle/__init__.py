@@ -161,6 +161,25 @@ def halfsweep(u2v, g2v, dims, parity, ndir=4, omega=1.68, nterms=3, active=None)
                          act.ctypes.data if act is not None else None, vs)
 
 
+def ref_halfsweep(u_su3, g_su3, dims, parity, ndir=4, omega=1.68):
+    """One half sweep in place on the reference's host layout (u: 4V su3,
+    g: V su3, 18 doubles each) by oracle/ref_sweep.c, built from the
+    reference's own host SU(3) routines; None where oracle/_ref is absent."""
+    ref = ref_lib()
+    if ref is None:
+        return None
+    nn = neighbours(dims)
+    ev, od = eo_lists(dims)
+    sites = ev if parity == 0 else od
+    vp = C.c_void_p
+    f = ref.ref_or_halfsweep
+    f.restype = None
+    f.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_double]
+    f(u_su3.ctypes.data, g_su3.ctypes.data, nn.ctypes.data, sites.ctypes.data,
+      volume(dims) // 2, ndir, omega)
+    return g_su3
+
+
 def functional(u2v, g2v, dims, ndir=4):
     """Returns (F, dAdA, maxdAdA) as calc_functional (overrelax.cu:610-671)."""
     nn = neighbours(dims)

@@ -55,6 +55,27 @@ def test_host_observables_bitexact(orc):
         assert orc.mean_plaq(a, dims) == gd[tag + "_plaq"]
 
 
+def test_reference_routine_sweep_matches_oracle(orc):
+    """The CPU baseline sweep built from the reference's own host SU(3)
+    routines on its 18-real layout (oracle/ref_sweep.c) and the oracle's
+    2-row sweep give the same g after an even and an odd half sweep (1e-13;
+    they differ in summation order and in rebuilding row 2)."""
+    if orc.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (no reference checkout)")
+    dims = (4, 4, 6, 8)
+    V = orc.volume(dims)
+    u = orc.random_init_gauge(V, 4501)
+    g = orc.random_init_trafo(V, 4502)
+    g_ref = g.copy()
+    for parity in (0, 1):
+        orc.ref_halfsweep(u, g_ref, dims, parity, 4, 1.7)
+    u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+    for parity in (0, 1):
+        orc.halfsweep(u2, g2, dims, parity, 4, 1.7)
+    err = np.abs(g_ref - orc.from_2v(g2)).max()
+    assert err < 1e-13 and np.abs(g_ref - g).max() > 1e-3
+
+
 def test_against_live_reference_build(orc):
     """Where oracle/_ref was built from the reference checkout: bigger lattice."""
     import ctypes as C
