@@ -301,7 +301,7 @@ def set_fp32_stall(iters=300, gate=1e-8):
 
 
 def set_packed_links(on=1):
-    """Links of a mixed Landau or Coulomb run: 1 packed (FP32 phase on 24-bit high
+    """Links of a mixed Landau, Coulomb or batch run: 1 packed (FP32 phase on 24-bit high
     parts within 2^-24, FP64 polish on high + 16-bit low parts within 2^-38
     of the FP64 links), 0 the FP32 and FP64 links."""
     _check(load().gfx_set_packed_links(int(on)), "set_packed_links")

@@ -136,7 +136,7 @@ int gfx_set_precision(int mode, double switch_theta);
  * and has not halved for `iters` iterations (the FP32 rounding floor, near
  * 5e-11 on 32^4); iters = 0 turns this off.  Default 300, 1e-8. */
 int gfx_set_fp32_stall(int iters, double gate);
-/* links read by the sweeps of a mixed Landau or Coulomb run: 1 (default)
+/* links read by the sweeps of a mixed Landau, Coulomb or batch run: 1 (default)
  * packed links, each real u of a link as a 24-bit integer hi = rint(u 2^23) plus an int16
  * lo = rint((u - hi 2^-23) 2^38).  The FP32 phase reads hi 2^-23 (within
  * 2^-24 of u, 36 instead of 48 bytes per link); the FP64 polish reads
