@@ -788,7 +788,8 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   // reads the FP32 or FP64 link planes
   auto below = [&](double m, bool f32, bool fused_check) {
     if (!s.packed_on || !fused_check) return m < eps;
-    const double r = std::sqrt(m) + (f32 ? PACK_DDELTA32 : PACK_DDELTA64);
+    // FP32 slices sweep with the short star, FP64 slices in the reference order
+    const double r = std::sqrt(m) + (f32 ? PACK_DDELTA32_SHORT : PACK_DDELTA64);
     return r * r < eps;
   };
   auto upload = [&]() {

@@ -490,8 +490,9 @@ __global__ __launch_bounds__(BLK) void k_batch_finalize(
   results[4 * q + 1] = d / V;
   results[4 * q + 2] = mx;
   // on packed links a lattice stops when the packing's worst effect on
-  // Delta keeps max dAdA below eps on the exact links as well
-  const double r = packed ? sqrt(mx) + (f64 ? PACK_DDELTA64 : PACK_DDELTA32) : 0.0;
+  // Delta (with the short star of the batch sweeps) keeps max dAdA below
+  // eps on the exact links as well
+  const double r = packed ? sqrt(mx) + (f64 ? PACK_DDELTA64_SHORT : PACK_DDELTA32_SHORT) : 0.0;
   const bool below = packed ? r * r < eps : mx < eps;
   if (f64) {
     if (below) {

@@ -224,12 +224,18 @@ constexpr double PACK_LO = 0x1p38;
 // by at most 4 sqrt(2) e per complex element, so |dU|_F <= sqrt(12 + 96) e
 // <= 10.4 e, and with 8 star terms of unitary g, |dDelta|_F <= 83.2 e.
 // theta(x) = |Delta(x)|_F^2, so theta on the exact links is below
-// (sqrt(theta on the packed links) + PACK_DDELTA)^2.
+// (sqrt(theta on the packed links) + PACK_DDELTA)^2.  The short form of the
+// star (star_terms_short: rows 0, 1 of each product, row 2 rebuilt) adds
+// the rebuild of the product's row 2: rows 0, 1 of U g^+ are off by at most
+// sqrt(6) e per complex element, its row 2 by 4 sqrt(6) e, so a term by
+// sqrt(12 + 288) e <= 17.4 e and Delta by 139.2 e (PACK_DDELTA*_SHORT).
 constexpr double PACK_DDELTA64 = 83.2 * 0x1p-38;
 constexpr double PACK_DDELTA32 = 83.2 * 0x1p-23;
+constexpr double PACK_DDELTA64_SHORT = 139.2 * 0x1p-38;
+constexpr double PACK_DDELTA32_SHORT = 139.2 * 0x1p-23;
 // runs whose stops are decided with that bound use packed links only for
 // eps above this (below it the bound would take most of eps)
-constexpr double PACK_EPS_MIN = 100.0 * PACK_DDELTA64 * PACK_DDELTA64;
+constexpr double PACK_EPS_MIN = 100.0 * PACK_DDELTA64_SHORT * PACK_DDELTA64_SHORT;
 struct PackedLinks {
   const uint4* a;       // field f, site i at [f * stride + i]
   const uint4* b;

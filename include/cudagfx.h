@@ -144,7 +144,7 @@ int gfx_set_fp32_stall(int iters, double gate);
  * with a Landau stop confirmed by a functional on the FP64 links (plain FP64
  * sweeps go on if it does not hold) and a Coulomb slice or batch lattice
  * stopped only when the packing's worst effect on Delta keeps its max dAdA
- * below eps (these two use packed links for eps > 9e-18 only).  0: the FP32
+ * below eps (these two use packed links for eps > 3e-17 only).  0: the FP32
  * and FP64 links. */
 int gfx_set_packed_links(int on);
 /* 0 silences the per-check log lines */
