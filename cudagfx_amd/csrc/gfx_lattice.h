@@ -209,11 +209,12 @@ __device__ __forceinline__ V* field_ptr(V* base, int field, int stride) {
 // point.  hi = rint(u 2^23), clamped to +-(2^23 - 1), is a signed 24-bit
 // integer; the 12 reals of a link take 288 bits = 9 dwords, stored as two
 // uint4 planes (dwords 0-3, 4-7) and one uint plane (dword 8), real k at bits
-// [24k, 24k + 24).  The FP32 bulk sweeps read hi 2^-23 (within 2^-24 of u, 36
-// instead of 48 bytes per link).  The FP64 polish adds lo = rint((u - hi
-// 2^-23) 2^38), an int16 (|u - hi 2^-23| <= 2^-23 even where hi is clamped, so
-// |lo| <= 2^15, clamped to 32767): hi 2^-23 + lo 2^-38 is within 2^-38 of u,
-// 60 instead of 96 bytes per link.  lo is held in three uint2 planes per
+// [24k, 24k + 24).  The FP32 bulk sweeps read hi 2^-23 (within 2^-24 of u,
+// 2^-23 where |u| rounds to 1 and hi is clamped; 36 instead of 48 bytes per
+// link).  The FP64 polish adds lo = rint((u - hi 2^-23) 2^38), an int16
+// (|u - hi 2^-23| <= 2^-23 even where hi is clamped, so |lo| <= 2^15,
+// clamped to 32767): hi 2^-23 + lo 2^-38 is within 2^-39 of u (2^-38 where
+// both clamp), 60 instead of 96 bytes per link.  lo is held in three uint2 planes per
 // field, four int16 per 8-byte word in the order of the reals.
 constexpr double PACK_HI = 0x1p23;
 constexpr double PACK_LO = 0x1p38;
