@@ -378,6 +378,24 @@ def packed_round(u2, lo=True):
     return hi * 2.0 ** -23 + q * 2.0 ** -38
 
 
+def test_packing_bound_on_theta(orc):
+    """The stop rule of runs on packed links takes the packing's worst
+    effect on Delta as 83.2 e (e = 2^-38 for high + low parts, 2^-23 for the
+    high parts alone): the max dAdA of the same g on packed-rounded and on
+    exact links differ by less than that in sqrt, on random links and g."""
+    dims = (4, 4, 4, 4)
+    V = orc.volume(dims)
+    for seed in (5101, 5102, 5103):
+        u2 = orc.su3_to_2v(orc.random_init_gauge(V, seed))
+        g2 = orc.su3_to_2v(orc.random_init_trafo(V, seed + 50))
+        m = orc.functional(u2, g2, dims)[2]
+        for lo, e in ((True, 2.0 ** -38), (False, 2.0 ** -23)):
+            mp = orc.functional(packed_round(u2, lo), g2, dims)[2]
+            d = abs(np.sqrt(mp) - np.sqrt(m))
+            assert d < 83.2 * e, (lo, d / e)
+            assert d > 1e-3 * e  # the rounding is seen
+
+
 def test_packed_round_bound(orc):
     """|packed(u) - u| <= 2^-39 for SU(3) links (2^-38 at |u| = 1, where hi
     and lo both clamp), the high parts alone within 2^-24 (2^-23 at |u| = 1),
