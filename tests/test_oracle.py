@@ -282,6 +282,99 @@ def test_functional_of_cold_and_pure_gauge(orc):
     assert abs(F - 3.0) < 1e-14 and M < 1e-25
 
 
+def np_site_theta(ut, dims, ndir=4, half=0.5, traceless=True):
+    """Per-site Re tr(Delta Delta^+) of host-layout links ut, restated in
+    numpy from observables.c:70-165 independently of oracle.c:
+    A(U) = (U - U^+)/(2i) made traceless, Delta(x) = sum_mu A(U_mu(x-mu)) -
+    A(U_mu(x)).  `half` and `traceless` exist only to plant mistakes."""
+    LX, LY, LZ, LT = dims
+    U = mats(ut).reshape(LT, LZ, LY, LX, 4, 3, 3)
+
+    def A(M):
+        h = (M - np.conj(np.swapaxes(M, -1, -2))) * (-1j * half)
+        if traceless:
+            h = h - (np.trace(h, axis1=-2, axis2=-1) / 3.0)[..., None, None] * np.eye(3)
+        return h
+
+    D = np.zeros(U.shape[:4] + (3, 3), dtype=complex)
+    for mu in range(ndir):
+        Um = U[..., mu, :, :]
+        # out[x] = Um[x - mu]: x is the last lattice axis, t the first
+        D += A(np.roll(Um, 1, axis=3 - mu)) - A(Um)
+    return (np.abs(D) ** 2).sum(axis=(-2, -1))
+
+
+def test_theta_pinned_on_unfixed_fields(orc):
+    """The stop quantity of the sweeps and of calc_functional (dAdA and max
+    dAdA of orc.functional: Delta(x) = -A(g(x) K(x)) from the star, the form
+    of overrelax.cu:482-605 that the fused checks use) equals the reference's
+    host dAdA (observables.c:70-165) of the transformed links on random,
+    not gauge-fixed fields: against the golden values written by the
+    reference build (4^4), then against the pinned host dAdA and a numpy
+    restatement on other shapes, where oracle/_ref is present also against
+    the live reference routine.  A dropped 1/2 in A or a dropped trace
+    subtraction changes the values far beyond the tolerance."""
+    gd = _gold()
+    dims = tuple(int(x) for x in gd["dims"])
+    F, D, M = orc.functional(orc.su3_to_2v(gd["gauge"]), orc.su3_to_2v(gd["trafo"]), dims)
+    assert abs(D - gd["ut_dAdA"]) <= 1e-13 * gd["ut_dAdA"]
+    assert abs(M - gd["ut_maxdAdA"]) <= 1e-13 * gd["ut_maxdAdA"]
+    assert abs(F - gd["ut_F"]) <= 1e-13
+    th = np_site_theta(gd["gauge_transformed"], dims)
+    assert abs(th.mean() - gd["ut_dAdA"]) <= 1e-13 * gd["ut_dAdA"]
+    assert abs(th.max() - gd["ut_maxdAdA"]) <= 1e-13 * gd["ut_maxdAdA"]
+    # the planted mistakes are seen
+    assert abs(np_site_theta(gd["gauge_transformed"], dims, half=1.0).mean() / gd["ut_dAdA"] - 4) < 1e-12
+    assert abs(np_site_theta(gd["gauge_transformed"], dims, traceless=False).mean() / gd["ut_dAdA"] - 1) > 1e-3
+    ref = orc.ref_lib()
+    for dims, seed in (((4, 4, 4, 4), 6101), ((4, 6, 4, 8), 6102), ((6, 4, 8, 4), 6103),
+                       ((4, 6, 4, 8), 6104)):
+        V = orc.volume(dims)
+        u = orc.random_init_gauge(V, seed)
+        g = orc.random_init_trafo(V, seed + 500)
+        ut = orc.g_trafo(u, g, dims)
+        F, D, M = orc.functional(orc.su3_to_2v(u), orc.su3_to_2v(g), dims)
+        d_host, m_host = orc.dAdA(ut, dims)
+        assert D > 1.0 and M > D
+        assert abs(D - d_host) <= 1e-13 * d_host and abs(M - m_host) <= 1e-13 * m_host
+        assert abs(F - orc.gauge_functional(ut, dims)) <= 1e-13
+        th = np_site_theta(ut, dims)
+        assert abs(th.mean() - d_host) <= 1e-13 * d_host and abs(th.max() - m_host) <= 1e-13 * m_host
+        if ref is not None:
+            import ctypes as C
+            for n, v in zip(["LX", "LY", "LZ", "T", "VOLUME"], list(dims) + [V]):
+                C.c_int.in_dll(ref, n).value = v
+            nn = orc.neighbours(dims)
+            C.c_void_p.in_dll(ref, "nn").value = nn.ctypes.data
+            ref.dAdA.restype = C.c_double
+            ref.dAdA.argtypes = [C.c_void_p]
+            d_ref = ref.dAdA(ut.ctypes.data_as(C.c_void_p))
+            m_ref = C.c_double.in_dll(ref, "maxDADA").value
+            assert abs(D - d_ref) <= 1e-13 * d_ref and abs(M - m_ref) <= 1e-13 * m_ref
+
+
+def test_theta_slices_pinned(orc):
+    """Per-slice F, dAdA and max dAdA of the Coulomb checks
+    (orc.functional_slices over the 3 spatial directions) against the numpy
+    restatement of the spatial Delta on the transformed links, per time
+    slice, on random fields that are not gauge fixed."""
+    for dims, seed in (((4, 4, 4, 6), 6201), ((6, 4, 4, 4), 6202)):
+        V = orc.volume(dims)
+        u = orc.random_init_gauge(V, seed)
+        g = orc.random_init_trafo(V, seed + 500)
+        ut = orc.g_trafo(u, g, dims)
+        F, D, M = orc.functional_slices(orc.su3_to_2v(u), orc.su3_to_2v(g), dims, 3)
+        th = np_site_theta(ut, dims, ndir=3).reshape(dims[3], -1)
+        Ut = mats(ut).reshape(dims[3], -1, 4, 3, 3)
+        f_np = np.trace(Ut[:, :, :3], axis1=-2, axis2=-1).real.sum(axis=(1, 2)) / (3 * th.shape[1])
+        assert np.all(D > 0.5)
+        assert np.allclose(D, th.mean(axis=1), rtol=1e-13, atol=0)
+        assert np.allclose(M, th.max(axis=1), rtol=1e-13, atol=0)
+        assert np.allclose(F, f_np, rtol=0, atol=1e-14)
+        # the 4-direction form differs: the test sees the direction count
+        assert not np.allclose(D, np_site_theta(ut, dims, ndir=4).reshape(dims[3], -1).mean(axis=1), rtol=1e-3)
+
+
 def test_stop_rule_quirks(orc):
     """overrelax.cu:775-822: the stop is checked only when i % ck == 0, so the
     stop iteration is a multiple of ck; maxit = 0 returns 0."""
