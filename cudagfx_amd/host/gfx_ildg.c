@@ -14,8 +14,10 @@
  *                      s = x + LX*(y + LY*(z + LZ*t)), mu-minor, row-major
  * Each record is written as its own message (MB = ME = 1).
  */
+#define _FILE_OFFSET_BITS 64
 #include <stdint.h>
 #include <stdio.h>
+#include <sys/types.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -79,6 +81,7 @@ static int xml_entry(const char *s, const char *tag, char *out, size_t cap) {
 static int text_entry(const char *s, const char *name, char *out, size_t cap) {
   char *dup = strdup(s), *save = NULL;
   int found = 0;
+  if (!dup) return 0;
   for (char *t = strtok_r(dup, " \n\t", &save); t; t = strtok_r(NULL, " \n\t", &save)) {
     if (strcmp(t, name) == 0) {
       char *eq = strtok_r(NULL, " \n\t", &save);
@@ -110,24 +113,77 @@ static int lime_write_record(FILE *f, const char *type, const void *data, uint64
   return 0;
 }
 
+/* LIME records are read from files that may be truncated or corrupt: every
+ * length is checked against the bytes that are left in the file before it
+ * is used, and metadata records (ildg-format, xlf-info) are held in memory
+ * only up to LIME_META_MAX bytes. */
+#define LIME_META_MAX (1u << 20)
+
+/* size of an open file, or -1 (also for input that cannot seek) */
+static int64_t file_size(FILE *f) {
+  if (fseeko(f, 0, SEEK_END) != 0) return -1;
+  const off_t n = ftello(f);
+  if (n < 0 || fseeko(f, 0, SEEK_SET) != 0) return -1;
+  return (int64_t)n;
+}
+
+/* the next record header: 1 with *len (checked to fit in the rest of the
+ * file) and the type, 0 at the end of the file, -1 for a short header, a
+ * wrong magic number or data running past the end of the file */
+static int lime_next(FILE *f, int64_t fsize, uint64_t *len, char type[129]) {
+  unsigned char h[LIME_HDR];
+  const size_t got = fread(h, 1, LIME_HDR, f);
+  if (got == 0 && feof(f)) return 0;
+  if (got != LIME_HDR || get_be32(h) != LIME_MAGIC) return -1;
+  const off_t pos = ftello(f);
+  if (pos < 0 || (int64_t)pos > fsize) return -1;
+  *len = get_be64(h + 8);
+  if (*len > (uint64_t)(fsize - (int64_t)pos)) return -1;
+  memcpy(type, h + 16, 128);
+  type[128] = 0;
+  return 1;
+}
+
+/* moves past the rest of a record whose data has length len (done bytes of
+ * it already read) and its padding to a multiple of 8; the padding of the
+ * last record may be cut short by the end of the file */
+static int lime_skip(FILE *f, int64_t fsize, uint64_t len, uint64_t done) {
+  const off_t pos = ftello(f);
+  if (pos < 0 || (int64_t)pos > fsize) return -1;
+  const uint64_t left = (uint64_t)(fsize - (int64_t)pos);
+  uint64_t skip = len - done + (8 - len % 8) % 8;
+  if (skip > left) skip = left;
+  return fseeko(f, (off_t)skip, SEEK_CUR) == 0 ? 0 : -1;
+}
+
+/* the data of a metadata record as a NUL-terminated string (NULL if it is
+ * longer than LIME_META_MAX or cannot be read) */
+static char *lime_read_text(FILE *f, uint64_t len) {
+  if (len > LIME_META_MAX) return NULL;
+  char *msg = (char *)calloc((size_t)len + 1, 1);
+  if (!msg) return NULL;
+  if (len && fread(msg, 1, (size_t)len, f) != (size_t)len) {
+    free(msg);
+    return NULL;
+  }
+  return msg;
+}
+
 int read_ildg_dims(char *filename, int dims[4]) {
   FILE *f = fopen(filename, "rb");
   if (!f) return -1;
-  unsigned char h[LIME_HDR];
+  const int64_t fsize = file_size(f);
   int rc = -1;
-  while (rc != 0 && fread(h, 1, LIME_HDR, f) == LIME_HDR && get_be32(h) == LIME_MAGIC) {
-    const uint64_t len = get_be64(h + 8);
-    char type[129];
-    memcpy(type, h + 16, 128);
-    type[128] = 0;
-    const uint64_t padded = len + (8 - len % 8) % 8;
+  uint64_t len;
+  char type[129];
+  while (fsize >= 0 && lime_next(f, fsize, &len, type) == 1) {
     if (!strcmp(type, "ildg-binary-data")) break;
-    if (strcmp(type, "ildg-format") != 0 || len > (1u << 20)) {
-      if (fseek(f, (long)padded, SEEK_CUR) != 0) break;
+    if (strcmp(type, "ildg-format") != 0 || len > LIME_META_MAX) {
+      if (lime_skip(f, fsize, len, 0) != 0) break;
       continue;
     }
-    char *msg = (char *)calloc(len + 1, 1);
-    if (msg && (!len || fread(msg, 1, len, f) == len)) {
+    char *msg = lime_read_text(f, len);
+    if (msg) {
       const char *tags[4] = {"lx", "ly", "lz", "lt"};
       char v[64];
       rc = 0;
@@ -152,31 +208,38 @@ int read_gf_ildg(gfx_su3 *gf, char *filename) {
   }
   printf("Reading ildg-format:\n");
   printf("Reading file '%s' in ildg-format\n", filename);
+  const int64_t fsize = file_size(f);
+  if (fsize < 0) {
+    fprintf(stderr, "Error in read_gf_ildg: cannot determine the size of %s\n", filename);
+    fclose(f);
+    return -1;
+  }
   int precision = 0, readprecision = 0, readformat = 0, have_data = 0;
-  uint64_t data_bytes = 0;
-  unsigned char h[LIME_HDR];
-  while (fread(h, 1, LIME_HDR, f) == LIME_HDR) {
-    if (get_be32(h) != LIME_MAGIC) {
+  uint64_t data_bytes = 0, len;
+  char type[129];
+  for (;;) {
+    const int st = lime_next(f, fsize, &len, type);
+    if (st == 0) break;
+    if (st < 0) {
       fprintf(stderr, "limeReaderNextRecord returned error with status = %d!\n", -1);
       break;
     }
-    const uint64_t len = get_be64(h + 8);
-    char type[129];
-    memcpy(type, h + 16, 128);
-    type[128] = 0;
-    const uint64_t padded = len + (8 - len % 8) % 8;
     if (!strcmp(type, "ildg-binary-data")) {
       data_bytes = len;
       have_data = 1;
       break;
     }
     if (!strcmp(type, "ildg-format") || !strcmp(type, "xlf-info")) {
-      char *msg = (char *)calloc(len + 1, 1);
-      if (len && fread(msg, 1, len, f) != len) {
+      char *msg = lime_read_text(f, len);
+      if (!msg) {
+        fprintf(stderr, "Error in read_gf_ildg: %s record of %llu bytes cannot be read\n", type,
+                (unsigned long long)len);
+        break;
+      }
+      if (lime_skip(f, fsize, len, len) != 0) {
         free(msg);
         break;
       }
-      if (padded > len) fseek(f, (long)(padded - len), SEEK_CUR);
       char v[512];
       if (!strcmp(type, "ildg-format")) {
         if (xml_entry(msg, "field", v, sizeof v)) printf("The field is of type: %s\n", v);
@@ -207,8 +270,8 @@ int read_gf_ildg(gfx_su3 *gf, char *filename) {
         printf("beta: %e\n", BETA);
       }
       free(msg);
-    } else {
-      fseek(f, (long)padded, SEEK_CUR);
+    } else if (lime_skip(f, fsize, len, 0) != 0) {
+      break;
     }
   }
   if (!have_data || !readformat) {
@@ -221,6 +284,10 @@ int read_gf_ildg(gfx_su3 *gf, char *filename) {
   if (data_bytes == n * sizeof(float) && (precision == 32 || !readprecision)) {
     float *buf = (float *)malloc(sizeof(float) * 18 * 4096);
     double *out = (double *)gf;
+    if (!buf) {
+      fclose(f);
+      return -1;
+    }
     for (uint64_t done = 0; done < n;) {
       const size_t chunk = (size_t)((n - done < 18 * 4096) ? n - done : 18 * 4096);
       if (fread(buf, sizeof(float), chunk, f) != chunk) {
@@ -290,6 +357,7 @@ int write_gf_ildg(gfx_su3 *gf, char *filename, const int prec) {
   const double *src = (const double *)gf;
   const size_t CH = 18 * 4096;
   void *buf = malloc(CH * 8);
+  if (!buf) rc = -1;
   for (uint64_t done = 0; done < n && rc == 0;) {
     const size_t chunk = (size_t)((n - done < CH) ? n - done : CH);
     if (prec == 64) {
