@@ -678,11 +678,16 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
     if (s.verbose)
       printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e\n", i, FUNC, DADA, maxdada);
     if (maxdada < eps) {
-      if (s.packed_on && !fp32) {
-        // a stop found by the polish on the packed links is decided on the
-        // FP64 links: state i with the per-link functional (a stop in the
-        // FP32 phase is taken as it is and converted after the loop)
-        canonical(false);
+      if (fp32 || s.packed_on) {
+        // a stop found by an FP32 sweep or by the polish on the packed
+        // links is decided on the FP64 links: state i (converted to FP64 g
+        // with an FP64 reunitarisation if it is an FP32 state) with the
+        // per-link functional
+        const bool from32 = fp32;
+        if (fp32)
+          to_fp64();
+        else
+          canonical(false);
         if (functional_now(s, false, 4, 1, eps)) return -1;
         FUNC = s.h_results[0];
         DADA = s.h_results[1];
@@ -691,8 +696,9 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
           printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e (FP64 links)\n", i,
                  FUNC, DADA, maxdada);
         if (!(maxdada < eps)) {
-          // not there on the FP64 links: iteration i+1 onwards in plain FP64
-          s.packed_on = false;
+          // not there on the FP64 links: iteration i+1 onwards in FP64, the
+          // polish form after an FP32 stop, plain FP64 after a polish stop
+          if (!from32) s.packed_on = false;
           continue;
         }
       }
@@ -781,15 +787,16 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
     tune_sweep(s, FORM_F64, 3);
   }
   s.packed_on = packed;
-  // a slice stops when its max dAdA is below eps.  A check fused into the
-  // sweeps on packed links decides that with the packing's worst effect on
-  // Delta added, so that the bound holds on the exact links too (a slice in
-  // the narrow band below eps stops one check later); a standalone check
-  // reads the FP32 or FP64 link planes
-  auto below = [&](double m, bool f32, bool fused_check) {
+  // an FP64 slice stops when its max dAdA is below eps.  A check fused
+  // into the sweeps on packed links decides that with the packing's worst
+  // effect on Delta added (the FP64 slices sweep in the reference order), so
+  // that the bound holds on the exact links too (a slice in the narrow band
+  // below eps stops one check later); a standalone check reads the FP64
+  // link planes.  An FP32 slice never stops: below eps (or the switch) it
+  // moves to FP64, whose check decides its stop.
+  auto below = [&](double m, bool fused_check) {
     if (!s.packed_on || !fused_check) return m < eps;
-    // FP32 slices sweep with the short star, FP64 slices in the reference order
-    const double r = std::sqrt(m) + (f32 ? PACK_DDELTA32_SHORT : PACK_DDELTA64);
+    const double r = std::sqrt(m) + PACK_DDELTA64;
     return r * r < eps;
   };
   auto upload = [&]() {
@@ -867,13 +874,13 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
           for (int k = 0; k < 4; k++) res[4 * t + k] = s.h_results[4 * t + k];
     }
     // decisions
-    std::vector<int> stop64, stop32, sw32;
+    std::vector<int> stop64, sw32;
     for (int t = 0; t < nt; t++) {
       const double m = res[4 * t + 2];
       if (act[0][t]) {
-        if (below(m, false, fused[0])) stop64.push_back(t);
+        if (below(m, fused[0])) stop64.push_back(t);
       } else if (act[1][t]) {
-        bool leave = m < s.switch_theta;
+        bool leave = m < s.switch_theta || m < eps;
         if (!leave) {
           if (m < 0.5 * best[t]) {
             best[t] = m;
@@ -882,8 +889,7 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
             leave = true;
           }
         }
-        if (below(m, true, fused[1])) stop32.push_back(t);
-        else if (leave) sw32.push_back(t);
+        if (leave) sw32.push_back(t);
       }
     }
     const bool last = i + 1 >= maxit;
@@ -895,18 +901,10 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
       even[0] = 2 - even[0];
       even_done[0] = true;
     }
-    // FP32 slices that stop or leave: to FP64.  A slice going on in FP64
-    // continues from the FP32 check sweep's update when the FP64 slices also
-    // took theirs (their next sweep is the odd half), else from the checked
-    // state.
+    // FP32 slices that leave: to FP64.  A slice going on in FP64 continues
+    // from the FP32 check sweep's update when the FP64 slices also took
+    // theirs (their next sweep is the odd half), else from the checked state.
     const bool swap32 = fused[1] && n[1] > 0 && !last;
-    for (int t : stop32) {
-      // a stopped slice's even sites must be in both FP64 even fields, which
-      // keep swapping roles
-      launch_g32_to_g64_range(field32(s, even[1]), field32(s, 1), field64(s, even[0]),
-                              field64(s, 1), s.geo, t * nsh, nsh, s.stream);
-      copy_slice(false, even[0], 2 - even[0], t);
-    }
     for (int t : sw32) {
       const bool from_update = swap32 && (even_done[0] || n[0] == 0);
       launch_g32_to_g64_range(field32(s, from_update ? 2 - even[1] : even[1]), field32(s, 1),
@@ -923,18 +921,12 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
       iters[t] = i;
       left--;
     }
-    for (int t : stop32) {
-      act[1][t] = 0;
-      phase[t] = -1;
-      iters[t] = i;
-      left--;
-    }
     for (int t : sw32) {
       act[1][t] = 0;
       act[0][t] = 1;
       phase[t] = 0;
     }
-    if (!stop64.empty() || !stop32.empty() || !sw32.empty()) {
+    if (!stop64.empty() || !sw32.empty()) {
       upload();
       maps();
     }

@@ -449,12 +449,12 @@ __global__ __launch_bounds__(BLK) void k_functional(const V* __restrict__ U,
 // results[4q..] = F/(ndir V), dAdA/V, max dAdA, stop iteration.
 // FP64: stop when max < eps, else take the check sweep's even field (swap
 // g[0] and galt) unless *iter + 1 == maxit (the run ends on the checked
-// state).  FP32: leave the phase when max < switch_theta or when the best
-// max has stayed below stall_gate without halving for stall_checks checks;
-// the lattice then continues in FP64 from the FP32 state after the check
-// sweep (conv = 1: even field from galt), or, when it already reached eps or
-// the run ends, stops with the checked state converted (conv = 2).  sw[q]
-// records the iteration of the switch.
+// state).  FP32: leave the phase when max < eps or max < switch_theta or
+// when the best max has stayed below stall_gate without halving for
+// stall_checks checks; the lattice then continues in FP64 from the FP32
+// state after the check sweep (conv = 1: even field from galt), or, when the
+// run ends, keeps the checked state converted (conv = 2, not stopped).
+// sw[q] records the iteration of the switch.
 __global__ __launch_bounds__(BLK) void k_batch_finalize(
     const double* __restrict__ pod64, const double* __restrict__ pev64,
     const double* __restrict__ pod32, const double* __restrict__ pev32,
@@ -489,13 +489,12 @@ __global__ __launch_bounds__(BLK) void k_batch_finalize(
   results[4 * q + 0] = F / (ndir * V);
   results[4 * q + 1] = d / V;
   results[4 * q + 2] = mx;
-  // on packed links a lattice stops when the packing's worst effect on
-  // Delta (with the short star of the batch sweeps) keeps max dAdA below
+  // on packed links an FP64 lattice stops when the packing's worst effect
+  // on Delta (with the short star of the batch sweeps) keeps max dAdA below
   // eps on the exact links as well
-  const double r = packed ? sqrt(mx) + (f64 ? PACK_DDELTA64_SHORT : PACK_DDELTA32_SHORT) : 0.0;
-  const bool below = packed ? r * r < eps : mx < eps;
   if (f64) {
-    if (below) {
+    const double r = packed ? sqrt(mx) + PACK_DDELTA64_SHORT : 0.0;
+    if (packed ? r * r < eps : mx < eps) {
       results[4 * q + 3] = (double)it;
       act64[q] = 0;
     } else if (!last) {
@@ -505,8 +504,9 @@ __global__ __launch_bounds__(BLK) void k_batch_finalize(
     }
     return;
   }
-  const bool stop = below;
-  bool leave = stop || mx < switch_theta;
+  // an FP32 lattice never stops: below eps (or the switch, or at its stall)
+  // it moves to FP64, whose check decides its stop
+  bool leave = mx < eps || mx < switch_theta;
   if (!leave && stall_checks > 0) {
     if (mx < 0.5 * best[q]) {
       best[q] = mx;
@@ -518,9 +518,8 @@ __global__ __launch_bounds__(BLK) void k_batch_finalize(
   if (leave) {
     act32[q] = 0;
     sw[q] = it;
-    if (stop || last) {
+    if (last) {
       conv[q] = 2;
-      if (stop) results[4 * q + 3] = (double)it;
     } else {
       conv[q] = 1;
       act64[q] = 1;

@@ -220,20 +220,22 @@ constexpr double PACK_HI = 0x1p23;
 constexpr double PACK_LO = 0x1p38;
 // Worst effect of the packing on a site's Delta = -A(g K) (A: traceless
 // antihermitian part, which does not increase the Frobenius norm).  A stored
-// real is off by at most e (2^-38 for high + low parts, 2^-23 for the high
-// parts alone, both where |u| rounds to 1); row 2 = conj(row0 x row1) then
-// by at most 4 sqrt(2) e per complex element, so |dU|_F <= sqrt(12 + 96) e
-// <= 10.4 e, and with 8 star terms of unitary g, |dDelta|_F <= 83.2 e.
-// theta(x) = |Delta(x)|_F^2, so theta on the exact links is below
-// (sqrt(theta on the packed links) + PACK_DDELTA)^2.  The short form of the
-// star (star_terms_short: rows 0, 1 of each product, row 2 rebuilt) adds
-// the rebuild of the product's row 2: rows 0, 1 of U g^+ are off by at most
-// sqrt(6) e per complex element, its row 2 by 4 sqrt(6) e, so a term by
-// sqrt(12 + 288) e <= 17.4 e and Delta by 139.2 e (PACK_DDELTA*_SHORT).
+// real is off by at most e (2^-38 for high + low parts, where |u| rounds to
+// 1); row 2 = conj(row0 x row1) then by at most 4 sqrt(2) e per complex
+// element, so |dU|_F <= sqrt(12 + 96) e <= 10.4 e, and with 8 star terms of
+// unitary g, |dDelta|_F <= 83.2 e.  theta(x) = |Delta(x)|_F^2, so theta on
+// the exact links is below (sqrt(theta on the packed links) + PACK_DDELTA)^2.
+// The short form of the star (star_terms_short: rows 0, 1 of each product,
+// row 2 rebuilt) adds the rebuild of the product's row 2: rows 0, 1 of U g^+
+// are off by at most sqrt(6) e per complex element, its row 2 by 4 sqrt(6)
+// e, so a term by sqrt(12 + 288) e <= 17.4 e and Delta by 139.2 e
+// (PACK_DDELTA64_SHORT).  These bound the FP64 checks, whose own rounding
+// (about 1e-15 in Delta) is far below them.  Checks made in FP32 (FP32 g
+// and FP32 arithmetic, rounding of order 1e-6 in Delta) decide no stop: a
+// lattice or slice that an FP32 check finds below eps moves to FP64, and an
+// FP64 check decides its stop.
 constexpr double PACK_DDELTA64 = 83.2 * 0x1p-38;
-constexpr double PACK_DDELTA32 = 83.2 * 0x1p-23;
 constexpr double PACK_DDELTA64_SHORT = 139.2 * 0x1p-38;
-constexpr double PACK_DDELTA32_SHORT = 139.2 * 0x1p-23;
 // runs whose stops are decided with that bound use packed links only for
 // eps above this (below it the bound would take most of eps)
 constexpr double PACK_EPS_MIN = 100.0 * PACK_DDELTA64_SHORT * PACK_DDELTA64_SHORT;

@@ -505,3 +505,61 @@ def test_packed_round_bound(orc):
     one = np.array([1.0, -1.0, 0.0])
     assert np.abs(packed_round(one) - one).max() == 2.0 ** -38
     assert np.abs(packed_round(one, lo=False) - one).max() == 2.0 ** -23
+
+
+def np_short_theta(u2, g2, dims):
+    """Per-site theta = |A(g(x) K(x))|_F^2 with the star K formed in the short
+    form of the batch sweeps and of the FP32 sweeps (star_terms_short in the
+    lattice header): each term U_mu(x) g^+(x+mu) and (g(x-mu) U_mu(x-mu))^+
+    from rows 0, 1 of the product with row 2 rebuilt as conj(row0 x row1),
+    link and g rows 0, 1 as stored (2-row layout), their row 2 rebuilt the
+    same way.  numpy, independent of oracle.c."""
+    LX, LY, LZ, LT = dims
+
+    def full(a2):
+        r = a2.reshape(-1, 2, 3, 2)
+        m = r[..., 0] + 1j * r[..., 1]
+        return np.concatenate([m, np.conj(np.cross(m[:, 0], m[:, 1]))[:, None]], axis=1)
+
+    def rebuild(p01):
+        return np.concatenate([p01, np.conj(np.cross(p01[..., 0, :], p01[..., 1, :]))[..., None, :]], axis=-2)
+
+    U = full(u2).reshape(LT, LZ, LY, LX, 4, 3, 3)
+    G = full(g2).reshape(LT, LZ, LY, LX, 3, 3)
+    K = np.zeros(G.shape, dtype=complex)
+    for mu in range(4):
+        ax = 3 - mu
+        Um = U[..., mu, :, :]
+        gf = np.roll(G, -1, axis=ax)                  # g(x + mu)
+        K += rebuild(Um[..., :2, :] @ np.conj(np.swapaxes(gf, -1, -2)))
+        gb = np.roll(G, 1, axis=ax)                   # g(x - mu)
+        Ub = np.roll(Um, 1, axis=ax)                  # U_mu(x - mu)
+        K += np.conj(np.swapaxes(rebuild(gb[..., :2, :] @ Ub), -1, -2))
+    M = G @ K
+    A = (M - np.conj(np.swapaxes(M, -1, -2))) * (-0.5j)
+    A = A - (np.trace(A, axis1=-2, axis2=-1) / 3.0)[..., None, None] * np.eye(3)
+    return (np.abs(A) ** 2).sum(axis=(-2, -1)).reshape(-1)
+
+
+def test_short_star_packing_bound(orc):
+    """The stop rule of batch lattices on packed links takes the packing's
+    worst effect on Delta with the short star as 139.2 e: the short-form
+    theta on packed-rounded links and the exact theta of the same g differ
+    by less than that in sqrt at every site, on random links and g
+    (e = 2^-38 for high + low parts; 2^-23 for the high parts alone).  The
+    short form on the exact links equals the reference form to rounding."""
+    dims = (4, 4, 4, 6)
+    V = orc.volume(dims)
+    for seed in (5201, 5202):
+        u = orc.random_init_gauge(V, seed)
+        g = orc.random_init_trafo(V, seed + 50)
+        u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+        exact = np_site_theta(orc.g_trafo(u, g, dims), dims).reshape(-1)
+        sh = np_short_theta(u2, g2, dims)
+        assert np.abs(np.sqrt(sh) - np.sqrt(exact)).max() < 1e-12
+        assert abs(sh.max() - orc.functional(u2, g2, dims)[2]) < 1e-12 * sh.max()
+        for lo, e in ((True, 2.0 ** -38), (False, 2.0 ** -23)):
+            shp = np_short_theta(packed_round(u2, lo), g2, dims)
+            d = np.abs(np.sqrt(shp) - np.sqrt(exact)).max()
+            assert d < 139.2 * e, (lo, d / e)
+            assert d > 1e-2 * e  # the rounding is seen
