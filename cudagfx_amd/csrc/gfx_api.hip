@@ -36,6 +36,7 @@ const char* err_text(int code) {
     case GFX_ERR_STATE: return "library not initialised for this call";
     case GFX_ERR_DEVICE: return "HIP runtime error";
     case GFX_ERR_MTDAT: return "MersenneTwister.dat missing or unreadable";
+    case GFX_ERR_SIZE: return "input larger than the static scratch of the entry point";
     default: return "unknown error";
   }
 }

@@ -46,6 +46,7 @@ extern "C" {
 #define GFX_ERR_STATE 600       /* called before init_* or after finalize_*           */
 #define GFX_ERR_DEVICE 700      /* a HIP runtime call failed                          */
 #define GFX_ERR_MTDAT 800       /* MersenneTwister.cu:81-90 (.dat missing or short)   */
+#define GFX_ERR_SIZE 900        /* input larger than a fixed-size device array allows */
 
 /* ---- lattice globals (global.h:77, 86-94), defined by the library ---- */
 extern int T, L, LX, LY, LZ, VOLUME;
@@ -270,6 +271,24 @@ int gfx_batch_anneal(int nsteps, double Tmin, double Tmax, double expo, int nmic
 /* after gfx_batch_run: the lattice with the largest F among those sharing
  * the links of `base` (the Gribov copy to keep) */
 int gfx_batch_best_copy(int base, int *q_best, double *F_best);
+
+/* ================= no allocation (no counterpart in the reference) =================
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  F, dAdA
+ * and max dAdA (calc_functional's three numbers, overrelax.cu:610-671) of
+ * caller-owned device fields in the reference's device layout (2-row
+ * storage, su3manip.c:212-230, 281-299: U_mu(s) at gf[12 (4 s + mu) ..],
+ * g(s) at trafo[12 s ..], 12 doubles each, s lexicographic), written to the
+ * device array result[3] = {F, dAdA, max dAdA} by kernels launched on
+ * `stream` (a hipStream_t; NULL the null stream) without waiting for them.
+ * Nothing is allocated: the reduction's partials live in a static device
+ * array of the library (1.5 MiB per GPU, held until the process ends), so at
+ * most 16,777,216 sites (64^4) are accepted (GFX_ERR_SIZE beyond).  Calls
+ * that overlap on one GPU share that array and race.  Independent of
+ * init_gaugefixing; returns GFX_OK, GFX_ERR_GEOMETRY, GFX_ERR_SIZE or
+ * GFX_ERR_DEVICE. */
+int gfx_calc_functional_dev(const double *gf, const double *trafo, int lx, int ly, int lz, int lt,
+                            double *result, void *stream);
 
 /* last error as text */
 const char *gfx_error_string(int code);
