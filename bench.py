@@ -7,28 +7,43 @@ from the Lehmer RNG with RandSeed = 351, stop at max_x dAdA(x) < 1e-12,
 checked every 10 iterations.
 
 A *step* is one complete gauge-fixing run from that fixed start to the stop
-decision in the mixed path (FP32 12-real sweeps until max dAdA < 1e-6,
-then FP64 reunitarisation and FP64 polish sweeps; both phases read packed
-links, each real as a 24-bit fixed-point high part (the FP32 phase, within
-2^-24) plus a 16-bit low part (the polish, within 2^-38 of the FP64 links);
-F/theta reductions in FP64; the stop confirmed by a functional on the FP64
-links).
-value = site-updates/s of those steps (iterations x V / wall time).  The
-plain FP64 run to the same bound is timed beside it (time-to-theta,
-speedup), with the error of F and of the links of the mixed result against
-it.  The roofline entry is the sweep kernel that dominates the mixed run,
-timed with HIP events on the library's stream; algorithmic bytes per
-site-update are 8 links + 3 g (FP32 g on packed high parts: 432 B, FP64 g
-on packed links: 768 B, FP32 on FP32 links: 528 B, FP64: 1056 B).
+decision in the mixed path: FP32 sweeps until max dAdA < 1e-6 on packed
+links (each real of a link as a 24-bit fixed-point integer, within 2^-24 of
+the FP64 link), then an FP64 reunitarisation of g and FP64 polish sweeps
+with links and g in 32-bit fixed point (within 2^-32; FP64 arithmetic),
+F/theta reductions in FP64, the stop confirmed by a functional on the FP64
+links and FP64 g.  value = site-updates/s of those steps (iterations x V /
+wall time).  The plain FP64 run to the same bound is timed beside it
+(time-to-theta, speedup), with the error of F and of g against it.
+
+roofline: the sweep kernel with the larger share of the mixed step
+(launches x time per launch), timed with HIP events on the library's stream;
+algorithmic bytes per site-update are 8 links + 3 g: FP32 g on 24-bit links
+432 B, fixed-point links and g 528 B, FP32 on FP32 links 528 B, FP64 g on
+24+16-bit links 768 B, FP64 1056 B.  The entry names the autotuned variant
+and takes its HBM traffic and its rocprofv3 mean duration from
+profiles/sweep_variants_32x4.json (made by --profile under rocprofv3 and
+--summarize-profile).
+
+gribov: the 64-copy Gribov workload at its stated schedule, run once: 64
+Haar-random copies of one 24^4 beta = 6.0 configuration, simulated annealing
+from T = 0.45 to 0.01 over 4000 steps of one SA sweep and 3 microcanonical
+sweeps, then mixed OR to theta < 1e-12 and the copy with the largest F.
 
 cpu_baseline: an OpenMP CPU half sweep built from the reference's own host
 SU(3) routines (su3manip.c, complex.c) on its 18-real AoS layout
 (oracle/ref_sweep.c, compiled into oracle/_ref), timed on the host cores on
 a bounded sample of the same 32^4 problem.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-gribov]
+       python bench.py --profile --plan FILE     (kernels for rocprofv3 counters)
+       python bench.py --trace-step --plan FILE  (one mixed and one FP64 step, bracketed)
+       python bench.py --summarize-profile FILE STATS FETCH WRITE
+       python bench.py --summarize-step FILE STATS
 """
 import argparse
+import csv
+import glob
 import json
 import os
 import sys
@@ -50,17 +65,19 @@ CK = 10
 RANDSEED = 351
 MTSEED = 4357
 HBM_PEAK_GBS = 8000.0
-BYTES_FP32 = 8 * 48 + 3 * 48
-BYTES_FP64 = 8 * 96 + 3 * 96
-BYTES_P32 = 8 * 36 + 3 * 48
-BYTES_P64 = 8 * 60 + 3 * 96
-# the roofline kernels by sweep form
-KERNEL_P32 = "k_or_sweep_packed<float4,float>"
-KERNEL_P64 = "k_or_sweep_packed<double2,double>"
-# sweep forms timed: name -> precision argument of time_sweeps (the values of
-# PREC_FP64, PREC_MIXED, SWEEP_PACKED64, SWEEP_PACKED32 in cudagfx_amd)
-FORMS = {"fp64": 0, "fp32": 1, "packed64": 2, "packed32": 3}
-FORM_BYTES = {"fp64": BYTES_FP64, "fp32": BYTES_FP32, "packed64": BYTES_P64, "packed32": BYTES_P32}
+# sweep forms timed: name -> precision argument of time_sweeps (PREC_FP64,
+# PREC_MIXED, SWEEP_PACKED64, SWEEP_PACKED32, SWEEP_FIXED64 in cudagfx_amd)
+FORMS = {"fp64": 0, "fp32": 1, "packed64": 2, "packed32": 3, "fixed64": 4}
+FORM_BYTES = {"fp64": 8 * 96 + 3 * 96, "fp32": 8 * 48 + 3 * 48, "packed64": 8 * 60 + 3 * 96,
+              "packed32": 8 * 36 + 3 * 48, "fixed64": 8 * 48 + 3 * 48}
+FORM_KERNEL = {"fp64": "k_or_sweep<double2, double>", "fp32": "k_or_sweep<float4, float>",
+               "packed64": "k_or_sweep_packed<double2, double>",
+               "packed32": "k_or_sweep_packed<float4, float>", "fixed64": "k_or_sweep<int4, double>"}
+# the two sweep forms of the default mixed Landau run
+MIXED_FORMS = ("packed32", "fixed64")
+PROFILE_JSON = os.path.join(ROOT, "profiles", "sweep_variants_32x4.json")
+GRIBOV = dict(dims=(24, 24, 24, 24), beta=6.0, ntherm=500, copies=64, steps=4000, Tmin=0.01,
+              Tmax=0.45, micro=3, switch=1e-6, mtseed=62)
 
 
 def synthetic_mt_params():
@@ -82,13 +99,13 @@ def synthetic_mt_params():
     return out.ravel()
 
 
-def thermalized_config(gfx):
-    gfx.set_lattice(*DIMS)
+def thermalized_config(gfx, dims=DIMS, beta=BETA, nsweep=None, mtseed=MTSEED):
+    gfx.set_lattice(*dims)
     u = gfx.unit_gauge()
     gfx.init_thermalization(u)
     gfx.set_mt_params(synthetic_mt_params())
-    gfx.set_mt_seed(MTSEED)
-    gfx.heatbath_sweeps(NTHERM, BETA)
+    gfx.set_mt_seed(mtseed)
+    gfx.heatbath_sweeps(NTHERM if nsweep is None else nsweep, beta)
     plaq = gfx.calc_plaquette(0)
     u = gfx.get_gauge()
     gfx.finalize_MT()
@@ -96,13 +113,13 @@ def thermalized_config(gfx):
     return u, plaq
 
 
-def read_traffic(kernel):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
-    path = os.path.join(ROOT, "profiles", "sweep_traffic_32x4.json")
-    if not os.path.exists(path):
+def profiled_variant(name, variant):
+    """Traffic and rocprofv3 duration of one sweep variant from the committed
+    profile, or None."""
+    if not os.path.exists(PROFILE_JSON):
         return None
     try:
-        return json.load(open(path)).get(kernel, {}).get("bytes_per_launch")
+        return json.load(open(PROFILE_JSON))["forms"][name]["variants"].get(str(variant))
     except Exception:
         return None
 
@@ -178,34 +195,290 @@ def align_global(g_a, g_b):
     return float(np.abs(A - np.einsum("ij,njk->nik", h, B)).max())
 
 
-def profile_run(gfx):
-    """The kernels of the timed region in isolation, for counter collection:
-    the autotuner's variant timing for the four sweep forms, then 20 FP64,
-    20 FP32, 20 packed FP64 and 20 packed FP32 sweeps of the picked variants
-    (the last 40 launches of each picked kernel), 5 standalone FP64 checks, a 31-iteration FP64 run with fused
-    checks every 10 iterations, and a plain copy of 512 MiB whose bytes are
-    known."""
-    global NTHERM
-    NTHERM = 20  # counters do not depend on the field values
-    u, _ = thermalized_config(gfx)
-    gfx.plant_seeds(RANDSEED)
+def _marker(gfx):
+    """A small streaming copy (k_stream_copy of 4 KiB): brackets a section
+    in a kernel trace."""
+    gfx.stream_copy(4096, 1)
+    gfx.sync()
+
+
+def _setup_32x4(gfx, nsweep=None):
     V = int(np.prod(DIMS))
+    u, plaq = thermalized_config(gfx, nsweep=nsweep)
+    gfx.plant_seeds(RANDSEED)
     g0 = gfx.random_init_trafo_host(V)
     gfx.init_gaugefixing(u, g0)
     gfx.set_omega(OMEGA)
-    for form in FORMS.values():
+    return u, g0, plaq
+
+
+def profile_run(gfx, plan_path):
+    """The kernels for counter collection, in a fixed order recorded in
+    plan_path: the autotuner's timing of every variant of the five sweep
+    forms, a marker copy, then 10 sweeps (20 launches) of every variant of
+    every form in turn (tiling is data, so the order tells the variants of
+    one kernel apart), a marker copy, 5 standalone FP64 checks, a
+    31-iteration FP64 run with fused checks, the stochastic kernels (5 link
+    heatbath sweeps with their twister refills, 5 SA steps with 3
+    microcanonical sweeps each), and 5 copies of 512 MiB whose bytes are
+    known (the counter calibration)."""
+    u, g0, _ = _setup_32x4(gfx, nsweep=20)  # counters do not depend on the field values
+    plan = {"lattice": list(DIMS), "forms": {}, "sequence": []}
+    for name, form in FORMS.items():
         gfx.time_sweeps(1, form)
-    t = {name: gfx.time_sweeps(20, form) for name, form in FORMS.items()}
+        v, blk, rolled, ms = gfx.sweep_variant(form, 4)
+        plan["forms"][name] = {"picked": v, "tune_ms": [float(x) for x in ms],
+                               "valid": [k for k in range(32) if ms[k] > 0]}
+    _marker(gfx)
+    for name, form in FORMS.items():
+        for v in plan["forms"][name]["valid"]:
+            gfx.set_sweep_variant(form, 4, v)
+            ms = gfx.time_sweeps(10, form)
+            plan["sequence"].append({"form": name, "variant": v, "launches": 20, "hip_event_ms": ms})
+        gfx.set_sweep_variant(form, 4, plan["forms"][name]["picked"])
+    _marker(gfx)
     for _ in range(5):
         gfx.calc_functional()
     gfx.set_trafo(g0)
     gfx.set_precision(gfx.PREC_FP64)
     gfx.overrelax_gauge(31, EPS, CK)
-    tc = gfx.stream_copy(512 << 20, 5)
-    out = {"profile": True, "sweep_ms": t, "copy_512MiB_ms": tc,
-           "variants": {name: list(gfx.sweep_variant(form, 4)[:3]) for name, form in FORMS.items()}}
+    gfx.sync()
+    # stochastic kernels: link heatbath (8 launches per sweep, a twister
+    # refill after each), then SA (an even and an odd heatbath half sweep of
+    # g per step, a refill after each, then the microcanonical sweeps)
     gfx.finalize_gaugefixing()
+    gfx.set_lattice(*DIMS)
+    gfx.init_thermalization(u)
+    gfx.set_mt_params(synthetic_mt_params())
+    gfx.set_mt_seed(MTSEED)
+    gfx.heatbath_sweeps(5, BETA)
+    gfx.finalize_MT()
+    gfx.finalize_thermalization()
+    gfx.set_lattice(*DIMS)
+    gfx.init_gaugefixing(u, g0)
+    gfx.set_mt_params(synthetic_mt_params())
+    gfx.set_mt_seed(MTSEED)
+    sa = gfx.saparam()
+    sa.Tmin, sa.Tmax, sa.expo, sa.N, sa.checkint = 0.01, 0.45, -1.0, 5, 1000
+    gfx.set_sa(0, 3, 4)
+    gfx.simannealing_gauge()
+    gfx.set_sa(0, 0, 4)
+    gfx.finalize_MT()
+    gfx.sync()
+    tc = gfx.stream_copy(512 << 20, 5)
+    gfx.finalize_gaugefixing()
+    plan["copy_512MiB_ms"] = tc
+    plan["stochastic"] = {"heatbath_sweeps": 5, "sa_steps": 5, "sa_micro": 3}
+    with open(plan_path, "w") as f:
+        json.dump(plan, f, indent=1)
+    return {"profile": True, "plan": plan_path}
+
+
+def trace_step(gfx, plan_path):
+    """One mixed step and one FP64 step of the 32^4 workload, each between
+    two marker copies, after the autotuner has run (for a kernel trace: the
+    kernels between the markers are exactly one step)."""
+    u, g0, _ = _setup_32x4(gfx)
+    for form in FORMS.values():
+        gfx.time_sweeps(1, form)
+    walls = {}
+    for name, prec in (("mixed", gfx.PREC_MIXED), ("fp64", gfx.PREC_FP64)):
+        gfx.set_trafo(g0)
+        gfx.set_precision(prec, SWITCH)
+        gfx.set_packed_links(1)
+        gfx.sync()
+        _marker(gfx)
+        t0 = time.perf_counter()
+        it = gfx.overrelax_gauge(20000, EPS, CK)
+        gfx.sync()
+        w = time.perf_counter() - t0
+        _marker(gfx)
+        ms, i32, i64 = gfx.last_run_stats()
+        walls[name] = {"wall_s": w, "lib_ms": ms, "stop": it, "iters_fp32": i32, "iters_fp64": i64}
+    variants = {}
+    for n, p in FORMS.items():
+        sv = gfx.sweep_variant(p, 4)
+        variants[n] = sv[0] if sv else None
+    gfx.finalize_gaugefixing()
+    with open(plan_path, "w") as f:
+        json.dump({"steps": walls, "variants": variants}, f, indent=1)
+    return {"trace_step": True, "plan": plan_path, "steps": walls}
+
+
+# ---------------------------------------------------------------- rocprofv3 summaries
+def _rows(d, suffix):
+    files = sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True))
+    if not files:
+        raise SystemExit(f"no *{suffix} under {d}")
+    rows = []
+    for fn in files:
+        with open(fn) as f:
+            rows += list(csv.DictReader(f))
+    return rows
+
+
+def _dispatches(d):
+    """Kernel dispatches of a --kernel-trace run in launch order: (name, us)."""
+    rows = _rows(d, "kernel_trace.csv")
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    return [(r["Kernel_Name"], (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3, r)
+            for r in rows]
+
+
+def _counter_dispatches(d, counter):
+    """Per dispatch (in launch order) the value of one counter of a --pmc run."""
+    rows = [r for r in _rows(d, "counter_collection.csv") if r["Counter_Name"].startswith(counter)]
+    acc = {}
+    for r in rows:
+        k = int(r["Dispatch_Id"])
+        a = acc.setdefault(k, [r["Kernel_Name"], 0.0])
+        a[1] += float(r["Counter_Value"])
+    return [(acc[k][0], acc[k][1]) for k in sorted(acc)]
+
+
+def _short(name):
+    return name.split("(")[0].replace("void ", "")
+
+
+def summarize_profile(plan_path, stats_dir, fetch_dir, write_dir):
+    plan = json.load(open(plan_path))
+    V = int(np.prod(plan["lattice"]))
+    trace = _dispatches(stats_dir)
+    fetch = _counter_dispatches(fetch_dir, "FETCH_SIZE")
+    write = _counter_dispatches(write_dir, "WRITE_SIZE")
+    out = {"lattice": plan["lattice"], "forms": {}, "stochastic": {}}
+    # calibration: the last 5 copies (512 MiB read and written each)
+    fc = [v for n, v in fetch if "k_stream_copy" in n][-5:]
+    wc = [v for n, v in write if "k_stream_copy" in n][-5:]
+    nbytes = 512 << 20
+    fscale = nbytes / (1024.0 * np.mean(fc))
+    wscale = nbytes / (1024.0 * np.mean(wc))
+    copy_us = [t for n, t, _ in trace if "k_stream_copy" in n][-5:]
+    out["calibration"] = {"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                          "FETCH_SIZE_KiB": float(np.mean(fc)), "WRITE_SIZE_KiB": float(np.mean(wc)),
+                          "fetch_scale": fscale, "write_scale": wscale,
+                          "copy_us": float(np.mean(copy_us)),
+                          "copy_GBs": 2 * nbytes / (np.mean(copy_us) * 1e-6) / 1e9}
+
+    def between_markers(seq, key):
+        m = [i for i, x in enumerate(seq) if "k_stream_copy" in x[0]]
+        return [x for x in seq[m[0] + 1:m[1]] if key(x[0])]
+
+    is_sweep = lambda n: "k_or_sweep" in n
+    tr = between_markers(trace, is_sweep)
+    fe = between_markers(fetch, is_sweep)
+    wr = between_markers(write, is_sweep)
+    k = 0
+    for ent in plan["sequence"]:
+        n = ent["launches"]
+        name, v = ent["form"], ent["variant"]
+        durs = [t for _, t, _ in tr[k:k + n]]
+        f = np.mean([x[1] for x in fe[k:k + n]]) * 1024 * fscale
+        w = np.mean([x[1] for x in wr[k:k + n]]) * 1024 * wscale
+        kern = _short(tr[k][0])
+        alg = (V // 2) * FORM_BYTES[name]
+        us = float(np.mean(durs))
+        out["forms"].setdefault(name, {"picked_on_profile_box": plan["forms"][name]["picked"],
+                                       "algorithmic_bytes_per_launch": alg, "variants": {}})
+        out["forms"][name]["variants"][str(v)] = {
+            "kernel": kern, "tiling": v >> 3, "launches": n, "us_rocprof": us,
+            "us_hip_events": ent["hip_event_ms"] * 1e3,
+            "fetch_bytes": float(f), "write_bytes": float(w), "traffic_bytes": float(f + w),
+            "traffic_over_algorithmic": float((f + w) / alg),
+            "GBs_algorithmic": alg / (us * 1e-6) / 1e9, "frac_of_8TBs": alg / (us * 1e-6) / 8e12}
+        k += n
+    assert k == len(tr) == len(fe) == len(wr), (k, len(tr), len(fe), len(wr))
+    # stochastic kernels and checks (after the second marker, before the calibration copies)
+    m = [i for i, x in enumerate(trace) if "k_stream_copy" in x[0]]
+    mf = [i for i, x in enumerate(fetch) if "k_stream_copy" in x[0]]
+    mw = [i for i, x in enumerate(write) if "k_stream_copy" in x[0]]
+    tail_t = trace[m[1] + 1:m[-5]]
+    tail_f = fetch[mf[1] + 1:mf[-5]]
+    tail_w = write[mw[1] + 1:mw[-5]]
+    assert [_short(x[0]) for x in tail_t] == [_short(x[0]) for x in tail_f] == [_short(x[0]) for x in tail_w]
+    groups = {}
+    for (n, t, _), (_, fv), (_, wv) in zip(tail_t, tail_f, tail_w):
+        g = groups.setdefault(_short(n), {"launches": 0, "us": [], "fetch": [], "write": []})
+        g["launches"] += 1
+        g["us"].append(t)
+        g["fetch"].append(fv * 1024 * fscale)
+        g["write"].append(wv * 1024 * wscale)
+    for n, g in groups.items():
+        us = float(np.mean(g["us"]))
+        fb, wb = float(np.mean(g["fetch"])), float(np.mean(g["write"]))
+        out["stochastic"][n] = {"launches": g["launches"], "us_rocprof": us, "fetch_bytes": fb,
+                                "write_bytes": wb, "traffic_GBs": (fb + wb) / (us * 1e-6) / 1e9}
     return out
+
+
+def summarize_step(plan_path, stats_dir):
+    plan = json.load(open(plan_path))
+    trace = _dispatches(stats_dir)
+    m = [i for i, x in enumerate(trace) if "k_stream_copy" in x[0]]
+    out = {"variants": plan["variants"], "steps": {}}
+    for k, name in enumerate(("mixed", "fp64")):
+        sec = trace[m[2 * k] + 1:m[2 * k + 1]]
+        first, last = sec[0][2], sec[-1][2]
+        span_us = (int(last["End_Timestamp"]) - int(first["Start_Timestamp"])) / 1e3
+        per = {}
+        for n, t, _ in sec:
+            p = per.setdefault(_short(n), [0, 0.0])
+            p[0] += 1
+            p[1] += t
+        busy = sum(p[1] for p in per.values())
+        out["steps"][name] = {"wall_s_host": plan["steps"][name]["wall_s"],
+                              "gpu_span_s": span_us * 1e-6, "kernel_time_s": busy * 1e-6,
+                              "stop": plan["steps"][name]["stop"],
+                              "iters_fp32": plan["steps"][name]["iters_fp32"],
+                              "iters_fp64": plan["steps"][name]["iters_fp64"],
+                              "kernels": {n: {"launches": p[0], "total_ms": p[1] / 1e3,
+                                              "mean_us": p[1] / p[0], "share": p[1] / busy}
+                                          for n, p in sorted(per.items(), key=lambda kv: -kv[1][1])}}
+    return out
+
+
+# ---------------------------------------------------------------- the bench line
+def gribov_run(gfx):
+    """The Gribov-copy workload at its stated schedule, once."""
+    G = GRIBOV
+    t0 = time.perf_counter()
+    u, plaq = thermalized_config(gfx, G["dims"], G["beta"], G["ntherm"], G["mtseed"])
+    gfx.set_omega(OMEGA)
+    gfx.set_mt_params(synthetic_mt_params())
+    gfx.set_mt_seed(G["mtseed"])
+    b = gfx.Batch()
+    base = b.add(G["dims"], u, gfx.unit_trafo())
+    for _ in range(G["copies"] - 1):
+        b.add_random_copy(base)
+    gfx.sync()
+    setup = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    b.anneal(G["steps"], G["Tmin"], G["Tmax"], -1.0, G["micro"], 0, steps_per_graph=20)
+    gfx.sync()
+    t_sa = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    rc = b.run(20000, EPS, CK, gfx.PREC_MIXED, G["switch"], 8)
+    gfx.sync()
+    t_or = time.perf_counter() - t2
+    res = [b.result(q) for q in range(G["copies"])]
+    st = b.stats()
+    qb, Fb = b.best_copy(base)
+    b.close()
+    gfx.finalize_MT()
+    V = int(np.prod(G["dims"]))
+    stops = [r[3] for r in res]
+    return {"workload": "64 Haar-random copies of one 24^4 beta=6.0 configuration (500 heatbath "
+                        "sweeps), SA T=0.45->0.01 over 4000 steps of 1 SA + 3 microcanonical sweeps, "
+                        "mixed OR (switch 1e-6) to theta<1e-12, best copy by F",
+            "rc": rc, "setup_s": setup, "sa_s": t_sa, "sa_ms_per_step": 1e3 * t_sa / G["steps"],
+            "sa_site_sweeps_per_s": G["copies"] * V * 4 * G["steps"] / t_sa,
+            "or_s": t_or, "or_site_updates": st["site_updates"],
+            "or_site_updates_per_s": st["site_updates"] / t_or,
+            "iters_fp32": st["iters_fp32"], "iters_fp64": st["iters_fp64"],
+            "stop_min": int(min(stops)), "stop_max": int(max(stops)), "stop_mean": float(np.mean(stops)),
+            "converged": int(sum(1 for r in res if r[2] < EPS and r[3] >= 0)),
+            "best_copy": qb, "best_F": Fb, "F_min": float(min(r[0] for r in res)),
+            "plaquette": plaq}
 
 
 def measure(args):
@@ -213,10 +486,12 @@ def measure(args):
 
     gfx.load()
     gfx.set_verbose(0)
-    assert FORMS == {"fp64": gfx.PREC_FP64, "fp32": gfx.PREC_MIXED,
-                     "packed64": gfx.SWEEP_PACKED64, "packed32": gfx.SWEEP_PACKED32}
+    assert FORMS == {"fp64": gfx.PREC_FP64, "fp32": gfx.PREC_MIXED, "packed64": gfx.SWEEP_PACKED64,
+                     "packed32": gfx.SWEEP_PACKED32, "fixed64": gfx.SWEEP_FIXED64}
     if args.profile:
-        return profile_run(gfx)
+        return profile_run(gfx, args.plan)
+    if args.trace_step:
+        return trace_step(gfx, args.plan)
     V = int(np.prod(DIMS))
     t_setup = time.perf_counter()
     u, plaq = thermalized_config(gfx)
@@ -272,23 +547,37 @@ def measure(args):
     gfx.set_trafo(g0)
     tk = {name: gfx.time_sweeps(50, form) for name, form in FORMS.items()}
     gbs = {name: (V // 2) * FORM_BYTES[name] / (tk[name] * 1e-3) / 1e9 for name in FORMS}
-    # the sweep kernel with the larger share of the mixed step
-    dom32 = last["it32"] * tk["packed32"] >= last["it64"] * tk["packed64"]
-    name = "packed32" if dom32 else "packed64"
-    kernel = KERNEL_P32 if dom32 else KERNEL_P64
-    roof = {"bound": "hbm", "achieved": gbs[name], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": gbs[name] / HBM_PEAK_GBS, "traffic": read_traffic(kernel), "kernel": kernel,
-            "ms_per_launch": tk[name],
-            "algorithmic_bytes_per_launch": (V // 2) * FORM_BYTES[name]}
-
-    link_err = align_global(g_mixed, g_fp64)
     variants = {}
     for name, prec in FORMS.items():
         sv = gfx.sweep_variant(prec, 4)
         if sv is not None:
             variants[name] = {"variant": sv[0], "block": sv[1], "rolled": sv[2],
                               "ms_per_variant": [round(x, 5) for x in sv[3]]}
+    # the sweep kernel with the larger share of the mixed step
+    share = {"packed32": 2 * last["it32"] * tk["packed32"], "fixed64": 2 * last["it64"] * tk["fixed64"]}
+    name = max(share, key=share.get)
+    v = variants[name]["variant"]
+    prof = profiled_variant(name, v)
+    roof = {"bound": "hbm", "achieved": gbs[name], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs[name] / HBM_PEAK_GBS,
+            "traffic": prof["traffic_bytes"] if prof else None,
+            "kernel": FORM_KERNEL[name], "form": name, "variant": v,
+            "ms_per_launch_hip_events": tk[name],
+            "ms_per_launch_rocprof_profile": prof["us_rocprof"] / 1e3 if prof else None,
+            "frac_rocprof_profile": prof["frac_of_8TBs"] if prof else None,
+            "traffic_over_algorithmic": prof["traffic_over_algorithmic"] if prof else None,
+            "algorithmic_bytes_per_launch": (V // 2) * FORM_BYTES[name],
+            "bytes_per_site_update": FORM_BYTES[name],
+            "step_share_ms": {k: x for k, x in share.items()}}
+    link_err = align_global(g_mixed, g_fp64)
     gfx.finalize_gaugefixing()
+
+    gribov = None
+    if not args.no_gribov:
+        try:
+            gribov = gribov_run(gfx)
+        except Exception as e:  # reported, the Landau line stands on its own
+            gribov = {"error": str(e)}
 
     cpu = None
     if not args.no_cpu:
@@ -308,13 +597,17 @@ def measure(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 sweeps + f64 reductions/polish",
+        "dtype": "f32 sweeps on 24-bit fixed-point links, then f64 arithmetic on 32-bit "
+                 "fixed-point links and g; f64 reductions and stop check on the f64 links",
         "data": "synthetic: cold start + 500 GPU heatbath sweeps at beta=6.0 (twister params "
                 "generated, srand(4357) seeds), Lehmer RandSeed=351 initial g",
         "config": {"workload": "Landau gauge fixing 32^4 beta=6.0, omega=1.7, theta<1e-12, ck=10",
-                   "lattice": list(DIMS), "site_updates_per_step": iters * V},
+                   "lattice": list(DIMS), "site_updates_per_step": iters * V,
+                   "link_formats": {"fp32_phase": "24-bit fixed point per real (36 B per link)",
+                                    "polish": "32-bit fixed point per real, links and g (48 B per matrix)"}},
         "time_to_theta_s": {"mixed": ms_step / 1000.0, "fp64": r64["wall_s"],
-                            "speedup": r64["wall_s"] / (ms_step / 1000.0)},
+                            "speedup": r64["wall_s"] / (ms_step / 1000.0),
+                            "mixed_median": float(np.median(walls))},
         "mixed_switch_1e-10": {"time_s": tuned["wall_s"], "speedup": r64["wall_s"] / tuned["wall_s"],
                                "iters_fp32": tuned["it32"], "iters_fp64": tuned["it64"],
                                "stop": tuned["iter"], "theta": tuned["max"],
@@ -326,8 +619,11 @@ def measure(args):
                                      "F_rel": abs(plain_links["F"] - r64["F"]) / r64["F"]},
         "iterations": {"mixed_fp32": last["it32"], "mixed_fp64": last["it64"],
                        "fp64": r64["it64"], "stop_mixed": last["iter"], "stop_fp64": r64["iter"]},
-        "sweep_kernels": {name: {"ms": tk[name], "GBs": gbs[name], "bytes_per_site_update": FORM_BYTES[name],
-                                 "site_updates_per_s": (V // 2) / (tk[name] * 1e-3)} for name in FORMS},
+        "sweep_kernels": {n: {"kernel": FORM_KERNEL[n], "ms": tk[n], "GBs": gbs[n],
+                              "frac": gbs[n] / HBM_PEAK_GBS,
+                              "bytes_per_site_update": FORM_BYTES[n],
+                              "site_updates_per_s": (V // 2) / (tk[n] * 1e-3),
+                              "variant": variants.get(n, {}).get("variant")} for n in FORMS},
         "accuracy_mixed_vs_fp64": {"F_rel": abs(last["F"] - r64["F"]) / r64["F"],
                                    "theta_mixed": last["max"], "theta_fp64": r64["max"],
                                    "dAdA_mixed": last["dAdA"], "dAdA_fp64": r64["dAdA"],
@@ -341,6 +637,7 @@ def measure(args):
         "plaquette": plaq,
         "setup_s": setup_s,
         "roofline": roof,
+        "gribov": gribov,
         "cpu_baseline": cpu,
     }
     return out
@@ -373,10 +670,22 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-gribov", action="store_true")
     ap.add_argument("--profile", action="store_true",
-                    help="short run for rocprofv3: autotuning, 20 sweeps of each form, "
-                         "checks, and a 512 MiB streaming copy as the byte-counter reference")
+                    help="kernels for rocprofv3 counters: every sweep variant, checks, stochastic "
+                         "kernels, and 512 MiB streaming copies as the byte-counter reference")
+    ap.add_argument("--trace-step", action="store_true",
+                    help="one mixed and one FP64 step between marker copies, for a kernel trace")
+    ap.add_argument("--plan", default=os.path.join(ROOT, "gpurun_out", "profile_plan.json"))
+    ap.add_argument("--summarize-profile", nargs=4, metavar=("PLAN", "STATS", "FETCH", "WRITE"))
+    ap.add_argument("--summarize-step", nargs=2, metavar=("PLAN", "STATS"))
     args = ap.parse_args()
+    if args.summarize_profile:
+        print(json.dumps(summarize_profile(*args.summarize_profile), indent=1))
+        return
+    if args.summarize_step:
+        print(json.dumps(summarize_step(*args.summarize_step), indent=1))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; the path would shard by "
                           "time-slab domain decomposition across GPUs, which is not built"}))

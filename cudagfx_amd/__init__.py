@@ -30,6 +30,8 @@ PREC_MIXED = 1
 # phase on high parts
 SWEEP_PACKED64 = 2
 SWEEP_PACKED32 = 3
+# the FP64 polish of the mixed Landau run on 32-bit fixed-point links and g
+SWEEP_FIXED64 = 4
 GAUGE_LANDAU = 0
 GAUGE_COULOMB = 1
 

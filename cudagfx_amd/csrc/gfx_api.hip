@@ -228,6 +228,7 @@ int ensure_f32_links(State& s) {
   launch_f64_to_f32(s.U64, s.U32, s.geo, 8, s.stream);
   s.U32_valid = true;
   s.packed_valid = false;
+  s.fx_valid = false;
   return GFX_OK;
 }
 
@@ -252,6 +253,23 @@ int ensure_packed_links(State& s) {
   return GFX_OK;
 }
 
+size_t field_bytes_fx(const Geom& g) { return (size_t)3 * g.stride * sizeof(int4); }
+
+// the 32-bit fixed-point links and g of the mixed Landau run's polish;
+// the links share the validity of the FP32 fields
+int ensure_fx_links(State& s) {
+  int rc = ensure_f32_links(s);
+  if (rc) return rc;
+  if (s.fx_valid) return GFX_OK;
+  if (!s.Ufx) {
+    if (dev_alloc((void**)&s.Ufx, 8 * field_bytes_fx(s.geo), "fixed-point links")) return GFX_ERR_ALLOC;
+    if (dev_alloc((void**)&s.gfx, 4 * field_bytes_fx(s.geo), "fixed-point trafo")) return GFX_ERR_ALLOC;
+  }
+  launch_f64_to_fx(s.U64, s.Ufx, s.geo, 8, s.stream);
+  s.fx_valid = true;
+  return GFX_OK;
+}
+
 void free_all(State& s) {
   if (s.stream) hipStreamSynchronize(s.stream);
   hipFree(s.U64);
@@ -261,6 +279,11 @@ void free_all(State& s) {
   hipFree(s.pk.a);
   s.pk = PackedArgs{};
   s.packed_valid = false;
+  hipFree(s.Ufx);
+  hipFree(s.gfx);
+  s.Ufx = nullptr;
+  s.gfx = nullptr;
+  s.fx_valid = false;
   hipFree(s.partials);
   hipFree(s.cpart);
   for (int bi = 0; bi < 2; bi++)
@@ -333,12 +356,13 @@ const int* sweep_map(State& s, int v) {
 
 double2* field64(State& s, int k) { return s.g64 + (size_t)k * 6 * s.geo.stride; }
 float4* field32(State& s, int k) { return s.g32 + (size_t)k * 3 * s.geo.stride; }
+int4* fieldfx(State& s, int k) { return s.gfx + (size_t)k * 3 * s.geo.stride; }
 
 // the form of the next sweep: FP32 or FP64, on the packed links while a
 // mixed run reads them
 int sweep_form(const State& s, bool fp32, int ndir) {
   (void)ndir;
-  if (s.packed_on) return fp32 ? FORM_P32 : FORM_P64;
+  if (s.packed_on) return fp32 ? FORM_P32 : s.fx_on ? FORM_X64 : FORM_P64;
   return fp32 ? FORM_F32 : FORM_F64;
 }
 
@@ -370,6 +394,11 @@ void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* ac
       launch_sweep_packed_f32(s.pk, field32(s, even), field32(s, odd), s.geo, parity, ndir,
                               s.omega, active, v, map, s.stream, chk, alt, part, nb);
   } else {
+    if (form == FORM_X64) {
+      launch_sweep_fx(s.Ufx, fieldfx(s, even), fieldfx(s, odd), s.geo, parity, s.omega, v, map,
+                      s.stream, chk, out < 0 ? nullptr : fieldfx(s, out), part, nb);
+      return;
+    }
     double2* alt = out < 0 ? nullptr : field64(s, out);
     if (form == FORM_P64)
       launch_sweep_packed_f64(s.pk, field64(s, even), field64(s, odd), s.geo, parity, ndir,
@@ -412,9 +441,11 @@ static void build_slice_map(State& s, bool fp32, const std::vector<unsigned char
 }
 
 // bring the even field back to slot 0 (the odd field to slot 1)
-static void canonical_slot(State& s, bool fp32, int slot, int home) {
+static void canonical_slot(State& s, bool fp32, int slot, int home, bool fx = false) {
   if (slot == home) return;
-  if (fp32)
+  if (fx)
+    hipMemcpyAsync(fieldfx(s, home), fieldfx(s, slot), field_bytes_fx(s.geo), hipMemcpyDeviceToDevice, s.stream);
+  else if (fp32)
     hipMemcpyAsync(field32(s, home), field32(s, slot), field_bytes32(s.geo), hipMemcpyDeviceToDevice, s.stream);
   else
     hipMemcpyAsync(field64(s, home), field64(s, slot), field_bytes64(s.geo), hipMemcpyDeviceToDevice, s.stream);
@@ -428,17 +459,23 @@ int tune_sweep(State& s, int form, int ndir) {
   const int pi = form, di = ndir == 4 ? 1 : 0;
   if (!s.autotune || s.tuned[pi][di]) return GFX_OK;
   const bool fp32 = form == FORM_F32 || form == FORM_P32;
-  const size_t bytes = 2 * (fp32 ? field_bytes32(s.geo) : field_bytes64(s.geo));
+  const size_t bytes = 2 * (form == FORM_X64 ? field_bytes_fx(s.geo)
+                            : fp32           ? field_bytes32(s.geo)
+                                             : field_bytes64(s.geo));
   void* scratch = nullptr;
   if (hipMalloc(&scratch, bytes) != hipSuccess) {
     s.tuned[pi][di] = true;  // keep the default variant
     return GFX_OK;
   }
-  hipMemcpyAsync(scratch, fp32 ? (void*)s.g32 : (void*)s.g64, bytes, hipMemcpyDeviceToDevice, s.stream);
+  hipMemcpyAsync(scratch, form == FORM_X64 ? (void*)s.gfx : fp32 ? (void*)s.g32 : (void*)s.g64, bytes,
+                 hipMemcpyDeviceToDevice, s.stream);
   float4* sc32 = static_cast<float4*>(scratch);
   double2* sc64 = static_cast<double2*>(scratch);
+  int4* scfx = static_cast<int4*>(scratch);
   auto sweep = [&](int v, const int* map, int p) {
-    if (form == FORM_F32)
+    if (form == FORM_X64)
+      launch_sweep_fx(s.Ufx, scfx, scfx + 3 * (size_t)s.geo.stride, s.geo, p, s.omega, v, map, s.stream);
+    else if (form == FORM_F32)
       launch_sweep_f32(s.U32, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
     else if (form == FORM_P32)
       launch_sweep_packed_f32(s.pk, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
@@ -451,7 +488,7 @@ int tune_sweep(State& s, int form, int ndir) {
   int bestv = s.variant[pi][di];
   for (int v = 0; v < SWEEP_VARIANTS; v++) {
     if (!s.tiling_ok[block_index(sweep_variant_block(v))][sweep_variant_tiling(v)]) continue;
-    if (form == FORM_P64 && !sweep_variant_rolled(v)) continue;
+    if ((form == FORM_P64 || form == FORM_X64) && !sweep_variant_rolled(v)) continue;
     const int* map = sweep_map(s, v);
     for (int rep = 0; rep < 2; rep++)
       for (int p = 0; p < 2; p++) sweep(v, map, p);
@@ -604,21 +641,35 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
   if (checkinterval < 1) checkinterval = 1;
   bool fp32 = (s.precision == GFX_PREC_MIXED);
   const bool packed = fp32 && s.packed_links;
+  // the polish of a run on packed links holds links and g in 32-bit fixed
+  // point, for eps well above what that resolves (below it: high + low parts
+  // and FP64 g)
+  const bool fx_polish = packed && eps > FX_EPS_MIN;
   // every exit leaves the packed forms off (gfx_halfsweep and the other
   // entry points sweep on the FP32 and FP64 links)
   struct PackedOff {
     State& s;
-    ~PackedOff() { s.packed_on = false; }
+    ~PackedOff() {
+      s.packed_on = false;
+      s.fx_on = false;
+    }
   } packed_off{s};
   s.packed_on = false;
+  s.fx_on = false;
   if (fp32) {
     if (ensure_f32_links(s)) return -1;
     launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
   }
   if (packed) {
     if (ensure_packed_links(s)) return -1;
+    if (fx_polish && ensure_fx_links(s)) return -1;
     tune_sweep(s, FORM_P32, 4);
-    tune_sweep(s, FORM_P64, 4);
+    if (fx_polish) {
+      launch_f64_to_fx(s.g64, s.gfx, s.geo, 2, s.stream);
+      tune_sweep(s, FORM_X64, 4);
+    } else {
+      tune_sweep(s, FORM_P64, 4);
+    }
   } else if (fp32) {
     tune_sweep(s, FORM_F32, 4);
   }
@@ -638,18 +689,32 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
   int i;
   hipStreamSynchronize(s.stream);
   const auto t0 = std::chrono::steady_clock::now();
-  // state i (the checked state) back into slots 0 and 1
-  auto canonical = [&](bool f32) {
-    canonical_even(s, f32, even);
-    canonical_slot(s, f32, odd, 1);
+  // the polish's g is in the fixed-point fields (gfx) while fx
+  bool fx = false;
+  // state i (the checked state) back into slots 0 and 1 of the fields the
+  // run sweeps now
+  auto canonical = [&]() {
+    canonical_slot(s, fp32, even, 0, fx);
+    canonical_slot(s, fp32, odd, 1, fx);
     even = 0;
     odd = 1;
   };
+  // state i into the FP64 fields g64 (reunitarised in FP64 when it comes
+  // from FP32 or fixed-point g), the run's sweeps then plain FP64
   auto to_fp64 = [&]() {
-    // leave FP32 at the checked state: reunitarise g in FP64
-    canonical(true);
-    launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
-    fp32 = false;
+    canonical();
+    if (fp32)
+      launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
+    else if (fx)
+      launch_fx_to_g64(s.gfx, s.g64, s.geo, s.stream);
+    fp32 = fx = s.fx_on = false;
+  };
+  // the FP64 state in g64 into the fields of the polish
+  auto to_polish = [&]() {
+    if (fx_polish) {
+      launch_f64_to_fx(s.g64, s.gfx, s.geo, 2, s.stream);
+      fx = s.fx_on = true;
+    }
   };
   for (i = 0; i < maxit; i++) {
     const bool chk = (i % checkinterval) == 0;
@@ -685,10 +750,7 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
         // with an FP64 reunitarisation if it is an FP32 state) with the
         // per-link functional
         const bool from32 = fp32;
-        if (fp32)
-          to_fp64();
-        else
-          canonical(false);
+        to_fp64();
         if (functional_now(s, false, 4, 1, eps)) return -1;
         FUNC = s.h_results[0];
         DADA = s.h_results[1];
@@ -699,7 +761,10 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
         if (!(maxdada < eps)) {
           // not there on the FP64 links: iteration i+1 onwards in FP64, the
           // polish form after an FP32 stop, plain FP64 after a polish stop
-          if (!from32) s.packed_on = false;
+          if (from32)
+            to_polish();
+          else
+            s.packed_on = false;
           continue;
         }
       }
@@ -715,7 +780,15 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
         stalled = true;
       }
       if (maxdada < s.switch_theta || stalled) {
-        to_fp64();
+        // leave FP32 at the checked state: g reunitarised in FP64
+        if (fx_polish) {
+          canonical();
+          launch_g32_to_fx(s.g32, s.gfx, s.geo, s.stream);
+          fp32 = false;
+          fx = s.fx_on = true;
+        } else {
+          to_fp64();
+        }
         continue;
       }
     }
@@ -725,9 +798,8 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
       even_done = odd_done = true;
     }
   }
-  if (fp32) to_fp64();
+  to_fp64();
   s.packed_on = false;
-  canonical(false);
   stop_clock_ms(s, t0);
   if (converged) {
     if (s.verbose) {
@@ -990,6 +1062,7 @@ static int form_of_precision(int precision) {
   if (precision == GFX_PREC_MIXED) return FORM_F32;
   if (precision == GFX_SWEEP_PACKED64) return FORM_P64;
   if (precision == GFX_SWEEP_PACKED32) return FORM_P32;
+  if (precision == GFX_SWEEP_FIXED64) return FORM_X64;
   return FORM_F64;
 }
 
@@ -998,22 +1071,30 @@ int gfx_time_sweeps(int nsweeps, int precision, double* ms) {
   if (!s.init) return GFX_ERR_STATE;
   const int form = form_of_precision(precision);
   const bool fp32 = form == FORM_F32 || form == FORM_P32;
-  const bool packed = form == FORM_P32 || form == FORM_P64;
-  int rc = packed ? ensure_packed_links(s) : fp32 ? ensure_f32_links(s) : GFX_OK;
+  const bool fx = form == FORM_X64;
+  const bool packed = form == FORM_P32 || form == FORM_P64 || fx;
+  int rc = fx ? ensure_fx_links(s) : packed ? ensure_packed_links(s) : fp32 ? ensure_f32_links(s) : GFX_OK;
   if (rc) return rc;
   if (fp32) launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
+  if (fx) launch_f64_to_fx(s.g64, s.gfx, s.geo, 2, s.stream);
   tune_sweep(s, form, 4);
   s.packed_on = packed;
+  s.fx_on = fx;
   hipEventRecord(s.ev0, s.stream);
   for (int k = 0; k < nsweeps; k++)
     for (int p = 0; p < 2; p++) do_sweep(s, fp32, p, 4, nullptr);
   hipEventRecord(s.ev1, s.stream);
   s.packed_on = false;
+  s.fx_on = false;
   if (hip_check(hipEventSynchronize(s.ev1), "timing")) return GFX_ERR_DEVICE;
   float t = 0.f;
   hipEventElapsedTime(&t, s.ev0, s.ev1);
   if (fp32) {
     launch_g32_to_g64(s.g32, s.g64, s.geo, s.stream);
+    hipStreamSynchronize(s.stream);
+  }
+  if (fx) {
+    launch_fx_to_g64(s.gfx, s.g64, s.geo, s.stream);
     hipStreamSynchronize(s.stream);
   }
   *ms = (double)t / (2.0 * nsweeps);

@@ -186,10 +186,11 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
   M3<R> h = G;
   cm_relax(h, K);
   m_normalize(h);
-  // the polish on packed links applies the series in the short form too: its
-  // star and relaxation keep the reference's order, which is what pins the
-  // packing to the oracle; the series differs from it by rounding only
-  if constexpr (SHORT || std::is_same<LK, PackedLinks>::value) {
+  // the polish on packed or fixed-point links applies the series in the
+  // short form too: its star and relaxation keep the reference's order,
+  // which is what pins the packing to the oracle; the series differs from
+  // it by rounding only
+  if constexpr (SHORT || std::is_same<LK, PackedLinks>::value || std::is_same<V, int4>::value) {
     overrelax_apply_short(G, h, c1, c2);
   } else {
     overrelax_apply(G, h, c1, c2);
@@ -798,6 +799,21 @@ __global__ __launch_bounds__(BLK) void k_g32_to_g64_range(const float4* __restri
   Store<double2>::store(dst, stride, i, M);
 }
 
+// nfield fields between storage forms, matrix by matrix (grid.y = field):
+// FP64 -> 32-bit fixed point (links and g of the fixed-point polish), and
+// back with an FP64 Gram-Schmidt reunitarisation; FP32 g -> fixed point with
+// the same reunitarisation (the FP32 phase handing over to the polish)
+template <typename VS, typename VD, bool NORM>
+__global__ __launch_bounds__(BLK) void k_convert(const VS* __restrict__ src, VD* __restrict__ dst,
+                                                 int nh, int stride) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  const int f = blockIdx.y;
+  if (i >= nh) return;
+  M3<double> M = load_m<VS, double>(field_ptr(src, f, stride), stride, i);
+  if (NORM) m_normalize(M);
+  Store<VD>::store(field_ptr(dst, f, stride), stride, i, M);
+}
+
 // ------------------------------------------------------------ launchers
 static inline dim3 grid1(long n) { return dim3((unsigned)((n + BLK - 1) / BLK)); }
 
@@ -953,6 +969,33 @@ void launch_sweep_packed_f64(const PackedArgs& p, double2* ge, double2* go, cons
                              double* partials, int nblocks) {
   launch_packed_t<double2, double>(p, {nullptr, ge, go, galt, geo, parity, active, bmap, nblocks, partials},
                                    ndir, omega, variant, chk, st);
+}
+
+// the FP64 polish on 32-bit fixed-point links and g (Landau): rolled
+// variants only (bit 1 of the variant ignored; unrolled, the FP64 star
+// spills)
+template <int CHK>
+static void launch_fx_chk(const SweepArgs<int4>& a, double c1, double c2, int variant,
+                          hipStream_t st) {
+  switch (variant & 5) {
+    case 0: launch_sweep_v<int4, double, 4, 256, true, false, CHK>(a, c1, c2, st); break;
+    case 1: launch_sweep_v<int4, double, 4, 64, true, false, CHK>(a, c1, c2, st); break;
+    case 4: launch_sweep_v<int4, double, 4, 256, true, true, CHK>(a, c1, c2, st); break;
+    default: launch_sweep_v<int4, double, 4, 64, true, true, CHK>(a, c1, c2, st); break;
+  }
+}
+void launch_sweep_fx(const int4* U, int4* ge, int4* go, const Geom& geo, int parity, double omega,
+                     int variant, const int* bmap, hipStream_t st, int chk, int4* galt,
+                     double* partials, int nblocks) {
+  const double c1 = (1.0 + omega - 1) / 1;
+  const double c2 = (1.0 + omega - 2) / 2;
+  const SweepArgs<int4> a{U, ge, go, galt, geo, parity, nullptr, bmap, nblocks, partials};
+  if (chk == 1)
+    launch_fx_chk<1>(a, c1, c2, variant, st);
+  else if (chk == 2)
+    launch_fx_chk<2>(a, c1, c2, variant, st);
+  else
+    launch_fx_chk<0>(a, c1, c2, variant, st);
 }
 
 int sweep_blocks(int nh, int variant) {
@@ -1144,6 +1187,21 @@ void launch_pack_links(const double2* src, const PackedArgs& p, const Geom& geo,
   dim3 grid((geo.nh + BLK - 1) / BLK, nfield);
   hipLaunchKernelGGL(k_pack_links, grid, dim3(BLK), 0, st, src, (uint4*)p.a, (uint4*)p.b,
                      (uint32_t*)p.c, (uint2*)p.lo, geo.nh, geo.stride, nfield);
+}
+void launch_f64_to_fx(const double2* src, int4* dst, const Geom& geo, int nfield, hipStream_t st) {
+  dim3 grid((geo.nh + BLK - 1) / BLK, nfield);
+  hipLaunchKernelGGL((k_convert<double2, int4, false>), grid, dim3(BLK), 0, st, src, dst, geo.nh,
+                     geo.stride);
+}
+void launch_fx_to_g64(const int4* src, double2* dst, const Geom& geo, hipStream_t st) {
+  dim3 grid((geo.nh + BLK - 1) / BLK, 2);
+  hipLaunchKernelGGL((k_convert<int4, double2, true>), grid, dim3(BLK), 0, st, src, dst, geo.nh,
+                     geo.stride);
+}
+void launch_g32_to_fx(const float4* src, int4* dst, const Geom& geo, hipStream_t st) {
+  dim3 grid((geo.nh + BLK - 1) / BLK, 2);
+  hipLaunchKernelGGL((k_convert<float4, int4, true>), grid, dim3(BLK), 0, st, src, dst, geo.nh,
+                     geo.stride);
 }
 void launch_g32_to_g64(const float4* src, double2* dst, const Geom& geo, hipStream_t st) {
   dim3 grid((geo.nh + BLK - 1) / BLK, 2);

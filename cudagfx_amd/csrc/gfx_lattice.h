@@ -178,6 +178,67 @@ struct Store<float4> {
   }
 };
 
+// ---- 32-bit fixed-point fields (links and g of the mixed Landau run's
+// FP64 polish) ----
+// The elements u of a unitary matrix lie in [-1, 1]: each real is held as
+// the int32 q = rint(u 2^31), clamped to +-(2^31 - 1), in 3 planes of int4
+// in the order of the FP32 planes, (u00,u01)(u02,u10)(u11,u12) as (re, im)
+// pairs, 48 bytes per matrix.  q 2^-31 is within 2^-32 of u (2^-31 where
+// |u| rounds to 1 and q is clamped).  The arithmetic on them is FP64.
+constexpr double FX_SCALE = 0x1p31;
+// A polish on fixed-point links and g is used for stop bounds eps above
+// this: the rounding of g after every update and of the links (2^-32 per
+// real) moves Delta by about 1e-9, so max dAdA cannot go much below 1e-17.
+constexpr double FX_EPS_MIN = 1e-15;
+__device__ __forceinline__ int fx_of(double u) {
+  return (int)fmin(fmax(rint(u * FX_SCALE), -2147483647.0), 2147483647.0);
+}
+typedef int nat_i4 __attribute__((ext_vector_type(4)));
+template <bool NT>
+__device__ __forceinline__ int4 ld16(const int4* p) {
+  if (NT) {
+    const nat_i4 v = __builtin_nontemporal_load(reinterpret_cast<const nat_i4*>(p));
+    return make_int4(v.x, v.y, v.z, v.w);
+  }
+  return *p;
+}
+template <typename R>
+__device__ __forceinline__ void fx_rows(const int4& a, const int4& b, const int4& c, Cx<R> r0[3],
+                                        Cx<R> r1[3]) {
+  const R sc = R(1.0 / FX_SCALE);
+  r0[0] = cx<R>((R)a.x * sc, (R)a.y * sc);
+  r0[1] = cx<R>((R)a.z * sc, (R)a.w * sc);
+  r0[2] = cx<R>((R)b.x * sc, (R)b.y * sc);
+  r1[0] = cx<R>((R)b.z * sc, (R)b.w * sc);
+  r1[1] = cx<R>((R)c.x * sc, (R)c.y * sc);
+  r1[2] = cx<R>((R)c.z * sc, (R)c.w * sc);
+}
+
+template <>
+struct Store<int4> {
+  static constexpr int NPLANE = 3;
+  template <typename R, bool NT = false>
+  __device__ __forceinline__ static void load(const int4* __restrict__ f, int stride, int i,
+                                              Cx<R> r0[3], Cx<R> r1[3]) {
+    const int4 a = ld16<NT>(f + (unsigned)i);
+    const int4 b = ld16<NT>(f + (size_t)stride + (unsigned)i);
+    const int4 c = ld16<NT>(f + (size_t)2 * stride + (unsigned)i);
+    fx_rows<R>(a, b, c, r0, r1);
+  }
+  template <typename R>
+  __device__ __forceinline__ static void store(int4* __restrict__ f, int stride, int i,
+                                               const M3<R>& m) {
+    f[(unsigned)i] = make_int4(fx_of((double)m.e[0][0].re), fx_of((double)m.e[0][0].im),
+                               fx_of((double)m.e[0][1].re), fx_of((double)m.e[0][1].im));
+    (f + (size_t)stride)[(unsigned)i] =
+        make_int4(fx_of((double)m.e[0][2].re), fx_of((double)m.e[0][2].im),
+                  fx_of((double)m.e[1][0].re), fx_of((double)m.e[1][0].im));
+    (f + (size_t)2 * stride)[(unsigned)i] =
+        make_int4(fx_of((double)m.e[1][1].re), fx_of((double)m.e[1][1].im),
+                  fx_of((double)m.e[1][2].re), fx_of((double)m.e[1][2].im));
+  }
+};
+
 template <typename V, typename R, bool NT = false>
 __device__ __forceinline__ M3<R> load_m(const V* f, int stride, int i) {
   M3<R> m;
@@ -313,6 +374,10 @@ template <typename R>
 struct Raw<const float4*, R> {
   float4 v[3];
 };
+template <typename R>
+struct Raw<const int4*, R> {
+  int4 v[3];
+};
 template <>
 struct Raw<PackedLinks, double> {
   uint4 a, b;
@@ -332,6 +397,11 @@ __device__ __forceinline__ void raw_load(const double2* f, int stride, int i, Ra
 }
 template <typename R, bool NT>
 __device__ __forceinline__ void raw_load(const float4* f, int stride, int i, Raw<const float4*, R>& r) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) r.v[c] = ld16<NT>(f + (size_t)c * stride + (unsigned)i);
+}
+template <typename R, bool NT>
+__device__ __forceinline__ void raw_load(const int4* f, int stride, int i, Raw<const int4*, R>& r) {
 #pragma unroll
   for (int c = 0; c < 3; c++) r.v[c] = ld16<NT>(f + (size_t)c * stride + (unsigned)i);
 }
@@ -371,6 +441,10 @@ __device__ __forceinline__ void raw_rows(const Raw<const float4*, R>& r, Cx<R> r
   r1[0] = cx<R>((R)b.z, (R)b.w);
   r1[1] = cx<R>((R)c.x, (R)c.y);
   r1[2] = cx<R>((R)c.z, (R)c.w);
+}
+template <typename R>
+__device__ __forceinline__ void raw_rows(const Raw<const int4*, R>& r, Cx<R> r0[3], Cx<R> r1[3]) {
+  fx_rows<R>(r.v[0], r.v[1], r.v[2], r0, r1);
 }
 template <typename R>
 __device__ __forceinline__ void raw_rows(const Raw<PackedLinks, R>& r, Cx<R> r0[3], Cx<R> r1[3]) {

@@ -11,10 +11,11 @@
 
 namespace gfx {
 
-// sweep forms: FP64 links and g, FP32 links and g, and the two forms of the
-// mixed Landau run on packed links: FP64 g on high + low parts, FP32 g on
-// high parts
-enum { FORM_F64 = 0, FORM_F32 = 1, FORM_P64 = 2, FORM_P32 = 3, NFORM = 4 };
+// sweep forms: FP64 links and g, FP32 links and g, the forms on packed
+// links (FP64 g on high + low parts: Coulomb and batch polish; FP32 g on
+// high parts: the FP32 phase of mixed runs), and the FP64 polish of the
+// mixed Landau run on 32-bit fixed-point links and g
+enum { FORM_F64 = 0, FORM_F32 = 1, FORM_P64 = 2, FORM_P32 = 3, FORM_X64 = 4, NFORM = 5 };
 
 struct State {
   int init = 0;  // 1 gauge fixing, 2 thermalization
@@ -35,6 +36,13 @@ struct State {
   // valid while U32 is (both are invalidated together)
   PackedArgs pk{};
   bool packed_valid = false;
+  // 32-bit fixed-point links (8 fields) and g (4 fields, slots as g64) of
+  // the mixed Landau run's FP64 polish; Ufx valid while U32 is
+  int4* Ufx = nullptr;
+  int4* gfx = nullptr;
+  bool fx_valid = false;
+  // the FP64 sweeps of the Landau run in progress read Ufx / gfx
+  bool fx_on = false;
 
   // fused-check partials: odd sweep [0, ncheck), even sweep [ncheck, 2 ncheck)
   double* cpart = nullptr;

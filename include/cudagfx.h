@@ -124,6 +124,9 @@ int apply_trafo(gfx_su3 *gf);
  * phase (FP32 g on the high parts) */
 #define GFX_SWEEP_PACKED64 2
 #define GFX_SWEEP_PACKED32 3
+/* the FP64 polish of the mixed Landau run: links and g in 32-bit fixed
+ * point (int32 = rint(u 2^31) per real, 48 bytes per matrix), FP64 arithmetic */
+#define GFX_SWEEP_FIXED64 4
 
 #define GFX_GAUGE_LANDAU 0
 #define GFX_GAUGE_COULOMB 1

@@ -70,6 +70,7 @@ def _load():
         "orc_su3_normalize": (None, [_dp]),
         "orc_cm_relax": (None, [_dp, _dp]),
         "orc_overrelax_step": (None, [_dp, _dp, C.c_double, C.c_int]),
+        "orc_overrelax_step_short": (None, [_dp, _dp, C.c_double]),
         "orc_reconstruct": (None, [_dp, _dp]),
         "orc_reconstruct_dagger": (None, [_dp, _dp]),
         "orc_mt_generate": (None, [C.POINTER(MTParams), _u32p, C.c_int, C.c_int, C.c_int,
@@ -149,6 +150,8 @@ def from_2v(a):
 
 # ---------------------------------------------------------------- sweeps
 def halfsweep(u2v, g2v, dims, parity, ndir=4, omega=1.68, nterms=3, active=None):
+    """One half sweep (overrelax.cu:335-446) on g2v in place; nterms = -3:
+    the 3-term series in the short form of the library's polish sweeps."""
     nn = neighbours(dims)
     ev, od = eo_lists(dims)
     sites = ev if parity == 0 else od

@@ -192,6 +192,34 @@ static void m_overrelax(m3 g, m3 gn, double w, int N) {
   m_copy(g, help);
 }
 
+/* The 3-term series of m_overrelax in the short form of the polish sweeps
+ * on packed and fixed-point links (the library's overrelax_apply_short):
+ * rows 0, 1 of g + c1 (h - g) + c1 c2 (h g^+ h - 2 h + g), which equals
+ * (1 + c1 R + c1 c2 R^2) g with R = h g^+ - 1 when g is unitary (the caller
+ * normalises).  c1, c2 are formed as overrelax.cu:167 forms them. */
+static void m_overrelax_short(m3 g, m3 h, double w) {
+  const double c1 = (1.0 + w - 1) / 1, c2 = (1.0 + w - 2) / 2;
+  const double cw = c1 * c2, cg = 1.0 - c1 + cw, ch = c1 - 2.0 * cw;
+  cpx t[2][3], u[2][3];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 3; j++) {
+      cpx s = c_mul(h[i][0], c_conj(g[j][0]));
+      s = c_add(s, c_mul(h[i][1], c_conj(g[j][1])));
+      t[i][j] = c_add(s, c_mul(h[i][2], c_conj(g[j][2])));
+    }
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 3; j++) {
+      cpx s = c_mul(t[i][0], h[0][j]);
+      s = c_add(s, c_mul(t[i][1], h[1][j]));
+      u[i][j] = c_add(s, c_mul(t[i][2], h[2][j]));
+    }
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 3; j++) {
+      g[i][j].re = cg * g[i][j].re + ch * h[i][j].re + cw * u[i][j].re;
+      g[i][j].im = cg * g[i][j].im + ch * h[i][j].im + cw * u[i][j].im;
+    }
+}
+
 /* overrelax.cu:451-473 (dev_vectorpotential): U -> (i/2)(U^+ - U), traceless */
 static void m_vecpot(m3 U) {
   m3 t;
@@ -351,7 +379,12 @@ static void site_update(const double *u2v, double *g2v, const int *nn, int pos,
   m_copy(h, gx);
   m_cm_relax(h, star);
   m_normalize(h);
-  m_overrelax(gx, h, omega, nterms);
+  /* nterms < 0: the 3-term series in the short form of the library's
+   * polish sweeps (m_overrelax_short) */
+  if (nterms < 0)
+    m_overrelax_short(gx, h, omega);
+  else
+    m_overrelax(gx, h, omega, nterms);
   m_normalize(gx);
   m_store(g2v + 12L * pos, gx);
 }
@@ -643,6 +676,14 @@ void orc_overrelax_step(double *g18, const double *gn18, double w, int N) {
   memcpy(g, g18, sizeof(m3));
   memcpy(gn, gn18, sizeof(m3));
   m_overrelax(g, gn, w, N);
+  memcpy(g18, g, sizeof(m3));
+}
+void orc_overrelax_step_short(double *g18, const double *h18, double w) {
+  m3 g, h;
+  memcpy(g, g18, sizeof(m3));
+  memcpy(h, h18, sizeof(m3));
+  m_overrelax_short(g, h, w);
+  m_normalize(g);
   memcpy(g18, g, sizeof(m3));
 }
 void orc_reconstruct(const double *r12, double *m18) {

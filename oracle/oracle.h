@@ -109,7 +109,10 @@ double orc_mean_plaq(const double *u_su3, const int *nn, int V);             /* 
 /* ---------------- small SU(3) helpers exposed for tests ---------------- */
 void orc_su3_normalize(double *m18);                  /* dev_su3.cu:1569-1634 */
 void orc_cm_relax(double *g18, const double *star18); /* overrelax.cu:203-269 */
-void orc_overrelax_step(double *g18, const double *gn18, double w, int N); /* 149-177 */
+void orc_overrelax_step(double *g18, const double *gn18, double w, int N);
+/* the short form of the 3-term series (the library's polish sweeps), then
+ * the normalisation */
+void orc_overrelax_step_short(double *g18, const double *h18, double w); /* 149-177 */
 void orc_reconstruct(const double *r12, double *m18);         /* dev_su3.cu:737-820 */
 void orc_reconstruct_dagger(const double *r12, double *m18);  /* dev_su3.cu:826-929 */
 

@@ -7,7 +7,7 @@ stop at the same iteration (+-2) with checkinterval 1.
 import numpy as np
 import pytest
 
-from test_oracle import packed_round
+from test_oracle import fx_round, packed_round
 
 pytestmark = pytest.mark.gpu
 
@@ -387,6 +387,48 @@ def test_packed_sweeps_match_oracle(gfx, orc):
         assert np.abs(g_gpu - g_cpu).max() < 1e-10, v
     gfx.finalize_gaugefixing()
     assert np.abs(g_cpu_packed - g_cpu).max() > 1e-14  # the rounding is visible
+
+
+def test_fixed_point_sweeps_match_oracle(gfx, orc):
+    """The sweep of the mixed Landau run's FP64 polish on 32-bit fixed-point
+    links and g (FP64 arithmetic, the series in the short form, g rounded to
+    the fixed-point grid after every update), each of its 16 variants,
+    against the oracle's FP64 sweeps with the short series on the same
+    rounded links, g rounded the same way after each half sweep and
+    reunitarised at the end: equal to FP64 rounding but for the rare element
+    whose value sits within rounding of a grid midpoint (one grid step,
+    2^-31, and its effect on the neighbours).  Against the oracle on the
+    exact links and g the difference is the effect of the 2^-32 rounding."""
+    dims = (16, 16, 16, 16)
+    V = orc.volume(dims)
+    u = orc.random_init_gauge(V, 4385)
+    g = orc.random_init_trafo(V, 4386)
+    u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+    g2x = fx_round(g2)
+    u2x = fx_round(u2)
+    for parity in (0, 1):
+        orc.halfsweep(u2, g2, dims, parity, 4, 1.7)
+        orc.halfsweep(u2x, g2x, dims, parity, 4, 1.7, nterms=-3)
+        g2x = fx_round(g2x)
+    g_cpu = orc.from_2v(g2)
+    g_cpu_fx = orc.from_2v(g2x).reshape(-1, 18)
+    for k in range(V):
+        orc.lib.orc_su3_normalize(g_cpu_fx[k])
+    g_cpu_fx = g_cpu_fx.ravel()
+    _setup(gfx, dims, u, g, 1.7)
+    for v in [v for v in range(32) if not v & 2]:
+        gfx.set_sweep_variant(gfx.SWEEP_FIXED64, 4, v)
+        gfx.set_trafo(g)
+        gfx.time_sweeps(1, gfx.SWEEP_FIXED64)
+        g_gpu = gfx.get_trafo()
+        d = np.abs(g_gpu - g_cpu_fx).reshape(-1, 18).max(axis=1)
+        print(f"variant {v}: max {d.max():.3e}, sites above 1e-13: {np.count_nonzero(d > 1e-13)}",
+              flush=True)
+        assert d.max() < 4 * 2.0 ** -31, (v, d.max())
+        assert np.count_nonzero(d > 1e-13) <= V // 200, (v, np.count_nonzero(d > 1e-13))
+        assert np.abs(g_gpu - g_cpu).max() < 1e-7, v
+    gfx.finalize_gaugefixing()
+    assert np.abs(g_cpu_fx - g_cpu).max() > 1e-11  # the rounding is visible
 
 
 def test_packed_fp32_sweeps_close_to_oracle(gfx, orc):

@@ -187,6 +187,30 @@ def test_cm_relax_does_not_decrease_functional(orc):
         assert np.real(np.trace(mats(ga)[0] @ K[k])) >= before - 1e-12
 
 
+@pytest.mark.parametrize("omega", [1.68, 1.7, 1.0])
+def test_short_series_equals_reference_series(orc, omega):
+    """The short form of the 3-term series (rows 0, 1 of g + c1 (h - g) +
+    c1 c2 (h g^+ h - 2 h + g), then normalised) equals the reference's
+    normalize(1 + c1 R + c1 c2 R^2) g with R = h g^+ - 1 for unitary g (to
+    rounding), and differs from it by the non-unitarity when g is not
+    unitary (the two forms then have to be told apart)."""
+    g = rand_su3(orc, 20, 18)
+    h = rand_su3(orc, 20, 19)
+    for k in range(20):
+        a, b = flat(g[k]), flat(g[k])
+        orc.lib.orc_overrelax_step(a, flat(h[k]), omega, 3)
+        orc.lib.orc_su3_normalize(a)
+        orc.lib.orc_overrelax_step_short(b, flat(h[k]), omega)
+        assert np.abs(a - b).max() < 1e-14
+        gp = g[k] * (1 + 1e-8 * np.cos(np.arange(9)).reshape(3, 3))
+        a, b = flat(gp), flat(gp)
+        orc.lib.orc_overrelax_step(a, flat(h[k]), omega, 3)
+        orc.lib.orc_su3_normalize(a)
+        orc.lib.orc_overrelax_step_short(b, flat(h[k]), omega)
+        if omega != 1.0:
+            assert np.abs(a - b).max() > 1e-11
+
+
 @pytest.mark.parametrize("omega", [1.0, 2.0, 1.68, 1.7])
 def test_overrelax_series_closed_form(orc, omega):
     """overrelax.cu:149-177 with N = 3: g -> normalize(1 + w R + w(w-1)/2 R^2) g,
@@ -469,6 +493,24 @@ def packed_round(u2, lo=True):
         return hi * 2.0 ** -23
     q = np.clip(np.rint((u2 - hi * 2.0 ** -23) * 2.0 ** 38), -32767, 32767)
     return hi * 2.0 ** -23 + q * 2.0 ** -38
+
+
+def fx_round(a):
+    """The 32-bit fixed-point form of the mixed Landau run's polish
+    restated in numpy: each real u as q = rint(u 2^31) clamped to
+    +-(2^31 - 1), read back as q 2^-31."""
+    return np.clip(np.rint(a * 2.0 ** 31), -2147483647, 2147483647) * 2.0 ** -31
+
+
+def test_fx_round_bound(orc):
+    """|fx(u) - u| <= 2^-32 for the reals of SU(3) matrices (2^-31 at |u| =
+    1, where q clamps); the rounding is seen on random links."""
+    u2 = orc.su3_to_2v(orc.random_init_gauge(4096, 4393))
+    err = np.abs(fx_round(u2) - u2)
+    assert err.max() <= 2.0 ** -32 and err.max() > 2.0 ** -36
+    one = np.array([1.0, -1.0, 0.0, 0.5])
+    assert np.abs(fx_round(one) - one).max() == 2.0 ** -31
+    assert fx_round(np.array([0.5]))[0] == 0.5
 
 
 def test_packing_bound_on_theta(orc):
