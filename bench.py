@@ -320,7 +320,7 @@ def _rows(d, suffix):
 def _dispatches(d):
     """Kernel dispatches of a --kernel-trace run in launch order: (name, us)."""
     rows = _rows(d, "kernel_trace.csv")
-    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
     return [(r["Kernel_Name"], (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3, r)
             for r in rows]
 
@@ -338,6 +338,19 @@ def _counter_dispatches(d, counter):
 
 def _short(name):
     return name.split("(")[0].replace("void ", "")
+
+
+def _marker_groups(seq):
+    """(first, end) index ranges of the runs of consecutive k_stream_copy
+    dispatches (a marker or the calibration copies) in a dispatch list."""
+    groups = []
+    for i, x in enumerate(seq):
+        if "k_stream_copy" in x[0]:
+            if groups and groups[-1][1] == i:
+                groups[-1][1] = i + 1
+            else:
+                groups.append([i, i + 1])
+    return groups
 
 
 def summarize_profile(plan_path, stats_dir, fetch_dir, write_dir):
@@ -361,8 +374,8 @@ def summarize_profile(plan_path, stats_dir, fetch_dir, write_dir):
                           "copy_GBs": 2 * nbytes / (np.mean(copy_us) * 1e-6) / 1e9}
 
     def between_markers(seq, key):
-        m = [i for i, x in enumerate(seq) if "k_stream_copy" in x[0]]
-        return [x for x in seq[m[0] + 1:m[1]] if key(x[0])]
+        g = _marker_groups(seq)
+        return [x for x in seq[g[0][1]:g[1][0]] if key(x[0])]
 
     is_sweep = lambda n: "k_or_sweep" in n
     tr = between_markers(trace, is_sweep)
@@ -389,35 +402,39 @@ def summarize_profile(plan_path, stats_dir, fetch_dir, write_dir):
         k += n
     assert k == len(tr) == len(fe) == len(wr), (k, len(tr), len(fe), len(wr))
     # stochastic kernels and checks (after the second marker, before the calibration copies)
-    m = [i for i, x in enumerate(trace) if "k_stream_copy" in x[0]]
-    mf = [i for i, x in enumerate(fetch) if "k_stream_copy" in x[0]]
-    mw = [i for i, x in enumerate(write) if "k_stream_copy" in x[0]]
-    tail_t = trace[m[1] + 1:m[-5]]
-    tail_f = fetch[mf[1] + 1:mf[-5]]
-    tail_w = write[mw[1] + 1:mw[-5]]
-    assert [_short(x[0]) for x in tail_t] == [_short(x[0]) for x in tail_f] == [_short(x[0]) for x in tail_w]
-    groups = {}
-    for (n, t, _), (_, fv), (_, wv) in zip(tail_t, tail_f, tail_w):
-        g = groups.setdefault(_short(n), {"launches": 0, "us": [], "fetch": [], "write": []})
-        g["launches"] += 1
-        g["us"].append(t)
-        g["fetch"].append(fv * 1024 * fscale)
-        g["write"].append(wv * 1024 * wscale)
-    for n, g in groups.items():
-        us = float(np.mean(g["us"]))
-        fb, wb = float(np.mean(g["fetch"])), float(np.mean(g["write"]))
-        out["stochastic"][n] = {"launches": g["launches"], "us_rocprof": us, "fetch_bytes": fb,
-                                "write_bytes": wb, "traffic_GBs": (fb + wb) / (us * 1e-6) / 1e9}
+    gt, gf, gw = _marker_groups(trace), _marker_groups(fetch), _marker_groups(write)
+    tail_t = trace[gt[1][1]:gt[2][0]]
+    tail_f = fetch[gf[1][1]:gf[2][0]]
+    tail_w = write[gw[1][1]:gw[2][0]]
+    # grouped by kernel name in each run (the sweeps of the fused-check run
+    # may use other variants from run to run: they are left out here)
+    def group(seq):
+        gr = {}
+        for x in seq:
+            n = _short(x[0])
+            if "k_or_sweep" in n:
+                continue
+            gr.setdefault(n, []).append(x[1])
+        return gr
+    gt_, gf_, gw_ = group(tail_t), group(tail_f), group(tail_w)
+    for n, us in gt_.items():
+        if n not in gf_ or n not in gw_ or not (len(us) == len(gf_[n]) == len(gw_[n])):
+            continue
+        t = float(np.mean(us))
+        fb = float(np.mean(gf_[n])) * 1024 * fscale
+        wb = float(np.mean(gw_[n])) * 1024 * wscale
+        out["stochastic"][n] = {"launches": len(us), "us_rocprof": t, "fetch_bytes": fb,
+                                "write_bytes": wb, "traffic_GBs": (fb + wb) / (t * 1e-6) / 1e9}
     return out
 
 
 def summarize_step(plan_path, stats_dir):
     plan = json.load(open(plan_path))
     trace = _dispatches(stats_dir)
-    m = [i for i, x in enumerate(trace) if "k_stream_copy" in x[0]]
+    g = _marker_groups(trace)
     out = {"variants": plan["variants"], "steps": {}}
     for k, name in enumerate(("mixed", "fp64")):
-        sec = trace[m[2 * k] + 1:m[2 * k + 1]]
+        sec = trace[g[2 * k][1]:g[2 * k + 1][0]]
         first, last = sec[0][2], sec[-1][2]
         span_us = (int(last["End_Timestamp"]) - int(first["Start_Timestamp"])) / 1e3
         per = {}
