@@ -109,9 +109,14 @@ def test_random_gribov_copies(gfx, orc):
         assert np.abs(m @ np.conj(np.swapaxes(m, -1, -2)) - np.eye(3)).max() < 1e-13
         assert np.abs(np.linalg.det(m) - 1).max() < 1e-13
     assert np.abs(g_start[0] - g_start[1]).max() > 0.1
-    # Haar: the mean of Re tr g over sites is ~0 (it is 3 for the identity)
-    tr = g_start[0].reshape(-1, 18)[:, [0, 8, 16]].sum(axis=1)
-    assert abs(tr.mean()) < 0.1
+    # Haar SU(3): E tr g = 0 and E |tr g|^2 = 1 (E |tr g|^4 = 2, so the mean
+    # over the 3 V samples of the three copies has standard deviation
+    # 1/sqrt(3 V)); the identity would give 3 and 9
+    a = np.concatenate([g.reshape(-1, 18) for g in g_start])
+    tr = a[:, [0, 8, 16]].sum(axis=1) + 1j * a[:, [1, 9, 17]].sum(axis=1)
+    n = tr.size
+    assert abs(tr.mean()) < 4 / np.sqrt(n)
+    assert abs((np.abs(tr) ** 2).mean() - 1.0) < 0.05 and 5 / np.sqrt(n) < 0.05
 
 
 @pytest.mark.parametrize("maxit,ck,periods", [(41, 5, 2), (9, 1, 3)])

@@ -33,9 +33,24 @@ def oracle_params(orc, p, srand_seed):
     return arr
 
 
+def with_real_records(p):
+    """p with the 16 records of the reference's MersenneTwister.dat held by
+    the words fixture (tests/golden/mt_dat_words.json) in their slots."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "mt_dat_words.json")))
+    p = p.copy()
+    for i, rec in zip(fx["indices"], fx["records"]):
+        p[i, :3] = rec
+    return p
+
+
 @pytest.mark.parametrize("n_gauss,n_unif", [(4096 * 6, 4096 * 20), (5000, 19 * 4096 + 7)])
 def test_mt_fields_bitexact(gfx, orc, n_gauss, n_unif):
-    p = synthetic_params()
+    """The GPU twister fields bit for bit against the oracle (whose words
+    for the reference's own records are pinned by the fixture), with 16 real
+    records among synthetic ones."""
+    p = with_real_records(synthetic_params())
     gfx.set_mt_params(p.ravel())
     gfx.set_mt_seed(4357)
     gfx.init_MT(n_gauss, n_unif)
@@ -86,10 +101,13 @@ def _thermalize(gfx, dims, beta, nsweep, seed, start_hot=False):
     return u
 
 
-@pytest.mark.parametrize("beta,want,tol", [(6.0, 0.5937, 0.006), (0.5, 0.5 / 18 + 0.25 / 216, 0.003)])
+@pytest.mark.parametrize("beta,want,tol", [(6.0, 0.5937, 0.006), (0.5, 0.5 / 18 + 0.25 / 216, 0.00087)])
 def test_heatbath_plaquette(gfx, orc, beta, want, tol):
     """SU(3) Wilson plaquette: beta = 6.0 gives 0.5937 (large volume; 8^4 is
-    within the tolerance); strong coupling <P> = beta/18 + beta^2/216 + ..."""
+    within the tolerance); strong coupling <P> = beta/18 + beta^2/216 + O(beta^3),
+    checked to 3% at beta = 0.5 (the statistical error of the mean of 60
+    sweeps over the 6 x 8^4 plaquettes, each of standard deviation about
+    1/sqrt(18), is about 2e-4)."""
     dims = (8, 8, 8, 8)
     _thermalize(gfx, dims, beta, 0, 4357)
     gfx.heatbath_sweeps(150, beta)

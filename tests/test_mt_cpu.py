@@ -94,3 +94,37 @@ def test_reference_parameter_table_drives_the_same_recurrence(orc):
     for i in range(16):
         want, _ = _mt607_reference_form(*(int(v) for v in p[i, :3]), 4357 + i, 40)
         assert list(words.reshape(40, 16)[:, i]) == want
+
+
+def test_real_table_words_fixture(orc):
+    """Known answers for the reference's own parameter table: the oracle's
+    words for 16 of its records (0..7, 4088..4095), seeded as seedMTGPU
+    seeds them (srand(4357) and srand(351), one rand() per twister,
+    MersenneTwister.cu:98-114), 40 numbers and then 19 after a restart.
+    The fixture (tests/golden/make_mt_golden.py) holds the records it used;
+    where the reference checkout is present they are checked against the
+    table as well."""
+    import ctypes as C
+    import json
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "mt_dat_words.json")))
+    idx, records = fx["indices"], fx["records"]
+    libc = C.CDLL("libc.so.6")
+    for seed, e in fx["seeds"].items():
+        libc.srand(int(seed))
+        r = [libc.rand() & 0xFFFFFFFF for _ in range(4096)]
+        assert [r[i] for i in idx] == e["twister_seeds"]
+        n = len(idx)
+        params = (orc.MTParams * n)()
+        for k, ((a, b, c), s) in enumerate(zip(records, e["twister_seeds"])):
+            params[k].matrix_a, params[k].mask_b, params[k].mask_c, params[k].seed = a, b, c, s
+        state = np.zeros(19 * n, dtype=np.uint32)
+        _, w1 = orc.mt_generate(params, state, fx["n_first"], 0, want_words=True)
+        _, w2 = orc.mt_generate(params, state, fx["n_restart"], 1, want_words=True)
+        assert w1.reshape(fx["n_first"], n).T.tolist() == e["words"]
+        assert w2.reshape(fx["n_restart"], n).T.tolist() == e["words_after_restart"]
+        for k in range(n):
+            want, _ = _mt607_reference_form(*records[k], e["twister_seeds"][k], fx["n_first"])
+            assert e["words"][k] == want
+    if os.path.exists(DAT):
+        p = np.fromfile(DAT, dtype="<u4").reshape(4096, 4)
+        assert [[int(v) for v in p[i, :3]] for i in idx] == records
