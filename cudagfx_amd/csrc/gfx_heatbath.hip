@@ -150,11 +150,12 @@ __device__ __forceinline__ M3<double> star_of(const double2* __restrict__ U,
   M3<double> K;
   m_zero(K);
   // direction loop kept rolled (one direction's operands live at a time:
-  // unrolled, these kernels spilled to scratch at 1 wave/SIMD), products in
-  // the short form (the stochastic updates have no bitwise target)
+  // unrolled, these kernels spilled to scratch at 1 wave/SIMD), the four
+  // operands of a direction loaded before any is decoded, products in the
+  // short form (the stochastic updates have no bitwise target)
 #pragma unroll 1
   for (int mu = 0; mu < NDIR; mu++)
-    star_terms_short<double2, double, false>(K, U, gop, geo, s, i, parity, mu);
+    star_terms_short<double2, double, false, const double2*, 1>(K, U, gop, geo, s, i, parity, mu);
   return K;
 }
 
@@ -258,11 +259,16 @@ __global__ __launch_bounds__(HB_BLK) void k_batch_micro_sweep(const BatchProblem
 
 // staple of U_nu(x): sum_{mu != nu} U_mu(x+nu) U_nu^+(x+mu) U_mu^+(x)
 //                                 + U_mu^+(x+nu-mu) U_nu^+(x-mu) U_mu(x-mu)
+// bmap: launch position -> block (a sweep workgroup order of HB_BLK-site
+// blocks, dealt to the XCDs); the update of a link depends only on links of
+// the other parity or of other directions, so the order leaves the result
+// unchanged and only decides how much of the staples' links L2 serves
 __global__ __launch_bounds__(HB_BLK) void k_link_heatbath(double2* __restrict__ U, Geom geo,
                                                           int parity, int nu, double beta,
                                                           const float* __restrict__ gauss,
-                                                          const float* __restrict__ unif) {
-  const int i = blockIdx.x * HB_BLK + threadIdx.x;
+                                                          const float* __restrict__ unif,
+                                                          const int* __restrict__ bmap) {
+  const int i = bmap[blockIdx.x] * HB_BLK + threadIdx.x;
   if (i >= geo.nh) return;
   const int S = geo.stride;
   const int p = parity, q = 1 - parity;
@@ -355,10 +361,10 @@ void launch_micro_sweep(const double2* U, double2* g, const Geom& geo, int parit
 }
 
 void launch_link_heatbath(double2* U, const Geom& geo, int parity, int nu, double beta,
-                          const float* gauss, const float* unif, hipStream_t st) {
+                          const float* gauss, const float* unif, const int* bmap, hipStream_t st) {
   dim3 grid((geo.nh + HB_BLK - 1) / HB_BLK);
   hipLaunchKernelGGL(k_link_heatbath, grid, dim3(HB_BLK), 0, st, U, geo, parity, nu, beta, gauss,
-                     unif);
+                     unif, bmap);
 }
 
 // the SU(2) samplers alone, for distribution tests: sample i draws alpha

@@ -79,6 +79,18 @@ struct State {
   float* rnd_gauss = nullptr;
   float* rnd_unif = nullptr;
   int n_per_gauss = 0, n_per_unif = 0;
+  // a second set of random fields and a stream of its own for the twister
+  // refills, so that the refill for the next link-heatbath launch runs
+  // beside the current one (same numbers, same order)
+  float* rnd_gauss2 = nullptr;
+  float* rnd_unif2 = nullptr;
+  int n2_gauss = 0, n2_unif = 0;
+  hipStream_t mt_stream = nullptr;
+  hipEvent_t ev_used[2] = {nullptr, nullptr};    // a launch finished reading buffer k
+  hipEvent_t ev_filled[2] = {nullptr, nullptr};  // buffer k refilled
+  // workgroup order (tiling of bmap[0]) of the link heatbath for this
+  // lattice, picked by timing on its first sweep (-1: not yet)
+  int hb_tiling = -1;
 
   // sweep kernel variant per [form: FORM_*][ndir
   // 3/4], picked by the autotuner on first use for the current lattice

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/cudagfx.h"
+#include "gfx_internal.h"
 #include "gfx_state.h"
 
 namespace gfx {
@@ -27,7 +28,7 @@ void launch_sa_sweep(const double2* U, double2* g, const Geom& geo, int parity, 
 void launch_micro_sweep(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
                         hipStream_t st);
 void launch_link_heatbath(double2* U, const Geom& geo, int parity, int nu, double beta,
-                          const float* gauss, const float* unif, hipStream_t st);
+                          const float* gauss, const float* unif, const int* bmap, hipStream_t st);
 int kp_gauss_per_site();
 int kp_unif_per_site();
 void launch_haar_trafo(double2* g, const Geom& geo, const float* gauss, hipStream_t st);
@@ -131,7 +132,46 @@ int mt_prepare(State& s, int need_g, int need_u) {
   return GFX_OK;
 }
 
+// the twister output of one refill into the given fields on stream st
+// (the same kernels and state as mt_refill)
+static void mt_refill_into(State& s, float* gauss, float* unif, hipStream_t st) {
+  launch_mt_generate(s.mt_params, s.mt_state, gauss, s.n_per_gauss, 1, st);
+  launch_box_muller(gauss, s.n_per_gauss, st);
+  launch_mt_generate(s.mt_params, s.mt_state, unif, s.n_per_unif, 1, st);
+}
+
+// the second set of fields (as large as the first), the refill stream and
+// its events
+static int mt_second_set(State& s) {
+  if (!s.mt_stream) {
+    if (hipStreamCreateWithFlags(&s.mt_stream, hipStreamNonBlocking) != hipSuccess) return GFX_ERR_DEVICE;
+    for (int k = 0; k < 2; k++) {
+      if (hipEventCreateWithFlags(&s.ev_used[k], hipEventDisableTiming) != hipSuccess ||
+          hipEventCreateWithFlags(&s.ev_filled[k], hipEventDisableTiming) != hipSuccess)
+        return GFX_ERR_DEVICE;
+    }
+  }
+  if (s.n2_gauss < s.n_per_gauss || !s.rnd_gauss2) {
+    hipFree(s.rnd_gauss2);
+    s.rnd_gauss2 = nullptr;
+    if (hipMalloc((void**)&s.rnd_gauss2, sizeof(float) * (size_t)MT_COUNT * s.n_per_gauss) != hipSuccess) return GFX_ERR_ALLOC;
+    s.n2_gauss = s.n_per_gauss;
+  }
+  if (s.n2_unif < s.n_per_unif || !s.rnd_unif2) {
+    hipFree(s.rnd_unif2);
+    s.rnd_unif2 = nullptr;
+    if (hipMalloc((void**)&s.rnd_unif2, sizeof(float) * (size_t)MT_COUNT * s.n_per_unif) != hipSuccess) return GFX_ERR_ALLOC;
+    s.n2_unif = s.n_per_unif;
+  }
+  return GFX_OK;
+}
+
 void mt_free(State& s) {
+  if (s.mt_stream) hipStreamSynchronize(s.mt_stream);
+  hipFree(s.rnd_gauss2);
+  hipFree(s.rnd_unif2);
+  s.rnd_gauss2 = s.rnd_unif2 = nullptr;
+  s.n2_gauss = s.n2_unif = 0;
   hipFree(s.rnd_gauss);
   hipFree(s.rnd_unif);
   hipFree(s.mt_params);
@@ -352,13 +392,57 @@ int gfx_heatbath_sweeps(int nsweep, double beta) {
     if (mt_fields(s, need_g, need_u)) return GFX_ERR_ALLOC;
     mt_refill(s, 1);
   }
-  for (int k = 0; k < nsweep; k++)
-    for (int nu = 0; nu < 4; nu++)
-      for (int p = 0; p < 2; p++) {
-        launch_link_heatbath(s.U64, s.geo, p, nu, beta / 3.0, s.rnd_gauss, s.rnd_unif, s.stream);
-        mt_refill(s, 1);
+  int rc = mt_second_set(s);
+  if (rc) return rc;
+  // Launch k reads fields[k % 2]; the refill that launch k+1 reads is made
+  // into fields[(k+1) % 2] on the twister stream while launch k runs (after
+  // launch k-1 has finished reading those fields), refills in order, so
+  // every launch gets the same numbers as with a refill after each launch.
+  float* fg[2] = {s.rnd_gauss, s.rnd_gauss2};
+  float* fu[2] = {s.rnd_unif, s.rnd_unif2};
+  hipEventRecord(s.ev_filled[0], s.stream);  // fields[0] hold the current numbers
+  hipStreamWaitEvent(s.mt_stream, s.ev_filled[0], 0);
+  // workgroup order: the first launches of the first call on this lattice
+  // try each tiling that applies (the result does not depend on it)
+  std::vector<int> tilings;
+  if (s.hb_tiling < 0)
+    for (int t = 0; t < TILINGS; t++)
+      if (s.tiling_ok[0][t]) tilings.push_back(t);
+  float best = 1e30f;
+  int bestt = 0;
+  const int n = 8 * nsweep;
+  for (int k = 0; k < n; k++) {
+    const int nu = (k >> 1) & 3, p = k & 1, b = k & 1;
+    const bool timing = k < (int)tilings.size();
+    const int t = timing ? tilings[k] : (s.hb_tiling >= 0 ? s.hb_tiling : 0);
+    if (k > 0) hipStreamWaitEvent(s.stream, s.ev_filled[b], 0);
+    if (timing) hipEventRecord(s.ev0, s.stream);
+    launch_link_heatbath(s.U64, s.geo, p, nu, beta / 3.0, fg[b], fu[b], s.bmap[0][t], s.stream);
+    if (timing) {
+      hipEventRecord(s.ev1, s.stream);
+      hipEventSynchronize(s.ev1);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, s.ev0, s.ev1);
+      if (ms < best) {
+        best = ms;
+        bestt = t;
       }
+      if (k + 1 == (int)tilings.size()) s.hb_tiling = bestt;
+    }
+    hipEventRecord(s.ev_used[b], s.stream);
+    // refill k+1 into the other fields once launch k-1 has read them
+    if (k > 0) hipStreamWaitEvent(s.mt_stream, s.ev_used[1 - b], 0);
+    mt_refill_into(s, fg[1 - b], fu[1 - b], s.mt_stream);
+    hipEventRecord(s.ev_filled[1 - b], s.mt_stream);
+  }
+  // the fields of the last refill become the current ones
+  hipStreamWaitEvent(s.stream, s.ev_filled[n & 1], 0);
+  if (n & 1) {
+    std::swap(s.rnd_gauss, s.rnd_gauss2);
+    std::swap(s.rnd_unif, s.rnd_unif2);
+  }
   s.U32_valid = false;
+  if (hip_check(hipStreamSynchronize(s.mt_stream), "twister refill")) return GFX_ERR_DEVICE;
   return hip_check(hipStreamSynchronize(s.stream), "heatbath");
 }
 
