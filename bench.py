@@ -195,6 +195,13 @@ def align_global(g_a, g_b):
     return float(np.abs(A - np.einsum("ij,njk->nik", h, B)).max())
 
 
+def _write_plan(path, plan):
+    d = os.path.dirname(os.path.abspath(path))
+    os.makedirs(d, exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(plan, f, indent=1)
+
+
 def _marker(gfx):
     """A small streaming copy (k_stream_copy of 4 KiB): brackets a section
     in a kernel trace."""
@@ -202,9 +209,9 @@ def _marker(gfx):
     gfx.sync()
 
 
-def _setup_32x4(gfx, nsweep=None):
-    V = int(np.prod(DIMS))
-    u, plaq = thermalized_config(gfx, nsweep=nsweep)
+def _setup_32x4(gfx, nsweep=None, dims=DIMS):
+    V = int(np.prod(dims))
+    u, plaq = thermalized_config(gfx, dims=dims, nsweep=nsweep)
     gfx.plant_seeds(RANDSEED)
     g0 = gfx.random_init_trafo_host(V)
     gfx.init_gaugefixing(u, g0)
@@ -212,7 +219,7 @@ def _setup_32x4(gfx, nsweep=None):
     return u, g0, plaq
 
 
-def profile_run(gfx, plan_path):
+def profile_run(gfx, plan_path, dims=DIMS, stochastic=True):
     """The kernels for counter collection, in a fixed order recorded in
     plan_path: the autotuner's timing of every variant of the five sweep
     forms, a marker copy, then 10 sweeps (20 launches) of every variant of
@@ -222,8 +229,8 @@ def profile_run(gfx, plan_path):
     heatbath sweeps with their twister refills, 5 SA steps with 3
     microcanonical sweeps each), and 5 copies of 512 MiB whose bytes are
     known (the counter calibration)."""
-    u, g0, _ = _setup_32x4(gfx, nsweep=20)  # counters do not depend on the field values
-    plan = {"lattice": list(DIMS), "forms": {}, "sequence": []}
+    u, g0, _ = _setup_32x4(gfx, nsweep=20, dims=dims)  # counters do not depend on the field values
+    plan = {"lattice": list(dims), "forms": {}, "sequence": []}
     for name, form in FORMS.items():
         gfx.time_sweeps(1, form)
         v, blk, rolled, ms = gfx.sweep_variant(form, 4)
@@ -243,18 +250,24 @@ def profile_run(gfx, plan_path):
     gfx.set_precision(gfx.PREC_FP64)
     gfx.overrelax_gauge(31, EPS, CK)
     gfx.sync()
+    if not stochastic:
+        tc = gfx.stream_copy(512 << 20, 5)
+        gfx.finalize_gaugefixing()
+        plan["copy_512MiB_ms"] = tc
+        _write_plan(plan_path, plan)
+        return {"profile": True, "plan": plan_path}
     # stochastic kernels: link heatbath (8 launches per sweep, a twister
     # refill after each), then SA (an even and an odd heatbath half sweep of
     # g per step, a refill after each, then the microcanonical sweeps)
     gfx.finalize_gaugefixing()
-    gfx.set_lattice(*DIMS)
+    gfx.set_lattice(*dims)
     gfx.init_thermalization(u)
     gfx.set_mt_params(synthetic_mt_params())
     gfx.set_mt_seed(MTSEED)
     gfx.heatbath_sweeps(5, BETA)
     gfx.finalize_MT()
     gfx.finalize_thermalization()
-    gfx.set_lattice(*DIMS)
+    gfx.set_lattice(*dims)
     gfx.init_gaugefixing(u, g0)
     gfx.set_mt_params(synthetic_mt_params())
     gfx.set_mt_seed(MTSEED)
@@ -269,8 +282,7 @@ def profile_run(gfx, plan_path):
     gfx.finalize_gaugefixing()
     plan["copy_512MiB_ms"] = tc
     plan["stochastic"] = {"heatbath_sweeps": 5, "sa_steps": 5, "sa_micro": 3}
-    with open(plan_path, "w") as f:
-        json.dump(plan, f, indent=1)
+    _write_plan(plan_path, plan)
     return {"profile": True, "plan": plan_path}
 
 
@@ -300,8 +312,7 @@ def trace_step(gfx, plan_path):
         sv = gfx.sweep_variant(p, 4)
         variants[n] = sv[0] if sv else None
     gfx.finalize_gaugefixing()
-    with open(plan_path, "w") as f:
-        json.dump({"steps": walls, "variants": variants}, f, indent=1)
+    _write_plan(plan_path, {"steps": walls, "variants": variants})
     return {"trace_step": True, "plan": plan_path, "steps": walls}
 
 
@@ -403,6 +414,8 @@ def summarize_profile(plan_path, stats_dir, fetch_dir, write_dir):
     assert k == len(tr) == len(fe) == len(wr), (k, len(tr), len(fe), len(wr))
     # stochastic kernels and checks (after the second marker, before the calibration copies)
     gt, gf, gw = _marker_groups(trace), _marker_groups(fetch), _marker_groups(write)
+    if len(gt) < 3:
+        return out
     tail_t = trace[gt[1][1]:gt[2][0]]
     tail_f = fetch[gf[1][1]:gf[2][0]]
     tail_w = write[gw[1][1]:gw[2][0]]
@@ -506,7 +519,8 @@ def measure(args):
     assert FORMS == {"fp64": gfx.PREC_FP64, "fp32": gfx.PREC_MIXED, "packed64": gfx.SWEEP_PACKED64,
                      "packed32": gfx.SWEEP_PACKED32, "fixed64": gfx.SWEEP_FIXED64}
     if args.profile:
-        return profile_run(gfx, args.plan)
+        L = args.lattice
+        return profile_run(gfx, args.plan, dims=(L, L, L, L), stochastic=L == DIMS[0])
     if args.trace_step:
         return trace_step(gfx, args.plan)
     V = int(np.prod(DIMS))
@@ -694,6 +708,8 @@ def main():
     ap.add_argument("--trace-step", action="store_true",
                     help="one mixed and one FP64 step between marker copies, for a kernel trace")
     ap.add_argument("--plan", default=os.path.join(ROOT, "gpurun_out", "profile_plan.json"))
+    ap.add_argument("--lattice", type=int, default=DIMS[0],
+                    help="--profile on an L^4 lattice (the stochastic kernels at 32^4 only)")
     ap.add_argument("--summarize-profile", nargs=4, metavar=("PLAN", "STATS", "FETCH", "WRITE"))
     ap.add_argument("--summarize-step", nargs=2, metavar=("PLAN", "STATS"))
     args = ap.parse_args()
