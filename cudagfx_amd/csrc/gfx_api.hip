@@ -396,8 +396,8 @@ void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* ac
                               s.omega, active, v, map, s.stream, chk, alt, part, nb);
   } else {
     if (form == FORM_X64) {
-      launch_sweep_fx(s.Ufx, fieldfx(s, even), fieldfx(s, odd), s.geo, parity, s.omega, v, map,
-                      s.stream, chk, out < 0 ? nullptr : fieldfx(s, out), part, nb);
+      launch_sweep_fx(s.Ufx, fieldfx(s, even), fieldfx(s, odd), s.geo, parity, ndir, s.omega,
+                      active, v, map, s.stream, chk, out < 0 ? nullptr : fieldfx(s, out), part, nb);
       return;
     }
     double2* alt = out < 0 ? nullptr : field64(s, out);
@@ -475,7 +475,8 @@ int tune_sweep(State& s, int form, int ndir) {
   int4* scfx = static_cast<int4*>(scratch);
   auto sweep = [&](int v, const int* map, int p) {
     if (form == FORM_X64)
-      launch_sweep_fx(s.Ufx, scfx, scfx + 3 * (size_t)s.geo.stride, s.geo, p, s.omega, v, map, s.stream);
+      launch_sweep_fx(s.Ufx, scfx, scfx + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v,
+                      map, s.stream);
     else if (form == FORM_F32)
       launch_sweep_f32(s.U32, sc32, sc32 + 3 * (size_t)s.geo.stride, s.geo, p, ndir, s.omega, nullptr, v, map, s.stream);
     else if (form == FORM_P32)
@@ -842,12 +843,20 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
                                        std::vector<unsigned char>(nt, mixed ? 1 : 0)};
   unsigned char* dact[2] = {s.active, s.active + nt};
   const bool packed = mixed && s.packed_links && eps > PACK_EPS_MIN;
+  // FP64 slices of a run on packed links are polished on fixed-point links
+  // and g when their stops can be decided by the fused checks (slices of
+  // whole workgroups of either size) and eps is well above that form's reach
+  const bool fxc = packed && eps > FX_EPS_MIN && nsh % 256 == 0;
   // every exit leaves the packed forms off
   struct PackedOff {
     State& s;
-    ~PackedOff() { s.packed_on = false; }
+    ~PackedOff() {
+      s.packed_on = false;
+      s.fx_on = false;
+    }
   } packed_off{s};
   s.packed_on = false;
+  s.fx_on = false;
   if (mixed) {
     if (ensure_f32_links(s)) return GFX_ERR_ALLOC;
     launch_f64_to_f32(s.g64, s.g32, s.geo, 2, s.stream);
@@ -855,22 +864,30 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   if (packed) {
     if (ensure_packed_links(s)) return GFX_ERR_ALLOC;
     tune_sweep(s, FORM_P32, 3);
-    tune_sweep(s, FORM_P64, 3);
+    if (fxc) {
+      if (ensure_fx_links(s)) return GFX_ERR_ALLOC;
+      launch_f64_to_fx(s.g64, s.gfx, s.geo, 2, s.stream);
+      tune_sweep(s, FORM_X64, 3);
+    } else {
+      tune_sweep(s, FORM_P64, 3);
+    }
   } else {
     if (mixed) tune_sweep(s, FORM_F32, 3);
     tune_sweep(s, FORM_F64, 3);
   }
   s.packed_on = packed;
+  s.fx_on = fxc;
   // an FP64 slice stops when its max dAdA is below eps.  A check fused
-  // into the sweeps on packed links decides that with the packing's worst
-  // effect on Delta added (the FP64 slices sweep in the reference order), so
-  // that the bound holds on the exact links too (a slice in the narrow band
-  // below eps stops one check later); a standalone check reads the FP64
-  // link planes.  An FP32 slice never stops: below eps (or the switch) it
-  // moves to FP64, whose check decides its stop.
+  // into the sweeps on packed or fixed-point fields decides that with the
+  // rounding's worst effect on Delta added (the FP64 slices sweep in the
+  // reference order), so that the bound holds on the exact links and the
+  // slice's FP64 g too (a slice in the narrow band below eps stops one check
+  // later); a standalone check reads the FP64 link planes.  An FP32 slice
+  // never stops: below eps (or the switch) it moves to FP64, whose check
+  // decides its stop.
   auto below = [&](double m, bool fused_check) {
     if (!s.packed_on || !fused_check) return m < eps;
-    const double r = std::sqrt(m) + PACK_DDELTA64;
+    const double r = std::sqrt(m) + (fxc ? FX_DDELTA : PACK_DDELTA64);
     return r * r < eps;
   };
   auto upload = [&]() {
@@ -903,12 +920,14 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   bool even_done[2] = {false, false};
   hipStreamSynchronize(s.stream);
   const auto t0 = std::chrono::steady_clock::now();
-  // copy the even sites of slice t from field `from` to field `to` (planes)
+  // copy the even sites of slice t from field `from` to field `to` (planes;
+  // the FP64 slices' fields are the fixed-point ones with fxc)
   auto copy_slice = [&](bool f32, int from, int to, int t) {
-    const int np = f32 ? 3 : 6;
-    const size_t vb = f32 ? sizeof(float4) : sizeof(double2);
-    char* src = f32 ? (char*)field32(s, from) : (char*)field64(s, from);
-    char* dst = f32 ? (char*)field32(s, to) : (char*)field64(s, to);
+    const bool fx = !f32 && fxc;
+    const int np = (f32 || fx) ? 3 : 6;
+    const size_t vb = f32 ? sizeof(float4) : fx ? sizeof(int4) : sizeof(double2);
+    char* src = f32 ? (char*)field32(s, from) : fx ? (char*)fieldfx(s, from) : (char*)field64(s, from);
+    char* dst = f32 ? (char*)field32(s, to) : fx ? (char*)fieldfx(s, to) : (char*)field64(s, to);
     for (int c = 0; c < np; c++) {
       const size_t off = ((size_t)c * s.geo.stride + (size_t)t * nsh) * vb;
       hipMemcpyAsync(dst + off, src + off, (size_t)nsh * vb, hipMemcpyDeviceToDevice, s.stream);
@@ -981,8 +1000,12 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
     const bool swap32 = fused[1] && n[1] > 0 && !last;
     for (int t : sw32) {
       const bool from_update = swap32 && (even_done[0] || n[0] == 0);
-      launch_g32_to_g64_range(field32(s, from_update ? 2 - even[1] : even[1]), field32(s, 1),
-                              field64(s, even[0]), field64(s, 1), s.geo, t * nsh, nsh, s.stream);
+      if (fxc)
+        launch_g32_to_fx_range(field32(s, from_update ? 2 - even[1] : even[1]), field32(s, 1),
+                               fieldfx(s, even[0]), fieldfx(s, 1), s.geo, t * nsh, nsh, s.stream);
+      else
+        launch_g32_to_g64_range(field32(s, from_update ? 2 - even[1] : even[1]), field32(s, 1),
+                                field64(s, even[0]), field64(s, 1), s.geo, t * nsh, nsh, s.stream);
     }
     if (!sw32.empty() && n[0] == 0 && swap32) even_done[0] = true;
     if (swap32) {
@@ -1014,7 +1037,13 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
       printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e\t slices left = %d\n", i, F / nt, D / nt, M, left);
     }
   }
-  // slices still in FP32 at maxit: their state to FP64
+  // the FP64 slices' fixed-point g to FP64 (reunitarised), then slices
+  // still in FP32 at maxit: their state to FP64
+  if (fxc) {
+    canonical_slot(s, false, even[0], 0, true);
+    even[0] = 0;
+    launch_fx_to_g64(s.gfx, s.g64, s.geo, s.stream);
+  }
   for (int t = 0; t < nt; t++)
     if (act[1][t])
       launch_g32_to_g64_range(field32(s, even[1]), field32(s, 1), field64(s, even[0]),

@@ -73,11 +73,12 @@ void launch_sweep_packed_f64(const PackedArgs& p, double2* ge, double2* go, cons
                              int parity, int ndir, double omega, const unsigned char* active,
                              int variant, const int* bmap, hipStream_t st, int chk = 0,
                              double2* galt = nullptr, double* partials = nullptr, int nblocks = 0);
-// the FP64 polish of the mixed Landau run on 32-bit fixed-point links and g
-// (int4 planes, Store<int4>), arguments as launch_sweep_f64 with ndir = 4
-void launch_sweep_fx(const int4* U, int4* ge, int4* go, const Geom& geo, int parity, double omega,
-                     int variant, const int* bmap, hipStream_t st, int chk = 0,
-                     int4* galt = nullptr, double* partials = nullptr, int nblocks = 0);
+// the FP64 polish of mixed Landau and Coulomb runs on 32-bit fixed-point
+// links and g (int4 planes, Store<int4>), arguments as launch_sweep_f64
+void launch_sweep_fx(const int4* U, int4* ge, int4* go, const Geom& geo, int parity, int ndir,
+                     double omega, const unsigned char* active, int variant, const int* bmap,
+                     hipStream_t st, int chk = 0, int4* galt = nullptr, double* partials = nullptr,
+                     int nblocks = 0);
 int sweep_blocks(int nh, int variant);
 // per-slice grid pass of a fused Coulomb check (bps blocks per slice; only
 // active slices are written)
@@ -154,6 +155,9 @@ void launch_g32_to_g64(const float4* src, double2* dst, const Geom& geo, hipStre
 void launch_f64_to_fx(const double2* src, int4* dst, const Geom& geo, int nfield, hipStream_t st);
 void launch_fx_to_g64(const int4* src, double2* dst, const Geom& geo, hipStream_t st);
 void launch_g32_to_fx(const float4* src, int4* dst, const Geom& geo, hipStream_t st);
+// sites [i0, i0 + n) of both parities, FP32 -> fixed point, fields given separately
+void launch_g32_to_fx_range(const float4* se, const float4* so, int4* de, int4* dd,
+                            const Geom& geo, int i0, int n, hipStream_t st);
 // sites [i0, i0 + n) of both parities, fields given separately
 void launch_g32_to_g64_range(const float4* se, const float4* so, double2* de, double2* dd,
                              const Geom& geo, int i0, int n, hipStream_t st);

@@ -814,6 +814,23 @@ __global__ __launch_bounds__(BLK) void k_convert(const VS* __restrict__ src, VD*
   Store<VD>::store(field_ptr(dst, f, stride), stride, i, M);
 }
 
+// the sites [i0, i0 + n) of each parity (one Coulomb time slice): FP32 g ->
+// fixed point with the FP64 reunitarisation (grid.y = parity, fields given
+// separately); and a plain copy of those sites between two fixed-point
+// fields
+__global__ __launch_bounds__(BLK) void k_g32_to_fx_range(const float4* __restrict__ se,
+                                                         const float4* __restrict__ so,
+                                                         int4* __restrict__ de,
+                                                         int4* __restrict__ dd, int i0, int n,
+                                                         int stride) {
+  const int k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= n) return;
+  const int i = i0 + k;
+  M3<double> M = load_m<float4, double>(blockIdx.y ? so : se, stride, i);
+  m_normalize(M);
+  Store<int4>::store(blockIdx.y ? dd : de, stride, i, M);
+}
+
 // ------------------------------------------------------------ launchers
 static inline dim3 grid1(long n) { return dim3((unsigned)((n + BLK - 1) / BLK)); }
 
@@ -971,31 +988,41 @@ void launch_sweep_packed_f64(const PackedArgs& p, double2* ge, double2* go, cons
                                    ndir, omega, variant, chk, st);
 }
 
-// the FP64 polish on 32-bit fixed-point links and g (Landau): rolled
-// variants only (bit 1 of the variant ignored; unrolled, the FP64 star
-// spills)
-template <int CHK>
+// the FP64 polish on 32-bit fixed-point links and g (Landau: 4 directions;
+// Coulomb: 3, active slices only): rolled variants only (bit 1 of the
+// variant ignored; unrolled, the FP64 star spills)
+template <int NDIR, int CHK>
 static void launch_fx_chk(const SweepArgs<int4>& a, double c1, double c2, int variant,
                           hipStream_t st) {
   switch (variant & 5) {
-    case 0: launch_sweep_v<int4, double, 4, 256, true, false, CHK>(a, c1, c2, st); break;
-    case 1: launch_sweep_v<int4, double, 4, 64, true, false, CHK>(a, c1, c2, st); break;
-    case 4: launch_sweep_v<int4, double, 4, 256, true, true, CHK>(a, c1, c2, st); break;
-    default: launch_sweep_v<int4, double, 4, 64, true, true, CHK>(a, c1, c2, st); break;
+    case 0: launch_sweep_v<int4, double, NDIR, 256, true, false, CHK>(a, c1, c2, st); break;
+    case 1: launch_sweep_v<int4, double, NDIR, 64, true, false, CHK>(a, c1, c2, st); break;
+    case 4: launch_sweep_v<int4, double, NDIR, 256, true, true, CHK>(a, c1, c2, st); break;
+    default: launch_sweep_v<int4, double, NDIR, 64, true, true, CHK>(a, c1, c2, st); break;
   }
 }
-void launch_sweep_fx(const int4* U, int4* ge, int4* go, const Geom& geo, int parity, double omega,
-                     int variant, const int* bmap, hipStream_t st, int chk, int4* galt,
-                     double* partials, int nblocks) {
+template <int NDIR>
+static void launch_fx_ndir(const SweepArgs<int4>& a, double c1, double c2, int variant, int chk,
+                           hipStream_t st) {
+  if (chk == 1)
+    launch_fx_chk<NDIR, 1>(a, c1, c2, variant, st);
+  else if (chk == 2)
+    launch_fx_chk<NDIR, 2>(a, c1, c2, variant, st);
+  else
+    launch_fx_chk<NDIR, 0>(a, c1, c2, variant, st);
+}
+void launch_sweep_fx(const int4* U, int4* ge, int4* go, const Geom& geo, int parity, int ndir,
+                     double omega, const unsigned char* active, int variant, const int* bmap,
+                     hipStream_t st, int chk, int4* galt, double* partials, int nblocks) {
   const double c1 = (1.0 + omega - 1) / 1;
   const double c2 = (1.0 + omega - 2) / 2;
-  const SweepArgs<int4> a{U, ge, go, galt, geo, parity, nullptr, bmap, nblocks, partials};
-  if (chk == 1)
-    launch_fx_chk<1>(a, c1, c2, variant, st);
-  else if (chk == 2)
-    launch_fx_chk<2>(a, c1, c2, variant, st);
-  else
-    launch_fx_chk<0>(a, c1, c2, variant, st);
+  if (ndir == 4) {
+    const SweepArgs<int4> a{U, ge, go, galt, geo, parity, nullptr, bmap, nblocks, partials};
+    launch_fx_ndir<4>(a, c1, c2, variant, chk, st);
+  } else {
+    const SweepArgs<int4> a{U, ge, go, galt, geo, parity, active, bmap, nblocks, partials};
+    launch_fx_ndir<3>(a, c1, c2, variant, chk, st);
+  }
 }
 
 int sweep_blocks(int nh, int variant) {
@@ -1206,6 +1233,12 @@ void launch_g32_to_fx(const float4* src, int4* dst, const Geom& geo, hipStream_t
 void launch_g32_to_g64(const float4* src, double2* dst, const Geom& geo, hipStream_t st) {
   dim3 grid((geo.nh + BLK - 1) / BLK, 2);
   hipLaunchKernelGGL(k_g32_to_g64, grid, dim3(BLK), 0, st, src, dst, geo.nh, geo.stride);
+}
+
+void launch_g32_to_fx_range(const float4* se, const float4* so, int4* de, int4* dd,
+                            const Geom& geo, int i0, int n, hipStream_t st) {
+  dim3 grid((n + BLK - 1) / BLK, 2);
+  hipLaunchKernelGGL(k_g32_to_fx_range, grid, dim3(BLK), 0, st, se, so, de, dd, i0, n, geo.stride);
 }
 
 void launch_g32_to_g64_range(const float4* se, const float4* so, double2* de, double2* dd,

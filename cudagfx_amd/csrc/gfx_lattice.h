@@ -190,6 +190,14 @@ constexpr double FX_SCALE = 0x1p31;
 // this: the rounding of g after every update and of the links (2^-32 per
 // real) moves Delta by about 1e-9, so max dAdA cannot go much below 1e-17.
 constexpr double FX_EPS_MIN = 1e-15;
+// Worst effect of the fixed-point rounding on a check's Delta = -A(G K)
+// when the result is g' = normalize(G) on the exact links (Coulomb slices,
+// whose stops are decided by the fused checks): the links contribute 83.2 e
+// (as for the packed links, e = 2^-32), and |g' - G|_F <= 31.2 e (G is
+// within 10.4 e of the unitary matrix it was rounded from, which
+// normalisation moves by at most twice that) enters G K once through G and
+// once through the 8 star terms, 16 x 31.2 e: 582 e, taken as 600 e.
+constexpr double FX_DDELTA = 600.0 * 0x1p-32;
 __device__ __forceinline__ int fx_of(double u) {
   return (int)fmin(fmax(rint(u * FX_SCALE), -2147483647.0), 2147483647.0);
 }

@@ -589,10 +589,13 @@ def test_coulomb_tiny_eps_on_plain_links(gfx, orc):
 @pytest.mark.parametrize("dims,ck", [((8, 8, 8, 6), 5), ((8, 8, 8, 4), 1)])
 def test_coulomb_packed_matches_plain_links(gfx, orc, dims, ck):
     """Mixed Coulomb on packed links (FP32 slices on the high parts, FP64
-    slices on high + low parts) and on the FP32 and FP64 links: every slice
-    stops below the bound on the exact links (host functional of the packed
-    run's result), per-slice F equal to 1e-12 relative, stops within two
-    checks of each other."""
+    slices on 32-bit fixed-point links and g, the slices holding whole
+    workgroups) and on the FP32 and FP64 links: every slice stops below the
+    bound on the exact links (host functional of the packed run's result),
+    per-slice F equal to 1e-12 relative, stops within two checks of each
+    other (within 5 iterations at checkinterval 1: a slice stops once its
+    max dAdA is below eps with the rounding's worst effect added, about 0.74
+    eps at eps = 1e-12)."""
     V = orc.volume(dims)
     u = smooth_config(orc, dims, 63, eps=0.4)
     g = orc.random_init_trafo(V, 64)
@@ -610,7 +613,7 @@ def test_coulomb_packed_matches_plain_links(gfx, orc, dims, ck):
     assert rc0 == 0 and rc1 == 0 and st1[1] > 0 and st1[2] > 0
     assert np.all(M0 < EPS12) and np.all(M1 < EPS12)
     assert np.max(np.abs(F1 - F0) / F0) < 1e-12
-    assert np.all(np.abs(it1 - it0) <= 2 * ck), (it0, it1)
+    assert np.all(np.abs(it1 - it0) <= max(2 * ck, 5)), (it0, it1)
     Fo, Do, Mo = orc.functional_slices(orc.su3_to_2v(u), orc.su3_to_2v(g1), dims, 3)
     assert np.all(Mo < EPS12 * (1 + 1e-9)) and np.max(np.abs(Fo - F1)) < 1e-12
 

@@ -605,3 +605,65 @@ def test_short_star_packing_bound(orc):
             d = np.abs(np.sqrt(shp) - np.sqrt(exact)).max()
             assert d < 139.2 * e, (lo, d / e)
             assert d > 1e-2 * e  # the rounding is seen
+
+
+def test_fixed_point_bound_on_theta(orc):
+    """The stop rule of Coulomb slices polished on 32-bit fixed-point links
+    and g takes the rounding's worst effect on Delta as 600 e (e = 2^-32): the
+    check computes Delta = -A(G K) from the rounded rows of g (row 2 rebuilt)
+    and the rounded links (reference-order star), while the slice's result
+    is g' = normalize(G) on the exact links.  Per site, sqrt(theta) of the
+    two differ by less than 600 e on random links and g (both with 3 and 4
+    directions).  The bound: links 83.2 e (as for the packed links), plus
+    |normalize(G) - G|_F <= 31.2 e entering G K once through G and once
+    through the 8 star terms (16 x 31.2 e)."""
+    dims = (4, 4, 4, 6)
+    V = orc.volume(dims)
+    e = 2.0 ** -32
+    for seed in (5301, 5302):
+        u = orc.random_init_gauge(V, seed)
+        g = orc.random_init_trafo(V, seed + 50)
+        u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
+        ux, gx = fx_round(u2), fx_round(g2)
+        # g' = normalize(G) with G = rounded rows, row 2 rebuilt
+        gn = orc.from_2v(gx).reshape(-1, 18)
+        for k in range(V):
+            orc.lib.orc_su3_normalize(gn[k])
+        gn = gn.ravel()
+        for ndir in (4, 3):
+            # the star form equals the per-link form on exact fields
+            ex = np_site_theta(orc.g_trafo(u, g, dims), dims, ndir=ndir).reshape(-1)
+            assert np.abs(np.sqrt(np_star_theta(u2, g2, dims, ndir)) - np.sqrt(ex)).max() < 1e-12
+            # the check's Delta: reference-order star on the rounded fields
+            th_c = np_star_theta(ux, gx, dims, ndir)
+            th_t = np_site_theta(orc.g_trafo(u, gn, dims), dims, ndir=ndir).reshape(-1)
+            d = np.abs(np.sqrt(th_c) - np.sqrt(th_t)).max()
+            assert d < 600 * e, (ndir, d / e)
+            assert d > 1e-2 * e
+
+
+def np_star_theta(u2, g2, dims, ndir=4):
+    """Per-site |A(G(x) K(x))|_F^2 with the star K = sum_mu U_mu(x) G^+(x+mu)
+    + U_mu^+(x-mu) G^+(x-mu) formed from full matrices whose row 2 is
+    rebuilt from the stored rows 0, 1 (the reference-order star of the
+    sweeps and their fused checks); numpy."""
+    LX, LY, LZ, LT = dims
+
+    def full(a2):
+        r = a2.reshape(-1, 2, 3, 2)
+        m = r[..., 0] + 1j * r[..., 1]
+        return np.concatenate([m, np.conj(np.cross(m[:, 0], m[:, 1]))[:, None]], axis=1)
+
+    U = full(u2).reshape(LT, LZ, LY, LX, 4, 3, 3)
+    G = full(g2).reshape(LT, LZ, LY, LX, 3, 3)
+    dag = lambda m: np.conj(np.swapaxes(m, -1, -2))
+    K = np.zeros(G.shape, dtype=complex)
+    for mu in range(ndir):
+        ax = 3 - mu
+        Um = U[..., mu, :, :]
+        K += Um @ dag(np.roll(G, -1, axis=ax))
+        K += dag(np.roll(Um, 1, axis=ax)) @ dag(np.roll(G, 1, axis=ax))
+    M = G @ K
+    A = (M - dag(M)) * (-0.5j)
+    A = A - (np.trace(A, axis1=-2, axis2=-1) / 3.0)[..., None, None] * np.eye(3)
+    return (np.abs(A) ** 2).sum(axis=(-2, -1)).reshape(-1)
