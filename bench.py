@@ -28,7 +28,8 @@ profiles/sweep_variants_32x4.json (made by --profile under rocprofv3 and
 gribov: the 64-copy Gribov workload at its stated schedule, run once: 64
 Haar-random copies of one 24^4 beta = 6.0 configuration, simulated annealing
 from T = 0.45 to 0.01 over 4000 steps of one SA sweep and 3 microcanonical
-sweeps, then mixed OR to theta < 1e-12 and the copy with the largest F.
+sweeps (in the mixed form: FP32 links and g), then mixed OR to theta < 1e-12
+and the copy with the largest F.
 
 cpu_baseline: an OpenMP CPU half sweep built from the reference's own host
 SU(3) routines (su3manip.c, complex.c) on its 18-real AoS layout
@@ -477,6 +478,8 @@ def gribov_run(gfx):
     gfx.set_mt_params(synthetic_mt_params())
     gfx.set_mt_seed(G["mtseed"])
     b = gfx.Batch()
+    # the mixed form throughout: the anneal on FP32 links and g as well
+    b.set_anneal_precision(gfx.PREC_MIXED)
     base = b.add(G["dims"], u, gfx.unit_trafo())
     for _ in range(G["copies"] - 1):
         b.add_random_copy(base)
@@ -493,13 +496,15 @@ def gribov_run(gfx):
     res = [b.result(q) for q in range(G["copies"])]
     st = b.stats()
     qb, Fb = b.best_copy(base)
+    b.set_anneal_precision(gfx.PREC_FP64)
     b.close()
     gfx.finalize_MT()
     V = int(np.prod(G["dims"]))
     stops = [r[3] for r in res]
     return {"workload": "64 Haar-random copies of one 24^4 beta=6.0 configuration (500 heatbath "
-                        "sweeps), SA T=0.45->0.01 over 4000 steps of 1 SA + 3 microcanonical sweeps, "
-                        "mixed OR (switch 1e-6) to theta<1e-12, best copy by F",
+                        "sweeps), SA T=0.45->0.01 over 4000 steps of 1 SA + 3 microcanonical sweeps "
+                        "(FP32 links and g, FP64 sampling), mixed OR (switch 1e-6) to theta<1e-12, "
+                        "best copy by F",
             "rc": rc, "setup_s": setup, "sa_s": t_sa, "sa_ms_per_step": 1e3 * t_sa / G["steps"],
             "sa_site_sweeps_per_s": G["copies"] * V * 4 * G["steps"] / t_sa,
             "or_s": t_or, "or_site_updates": st["site_updates"],

@@ -403,6 +403,10 @@ class Batch:
         rc = load().gfx_batch_variant(int(precision), C.byref(v), ms.ctypes.data_as(C.c_void_p))
         return (v.value, ms) if rc == 0 else None
 
+    def set_anneal_precision(self, mode):
+        """PREC_FP64 (default) or PREC_MIXED: the anneal on FP32 links and g."""
+        _check(load().gfx_batch_set_anneal_precision(int(mode)), "anneal precision")
+
     def anneal(self, nsteps, Tmin, Tmax, expo=-1.0, nmicro=0, sampler=0, steps_per_graph=8):
         """Simulated annealing of every lattice's g (see gfx_batch_anneal)."""
         _check(load().gfx_batch_anneal(int(nsteps), float(Tmin), float(Tmax), float(expo),

@@ -76,6 +76,7 @@ struct Batch {
   int variant[2] = {0, 0};
   bool tuned[2] = {false, false};
   bool packed = false;         // the current run sweeps on packed links
+  bool anneal_fp32 = false;    // gfx_batch_anneal on FP32 links and g
   bool tuned_packed = false;   // the link form the variants were timed on
   float tune_ms[2][SWEEP_VARIANTS] = {};
   // statistics of the last run
@@ -734,14 +735,26 @@ int gfx_batch_anneal(int nsteps, double Tmin, double Tmax, double expo, int nmic
   const int k = block_index(sa_block_size());
   const int2* map = b.d_bmap[k][0];
   const int nblk = b.nbmap[k];
+  // FP32 anneal: every lattice's g (and its links) in FP32 for the whole
+  // schedule, reunitarised into FP64 at the end
+  const bool f32 = b.anneal_fp32;
+  if (f32) {
+    if ((rc = ensure_fp32(b))) {
+      hipFree(d_betas);
+      hipFree(d_step);
+      return rc;
+    }
+    set_pointers(b);
+  }
+  const BatchProblem* probs = b.d_probs[f32 ? 1 : 0];
   auto one_step = [&]() {
     for (int p = 0; p < 2; p++) {
-      launch_batch_sa_sweep(b.d_probs[0], map, nblk, b.d_roff, p, d_betas, d_step, sampler,
-                            s.rnd_gauss, s.rnd_unif, s.stream);
+      launch_batch_sa_sweep(probs, map, nblk, b.d_roff, p, d_betas, d_step, sampler, s.rnd_gauss,
+                            s.rnd_unif, s.stream, f32);
       mt_refill(s, 1);
     }
     for (int m = 0; m < nmicro; m++)
-      for (int p = 0; p < 2; p++) launch_batch_micro_sweep(b.d_probs[0], map, nblk, p, s.stream);
+      for (int p = 0; p < 2; p++) launch_batch_micro_sweep(probs, map, nblk, p, s.stream, f32);
     launch_batch_advance(d_step, 1, s.stream);
   };
   const auto t0 = std::chrono::steady_clock::now();
@@ -768,11 +781,21 @@ int gfx_batch_anneal(int nsteps, double Tmin, double Tmax, double expo, int nmic
     one_step();
     done++;
   }
+  if (f32)
+    for (const BProbHost& p : b.probs) launch_g32_to_g64(p.g32, p.g64, p.geo, s.stream);
   if (hip_check(hipStreamSynchronize(s.stream), "anneal") && rc == GFX_OK) rc = GFX_ERR_DEVICE;
   b.run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   hipFree(d_betas);
   hipFree(d_step);
   return rc;
+}
+
+// gfx_batch_anneal in FP64 (GFX_PREC_FP64, default: the reference's
+// arithmetic) or on FP32 links and g with FP32 arithmetic (GFX_PREC_MIXED)
+int gfx_batch_set_anneal_precision(int mode) {
+  if (mode != GFX_PREC_FP64 && mode != GFX_PREC_MIXED) return GFX_ERR_STATE;
+  g_b.anneal_fp32 = mode == GFX_PREC_MIXED;
+  return GFX_OK;
 }
 
 // the lattice with the largest F among those sharing lattice base's links

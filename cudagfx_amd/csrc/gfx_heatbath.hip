@@ -119,8 +119,10 @@ __device__ __forceinline__ bool sample_kp(double alpha[4], const double w[4], do
   return true;
 }
 
-template <int SAMPLER>
-__device__ __forceinline__ void cm_heatbath(M3<double>& h, const M3<double>& K, double beta,
+// one heatbath update of h in each SU(2) subgroup; the subgroup vector w
+// comes from h K in the storage arithmetic R, the sampling runs in FP64
+template <int SAMPLER, typename R>
+__device__ __forceinline__ void cm_heatbath(M3<R>& h, const M3<R>& K, double beta,
                                             const float* __restrict__ gauss,
                                             const float* __restrict__ unif, int i) {
   int sub = 0;
@@ -128,26 +130,28 @@ __device__ __forceinline__ void cm_heatbath(M3<double>& h, const M3<double>& K, 
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = a + 1; b < 3; b++) {
+      R wr[4];
       double w[4], al[4];
-      su2_subvec(h, K, a, b, w[0], w[1], w[2], w[3]);
+      su2_subvec(h, K, a, b, wr[0], wr[1], wr[2], wr[3]);
+#pragma unroll
+      for (int c = 0; c < 4; c++) w[c] = (double)wr[c];
       if (SAMPLER == 0) {
         sample_ref(al, w, beta, gauss, unif, i);
-        su2_left(h, a, b, al[0], al[1], al[2], al[3]);
+        su2_left(h, a, b, (R)al[0], (R)al[1], (R)al[2], (R)al[3]);
       } else {
         const float* g = gauss + (size_t)KP_NG * i + sub * KP_NT;
         const float* u = unif + (size_t)KP_NU * i + sub * (2 * KP_NT + 2);
-        if (sample_kp(al, w, beta, g, u)) su2_left(h, a, b, al[0], al[1], al[2], al[3]);
+        if (sample_kp(al, w, beta, g, u)) su2_left(h, a, b, (R)al[0], (R)al[1], (R)al[2], (R)al[3]);
       }
       sub++;
     }
 }
 
 // star K of g at site i (same as the overrelaxation sweep)
-template <int NDIR>
-__device__ __forceinline__ M3<double> star_of(const double2* __restrict__ U,
-                                              const double2* __restrict__ gop, const Geom& geo,
-                                              const Site& s, int i, int parity) {
-  M3<double> K;
+template <int NDIR, typename V = double2, typename R = double>
+__device__ __forceinline__ M3<R> star_of(const V* __restrict__ U, const V* __restrict__ gop,
+                                         const Geom& geo, const Site& s, int i, int parity) {
+  M3<R> K;
   m_zero(K);
   // direction loop kept rolled (one direction's operands live at a time:
   // unrolled, these kernels spilled to scratch at 1 wave/SIMD), the four
@@ -155,7 +159,7 @@ __device__ __forceinline__ M3<double> star_of(const double2* __restrict__ U,
   // short form (the stochastic updates have no bitwise target)
 #pragma unroll 1
   for (int mu = 0; mu < NDIR; mu++)
-    star_terms_short<double2, double, false, const double2*, 1>(K, U, gop, geo, s, i, parity, mu);
+    star_terms_short<V, R, false, const V*, 1>(K, U, gop, geo, s, i, parity, mu);
   return K;
 }
 
@@ -179,17 +183,18 @@ __global__ __launch_bounds__(HB_BLK) void k_sa_sweep(const double2* __restrict__
 
 // microcanonical reflection in each SU(2) subgroup:
 // heatbath_thermalization.cu:318-338, alpha = 2 w0 w/|w|^2 - e0
-__device__ __forceinline__ void micro_reflect(M3<double>& h, const M3<double>& K) {
+template <typename R>
+__device__ __forceinline__ void micro_reflect(M3<R>& h, const M3<R>& K) {
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = a + 1; b < 3; b++) {
-      double w[4];
+      R w[4];
       su2_subvec(h, K, a, b, w[0], w[1], w[2], w[3]);
-      const double ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
-      if (!(fabs(ww) < 1.0e-9)) {
-        const double f = 2.0 * w[0] / ww;
-        su2_left(h, a, b, f * w[0] - 1.0, f * w[1], f * w[2], f * w[3]);
+      const R ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2] + w[3] * w[3];
+      if (!(fabs(ww) < R(1.0e-9))) {
+        const R f = R(2) * w[0] / ww;
+        su2_left(h, a, b, f * w[0] - R(1), f * w[1], f * w[2], f * w[3]);
       }
     }
 }
@@ -216,7 +221,7 @@ __global__ __launch_bounds__(HB_BLK) void k_micro_sweep(const double2* __restric
 // numbers of lattice q start at site roff[q] of the fields; the inverse
 // temperature of the current step is betas[*step] (step counter on the
 // device, so a captured graph of several steps replays unchanged).
-template <int SAMPLER>
+template <int SAMPLER, typename V, typename R>
 __global__ __launch_bounds__(HB_BLK) void k_batch_sa_sweep(const BatchProblem* __restrict__ probs,
                                                            const int2* __restrict__ bmap,
                                                            const int* __restrict__ roff, int parity,
@@ -230,15 +235,16 @@ __global__ __launch_bounds__(HB_BLK) void k_batch_sa_sweep(const BatchProblem* _
   const int i = m.y * HB_BLK + threadIdx.x;
   if (i >= geo.nh) return;
   const Site s = site_coords(geo, i, parity);
-  double2* gme = static_cast<double2*>(P.g[parity]);
-  const M3<double> K = star_of<4>(static_cast<const double2*>(P.U),
-                                  static_cast<const double2*>(P.g[1 - parity]), geo, s, i, parity);
-  M3<double> h = load_m<double2, double>(gme, geo.stride, i);
+  V* gme = static_cast<V*>(P.g[parity]);
+  const M3<R> K = star_of<4, V, R>(static_cast<const V*>(P.U), static_cast<const V*>(P.g[1 - parity]),
+                                   geo, s, i, parity);
+  M3<R> h = load_m<V, R>(gme, geo.stride, i);
   cm_heatbath<SAMPLER>(h, K, betas[*step], gauss, unif, roff[m.x] + i);
   m_normalize(h);
-  Store<double2>::store(gme, geo.stride, i, h);
+  Store<V>::store(gme, geo.stride, i, h);
 }
 
+template <typename V, typename R>
 __global__ __launch_bounds__(HB_BLK) void k_batch_micro_sweep(const BatchProblem* __restrict__ probs,
                                                               const int2* __restrict__ bmap,
                                                               int parity) {
@@ -248,13 +254,13 @@ __global__ __launch_bounds__(HB_BLK) void k_batch_micro_sweep(const BatchProblem
   const int i = m.y * HB_BLK + threadIdx.x;
   if (i >= geo.nh) return;
   const Site s = site_coords(geo, i, parity);
-  double2* gme = static_cast<double2*>(P.g[parity]);
-  const M3<double> K = star_of<4>(static_cast<const double2*>(P.U),
-                                  static_cast<const double2*>(P.g[1 - parity]), geo, s, i, parity);
-  M3<double> h = load_m<double2, double>(gme, geo.stride, i);
+  V* gme = static_cast<V*>(P.g[parity]);
+  const M3<R> K = star_of<4, V, R>(static_cast<const V*>(P.U), static_cast<const V*>(P.g[1 - parity]),
+                                   geo, s, i, parity);
+  M3<R> h = load_m<V, R>(gme, geo.stride, i);
   micro_reflect(h, K);
   m_normalize(h);
-  Store<double2>::store(gme, geo.stride, i, h);
+  Store<V>::store(gme, geo.stride, i, h);
 }
 
 // staple of U_nu(x): sum_{mu != nu} U_mu(x+nu) U_nu^+(x+mu) U_mu^+(x)
@@ -398,20 +404,38 @@ void launch_su2_sample(int sampler, double beta, double wnorm, int n, const floa
     hipLaunchKernelGGL(k_su2_sample<1>, grid, dim3(HB_BLK), 0, st, beta, wnorm, n, gauss, unif, out);
 }
 
+template <typename V, typename R>
+static void launch_bsa(const BatchProblem* probs, const int2* bmap, int nblocks, const int* roff,
+                       int parity, const double* betas, const int* step, int sampler,
+                       const float* gauss, const float* unif, hipStream_t st) {
+  if (sampler == 0)
+    hipLaunchKernelGGL((k_batch_sa_sweep<0, V, R>), dim3(nblocks), dim3(HB_BLK), 0, st, probs, bmap,
+                       roff, parity, betas, step, gauss, unif);
+  else
+    hipLaunchKernelGGL((k_batch_sa_sweep<1, V, R>), dim3(nblocks), dim3(HB_BLK), 0, st, probs, bmap,
+                       roff, parity, betas, step, gauss, unif);
+}
+
+// fp32: the lattices' FP32 problems (FP32 links and g); else FP64
 void launch_batch_sa_sweep(const BatchProblem* probs, const int2* bmap, int nblocks,
                            const int* roff, int parity, const double* betas, const int* step,
-                           int sampler, const float* gauss, const float* unif, hipStream_t st) {
-  if (sampler == 0)
-    hipLaunchKernelGGL(k_batch_sa_sweep<0>, dim3(nblocks), dim3(HB_BLK), 0, st, probs, bmap, roff,
-                       parity, betas, step, gauss, unif);
+                           int sampler, const float* gauss, const float* unif, hipStream_t st,
+                           bool fp32) {
+  if (fp32)
+    launch_bsa<float4, float>(probs, bmap, nblocks, roff, parity, betas, step, sampler, gauss, unif, st);
   else
-    hipLaunchKernelGGL(k_batch_sa_sweep<1>, dim3(nblocks), dim3(HB_BLK), 0, st, probs, bmap, roff,
-                       parity, betas, step, gauss, unif);
+    launch_bsa<double2, double>(probs, bmap, nblocks, roff, parity, betas, step, sampler, gauss, unif,
+                                st);
 }
 
 void launch_batch_micro_sweep(const BatchProblem* probs, const int2* bmap, int nblocks, int parity,
-                              hipStream_t st) {
-  hipLaunchKernelGGL(k_batch_micro_sweep, dim3(nblocks), dim3(HB_BLK), 0, st, probs, bmap, parity);
+                              hipStream_t st, bool fp32) {
+  if (fp32)
+    hipLaunchKernelGGL((k_batch_micro_sweep<float4, float>), dim3(nblocks), dim3(HB_BLK), 0, st, probs,
+                       bmap, parity);
+  else
+    hipLaunchKernelGGL((k_batch_micro_sweep<double2, double>), dim3(nblocks), dim3(HB_BLK), 0, st,
+                       probs, bmap, parity);
 }
 
 int sa_block_size() { return HB_BLK; }

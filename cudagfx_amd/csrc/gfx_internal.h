@@ -120,11 +120,13 @@ void launch_batch_convert(const BatchProblem* probs32, const BatchProblem* probs
 int batch_convert_block();
 void launch_batch_advance(int* iter, int step, hipStream_t st);
 // batched annealing (gfx_heatbath.hip); bmap of sa_block_size() blocks
+// fp32: the FP32 problems (FP32 links and g, FP32 arithmetic; sampling in FP64)
 void launch_batch_sa_sweep(const BatchProblem* probs, const int2* bmap, int nblocks,
                            const int* roff, int parity, const double* betas, const int* step,
-                           int sampler, const float* gauss, const float* unif, hipStream_t st);
+                           int sampler, const float* gauss, const float* unif, hipStream_t st,
+                           bool fp32 = false);
 void launch_batch_micro_sweep(const BatchProblem* probs, const int2* bmap, int nblocks, int parity,
-                              hipStream_t st);
+                              hipStream_t st, bool fp32 = false);
 int sa_block_size();
 int kp_gauss_per_site();
 int kp_unif_per_site();

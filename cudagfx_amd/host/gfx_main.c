@@ -114,6 +114,8 @@ static int run_batch(const char *first) {
     printf("Starting simulated annealing of every start...\n");
     printf("Tmin = %f, Tmax = %f, N = %d, expo = %f\n", saparam.Tmin, saparam.Tmax, saparam.N,
            saparam.expo);
+    /* a mixed-precision batch anneals on FP32 links and g as well */
+    gfx_batch_set_anneal_precision(orxprecision);
     if ((rc = gfx_batch_anneal(saparam.N, saparam.Tmin, saparam.Tmax, saparam.expo, samicro,
                                sasampler, 8)))
       return rc;

@@ -275,6 +275,12 @@ int gfx_batch_variant(int precision, int *variant, double *ms_per_variant);
  * over the batch, steps replayed as a HIP graph of steps_per_graph steps */
 int gfx_batch_anneal(int nsteps, double Tmin, double Tmax, double expo, int nmicro, int sampler,
                      int steps_per_graph);
+/* precision of gfx_batch_anneal: GFX_PREC_FP64 (default; the reference's SA
+ * arithmetic, simulated_annealing.cu:5-257) or GFX_PREC_MIXED: links and g in
+ * FP32 with FP32 arithmetic (the SU(2) sampling stays FP64) for the whole
+ * schedule, g reunitarised into FP64 at the end.  Kept until changed (also
+ * across gfx_batch_create). */
+int gfx_batch_set_anneal_precision(int mode);
 /* after gfx_batch_run: the lattice with the largest F among those sharing
  * the links of `base` (the Gribov copy to keep) */
 int gfx_batch_best_copy(int base, int *q_best, double *F_best);

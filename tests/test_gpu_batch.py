@@ -227,6 +227,46 @@ def test_gribov_batch_anneal_then_fix(gfx, orc):
     assert Fb == max(r[0] for r in res) and res[qb][0] == Fb
 
 
+def test_gribov_batch_fp32_anneal_then_fix(gfx, orc):
+    """The anneal on FP32 links and g (gfx_batch_set_anneal_precision): every
+    annealed g comes back unitary in FP64, F rises as much as in the FP64
+    anneal from the same starts and twister numbers (mean over the copies
+    within 0.02), and every copy then reaches the bound in the mixed OR."""
+    dims = (8, 8, 8, 8)
+    V = orc.volume(dims)
+    u = _thermalized(gfx, dims, 6.0, 40, 51)
+    gfx.set_omega(1.7)
+    out = {}
+    for mode in (gfx.PREC_FP64, gfx.PREC_MIXED):
+        gfx.set_mt_params(synthetic_params(52).ravel())
+        gfx.set_mt_seed(52)
+        b = gfx.Batch()
+        b.set_anneal_precision(mode)
+        base = b.add(dims, u, orc.random_init_trafo(V, 53))
+        for _ in range(7):
+            b.add_random_copy(base)
+        starts = [b.trafo(q) for q in range(8)]
+        b.anneal(200, 0.01, 0.45, -1.0, 3, 0, steps_per_graph=16)
+        annealed = [b.trafo(q) for q in range(8)]
+        rc = b.run(8000, EPS12, 10, gfx.PREC_MIXED, 1e-6, 4)
+        res = [b.result(q) for q in range(8)]
+        b.set_anneal_precision(gfx.PREC_FP64)
+        b.close()
+        gfx.finalize_MT()
+        out[mode] = (starts, annealed, rc, res)
+    F_after = {}
+    for mode, (starts, annealed, rc, res) in out.items():
+        assert rc == 0 and all(r[2] < EPS12 for r in res)
+        F_after[mode] = np.mean([_host_F(orc, u, a, dims) for a in annealed])
+        for q in range(8):
+            assert _host_F(orc, u, annealed[q], dims) > _host_F(orc, u, starts[q], dims) + 0.3
+            m = annealed[q].reshape(-1, 3, 3, 2)
+            m = m[..., 0] + 1j * m[..., 1]
+            assert np.abs(m @ np.conj(np.swapaxes(m, -1, -2)) - np.eye(3)).max() < 1e-13
+    assert np.array_equal(out[gfx.PREC_FP64][0][1], out[gfx.PREC_MIXED][0][1])  # same starts
+    assert abs(F_after[gfx.PREC_MIXED] - F_after[gfx.PREC_FP64]) < 0.02, F_after
+
+
 def test_gribov_batch_full_shape(gfx, orc):
     """The Gribov-copy workload at its real shape: 64 Haar-random copies of
     one 24^4 beta = 6.0 configuration, annealed from T = 0.45 to 0.01 with 3
