@@ -845,8 +845,9 @@ int gfx_coulomb_gauge(int maxit, double eps, int checkinterval, int* iters) {
   const bool packed = mixed && s.packed_links && eps > PACK_EPS_MIN;
   // FP64 slices of a run on packed links are polished on fixed-point links
   // and g when their stops can be decided by the fused checks (slices of
-  // whole workgroups of either size) and eps is well above that form's reach
-  const bool fxc = packed && eps > FX_EPS_MIN && nsh % 256 == 0;
+  // whole workgroups of either size) and eps is well above the fixed-point
+  // rounding bound those stops carry
+  const bool fxc = packed && eps > FX_EPS_BOUND_MIN && nsh % 256 == 0;
   // every exit leaves the packed forms off
   struct PackedOff {
     State& s;

@@ -33,6 +33,10 @@ struct BProbHost {
   PackedArgs pk{};  // packed links (owner of the links only; mixed runs)
   double2* g64 = nullptr;
   float4* g32 = nullptr;
+  // 32-bit fixed-point links (owner only; 8 fields) and g (3 fields) of the
+  // FP64 phase of mixed runs
+  int4* Ufx = nullptr;
+  int4* gfx = nullptr;
   int ushare = 0;  // problem whose links this one uses (itself if it owns them)
   int stop_iter = -1;  // iteration of the stop check (-1: not converged)
   int switch_iter = -1;  // mixed: iteration of the check that left FP32
@@ -71,13 +75,15 @@ struct Batch {
   int* d_stall = nullptr;     // checks since it last halved
   double* d_results = nullptr;
   double* h_results = nullptr;
+  double* d_fres = nullptr;   // functionals of the final states (4 per problem)
   int* d_iter = nullptr;
   bool built = false;
   int variant[2] = {0, 0};
   bool tuned[2] = {false, false};
   bool packed = false;         // the current run sweeps on packed links
+  bool fixed = false;          // and its FP64 phase on fixed-point links and g
   bool anneal_fp32 = false;    // gfx_batch_anneal on FP32 links and g
-  bool tuned_packed = false;   // the link form the variants were timed on
+  int tuned_links = -1;        // the link form (BLINK_*) the variants were timed on
   float tune_ms[2][SWEEP_VARIANTS] = {};
   // statistics of the last run
   double run_ms = 0.0;
@@ -121,6 +127,7 @@ static void batch_free_maps(Batch& b) {
   hipFree(b.d_roff);
   hipFree(b.d_stall);
   hipFree(b.d_results);
+  hipFree(b.d_fres);
   hipFree(b.d_iter);
   if (b.h_act) hipHostFree(b.h_act);
   if (b.h_sw) hipHostFree(b.h_sw);
@@ -130,6 +137,7 @@ static void batch_free_maps(Batch& b) {
   b.d_roff = nullptr;
   b.d_stall = nullptr;
   b.d_results = nullptr;
+  b.d_fres = nullptr;
   b.d_iter = nullptr;
   b.h_act = b.h_sw = nullptr;
   b.h_results = nullptr;
@@ -141,10 +149,12 @@ static void batch_free(Batch& b) {
   for (auto& p : b.probs) {
     hipFree(p.g64);
     hipFree(p.g32);
+    hipFree(p.gfx);
     if (&p - &b.probs[0] == p.ushare) {
       hipFree(p.U64);
       hipFree(p.U32);
       hipFree(p.pk.a);
+      hipFree(p.Ufx);
     }
   }
   b.probs.clear();
@@ -214,6 +224,7 @@ static int batch_build(Batch& b) {
       dev_alloc((void**)&b.d_conv, sizeof(int) * n, "batch conversions") ||
       dev_alloc((void**)&b.d_sw, sizeof(int) * n, "batch switches") ||
       dev_alloc((void**)&b.d_results, sizeof(double) * 4 * n, "batch results") ||
+      dev_alloc((void**)&b.d_fres, sizeof(double) * 4 * n, "batch results") ||
       dev_alloc((void**)&b.d_iter, sizeof(int), "batch iter"))
     return GFX_ERR_ALLOC;
   if (hipHostMalloc((void**)&b.h_act, sizeof(int) * 2 * n, 0) != hipSuccess ||
@@ -230,7 +241,8 @@ static V* bfield(V* g, const Geom& geo, int k) {
 }
 
 // canonical field pointers in both precisions: even = field 0, odd =
-// field 1, spare = field 2 (FP32 pointers only where FP32 fields exist)
+// field 1, spare = field 2 (FP32 pointers only where FP32 fields exist; the
+// FP64 phase's fields are the fixed-point ones in a run with b.fixed)
 static void set_pointers(Batch& b) {
   const size_t n = b.probs.size();
   std::vector<BatchProblem> h64(n), h32(n);
@@ -238,16 +250,22 @@ static void set_pointers(Batch& b) {
     const BProbHost& p = b.probs[q];
     const BProbHost& o = b.probs[p.ushare];
     h64[q].geo = h32[q].geo = p.geo;
-    h64[q].U = o.U64;
+    h64[q].U = b.fixed ? (const void*)o.Ufx : (const void*)o.U64;
     h32[q].U = o.U32;
     const PackedLinks pk = b.packed ? PackedLinks{o.pk.a, o.pk.b, o.pk.c, o.pk.lo} : PackedLinks{};
-    h64[q].pk = h32[q].pk = pk;
-    for (int k = 0; k < 2; k++) {
-      h64[q].g[k] = bfield(p.g64, p.geo, k);
-      h32[q].g[k] = p.g32 ? (void*)bfield(p.g32, p.geo, k) : nullptr;
+    h64[q].pk = b.fixed ? PackedLinks{} : pk;
+    h32[q].pk = pk;
+    for (int k = 0; k < 3; k++) {
+      void* f64 = b.fixed ? (void*)bfield(p.gfx, p.geo, k) : (void*)bfield(p.g64, p.geo, k);
+      void* f32 = p.g32 ? (void*)bfield(p.g32, p.geo, k) : nullptr;
+      if (k < 2) {
+        h64[q].g[k] = f64;
+        h32[q].g[k] = f32;
+      } else {
+        h64[q].galt = f64;
+        h32[q].galt = f32;
+      }
     }
-    h64[q].galt = bfield(p.g64, p.geo, 2);
-    h32[q].galt = p.g32 ? (void*)bfield(p.g32, p.geo, 2) : nullptr;
   }
   hipStreamSynchronize(g_st.stream);
   hipMemcpy(b.d_probs[0], h64.data(), sizeof(BatchProblem) * n, hipMemcpyHostToDevice);
@@ -255,7 +273,10 @@ static void set_pointers(Batch& b) {
 }
 
 // after a run: copy each FP64 even field the checks left in the spare slot
-// back to field 0
+// back to field 0.  Fixed-point g (b.fixed) then goes to the FP64 fields,
+// reunitarised (the state the stop bound holds for), and each lattice's F,
+// dAdA and max dAdA are evaluated again on its FP64 links and g (the checks
+// saw the rounded fields); the stop iterations stay.
 static void batch_canonical(Batch& b) {
   State& s = g_st;
   std::vector<BatchProblem> h(b.probs.size());
@@ -263,8 +284,17 @@ static void batch_canonical(Batch& b) {
   hipMemcpy(h.data(), b.d_probs[0], sizeof(BatchProblem) * h.size(), hipMemcpyDeviceToHost);
   for (size_t q = 0; q < b.probs.size(); q++) {
     const BProbHost& p = b.probs[q];
-    if (h[q].g[0] != (void*)p.g64)
-      hipMemcpyAsync(p.g64, h[q].g[0], field_bytes64(p.geo), hipMemcpyDeviceToDevice, s.stream);
+    void* f0 = b.fixed ? (void*)p.gfx : (void*)p.g64;
+    const size_t bytes = b.fixed ? field_bytes_fx(p.geo) : field_bytes64(p.geo);
+    if (h[q].g[0] != f0) hipMemcpyAsync(f0, h[q].g[0], bytes, hipMemcpyDeviceToDevice, s.stream);
+    if (!b.fixed) continue;
+    launch_fx_to_g64(p.gfx, p.g64, p.geo, s.stream);
+    // partials: the FP64 odd-check slots, ceil(nh / 64) blocks per lattice
+    // in all, at least the functional's blocks of the largest lattice
+    launch_functional_f64(b.probs[p.ushare].U64, bfield(p.g64, p.geo, 0), bfield(p.g64, p.geo, 1),
+                          p.geo, 4, 1, b.d_pod[0], 0.0, b.d_fres + 4 * q, s.stream);
+    hipMemcpyAsync(b.d_results + 4 * q, b.d_fres + 4 * q, 3 * sizeof(double),
+                   hipMemcpyDeviceToDevice, s.stream);
   }
   hipStreamSynchronize(s.stream);
 }
@@ -304,6 +334,24 @@ static int ensure_packed(Batch& b) {
   return GFX_OK;
 }
 
+// the fixed-point links of every link set (8 fields) and the fixed-point
+// g fields (3) of every lattice, made from the FP64 links; g is written by
+// the conversions out of FP32
+static int ensure_fixed(Batch& b) {
+  State& s = g_st;
+  for (size_t q = 0; q < b.probs.size(); q++) {
+    BProbHost& p = b.probs[q];
+    if ((int)q == p.ushare) {
+      if (!p.Ufx && dev_alloc((void**)&p.Ufx, 8 * field_bytes_fx(p.geo), "batch fixed-point links"))
+        return GFX_ERR_ALLOC;
+      launch_f64_to_fx(p.U64, p.Ufx, p.geo, 8, s.stream);
+    }
+    if (!p.gfx && dev_alloc((void**)&p.gfx, 3 * field_bytes_fx(p.geo), "batch fixed-point trafo"))
+      return GFX_ERR_ALLOC;
+  }
+  return GFX_OK;
+}
+
 // the workgroup map of precision pr for variant v: the blocks of the
 // lattices sweeping in pr at the last census, in the variant's order, dealt
 // to the XCDs (stopped or other-precision lattices cost no workgroups)
@@ -325,6 +373,10 @@ static const Batch::CMap& batch_map(Batch& b, int pr, int v) {
   return m;
 }
 
+static int links_of(const Batch& b) {
+  return b.fixed ? BLINK_FIXED : b.packed ? BLINK_PACKED : BLINK_F64;
+}
+
 // one half sweep in precision pr (0 FP64, 1 FP32) of the lattices sweeping
 // in it; chk as in launch_batch_sweep
 static void batch_half(Batch& b, int pr, int parity, int chk) {
@@ -334,7 +386,7 @@ static void batch_half(Batch& b, int pr, int parity, int chk) {
   if (m.n == 0) return;
   launch_batch_sweep(pr == 1, b.d_probs[pr], m.d, m.n, b.d_act[pr], b.d_poff[vblock_index(v)],
                      parity, s.omega, v, chk, chk == 2 ? b.d_pev[pr] : b.d_pod[pr], s.stream,
-                     b.packed);
+                     links_of(b));
 }
 
 struct RunParams {
@@ -365,14 +417,14 @@ static void batch_check_tail(Batch& b, const bool use[2], const RunParams& rp) {
   c.iter = b.d_iter;
   c.best = b.d_best;
   c.stall = b.d_stall;
-  c.packed = b.packed;
+  c.links = links_of(b);
   const int n = (int)b.probs.size();
   launch_batch_finalize(c, n, 4, rp.eps, rp.switch_theta, rp.maxit, rp.stall_checks, s.stall_gate,
                         s.stream);
   if (use[1]) {
     const int k = block_index(batch_convert_block());
     launch_batch_convert(b.d_probs[1], b.d_probs[0], b.d_bmap[k][0], b.nbmap[k], b.d_conv,
-                         s.stream);
+                         s.stream, b.fixed);
     launch_fill_i32(b.d_conv, n, 0, s.stream);
   }
 }
@@ -559,7 +611,7 @@ static int batch_loop(Batch& b, bool mixed, const RunParams& rp, int* it_end) {
       hipMemcpy(b.d_conv, cv.data(), sizeof(int) * n, hipMemcpyHostToDevice);
       const int k = block_index(batch_convert_block());
       launch_batch_convert(b.d_probs[1], b.d_probs[0], b.d_bmap[k][0], b.nbmap[k], b.d_conv,
-                           s.stream);
+                           s.stream, b.fixed);
       launch_fill_i32(b.d_conv, n, 0, s.stream);
     }
   }
@@ -650,10 +702,14 @@ int gfx_batch_run(int maxit, double eps, int checkinterval, int precision, doubl
   if (mixed && ensure_fp32(b)) return GFX_ERR_ALLOC;
   b.packed = mixed && s.packed_links && eps > PACK_EPS_MIN;
   if (b.packed && ensure_packed(b)) return GFX_ERR_ALLOC;
-  if (b.packed != b.tuned_packed) {
+  // the FP64 phase of a run on packed links sweeps on fixed-point links and
+  // g when eps is well above the fixed-point bound its stops carry
+  b.fixed = b.packed && eps > FX_EPS_BOUND_MIN;
+  if (b.fixed && ensure_fixed(b)) return GFX_ERR_ALLOC;
+  if (links_of(b) != b.tuned_links) {
     // the sweep variants are timed again on the other link form
     b.tuned[0] = b.tuned[1] = false;
-    b.tuned_packed = b.packed;
+    b.tuned_links = links_of(b);
   }
   b.graph_launches = b.iters32 = b.iters64 = 0;
   for (auto& p : b.probs) {
@@ -716,6 +772,9 @@ int gfx_batch_anneal(int nsteps, double Tmin, double Tmax, double expo, int nmic
   if (nu > (1L << 31) - 4096 * 2) return GFX_ERR_GEOMETRY;
   int rc = mt_prepare(s, (int)ng, (int)nu);
   if (rc) return rc;
+  // the anneal works on the FP64 (or FP32) fields, whatever the last run's
+  // FP64 phase swept on
+  b.fixed = false;
   set_pointers(b);
   std::vector<double> betas(nsteps > 0 ? nsteps : 1);
   for (int i = 0; i < nsteps; i++) betas[i] = 1.0 / gfx_sa_temperature(i, Tmin, Tmax, expo, nsteps);

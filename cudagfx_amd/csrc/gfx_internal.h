@@ -93,9 +93,13 @@ int functional_block_size();
 void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
 
 // batched lattices (gfx_batch_api.hip)
+// the FP64 problems' fields: FP64 links and g (BLINK_F64), packed links and
+// FP64 g (BLINK_PACKED), or 32-bit fixed-point links and g (BLINK_FIXED);
+// the FP32 problems read FP32 links (BLINK_F64) or the packed high parts
+enum { BLINK_F64 = 0, BLINK_PACKED = 1, BLINK_FIXED = 2 };
 void launch_batch_sweep(bool fp32, const BatchProblem* probs, const int2* bmap, int nblocks,
                         const int* active, const int* poff, int parity, double omega, int variant,
-                        int chk, double* partials, hipStream_t st, bool packed);
+                        int chk, double* partials, hipStream_t st, int links);
 // device arrays of a batch check, index 0 = FP64 phase, 1 = FP32 phase
 struct BatchCheck {
   const double* pod[2];  // block partials of the odd check sweeps
@@ -110,13 +114,14 @@ struct BatchCheck {
   const int* iter;
   double* best;
   int* stall;
-  bool packed;  // the lattices sweep on packed links (stop rule with the packing bound)
+  int links;  // BLINK_*: the FP64 stops take the packing or fixed-point bound
 };
 void launch_batch_finalize(const BatchCheck& c, int nprob, int ndir, double eps,
                            double switch_theta, int maxit, int stall_checks, double stall_gate,
                            hipStream_t st);
 void launch_batch_convert(const BatchProblem* probs32, const BatchProblem* probs64,
-                          const int2* bmap, int nblocks, const int* conv, hipStream_t st);
+                          const int2* bmap, int nblocks, const int* conv, hipStream_t st,
+                          bool fixed = false);
 int batch_convert_block();
 void launch_batch_advance(int* iter, int step, hipStream_t st);
 // batched annealing (gfx_heatbath.hip); bmap of sa_block_size() blocks

@@ -279,9 +279,12 @@ __global__ __launch_bounds__(BLOCK) void k_or_sweep_packed(const uint4* __restri
 // configurations of different sizes) in one launch.  bmap[block] = (problem,
 // block inside the problem); a problem whose active flag is 0 has converged
 // and its blocks return at once.  Check partials of block (q, k) go to
-// partials[3 * (poff[q] + k)].  Both precisions use the short form of the
-// products (see sweep_site).  PK: the links are read from the packed planes
-// P.pk (FP32: high parts, FP64: high + low parts).
+// partials[3 * (poff[q] + k)].  FP32 and FP64 on FP64 or packed links use
+// the short form of the products (see sweep_site); on fixed-point links and
+// g (V = int4) the star and relaxation keep the reference's order, as in the
+// single-lattice polish, which the fixed-point stop bound assumes.  PK: the
+// links are read from the packed planes P.pk (FP32: high parts, FP64: high
+// + low parts).
 template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK, bool PK>
 __global__ __launch_bounds__(BLOCK) void k_batch_sweep(const BatchProblem* __restrict__ probs,
                                                         const int2* __restrict__ bmap,
@@ -298,13 +301,14 @@ __global__ __launch_bounds__(BLOCK) void k_batch_sweep(const BatchProblem* __res
   V* gout = (CHK == 2) ? static_cast<V*>(P.galt) : gme;
   const int i = m.y * BLOCK + threadIdx.x;
   double F = 0.0, d = 0.0;
+  constexpr bool FSHORT = !std::is_same<V, int4>::value;
   auto site = [&](const Site& s) {
     if constexpr (PK)
       sweep_site<V, R, NDIR, ROLLED, NT, CHK, PackedLinks, true>(P.pk, gop, gme, gout, geo, s, i,
                                                                  parity, c1, c2, F, d);
     else
-      sweep_site<V, R, NDIR, ROLLED, NT, CHK, const V*, true>(U, gop, gme, gout, geo, s, i, parity,
-                                                              c1, c2, F, d);
+      sweep_site<V, R, NDIR, ROLLED, NT, CHK, const V*, FSHORT>(U, gop, gme, gout, geo, s, i,
+                                                                parity, c1, c2, F, d);
   };
   if (CHK == 0) {
     if (i >= geo.nh) return;
@@ -465,7 +469,7 @@ __global__ __launch_bounds__(BLK) void k_batch_finalize(
     double switch_theta, int* __restrict__ act64, int* __restrict__ act32,
     int* __restrict__ conv, int* __restrict__ sw, double* __restrict__ results,
     const int* __restrict__ iter, int maxit, double* __restrict__ best, int* __restrict__ stall,
-    int stall_checks, double stall_gate, int packed) {
+    int stall_checks, double stall_gate, int links) {
   const int q = blockIdx.x;
   const bool f64 = act64[q] != 0;
   if (!f64 && !act32[q]) return;
@@ -490,12 +494,15 @@ __global__ __launch_bounds__(BLK) void k_batch_finalize(
   results[4 * q + 0] = F / (ndir * V);
   results[4 * q + 1] = d / V;
   results[4 * q + 2] = mx;
-  // on packed links an FP64 lattice stops when the packing's worst effect
-  // on Delta (with the short star of the batch sweeps) keeps max dAdA below
-  // eps on the exact links as well
+  // on packed or fixed-point fields an FP64 lattice stops when the
+  // rounding's worst effect on Delta (packed links: with the short star of
+  // the batch sweeps; fixed-point links and g: reference-order star, g
+  // reunitarised at the end) keeps max dAdA below eps on the exact links
+  // and the final FP64 g as well
   if (f64) {
-    const double r = packed ? sqrt(mx) + PACK_DDELTA64_SHORT : 0.0;
-    if (packed ? r * r < eps : mx < eps) {
+    const double bound = links == 1 ? PACK_DDELTA64_SHORT : links == 2 ? FX_DDELTA : 0.0;
+    const double r = sqrt(mx) + bound;
+    if (links ? r * r < eps : mx < eps) {
       results[4 * q + 3] = (double)it;
       act64[q] = 0;
     } else if (!last) {
@@ -533,9 +540,11 @@ __global__ __launch_bounds__(BLK) void k_batch_finalize(
 }
 
 // FP32 -> FP64 g of the lattices with conv[q] != 0 (FP64 Gram-Schmidt
-// reunitarisation, as k_g32_to_g64): bmap of BLK-site blocks, y = parity.
+// reunitarisation, as k_g32_to_g64), stored as FP64 (VD = double2) or as
+// 32-bit fixed point (VD = int4): bmap of BLK-site blocks, y = parity.
 // conv 1 takes the even field from galt (the check sweep's update), conv 2
 // from g[0] (the checked state).
+template <typename VD>
 __global__ __launch_bounds__(BLK) void k_batch_convert(const BatchProblem* __restrict__ probs32,
                                                        const BatchProblem* __restrict__ probs64,
                                                        const int2* __restrict__ bmap,
@@ -548,10 +557,10 @@ __global__ __launch_bounds__(BLK) void k_batch_convert(const BatchProblem* __res
   const int i = m.y * BLK + threadIdx.x;
   if (i >= P.geo.nh) return;
   const float4* src = static_cast<const float4*>(parity ? P.g[1] : (cv == 1 ? P.galt : P.g[0]));
-  double2* dst = static_cast<double2*>(probs64[m.x].g[parity]);
+  VD* dst = static_cast<VD*>(probs64[m.x].g[parity]);
   M3<double> M = load_m<float4, double>(src, P.geo.stride, i);
   m_normalize(M);
-  Store<double2>::store(dst, P.geo.stride, i, M);
+  Store<VD>::store(dst, P.geo.stride, i, M);
 }
 
 __global__ void k_batch_advance(int* iter, int step) {
@@ -1058,27 +1067,42 @@ struct BSweepArgs {
 
 template <typename V, typename R, int BLOCK, bool ROLLED, bool NT, int CHK>
 static void launch_bsweep_v(const BSweepArgs& a, R c1, R c2, hipStream_t st) {
-  if (a.packed)
-    hipLaunchKernelGGL((k_batch_sweep<V, R, 4, BLOCK, ROLLED, NT, CHK, true>), dim3(a.nblocks),
-                       dim3(BLOCK), 0, st, a.probs, a.bmap, a.active, a.poff, a.parity, c1, c2,
-                       a.partials);
-  else
+  if constexpr (std::is_same<V, int4>::value) {
     hipLaunchKernelGGL((k_batch_sweep<V, R, 4, BLOCK, ROLLED, NT, CHK, false>), dim3(a.nblocks),
                        dim3(BLOCK), 0, st, a.probs, a.bmap, a.active, a.poff, a.parity, c1, c2,
                        a.partials);
+  } else if (a.packed) {
+    hipLaunchKernelGGL((k_batch_sweep<V, R, 4, BLOCK, ROLLED, NT, CHK, true>), dim3(a.nblocks),
+                       dim3(BLOCK), 0, st, a.probs, a.bmap, a.active, a.poff, a.parity, c1, c2,
+                       a.partials);
+  } else {
+    hipLaunchKernelGGL((k_batch_sweep<V, R, 4, BLOCK, ROLLED, NT, CHK, false>), dim3(a.nblocks),
+                       dim3(BLOCK), 0, st, a.probs, a.bmap, a.active, a.poff, a.parity, c1, c2,
+                       a.partials);
+  }
 }
 
 template <typename V, typename R, int CHK>
 static void launch_bsweep_chk(const BSweepArgs& a, R c1, R c2, int variant, hipStream_t st) {
-  switch (variant & 7) {
-    case 0: launch_bsweep_v<V, R, 256, true, false, CHK>(a, c1, c2, st); break;
-    case 1: launch_bsweep_v<V, R, 64, true, false, CHK>(a, c1, c2, st); break;
-    case 2: launch_bsweep_v<V, R, 256, false, false, CHK>(a, c1, c2, st); break;
-    case 3: launch_bsweep_v<V, R, 64, false, false, CHK>(a, c1, c2, st); break;
-    case 4: launch_bsweep_v<V, R, 256, true, true, CHK>(a, c1, c2, st); break;
-    case 5: launch_bsweep_v<V, R, 64, true, true, CHK>(a, c1, c2, st); break;
-    case 6: launch_bsweep_v<V, R, 256, false, true, CHK>(a, c1, c2, st); break;
-    default: launch_bsweep_v<V, R, 64, false, true, CHK>(a, c1, c2, st); break;
+  if constexpr (std::is_same<V, int4>::value) {
+    // fixed-point fields: rolled variants only, as the single-lattice polish
+    switch (variant & 5) {
+      case 0: launch_bsweep_v<V, R, 256, true, false, CHK>(a, c1, c2, st); break;
+      case 1: launch_bsweep_v<V, R, 64, true, false, CHK>(a, c1, c2, st); break;
+      case 4: launch_bsweep_v<V, R, 256, true, true, CHK>(a, c1, c2, st); break;
+      default: launch_bsweep_v<V, R, 64, true, true, CHK>(a, c1, c2, st); break;
+    }
+  } else {
+    switch (variant & 7) {
+      case 0: launch_bsweep_v<V, R, 256, true, false, CHK>(a, c1, c2, st); break;
+      case 1: launch_bsweep_v<V, R, 64, true, false, CHK>(a, c1, c2, st); break;
+      case 2: launch_bsweep_v<V, R, 256, false, false, CHK>(a, c1, c2, st); break;
+      case 3: launch_bsweep_v<V, R, 64, false, false, CHK>(a, c1, c2, st); break;
+      case 4: launch_bsweep_v<V, R, 256, true, true, CHK>(a, c1, c2, st); break;
+      case 5: launch_bsweep_v<V, R, 64, true, true, CHK>(a, c1, c2, st); break;
+      case 6: launch_bsweep_v<V, R, 256, false, true, CHK>(a, c1, c2, st); break;
+      default: launch_bsweep_v<V, R, 64, false, true, CHK>(a, c1, c2, st); break;
+    }
   }
 }
 
@@ -1097,10 +1121,12 @@ static void launch_bsweep_t(const BSweepArgs& a, double omega, int variant, int 
 
 void launch_batch_sweep(bool fp32, const BatchProblem* probs, const int2* bmap, int nblocks,
                         const int* active, const int* poff, int parity, double omega, int variant,
-                        int chk, double* partials, hipStream_t st, bool packed) {
-  const BSweepArgs a{probs, bmap, nblocks, active, poff, parity, partials, packed};
+                        int chk, double* partials, hipStream_t st, int links) {
+  const BSweepArgs a{probs, bmap, nblocks, active, poff, parity, partials, links != BLINK_F64};
   if (fp32)
     launch_bsweep_t<float4, float>(a, omega, variant, chk, st);
+  else if (links == BLINK_FIXED)
+    launch_bsweep_t<int4, double>(a, omega, variant, chk, st);
   else
     launch_bsweep_t<double2, double>(a, omega, variant, chk, st);
 }
@@ -1112,13 +1138,18 @@ void launch_batch_finalize(const BatchCheck& c, int nprob, int ndir, double eps,
                      c.pev[1], c.poff[0], c.poff[1], c.inmap[0], c.inmap[1],
                      c.probs[0], c.probs[1], ndir, eps,
                      switch_theta, c.act[0], c.act[1], c.conv, c.sw, c.results, c.iter, maxit,
-                     c.best, c.stall, stall_checks, stall_gate, c.packed ? 1 : 0);
+                     c.best, c.stall, stall_checks, stall_gate, c.links);
 }
 
 void launch_batch_convert(const BatchProblem* probs32, const BatchProblem* probs64,
-                          const int2* bmap, int nblocks, const int* conv, hipStream_t st) {
-  hipLaunchKernelGGL(k_batch_convert, dim3(nblocks, 2), dim3(BLK), 0, st, probs32, probs64, bmap,
-                     conv);
+                          const int2* bmap, int nblocks, const int* conv, hipStream_t st,
+                          bool fixed) {
+  if (fixed)
+    hipLaunchKernelGGL(k_batch_convert<int4>, dim3(nblocks, 2), dim3(BLK), 0, st, probs32, probs64,
+                       bmap, conv);
+  else
+    hipLaunchKernelGGL(k_batch_convert<double2>, dim3(nblocks, 2), dim3(BLK), 0, st, probs32,
+                       probs64, bmap, conv);
 }
 
 int batch_convert_block() { return BLK; }

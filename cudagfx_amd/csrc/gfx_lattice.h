@@ -198,6 +198,11 @@ constexpr double FX_EPS_MIN = 1e-15;
 // normalisation moves by at most twice that) enters G K once through G and
 // once through the 8 star terms, 16 x 31.2 e: 582 e, taken as 600 e.
 constexpr double FX_DDELTA = 600.0 * 0x1p-32;
+// runs whose stops are decided with that bound (Coulomb slices, batch
+// lattices) use the fixed-point polish only for eps above this: the bound
+// then takes at most a quarter of sqrt(eps) (below it, sqrt(eps) would
+// approach FX_DDELTA and no check could stop)
+constexpr double FX_EPS_BOUND_MIN = 16.0 * FX_DDELTA * FX_DDELTA;
 __device__ __forceinline__ int fx_of(double u) {
   return (int)fmin(fmax(rint(u * FX_SCALE), -2147483647.0), 2147483647.0);
 }

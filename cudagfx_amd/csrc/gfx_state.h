@@ -125,6 +125,7 @@ int ensure_common(int need_links32);
 int dev_alloc(void** p, size_t bytes, const char* what);
 size_t field_bytes64(const Geom& g);
 size_t field_bytes32(const Geom& g);
+size_t field_bytes_fx(const Geom& g);
 int upload_aos_geo(State& s, const Geom& geo, const double* host, double2* dst, int nmat);
 // twisters (gfx_stoch_api.hip): seed if needed and size the fields; refill
 int mt_prepare(State& s, int need_g, int need_u);

@@ -322,6 +322,8 @@ def test_mixed_batch_lattices_switch_on_their_own(gfx, orc):
         it, hist = orc.overrelax_gauge(u2, g2, dims, 20000, EPS12, 10, 1.7)
         assert res[q][2] < EPS12
         assert abs(res[q][0] - hist[-1, 0]) / hist[-1, 0] < 1e-10
+        # below the bound on the exact links with the returned FP64 g
+        assert orc.dAdA(orc.g_trafo(u, gs[q], dims), dims)[1] < EPS12
         # the same lattice alone, mixed, single-lattice path
         gfx.set_lattice(*dims)
         gfx.set_verbose(0)

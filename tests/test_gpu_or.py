@@ -586,6 +586,29 @@ def test_coulomb_tiny_eps_on_plain_links(gfx, orc):
     assert np.all(Mo < 10 * eps)
 
 
+@pytest.mark.parametrize("eps", [1e-14, 1e-13])
+def test_coulomb_mixed_eps_below_fixed_point_reach(gfx, orc, eps):
+    """For eps below 16 (600 2^-32)^2 = 3.1e-13 the FP64 slices of a mixed
+    Coulomb run are not polished on fixed-point fields, whose stop bound
+    (600 2^-32 added to sqrt(max dAdA)) would take most or all of sqrt(eps):
+    at eps = 1e-14 no check could stop a slice.  Every slice stops below eps,
+    also on the exact links."""
+    dims = (8, 8, 8, 4)
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, 131, eps=0.4)
+    g = orc.random_init_trafo(V, 132)
+    _setup(gfx, dims, u, g, 1.7, prec=gfx.PREC_MIXED)
+    rc, it = gfx.coulomb_gauge(20000, eps, 5)
+    F, D, M = gfx.functional_slices()
+    g1 = gfx.get_trafo()
+    st = gfx.last_run_stats()
+    gfx.set_precision(gfx.PREC_FP64)
+    gfx.finalize_gaugefixing()
+    assert rc == 0 and np.all(it > 0) and np.all(M < eps) and st[1] > 0 and st[2] > 0
+    Fo, Do, Mo = orc.functional_slices(orc.su3_to_2v(u), orc.su3_to_2v(g1), dims, 3)
+    assert np.all(Mo < eps * (1 + 1e-9)), Mo
+
+
 @pytest.mark.parametrize("dims,ck", [((8, 8, 8, 6), 5), ((8, 8, 8, 4), 1)])
 def test_coulomb_packed_matches_plain_links(gfx, orc, dims, ck):
     """Mixed Coulomb on packed links (FP32 slices on the high parts, FP64
