@@ -36,7 +36,12 @@ SU(3) routines (su3manip.c, complex.c) on its 18-real AoS layout
 (oracle/ref_sweep.c, compiled into oracle/_ref), timed on the host cores on
 a bounded sample of the same 32^4 problem.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-gribov]
+uneven: the uneven-size batch, run once: 16 configurations (4 x 16^4,
+4 x 24^4, 4 x 32^3x64, 4 x 48^4) at beta 5.8, 6.0, 6.2 in turn, 200
+heatbath sweeps each, fixed together in the mixed form (FP32 on 24-bit links,
+FP64 polish on 32-bit fixed-point links and g) to theta < 1e-12.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-gribov] [--no-uneven]
        python bench.py --profile --plan FILE     (kernels for rocprofv3 counters)
        python bench.py --trace-step --plan FILE  (one mixed and one FP64 step, bracketed)
        python bench.py --summarize-profile FILE STATS FETCH WRITE
@@ -79,6 +84,11 @@ MIXED_FORMS = ("packed32", "fixed64")
 PROFILE_JSON = os.path.join(ROOT, "profiles", "sweep_variants_32x4.json")
 GRIBOV = dict(dims=(24, 24, 24, 24), beta=6.0, ntherm=500, copies=64, steps=4000, Tmin=0.01,
               Tmax=0.45, micro=3, switch=1e-6, mtseed=62)
+# the uneven batch: 4 lattices of each shape, beta cycling through BETAS,
+# twister seed 800 + k and Lehmer RandSeed 900 + k for configuration k
+UNEVEN = dict(shapes=[(16, 16, 16, 16)] * 4 + [(24, 24, 24, 24)] * 4 + [(32, 32, 32, 64)] * 4
+              + [(48, 48, 48, 48)] * 4, betas=(5.8, 6.0, 6.2), ntherm=200, mtseed=800,
+              randseed=900, switch=1e-6, periods=8)
 
 
 def synthetic_mt_params():
@@ -516,6 +526,40 @@ def gribov_run(gfx):
             "plaquette": plaq}
 
 
+def uneven_run(gfx):
+    """The uneven-size batch, once: 16 heatbath-thermalized configurations
+    of four shapes fixed together in the mixed form."""
+    U = UNEVEN
+    t0 = time.perf_counter()
+    cases = []
+    for k, dims in enumerate(U["shapes"]):
+        u, _ = thermalized_config(gfx, dims, U["betas"][k % 3], U["ntherm"], U["mtseed"] + k)
+        gfx.plant_seeds(U["randseed"] + k)
+        cases.append((dims, u, gfx.random_init_trafo_host(int(np.prod(dims)))))
+    gfx.set_omega(OMEGA)
+    b = gfx.Batch()
+    for dims, u, g in cases:
+        b.add(dims, u, g)
+    gfx.sync()
+    setup = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    rc = b.run(60000, EPS, CK, gfx.PREC_MIXED, U["switch"], U["periods"])
+    gfx.sync()
+    wall = time.perf_counter() - t1
+    res = [b.result(q) for q in range(len(cases))]
+    st = b.stats()
+    b.close()
+    return {"workload": "16 configurations 4x16^4, 4x24^4, 4x32^3x64, 4x48^4 (31.2 M sites), beta "
+                        "5.8/6.0/6.2 in turn, 200 heatbath sweeps each from per-configuration "
+                        "twister seeds; mixed Landau OR (switch 1e-6) to theta<1e-12 in one batch",
+            "rc": rc, "setup_s": setup, "wall_s": wall, "site_updates": st["site_updates"],
+            "site_updates_per_s": st["site_updates"] / wall, "graph_launches": st["graph_launches"],
+            "iters_fp32": st["iters_fp32"], "iters_fp64": st["iters_fp64"],
+            "converged": int(sum(1 for r in res if r[2] < EPS and r[3] >= 0)),
+            "stops": [int(r[3]) for r in res], "F": [r[0] for r in res],
+            "max_dAdA": max(r[2] for r in res)}
+
+
 def measure(args):
     import cudagfx_amd as gfx
 
@@ -615,6 +659,13 @@ def measure(args):
         except Exception as e:  # reported, the Landau line stands on its own
             gribov = {"error": str(e)}
 
+    uneven = None
+    if not args.no_uneven:
+        try:
+            uneven = uneven_run(gfx)
+        except Exception as e:  # reported, the Landau line stands on its own
+            uneven = {"error": str(e)}
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -674,6 +725,7 @@ def measure(args):
         "setup_s": setup_s,
         "roofline": roof,
         "gribov": gribov,
+        "uneven": uneven,
         "cpu_baseline": cpu,
     }
     return out
@@ -707,6 +759,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gribov", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="kernels for rocprofv3 counters: every sweep variant, checks, stochastic "
                          "kernels, and 512 MiB streaming copies as the byte-counter reference")
