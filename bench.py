@@ -44,6 +44,7 @@ FP64 polish on 32-bit fixed-point links and g) to theta < 1e-12.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-gribov] [--no-uneven]
        python bench.py --profile --plan FILE     (kernels for rocprofv3 counters)
        python bench.py --trace-step --plan FILE  (one mixed and one FP64 step, bracketed)
+       python bench.py --only gribov|uneven      (one batch workload alone)
        python bench.py --summarize-profile FILE STATS FETCH WRITE
        python bench.py --summarize-step FILE STATS
 """
@@ -572,6 +573,10 @@ def measure(args):
         return profile_run(gfx, args.plan, dims=(L, L, L, L), stochastic=L == DIMS[0])
     if args.trace_step:
         return trace_step(gfx, args.plan)
+    if args.only == "gribov":
+        return {"gribov": gribov_run(gfx)}
+    if args.only == "uneven":
+        return {"uneven": uneven_run(gfx)}
     V = int(np.prod(DIMS))
     t_setup = time.perf_counter()
     u, plaq = thermalized_config(gfx)
@@ -760,6 +765,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gribov", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--only", choices=("gribov", "uneven"),
+                    help="run one of the batch workloads alone (for a kernel trace)")
     ap.add_argument("--profile", action="store_true",
                     help="kernels for rocprofv3 counters: every sweep variant, checks, stochastic "
                          "kernels, and 512 MiB streaming copies as the byte-counter reference")
