@@ -143,17 +143,19 @@ int gfx_set_fp32_stall(int iters, double gate);
 /* links read by the sweeps of a mixed Landau, Coulomb or batch run: 1 (default)
  * packed links, each real u of a link as a 24-bit integer hi = rint(u 2^23) plus an int16
  * lo = rint((u - hi 2^-23) 2^38).  The FP32 phase reads hi 2^-23 (within
- * 2^-24 of u, 36 instead of 48 bytes per link).  The FP64 polish of Landau
- * runs, and of Coulomb runs whose slices hold whole workgroups, holds links
+ * 2^-24 of u, 36 instead of 48 bytes per link).  The FP64 polish holds links
  * and g in 32-bit fixed point (rint(u 2^31) per real, 48 bytes per matrix,
- * FP64 arithmetic) for eps > 1e-15; otherwise, and for batch runs, it reads
+ * FP64 arithmetic): Landau runs for eps > 1e-15, batch runs and Coulomb runs
+ * whose slices hold whole workgroups for eps > 3.1e-13; otherwise it reads
  * hi 2^-23 + lo 2^-38 (within 2^-38 of u, 60 instead of 96 bytes per link)
  * with FP64 g.  A Landau stop found in FP32 or on packed or fixed-point
  * fields is confirmed by a functional on the FP64 links (the sweeps go on if
  * it does not hold); an FP32 check never stops a Coulomb slice or batch
  * lattice (it moves to FP64), and an FP64 one stops it only when the
  * rounding's worst effect on Delta keeps its max dAdA below eps (packed
- * links for eps > 3e-17 only).  0: the FP32 and FP64 links. */
+ * links for eps > 3e-17 only).  A batch run's reported F, dAdA and max dAdA
+ * are those of its returned g on the FP64 links.  0: the FP32 and FP64
+ * links. */
 int gfx_set_packed_links(int on);
 /* 0 silences the per-check log lines */
 int gfx_set_verbose(int verbose);

@@ -65,24 +65,35 @@ def test_gribov_copies_share_links(gfx, orc):
         assert np.abs(gs[k] - g_cpu).max() < 1e-10
 
 
-def test_mixed_batch_matches_fp64_batch(gfx, orc):
+@pytest.mark.parametrize("eps", [EPS12, 1e-14])
+def test_mixed_batch_matches_fp64_batch(gfx, orc, eps):
+    """The mixed batch reaches the FP64 batch's F; its FP64 phase runs on
+    fixed-point links and g at eps = 1e-12 and on 24 + 16-bit links with FP64
+    g at 1e-14 (below 3.1e-13 the fixed-point stop bound would take more
+    than a quarter of sqrt(eps)).  Every returned g is below eps on the
+    exact links, with the reported F."""
     gfx.set_omega(1.7)
     shapes = [(8, 8, 8, 8), (8, 8, 4, 8)]
+    cases = [(dims, smooth_config(orc, dims, 500 + k, eps=0.35),
+              orc.random_init_trafo(orc.volume(dims), 600 + k)) for k, dims in enumerate(shapes)]
     runs = []
     for prec in (gfx.PREC_FP64, gfx.PREC_MIXED):
         b = gfx.Batch()
-        for k, dims in enumerate(shapes):
-            b.add(dims, smooth_config(orc, dims, 500 + k, eps=0.35),
-                  orc.random_init_trafo(orc.volume(dims), 600 + k))
-        rc = b.run(5000, EPS12, 10, prec, 1e-6, 2)
-        runs.append(([b.result(q) for q in range(len(shapes))], b.stats()))
+        for dims, u, g in cases:
+            b.add(dims, u, g)
+        rc = b.run(8000, eps, 10, prec, 1e-6, 2)
+        runs.append(([b.result(q) for q in range(len(shapes))], b.stats(),
+                     [b.trafo(q) for q in range(len(shapes))]))
         b.close()
         assert rc == 0
-    (r64, s64), (rmx, smx) = runs
+    (r64, s64, _), (rmx, smx, gmx) = runs
     assert smx["iters_fp32"] > 0 and smx["iters_fp64"] > 0
-    for q in range(len(shapes)):
-        assert rmx[q][2] < EPS12
+    for q, (dims, u, _) in enumerate(cases):
+        assert rmx[q][2] < eps
         assert abs(rmx[q][0] - r64[q][0]) / r64[q][0] < 1e-10
+        uf = orc.g_trafo(u, gmx[q], dims)
+        assert orc.dAdA(uf, dims)[1] < eps * (1 + 1e-9)
+        assert abs(orc.gauge_functional(uf, dims) - rmx[q][0]) < 1e-12
 
 
 def test_random_gribov_copies(gfx, orc):
