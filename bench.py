@@ -8,8 +8,8 @@ checked every 10 iterations.
 
 A *step* is one complete gauge-fixing run from that fixed start to the stop
 decision in the mixed path: FP32 sweeps until max dAdA < 1e-6 on packed
-links (each real of a link as a 24-bit fixed-point integer, within 2^-24 of
-the FP64 link), then an FP64 reunitarisation of g and FP64 polish sweeps
+links (each real of a link as a 21-bit fixed-point integer, within 2^-21 of
+the FP64 link, 32 bytes per link), then an FP64 reunitarisation of g and FP64 polish sweeps
 with links and g in 32-bit fixed point (within 2^-32; FP64 arithmetic),
 F/theta reductions in FP64, the stop confirmed by a functional on the FP64
 links and FP64 g.  value = site-updates/s of those steps (iterations x V /
@@ -18,8 +18,8 @@ wall time).  The plain FP64 run to the same bound is timed beside it
 
 roofline: the sweep kernel with the larger share of the mixed step
 (launches x time per launch), timed with HIP events on the library's stream;
-algorithmic bytes per site-update are 8 links + 3 g: FP32 g on 24-bit links
-432 B, fixed-point links and g 528 B, FP32 on FP32 links 528 B, FP64 g on
+algorithmic bytes per site-update are 8 links + 3 g: FP32 g on 21-bit links
+400 B, fixed-point links and g 528 B, FP32 on FP32 links 528 B, FP64 g on
 24+16-bit links 768 B, FP64 1056 B.  The entry names the autotuned variant
 and takes its HBM traffic and its rocprofv3 mean duration from
 profiles/sweep_variants_32x4.json (made by --profile under rocprofv3 and
@@ -38,7 +38,7 @@ a bounded sample of the same 32^4 problem.
 
 uneven: the uneven-size batch, run once: 16 configurations (4 x 16^4,
 4 x 24^4, 4 x 32^3x64, 4 x 48^4) at beta 5.8, 6.0, 6.2 in turn, 200
-heatbath sweeps each, fixed together in the mixed form (FP32 on 24-bit links,
+heatbath sweeps each, fixed together in the mixed form (FP32 on 21-bit links,
 FP64 polish on 32-bit fixed-point links and g) to theta < 1e-12.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-gribov] [--no-uneven]
@@ -76,7 +76,7 @@ HBM_PEAK_GBS = 8000.0
 # PREC_MIXED, SWEEP_PACKED64, SWEEP_PACKED32, SWEEP_FIXED64 in cudagfx_amd)
 FORMS = {"fp64": 0, "fp32": 1, "packed64": 2, "packed32": 3, "fixed64": 4}
 FORM_BYTES = {"fp64": 8 * 96 + 3 * 96, "fp32": 8 * 48 + 3 * 48, "packed64": 8 * 60 + 3 * 96,
-              "packed32": 8 * 36 + 3 * 48, "fixed64": 8 * 48 + 3 * 48}
+              "packed32": 8 * 32 + 3 * 48, "fixed64": 8 * 48 + 3 * 48}
 FORM_KERNEL = {"fp64": "k_or_sweep<double2, double>", "fp32": "k_or_sweep<float4, float>",
                "packed64": "k_or_sweep_packed<double2, double>",
                "packed32": "k_or_sweep_packed<float4, float>", "fixed64": "k_or_sweep<int4, double>"}
@@ -689,13 +689,13 @@ def measure(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 sweeps on 24-bit fixed-point links, then f64 arithmetic on 32-bit "
+        "dtype": "f32 sweeps on 21-bit fixed-point links, then f64 arithmetic on 32-bit "
                  "fixed-point links and g; f64 reductions and stop check on the f64 links",
         "data": "synthetic: cold start + 500 GPU heatbath sweeps at beta=6.0 (twister params "
                 "generated, srand(4357) seeds), Lehmer RandSeed=351 initial g",
         "config": {"workload": "Landau gauge fixing 32^4 beta=6.0, omega=1.7, theta<1e-12, ck=10",
                    "lattice": list(DIMS), "site_updates_per_step": iters * V,
-                   "link_formats": {"fp32_phase": "24-bit fixed point per real (36 B per link)",
+                   "link_formats": {"fp32_phase": "21-bit fixed point per real (32 B per link)",
                                     "polish": "32-bit fixed point per real, links and g (48 B per matrix)"}},
         "time_to_theta_s": {"mixed": ms_step / 1000.0, "fp64": r64["wall_s"],
                             "speedup": r64["wall_s"] / (ms_step / 1000.0),

@@ -72,6 +72,9 @@ Geom make_geom(int l0, int l1, int l2, int l3) {
   return g;
 }
 
+// even extents, at most 2^29 sites (2^28 per parity: the kernels address a
+// plane's elements with 32-bit byte offsets; the FP64 links of 2^29 sites
+// alone take 206 GB)
 bool geometry_ok(int l0, int l1, int l2, int l3) {
   const int d[4] = {l0, l1, l2, l3};
   long v = 1;
@@ -79,7 +82,7 @@ bool geometry_ok(int l0, int l1, int l2, int l3) {
     if (d[k] < 2 || (d[k] & 1)) return false;
     v *= d[k];
   }
-  return v <= (1L << 30);
+  return v <= (1L << 29);
 }
 
 int ensure_common(int need_links32) {
@@ -242,11 +245,13 @@ int ensure_packed_links(State& s) {
   if (!s.pk.a) {
     const size_t n = (size_t)8 * s.geo.stride;
     char* base = nullptr;
-    if (dev_alloc((void**)&base, n * (16 + 16 + 4 + 24), "packed links")) return GFX_ERR_ALLOC;
+    if (dev_alloc((void**)&base, n * (16 + 16 + 4 + 24 + 32), "packed links")) return GFX_ERR_ALLOC;
     s.pk.a = reinterpret_cast<uint4*>(base);
     s.pk.b = reinterpret_cast<uint4*>(base + n * 16);
     s.pk.c = reinterpret_cast<uint32_t*>(base + n * 32);
     s.pk.lo = reinterpret_cast<uint2*>(base + n * 36);
+    s.pk.a21 = reinterpret_cast<uint4*>(base + n * 60);
+    s.pk.b21 = reinterpret_cast<uint4*>(base + n * 76);
   }
   launch_pack_links(s.U64, s.pk, s.geo, 8, s.stream);
   s.packed_valid = true;

@@ -252,7 +252,7 @@ static void set_pointers(Batch& b) {
     h64[q].geo = h32[q].geo = p.geo;
     h64[q].U = b.fixed ? (const void*)o.Ufx : (const void*)o.U64;
     h32[q].U = o.U32;
-    const PackedLinks pk = b.packed ? PackedLinks{o.pk.a, o.pk.b, o.pk.c, o.pk.lo} : PackedLinks{};
+    const PackedLinks pk = b.packed ? PackedLinks{o.pk.a, o.pk.b, o.pk.c, o.pk.lo, o.pk.a21, o.pk.b21} : PackedLinks{};
     h64[q].pk = b.fixed ? PackedLinks{} : pk;
     h32[q].pk = pk;
     for (int k = 0; k < 3; k++) {
@@ -323,11 +323,13 @@ static int ensure_packed(Batch& b) {
     if (!p.pk.a) {
       const size_t n = (size_t)8 * p.geo.stride;
       char* base = nullptr;
-      if (dev_alloc((void**)&base, n * (16 + 16 + 4 + 24), "batch packed links")) return GFX_ERR_ALLOC;
+      if (dev_alloc((void**)&base, n * (16 + 16 + 4 + 24 + 32), "batch packed links")) return GFX_ERR_ALLOC;
       p.pk.a = reinterpret_cast<uint4*>(base);
       p.pk.b = reinterpret_cast<uint4*>(base + n * 16);
       p.pk.c = reinterpret_cast<uint32_t*>(base + n * 32);
       p.pk.lo = reinterpret_cast<uint2*>(base + n * 36);
+      p.pk.a21 = reinterpret_cast<uint4*>(base + n * 60);
+      p.pk.b21 = reinterpret_cast<uint4*>(base + n * 76);
     }
     launch_pack_links(p.U64, p.pk, p.geo, 8, s.stream);
   }

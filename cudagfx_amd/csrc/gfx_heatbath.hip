@@ -121,7 +121,8 @@ __device__ __forceinline__ bool sample_kp(double alpha[4], const double w[4], do
 
 // one heatbath update of h in each SU(2) subgroup; the subgroup vector w
 // comes from h K in the storage arithmetic R, the sampling runs in FP64
-template <int SAMPLER, typename R>
+// (F: products through fused multiply-adds, see gfx_su3.h)
+template <int SAMPLER, typename R, bool F = true>
 __device__ __forceinline__ void cm_heatbath(M3<R>& h, const M3<R>& K, double beta,
                                             const float* __restrict__ gauss,
                                             const float* __restrict__ unif, int i) {
@@ -132,16 +133,17 @@ __device__ __forceinline__ void cm_heatbath(M3<R>& h, const M3<R>& K, double bet
     for (int b = a + 1; b < 3; b++) {
       R wr[4];
       double w[4], al[4];
-      su2_subvec(h, K, a, b, wr[0], wr[1], wr[2], wr[3]);
+      su2_subvec<R, F>(h, K, a, b, wr[0], wr[1], wr[2], wr[3]);
 #pragma unroll
       for (int c = 0; c < 4; c++) w[c] = (double)wr[c];
       if (SAMPLER == 0) {
         sample_ref(al, w, beta, gauss, unif, i);
-        su2_left(h, a, b, (R)al[0], (R)al[1], (R)al[2], (R)al[3]);
+        su2_left<R, F>(h, a, b, (R)al[0], (R)al[1], (R)al[2], (R)al[3]);
       } else {
         const float* g = gauss + (size_t)KP_NG * i + sub * KP_NT;
         const float* u = unif + (size_t)KP_NU * i + sub * (2 * KP_NT + 2);
-        if (sample_kp(al, w, beta, g, u)) su2_left(h, a, b, (R)al[0], (R)al[1], (R)al[2], (R)al[3]);
+        if (sample_kp(al, w, beta, g, u))
+          su2_left<R, F>(h, a, b, (R)al[0], (R)al[1], (R)al[2], (R)al[3]);
       }
       sub++;
     }
@@ -265,6 +267,8 @@ __global__ __launch_bounds__(HB_BLK) void k_batch_micro_sweep(const BatchProblem
 
 // staple of U_nu(x): sum_{mu != nu} U_mu(x+nu) U_nu^+(x+mu) U_mu^+(x)
 //                                 + U_mu^+(x+nu-mu) U_nu^+(x-mu) U_mu(x-mu)
+// (the products in their earlier, unfused form: F = false, so that a
+// thermalized configuration stays the same for the same twister seeds)
 // bmap: launch position -> block (a sweep workgroup order of HB_BLK-site
 // blocks, dealt to the XCDs); the update of a link depends only on links of
 // the other parity or of other directions, so the order leaves the result
@@ -290,22 +294,22 @@ __global__ __launch_bounds__(HB_BLK) void k_link_heatbath(double2* __restrict__ 
     const int xmmu = nb_bwd(geo, s, i, mu);        // parity q
     const int xpnumu = nb_bwd(geo, snu, xpnu, mu); // x+nu-mu, parity p
     // forward staple
-    M3<double> a = load_m<double2, double>(field_ptr(U, q * 4 + mu, S), S, xpnu);
-    M3<double> b = load_m_dagger<double2, double>(field_ptr(U, q * 4 + nu, S), S, xpmu);
-    M3<double> c = m_mul(a, b);
-    M3<double> d = load_m_dagger<double2, double>(field_ptr(U, p * 4 + mu, S), S, i);
-    m_addmul(st, c, d);
+    M3<double> a = load_m<double2, double, false, false>(field_ptr(U, q * 4 + mu, S), S, xpnu);
+    M3<double> b = load_m_dagger<double2, double, false, false>(field_ptr(U, q * 4 + nu, S), S, xpmu);
+    M3<double> c = m_mul<double, false>(a, b);
+    M3<double> d = load_m_dagger<double2, double, false, false>(field_ptr(U, p * 4 + mu, S), S, i);
+    m_addmul<double, false>(st, c, d);
     // backward staple
-    a = load_m_dagger<double2, double>(field_ptr(U, p * 4 + mu, S), S, xpnumu);
-    b = load_m_dagger<double2, double>(field_ptr(U, q * 4 + nu, S), S, xmmu);
-    c = m_mul(a, b);
-    d = load_m<double2, double>(field_ptr(U, q * 4 + mu, S), S, xmmu);
-    m_addmul(st, c, d);
+    a = load_m_dagger<double2, double, false, false>(field_ptr(U, p * 4 + mu, S), S, xpnumu);
+    b = load_m_dagger<double2, double, false, false>(field_ptr(U, q * 4 + nu, S), S, xmmu);
+    c = m_mul<double, false>(a, b);
+    d = load_m<double2, double, false, false>(field_ptr(U, q * 4 + mu, S), S, xmmu);
+    m_addmul<double, false>(st, c, d);
   }
   double2* Un = field_ptr(U, p * 4 + nu, S);
-  M3<double> h = load_m<double2, double>(Un, S, i);
-  cm_heatbath<1>(h, st, beta, gauss, unif, i);
-  m_normalize(h);
+  M3<double> h = load_m<double2, double, false, false>(Un, S, i);
+  cm_heatbath<1, double, false>(h, st, beta, gauss, unif, i);
+  m_normalize<double, false>(h);
   Store<double2>::store(Un, S, i, h);
 }
 
@@ -331,7 +335,7 @@ __global__ __launch_bounds__(HB_BLK) void k_haar_trafo(double2* __restrict__ g, 
     m.e[0][c] = cx<double>((double)z[2 * c], (double)z[2 * c + 1]);
     m.e[1][c] = cx<double>((double)z[6 + 2 * c], (double)z[6 + 2 * c + 1]);
   }
-  m_normalize(m);
+  m_normalize<double, false>(m);  // earlier, unfused form: the same starts as before
   Store<double2>::store(field_ptr(g, p, geo.stride), geo.stride, s >> 1, m);
 }
 

@@ -256,22 +256,18 @@ __global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* 
 
 // The half sweeps of the mixed path on packed links (PackedLinks), Landau
 // (NDIR = 4) and Coulomb (NDIR = 3, active slices only):
-// R = float, g in FP32 planes, links read as 24-bit fixed point (432 instead
+// R = float, g in FP32 planes, links read as 21-bit fixed point (400 instead
 // of 528 bytes per site update); R = double, g in FP64 planes and the
 // reference's FP64 operation order, links as 24 + 16-bit fixed point within
 // 2^-38 of the FP64 links (768 instead of 1056 bytes per site update).
 template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK>
-__global__ __launch_bounds__(BLOCK) void k_or_sweep_packed(const uint4* __restrict__ Ua,
-                                                            const uint4* __restrict__ Ub,
-                                                            const uint32_t* __restrict__ Uc,
-                                                            const uint2* __restrict__ Ulo, V* ge,
-                                                            V* go, V* galt, Geom geo, int parity,
-                                                            R c1, R c2,
+__global__ __launch_bounds__(BLOCK) void k_or_sweep_packed(PackedLinks U, V* ge, V* go, V* galt,
+                                                            Geom geo, int parity, R c1, R c2,
                                                             const unsigned char* __restrict__ active,
                                                             const int* __restrict__ bmap,
                                                             double* __restrict__ partials) {
-  or_sweep_block<V, R, NDIR, BLOCK, ROLLED, NT, CHK>(PackedLinks{Ua, Ub, Uc, Ulo}, ge, go, galt,
-                                                     geo, parity, c1, c2, active, bmap, partials);
+  or_sweep_block<V, R, NDIR, BLOCK, ROLLED, NT, CHK>(U, ge, go, galt, geo, parity, c1, c2, active,
+                                                     bmap, partials);
 }
 
 // ------------------------------------------------------------ batched sweep
@@ -741,11 +737,14 @@ __global__ __launch_bounds__(BLK) void k_f64_to_f32(const double2* __restrict__ 
 }
 
 // nfield FP64 link fields -> packed links (PackedLinks): 24-bit hi parts in
-// the a/b/c planes, 16-bit lo parts in the lo planes
+// the a/b/c planes, 16-bit lo parts in the lo planes, 21-bit values in the
+// a21/b21 planes
 __global__ __launch_bounds__(BLK) void k_pack_links(const double2* __restrict__ src,
                                                     uint4* __restrict__ pa, uint4* __restrict__ pb,
                                                     uint32_t* __restrict__ pc,
-                                                    uint2* __restrict__ lo, int nh, int stride,
+                                                    uint2* __restrict__ lo,
+                                                    uint4* __restrict__ pa21,
+                                                    uint4* __restrict__ pb21, int nh, int stride,
                                                     int nfield) {
   const int i = blockIdx.x * BLK + threadIdx.x;
   const int f = blockIdx.y;
@@ -759,6 +758,7 @@ __global__ __launch_bounds__(BLK) void k_pack_links(const double2* __restrict__ 
     r[2 * c + 1] = v.y;
   }
   uint32_t w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int q[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) {
@@ -768,11 +768,17 @@ __global__ __launch_bounds__(BLK) void k_pack_links(const double2* __restrict__ 
     const int o = 24 * k, d = o / 32, sh = o % 32;
     w[d] |= hv << sh;
     if (sh > 8) w[d + 1] |= hv >> (32 - sh);
+    const uint32_t v21 = (uint32_t)pack21_of(r[k]) & 0x1fffffu;
+    const int o2 = 21 * k, d2 = o2 / 32, sh2 = o2 % 32;
+    x[d2] |= v21 << sh2;
+    if (sh2 > 11) x[d2 + 1] |= v21 >> (32 - sh2);
   }
   const size_t fi = (size_t)f * stride + i;
   pa[fi] = make_uint4(w[0], w[1], w[2], w[3]);
   pb[fi] = make_uint4(w[4], w[5], w[6], w[7]);
   pc[fi] = w[8];
+  pa21[fi] = make_uint4(x[0], x[1], x[2], x[3]);
+  pb21[fi] = make_uint4(x[4], x[5], x[6], x[7]);
   uint2* l = lo + (size_t)f * 3 * stride + i;
 #pragma unroll
   for (int p = 0; p < 3; p++)
@@ -930,9 +936,10 @@ template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int
 static void launch_packed_v(const PackedArgs& p, const SweepArgs<V>& a, R c1, R c2,
                             hipStream_t st) {
   dim3 grid(a.nblocks > 0 ? a.nblocks : (a.geo.nh + BLOCK - 1) / BLOCK);
+  const PackedLinks U{p.a, p.b, p.c, p.lo, p.a21, p.b21};
   hipLaunchKernelGGL((k_or_sweep_packed<V, R, NDIR, BLOCK, ROLLED, NT, CHK>), grid, dim3(BLOCK), 0,
-                     st, p.a, p.b, p.c, p.lo, a.ge, a.go, a.galt, a.geo, a.parity, c1, c2,
-                     a.active, a.bmap, a.partials);
+                     st, U, a.ge, a.go, a.galt, a.geo, a.parity, c1, c2, a.active, a.bmap,
+                     a.partials);
 }
 
 // FP64 on packed links: rolled variants only (bit 1 of the variant ignored)
@@ -1244,7 +1251,7 @@ void launch_pack_links(const double2* src, const PackedArgs& p, const Geom& geo,
                        hipStream_t st) {
   dim3 grid((geo.nh + BLK - 1) / BLK, nfield);
   hipLaunchKernelGGL(k_pack_links, grid, dim3(BLK), 0, st, src, (uint4*)p.a, (uint4*)p.b,
-                     (uint32_t*)p.c, (uint2*)p.lo, geo.nh, geo.stride, nfield);
+                     (uint32_t*)p.c, (uint2*)p.lo, p.a21, p.b21, geo.nh, geo.stride, nfield);
 }
 void launch_f64_to_fx(const double2* src, int4* dst, const Geom& geo, int nfield, hipStream_t st) {
   dim3 grid((geo.nh + BLK - 1) / BLK, nfield);

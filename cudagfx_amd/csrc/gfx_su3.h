@@ -38,6 +38,53 @@ template <typename R>
 __device__ __forceinline__ Cx<R> cmulc(Cx<R> a, Cx<R> b) {
   return cx<R>(a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im);
 }
+// Products accumulate through explicit fused multiply-adds (one instruction
+// each, the sign of an operand free): a complex multiply-accumulate is four
+// of them instead of two multiplies, two multiply-adds and two adds.  The
+// k-order of every sum is the reference's; the roundings are those of the
+// fused operations.
+template <typename R>
+__device__ __forceinline__ R fmaR(R a, R b, R c);
+template <>
+__device__ __forceinline__ double fmaR<double>(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+template <>
+__device__ __forceinline__ float fmaR<float>(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+// s += a b
+template <typename R>
+__device__ __forceinline__ void cmac(Cx<R>& s, Cx<R> a, Cx<R> b) {
+  s.re = fmaR(a.re, b.re, s.re);
+  s.re = fmaR(-a.im, b.im, s.re);
+  s.im = fmaR(a.re, b.im, s.im);
+  s.im = fmaR(a.im, b.re, s.im);
+}
+// s += a conj(b)
+template <typename R>
+__device__ __forceinline__ void cmacc(Cx<R>& s, Cx<R> a, Cx<R> b) {
+  s.re = fmaR(a.re, b.re, s.re);
+  s.re = fmaR(a.im, b.im, s.re);
+  s.im = fmaR(a.im, b.re, s.im);
+  s.im = fmaR(-a.re, b.im, s.im);
+}
+// s -= a b
+template <typename R>
+__device__ __forceinline__ void cmsb(Cx<R>& s, Cx<R> a, Cx<R> b) {
+  s.re = fmaR(-a.re, b.re, s.re);
+  s.re = fmaR(a.im, b.im, s.re);
+  s.im = fmaR(-a.re, b.im, s.im);
+  s.im = fmaR(-a.im, b.re, s.im);
+}
+// conj(a b - c d)
+template <typename R>
+__device__ __forceinline__ Cx<R> cross_conj(Cx<R> a, Cx<R> b, Cx<R> c, Cx<R> d) {
+  Cx<R> t = cmul(a, b);
+  cmsb(t, c, d);
+  return cconj(t);
+}
+
 template <typename R>
 __device__ __forceinline__ Cx<R> cadd(Cx<R> a, Cx<R> b) {
   return cx<R>(a.re + b.re, a.im + b.im);
@@ -64,12 +111,23 @@ __device__ __forceinline__ void m_zero(M3<R>& m) {
     for (int j = 0; j < 3; j++) m.e[i][j] = cx<R>(R(0), R(0));
 }
 
+// F (default): products accumulated through the fused multiply-adds above;
+// F = false: the separate products and sums of the earlier form of these
+// routines, kept for the link heatbath and the Haar-random g, so that the
+// configurations and random starts they generate stay what they were.
+
 // third row = conj(row0 x row1)
-template <typename R>
+template <typename R, bool F = true>
 __device__ __forceinline__ void m_row2(M3<R>& u) {
-  u.e[2][0] = cconj(csub(cmul(u.e[0][1], u.e[1][2]), cmul(u.e[0][2], u.e[1][1])));
-  u.e[2][1] = cconj(csub(cmul(u.e[0][2], u.e[1][0]), cmul(u.e[0][0], u.e[1][2])));
-  u.e[2][2] = cconj(csub(cmul(u.e[0][0], u.e[1][1]), cmul(u.e[0][1], u.e[1][0])));
+  if constexpr (F) {
+    u.e[2][0] = cross_conj(u.e[0][1], u.e[1][2], u.e[0][2], u.e[1][1]);
+    u.e[2][1] = cross_conj(u.e[0][2], u.e[1][0], u.e[0][0], u.e[1][2]);
+    u.e[2][2] = cross_conj(u.e[0][0], u.e[1][1], u.e[0][1], u.e[1][0]);
+  } else {
+    u.e[2][0] = cconj(csub(cmul(u.e[0][1], u.e[1][2]), cmul(u.e[0][2], u.e[1][1])));
+    u.e[2][1] = cconj(csub(cmul(u.e[0][2], u.e[1][0]), cmul(u.e[0][0], u.e[1][2])));
+    u.e[2][2] = cconj(csub(cmul(u.e[0][0], u.e[1][1]), cmul(u.e[0][1], u.e[1][0])));
+  }
 }
 
 // The same third row without fused multiply-adds: bit for bit what the
@@ -95,35 +153,47 @@ __device__ __forceinline__ void m_row2_unfused(M3<R>& u) {
 // Dagger of a 2-row matrix whose rows sit in u.e[0..1]: the result's
 // columns 0,1 are the conjugated rows, column 2 the conjugated cross product
 // (dev_su3.cu:826-929).
-template <typename R>
+template <typename R, bool F = true>
 __device__ __forceinline__ void m_dagger_from_rows(M3<R>& d, const Cx<R> r0[3], const Cx<R> r1[3]) {
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     d.e[i][0] = cconj(r0[i]);
     d.e[i][1] = cconj(r1[i]);
   }
-  d.e[0][2] = csub(cconj(cmul(d.e[1][0], d.e[2][1])), cconj(cmul(d.e[2][0], d.e[1][1])));
-  d.e[1][2] = csub(cconj(cmul(d.e[2][0], d.e[0][1])), cconj(cmul(d.e[0][0], d.e[2][1])));
-  d.e[2][2] = csub(cconj(cmul(d.e[0][0], d.e[1][1])), cconj(cmul(d.e[1][0], d.e[0][1])));
+  if constexpr (F) {
+    d.e[0][2] = cross_conj(d.e[1][0], d.e[2][1], d.e[2][0], d.e[1][1]);
+    d.e[1][2] = cross_conj(d.e[2][0], d.e[0][1], d.e[0][0], d.e[2][1]);
+    d.e[2][2] = cross_conj(d.e[0][0], d.e[1][1], d.e[1][0], d.e[0][1]);
+  } else {
+    d.e[0][2] = csub(cconj(cmul(d.e[1][0], d.e[2][1])), cconj(cmul(d.e[2][0], d.e[1][1])));
+    d.e[1][2] = csub(cconj(cmul(d.e[2][0], d.e[0][1])), cconj(cmul(d.e[0][0], d.e[2][1])));
+    d.e[2][2] = csub(cconj(cmul(d.e[0][0], d.e[1][1])), cconj(cmul(d.e[1][0], d.e[0][1])));
+  }
 }
 
 // u += v * w   (dev_add_su3_ti_su3)
-template <typename R>
+template <typename R, bool F = true>
 __device__ __forceinline__ void m_addmul(M3<R>& u, const M3<R>& v, const M3<R>& w) {
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      Cx<R> s = cmul(v.e[i][0], w.e[0][j]);
-      s = cadd(cmul(v.e[i][1], w.e[1][j]), s);
-      s = cadd(cmul(v.e[i][2], w.e[2][j]), s);
-      u.e[i][j].re += s.re;
-      u.e[i][j].im += s.im;
+      if constexpr (F) {
+        cmac(u.e[i][j], v.e[i][0], w.e[0][j]);
+        cmac(u.e[i][j], v.e[i][1], w.e[1][j]);
+        cmac(u.e[i][j], v.e[i][2], w.e[2][j]);
+      } else {
+        Cx<R> s = cmul(v.e[i][0], w.e[0][j]);
+        s = cadd(cmul(v.e[i][1], w.e[1][j]), s);
+        s = cadd(cmul(v.e[i][2], w.e[2][j]), s);
+        u.e[i][j].re += s.re;
+        u.e[i][j].im += s.im;
+      }
     }
 }
 
 // u = v * w   (dev_su3_ti_su3)
-template <typename R>
+template <typename R, bool F = true>
 __device__ __forceinline__ M3<R> m_mul(const M3<R>& v, const M3<R>& w) {
   M3<R> u;
 #pragma unroll
@@ -131,8 +201,13 @@ __device__ __forceinline__ M3<R> m_mul(const M3<R>& v, const M3<R>& w) {
 #pragma unroll
     for (int j = 0; j < 3; j++) {
       Cx<R> s = cmul(v.e[i][0], w.e[0][j]);
-      s = cadd(cmul(v.e[i][1], w.e[1][j]), s);
-      s = cadd(cmul(v.e[i][2], w.e[2][j]), s);
+      if constexpr (F) {
+        cmac(s, v.e[i][1], w.e[1][j]);
+        cmac(s, v.e[i][2], w.e[2][j]);
+      } else {
+        s = cadd(cmul(v.e[i][1], w.e[1][j]), s);
+        s = cadd(cmul(v.e[i][2], w.e[2][j]), s);
+      }
       u.e[i][j] = s;
     }
   return u;
@@ -147,19 +222,24 @@ __device__ __forceinline__ M3<R> m_mul_d(const M3<R>& v, const M3<R>& w) {
 #pragma unroll
     for (int j = 0; j < 3; j++) {
       Cx<R> s = cmulc(v.e[i][0], w.e[j][0]);
-      s = cadd(cmulc(v.e[i][1], w.e[j][1]), s);
-      s = cadd(cmulc(v.e[i][2], w.e[j][2]), s);
+      cmacc(s, v.e[i][1], w.e[j][1]);
+      cmacc(s, v.e[i][2], w.e[j][2]);
       u.e[i][j] = s;
     }
   return u;
 }
 
 // element (i,j) of v * w
-template <typename R>
+template <typename R, bool F = true>
 __device__ __forceinline__ Cx<R> m_mul_elem(const M3<R>& v, const M3<R>& w, int i, int j) {
   Cx<R> s = cmul(v.e[i][0], w.e[0][j]);
-  s = cadd(cmul(v.e[i][1], w.e[1][j]), s);
-  s = cadd(cmul(v.e[i][2], w.e[2][j]), s);
+  if constexpr (F) {
+    cmac(s, v.e[i][1], w.e[1][j]);
+    cmac(s, v.e[i][2], w.e[2][j]);
+  } else {
+    s = cadd(cmul(v.e[i][1], w.e[1][j]), s);
+    s = cadd(cmul(v.e[i][2], w.e[2][j]), s);
+  }
   return s;
 }
 
@@ -175,7 +255,7 @@ __device__ __forceinline__ float rsqrt_t<float>(float x) {
 }
 
 // Gram-Schmidt on rows 0,1; row 2 from the cross product (dev_su3.cu:1569-1634)
-template <typename R>
+template <typename R, bool F = true>
 __device__ __forceinline__ void m_normalize(M3<R>& u) {
   R len = (u.e[0][0].re * u.e[0][0].re + u.e[0][0].im * u.e[0][0].im) +
           (u.e[0][1].re * u.e[0][1].re + u.e[0][1].im * u.e[0][1].im) +
@@ -187,10 +267,17 @@ __device__ __forceinline__ void m_normalize(M3<R>& u) {
     u.e[0][i].im = u.e[0][i].im * len;
   }
   Cx<R> proj = cmulc(u.e[1][0], u.e[0][0]);
-  proj = cadd(proj, cmulc(u.e[1][1], u.e[0][1]));
-  proj = cadd(proj, cmulc(u.e[1][2], u.e[0][2]));
+  if constexpr (F) {
+    cmacc(proj, u.e[1][1], u.e[0][1]);
+    cmacc(proj, u.e[1][2], u.e[0][2]);
 #pragma unroll
-  for (int i = 0; i < 3; i++) u.e[1][i] = csub(u.e[1][i], cmul(proj, u.e[0][i]));
+    for (int i = 0; i < 3; i++) cmsb(u.e[1][i], proj, u.e[0][i]);
+  } else {
+    proj = cadd(proj, cmulc(u.e[1][1], u.e[0][1]));
+    proj = cadd(proj, cmulc(u.e[1][2], u.e[0][2]));
+#pragma unroll
+    for (int i = 0; i < 3; i++) u.e[1][i] = csub(u.e[1][i], cmul(proj, u.e[0][i]));
+  }
   len = (u.e[1][0].re * u.e[1][0].re + u.e[1][0].im * u.e[1][0].im) +
         (u.e[1][1].re * u.e[1][1].re + u.e[1][1].im * u.e[1][1].im) +
         (u.e[1][2].re * u.e[1][2].re + u.e[1][2].im * u.e[1][2].im);
@@ -200,31 +287,40 @@ __device__ __forceinline__ void m_normalize(M3<R>& u) {
     u.e[1][i].re = u.e[1][i].re * len;
     u.e[1][i].im = u.e[1][i].im * len;
   }
-  m_row2(u);
+  m_row2<R, F>(u);
 }
 
 // Left multiplication of rows a,b of g by the SU(2) element
 // [[w0 + i w3, w2 + i w1], [-w2 + i w1, w0 - i w3]]  (overrelax.cu:244-265)
-template <typename R>
+template <typename R, bool F = true>
 __device__ __forceinline__ void su2_left(M3<R>& g, int a, int b, R w0, R w1, R w2, R w3) {
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    Cx<R> d = cadd(cmul(cx<R>(w0, w3), g.e[a][c]), cmul(cx<R>(w2, w1), g.e[b][c]));
-    g.e[b][c] = cadd(cmul(cx<R>(-w2, w1), g.e[a][c]), cmul(cx<R>(w0, -w3), g.e[b][c]));
-    g.e[a][c] = d;
+    if constexpr (F) {
+      Cx<R> d = cmul(cx<R>(w0, w3), g.e[a][c]);
+      cmac(d, cx<R>(w2, w1), g.e[b][c]);
+      Cx<R> e = cmul(cx<R>(-w2, w1), g.e[a][c]);
+      cmac(e, cx<R>(w0, -w3), g.e[b][c]);
+      g.e[b][c] = e;
+      g.e[a][c] = d;
+    } else {
+      Cx<R> d = cadd(cmul(cx<R>(w0, w3), g.e[a][c]), cmul(cx<R>(w2, w1), g.e[b][c]));
+      g.e[b][c] = cadd(cmul(cx<R>(-w2, w1), g.e[a][c]), cmul(cx<R>(w0, -w3), g.e[b][c]));
+      g.e[a][c] = d;
+    }
   }
 }
 
 // The SU(2) subgroup vector w of X = g*K for rows (a,b)  (overrelax.cu:217-223).
 // Only the four entries of X that enter are formed; each with the same
 // k-order as the full product, so the values equal those of the full product.
-template <typename R>
+template <typename R, bool F = true>
 __device__ __forceinline__ void su2_subvec(const M3<R>& g, const M3<R>& K, int a, int b,
                                            R& w0, R& w1, R& w2, R& w3) {
-  Cx<R> xaa = m_mul_elem(g, K, a, a);
-  Cx<R> xbb = m_mul_elem(g, K, b, b);
-  Cx<R> xab = m_mul_elem(g, K, a, b);
-  Cx<R> xba = m_mul_elem(g, K, b, a);
+  Cx<R> xaa = m_mul_elem<R, F>(g, K, a, a);
+  Cx<R> xbb = m_mul_elem<R, F>(g, K, b, b);
+  Cx<R> xab = m_mul_elem<R, F>(g, K, a, b);
+  Cx<R> xba = m_mul_elem<R, F>(g, K, b, a);
   w0 = xaa.re + xbb.re;
   w3 = -xaa.im + xbb.im;
   w1 = -xab.im - xba.im;
@@ -294,8 +390,11 @@ __device__ __forceinline__ void rows_mul_dagger(Cx<R> p[2][3], const Cx<R> a[2][
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++)
-      p[i][j] = cadd(cadd(cmulc(a[i][0], b[j][0]), cmulc(a[i][1], b[j][1])), cmulc(a[i][2], b[j][2]));
+    for (int j = 0; j < 3; j++) {
+      p[i][j] = cmulc(a[i][0], b[j][0]);
+      cmacc(p[i][j], a[i][1], b[j][1]);
+      cmacc(p[i][j], a[i][2], b[j][2]);
+    }
 }
 // rows 0,1 of A B
 template <typename R>
@@ -303,15 +402,18 @@ __device__ __forceinline__ void rows_mul(Cx<R> p[2][3], const Cx<R> a[2][3], con
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++)
-      p[i][j] = cadd(cadd(cmul(a[i][0], b[0][j]), cmul(a[i][1], b[1][j])), cmul(a[i][2], b[2][j]));
+    for (int j = 0; j < 3; j++) {
+      p[i][j] = cmul(a[i][0], b[0][j]);
+      cmac(p[i][j], a[i][1], b[1][j]);
+      cmac(p[i][j], a[i][2], b[2][j]);
+    }
 }
 // r2 = conj(r0 x r1)
 template <typename R>
 __device__ __forceinline__ void row2_of(Cx<R> r2[3], const Cx<R> r0[3], const Cx<R> r1[3]) {
-  r2[0] = cconj(csub(cmul(r0[1], r1[2]), cmul(r0[2], r1[1])));
-  r2[1] = cconj(csub(cmul(r0[2], r1[0]), cmul(r0[0], r1[2])));
-  r2[2] = cconj(csub(cmul(r0[0], r1[1]), cmul(r0[1], r1[0])));
+  r2[0] = cross_conj(r0[1], r1[2], r0[2], r1[1]);
+  r2[1] = cross_conj(r0[2], r1[0], r0[0], r1[2]);
+  r2[2] = cross_conj(r0[0], r1[1], r0[1], r1[0]);
 }
 
 // g <- normalize(Rw g) with Rw = 1 + c1 R + c1 c2 R^2, R = h g^+ - 1, in the

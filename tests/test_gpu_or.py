@@ -7,7 +7,7 @@ stop at the same iteration (+-2) with checkinterval 1.
 import numpy as np
 import pytest
 
-from test_oracle import fx_round, packed_round
+from test_oracle import fx_round, p21_round, packed_round
 
 pytestmark = pytest.mark.gpu
 
@@ -432,26 +432,30 @@ def test_fixed_point_sweeps_match_oracle(gfx, orc):
 
 
 def test_packed_fp32_sweeps_close_to_oracle(gfx, orc):
-    """The FP32 sweep of the mixed run's first phase on the 24-bit high parts
-    of the packed links, each of its 32 variants, against the oracle's FP64
-    sweep on the same high-part links: equal to FP32 rounding.  (The high
-    parts themselves are pinned to the bit by the FP64 packed sweep test.)"""
+    """The FP32 sweep of the mixed run's first phase on the 21-bit links
+    (q = rint(u 2^20) per real), each of its 32 variants, against the
+    oracle's FP64 sweep on the same rounded links: equal to FP32 rounding."""
     dims = (8, 8, 8, 16)
     V = orc.volume(dims)
     u = orc.random_init_gauge(V, 4391)
     g = orc.random_init_trafo(V, 4392)
     u2, g2 = orc.su3_to_2v(u), orc.su3_to_2v(g)
-    links = packed_round(u2, lo=False)
+    links = p21_round(u2)
+    g2e = g2.copy()
     for parity in (0, 1):
         orc.halfsweep(links, g2, dims, parity, 4, 1.7)
+        orc.halfsweep(u2, g2e, dims, parity, 4, 1.7)
     g_cpu = orc.from_2v(g2)
+    g_exact = orc.from_2v(g2e)
     _setup(gfx, dims, u, g, 1.7)
     for v in range(32):
         gfx.set_sweep_variant(gfx.SWEEP_PACKED32, 4, v)
         gfx.set_trafo(g)
         gfx.time_sweeps(1, gfx.SWEEP_PACKED32)
-        err = np.abs(gfx.get_trafo() - g_cpu).max()
+        g_gpu = gfx.get_trafo()
+        err = np.abs(g_gpu - g_cpu).max()
         assert 1e-9 < err < 2e-5, (v, err)
+    assert np.abs(g_cpu - g_exact).max() > 1e-8  # the link rounding is visible
     gfx.finalize_gaugefixing()
 
 

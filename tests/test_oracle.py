@@ -495,6 +495,32 @@ def packed_round(u2, lo=True):
     return hi * 2.0 ** -23 + q * 2.0 ** -38
 
 
+def p21_round(a):
+    """The 21-bit links of the FP32 phase of mixed runs restated in numpy:
+    each real u as q = rint(u 2^20) clamped to +-(2^20 - 1), read back as
+    q 2^-20."""
+    return np.clip(np.rint(a * 2.0 ** 20), -1048575, 1048575) * 2.0 ** -20
+
+
+def test_p21_round_bound(orc):
+    """|p21(u) - u| <= 2^-21 for the reals of SU(3) matrices (2^-20 at
+    |u| = 1, where q clamps); and the worst effect of that rounding on a
+    site's sqrt(theta), 83.2 2^-21 = 4e-5, is far below the 1e-3 at which the
+    FP32 phase ends by default (switch theta = 1e-6): measured on random
+    links and g it stays below that bound."""
+    u2 = orc.su3_to_2v(orc.random_init_gauge(4096, 4394))
+    err = np.abs(p21_round(u2) - u2)
+    assert err.max() <= 2.0 ** -21 and err.max() > 2.0 ** -25
+    one = np.array([1.0, -1.0, 0.0, 0.5])
+    assert np.abs(p21_round(one) - one).max() == 2.0 ** -20
+    dims = (4, 4, 4, 4)
+    V = orc.volume(dims)
+    u = orc.su3_to_2v(orc.random_init_gauge(V, 4395))
+    g = orc.su3_to_2v(orc.random_init_trafo(V, 4396))
+    d = np.abs(np.sqrt(np_star_theta(p21_round(u), g, dims)) - np.sqrt(np_star_theta(u, g, dims)))
+    assert 1e-3 * 2.0 ** -21 < d.max() < 83.2 * 2.0 ** -21
+
+
 def fx_round(a):
     """The 32-bit fixed-point form of the mixed Landau run's polish
     restated in numpy: each real u as q = rint(u 2^31) clamped to
