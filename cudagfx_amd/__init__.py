@@ -26,8 +26,8 @@ DRIVER_PATH = os.path.join(_HERE, "bin", "cudagaugefix")
 PREC_FP64 = 0
 PREC_MIXED = 1
 # the sweep forms of a mixed run on packed links, for time_sweeps /
-# sweep_variant / set_sweep_variant: FP64 polish on high + low parts, FP32
-# phase on high parts
+# sweep_variant / set_sweep_variant: FP64 polish on 24-bit high + 16-bit low
+# parts, FP32 phase on the 21-bit links
 SWEEP_PACKED64 = 2
 SWEEP_PACKED32 = 3
 # the FP64 polish of the mixed Landau run on 32-bit fixed-point links and g
@@ -303,9 +303,10 @@ def set_fp32_stall(iters=300, gate=1e-8):
 
 
 def set_packed_links(on=1):
-    """Links of a mixed Landau, Coulomb or batch run: 1 packed (FP32 phase on 24-bit high
-    parts within 2^-24, FP64 polish on high + 16-bit low parts within 2^-38
-    of the FP64 links), 0 the FP32 and FP64 links."""
+    """Links of a mixed Landau, Coulomb or batch run: 1 packed (FP32 phase on
+    21-bit links within 2^-21, FP64 polish on 32-bit fixed-point links and g,
+    or below its reach on 24-bit high + 16-bit low parts within 2^-38 of the
+    FP64 links), 0 the FP32 and FP64 links."""
     _check(load().gfx_set_packed_links(int(on)), "set_packed_links")
 
 

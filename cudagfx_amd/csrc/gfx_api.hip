@@ -235,9 +235,10 @@ int ensure_f32_links(State& s) {
   return GFX_OK;
 }
 
-// the packed links of the mixed Landau run (60 bytes per link in one
-// allocation: a, b, c planes of the high parts, then the low parts); also
-// makes the FP32 fields, whose validity they share
+// the packed links of the mixed runs (92 bytes per link in one allocation:
+// a, b, c planes of the 24-bit high parts, the low parts, then the two
+// planes of the 21-bit links); also makes the FP32 fields, whose validity
+// they share
 int ensure_packed_links(State& s) {
   int rc = ensure_f32_links(s);
   if (rc) return rc;

@@ -63,8 +63,8 @@ void launch_sweep_f32(const float4* U, float4* ge, float4* go, const Geom& geo, 
                       const int* bmap, hipStream_t st, int chk = 0, float4* galt = nullptr,
                       double* partials = nullptr, int nblocks = 0);
 // half sweeps on packed links (Landau and Coulomb, arguments as above): FP32
-// g on the high parts (every variant), FP64 g on high + low parts (rolled
-// variants only, bit 1 of the variant ignored)
+// g on the 21-bit links (every variant), FP64 g on 24-bit high + 16-bit low
+// parts (rolled variants only, bit 1 of the variant ignored)
 void launch_sweep_packed_f32(const PackedArgs& p, float4* ge, float4* go, const Geom& geo,
                              int parity, int ndir, double omega, const unsigned char* active,
                              int variant, const int* bmap, hipStream_t st, int chk = 0,
@@ -95,7 +95,7 @@ void launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t st
 // batched lattices (gfx_batch_api.hip)
 // the FP64 problems' fields: FP64 links and g (BLINK_F64), packed links and
 // FP64 g (BLINK_PACKED), or 32-bit fixed-point links and g (BLINK_FIXED);
-// the FP32 problems read FP32 links (BLINK_F64) or the packed high parts
+// the FP32 problems read FP32 links (BLINK_F64) or the packed 21-bit links
 enum { BLINK_F64 = 0, BLINK_PACKED = 1, BLINK_FIXED = 2 };
 void launch_batch_sweep(bool fp32, const BatchProblem* probs, const int2* bmap, int nblocks,
                         const int* active, const int* poff, int parity, double omega, int variant,

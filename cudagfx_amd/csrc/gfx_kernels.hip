@@ -279,8 +279,8 @@ __global__ __launch_bounds__(BLOCK) void k_or_sweep_packed(PackedLinks U, V* ge,
 // the short form of the products (see sweep_site); on fixed-point links and
 // g (V = int4) the star and relaxation keep the reference's order, as in the
 // single-lattice polish, which the fixed-point stop bound assumes.  PK: the
-// links are read from the packed planes P.pk (FP32: high parts, FP64: high
-// + low parts).
+// links are read from the packed planes P.pk (FP32: the 21-bit planes,
+// FP64: high + low parts).
 template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK, bool PK>
 __global__ __launch_bounds__(BLOCK) void k_batch_sweep(const BatchProblem* __restrict__ probs,
                                                         const int2* __restrict__ bmap,

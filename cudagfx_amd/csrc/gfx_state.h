@@ -12,9 +12,9 @@
 namespace gfx {
 
 // sweep forms: FP64 links and g, FP32 links and g, the forms on packed
-// links (FP64 g on high + low parts: Coulomb and batch polish; FP32 g on
-// high parts: the FP32 phase of mixed runs), and the FP64 polish of the
-// mixed Landau run on 32-bit fixed-point links and g
+// links (FP64 g on 24-bit high + 16-bit low parts: Coulomb and batch polish
+// below the fixed-point polish's reach; FP32 g on the 21-bit links: the FP32
+// phase of mixed runs), and the FP64 polish on 32-bit fixed-point links and g
 enum { FORM_F64 = 0, FORM_F32 = 1, FORM_P64 = 2, FORM_P32 = 3, FORM_X64 = 4, NFORM = 5 };
 
 struct State {

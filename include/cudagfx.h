@@ -120,8 +120,8 @@ int apply_trafo(gfx_su3 *gf);
 /* not precision modes: the two sweep forms of a mixed run on packed
  * links (gfx_set_packed_links), as the `precision` argument of
  * gfx_time_sweeps / gfx_get_sweep_variant / gfx_set_sweep_variant:
- * PACKED64 the FP64 polish (FP64 g on high + low parts), PACKED32 the FP32
- * phase (FP32 g on the high parts) */
+ * PACKED64 the FP64 polish (FP64 g on 24-bit high + 16-bit low parts),
+ * PACKED32 the FP32 phase (FP32 g on the 21-bit links) */
 #define GFX_SWEEP_PACKED64 2
 #define GFX_SWEEP_PACKED32 3
 /* the FP64 polish of the mixed Landau run: links and g in 32-bit fixed
@@ -141,14 +141,15 @@ int gfx_set_precision(int mode, double switch_theta);
  * 5e-11 on 32^4); iters = 0 turns this off.  Default 300, 1e-8. */
 int gfx_set_fp32_stall(int iters, double gate);
 /* links read by the sweeps of a mixed Landau, Coulomb or batch run: 1 (default)
- * packed links, each real u of a link as a 24-bit integer hi = rint(u 2^23) plus an int16
- * lo = rint((u - hi 2^-23) 2^38).  The FP32 phase reads hi 2^-23 (within
- * 2^-24 of u, 36 instead of 48 bytes per link).  The FP64 polish holds links
+ * packed links.  The FP32 phase reads each real u of a link as the 21-bit
+ * integer rint(u 2^20) times 2^-20 (within 2^-21 of u, 32 instead of 48 bytes
+ * per link).  The FP64 polish holds links
  * and g in 32-bit fixed point (rint(u 2^31) per real, 48 bytes per matrix,
  * FP64 arithmetic): Landau runs for eps > 1e-15, batch runs and Coulomb runs
  * whose slices hold whole workgroups for eps > 3.1e-13; otherwise it reads
- * hi 2^-23 + lo 2^-38 (within 2^-38 of u, 60 instead of 96 bytes per link)
- * with FP64 g.  A Landau stop found in FP32 or on packed or fixed-point
+ * a 24-bit hi = rint(u 2^23) plus an int16 lo = rint((u - hi 2^-23) 2^38),
+ * i.e. hi 2^-23 + lo 2^-38 (within 2^-38 of u, 60 instead of 96 bytes per
+ * link), with FP64 g.  A Landau stop found in FP32 or on packed or fixed-point
  * fields is confirmed by a functional on the FP64 links (the sweeps go on if
  * it does not hold); an FP32 check never stops a Coulomb slice or batch
  * lattice (it moves to FP64), and an FP64 one stops it only when the

@@ -462,8 +462,9 @@ def test_packed_fp32_sweeps_close_to_oracle(gfx, orc):
 @pytest.mark.parametrize("dims,seed", [((8, 8, 8, 16), 90), ((8, 8, 8, 8), 911),
                                        ((6, 6, 6, 6), 912)])
 def test_packed_run_matches_plain_mixed_run(gfx, orc, dims, seed):
-    """The mixed run on packed links (FP32 phase on the high parts, FP64
-    polish on high + low parts) and on the FP32 and FP64 links: both stop
+    """The mixed run on packed links (FP32 phase on the 21-bit links, FP64
+    polish on 32-bit fixed-point links and g) and on the FP32 and FP64
+    links: both stop
     below the bound, at iterations within 10%, with F equal to 1e-12
     relative and g equal up to a constant rotation; the bound holds on the
     FP64 links (host functional of the packed run's result)."""
@@ -615,7 +616,7 @@ def test_coulomb_mixed_eps_below_fixed_point_reach(gfx, orc, eps):
 
 @pytest.mark.parametrize("dims,ck", [((8, 8, 8, 6), 5), ((8, 8, 8, 4), 1)])
 def test_coulomb_packed_matches_plain_links(gfx, orc, dims, ck):
-    """Mixed Coulomb on packed links (FP32 slices on the high parts, FP64
+    """Mixed Coulomb on packed links (FP32 slices on the 21-bit links, FP64
     slices on 32-bit fixed-point links and g, the slices holding whole
     workgroups) and on the FP32 and FP64 links: every slice stops below the
     bound on the exact links (host functional of the packed run's result),
