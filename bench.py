@@ -619,6 +619,12 @@ def measure(args):
     g_mixed = gfx.get_trafo()
     t_down = time.perf_counter() - t_down
     ms_step = 1000.0 * max(walls)
+    # kernel timing of the sweeps right after the timed steps (HIP events on
+    # the library stream, back-to-back launches from the start g)
+    gfx.set_trafo(g0)
+    tk = {}
+    for name in ("packed32", "fixed64", "fp64", "fp32", "packed64"):
+        tk[name] = gfx.time_sweeps(50, FORMS[name])
     # the same run with the switch at 1e-10 (FP32 rounding floor of this
     # lattice ~5e-11; the stall guard covers lattices whose floor is higher)
     tuned = run(gfx.PREC_MIXED, SWITCH_LOW)
@@ -627,10 +633,6 @@ def measure(args):
     gfx.set_packed_links(1)
     iters = last["it32"] + last["it64"]
     value = iters * V / (ms_step / 1000.0)
-
-    # kernel timing of the sweeps (HIP events on the library stream)
-    gfx.set_trafo(g0)
-    tk = {name: gfx.time_sweeps(50, form) for name, form in FORMS.items()}
     gbs = {name: (V // 2) * FORM_BYTES[name] / (tk[name] * 1e-3) / 1e9 for name in FORMS}
     variants = {}
     for name, prec in FORMS.items():
