@@ -17,7 +17,10 @@ wall time).  The plain FP64 run to the same bound is timed beside it
 (time-to-theta, speedup), with the error of F and of g against it.
 
 roofline: the sweep kernel with the larger share of the mixed step
-(launches x time per launch), timed with HIP events on the library's stream;
+(launches x time per launch), its launch time the mean over HIP event pairs
+on the library's stream around the runs of consecutive half sweeps between
+two checks (about 17 launches each) of one more (untimed) mixed step run
+right after the timed ones;
 algorithmic bytes per site-update are 8 links + 3 g: FP32 g on 21-bit links
 400 B, fixed-point links and g 528 B, FP32 on FP32 links 528 B, FP64 g on
 24+16-bit links 768 B, FP64 1056 B.  The entry names the autotuned variant
@@ -619,11 +622,23 @@ def measure(args):
     g_mixed = gfx.get_trafo()
     t_down = time.perf_counter() - t_down
     ms_step = 1000.0 * max(walls)
-    # kernel timing of the sweeps right after the timed steps (HIP events on
-    # the library stream, back-to-back launches from the start g)
+    # launch times inside a step: one more mixed step (untimed) with an event
+    # pair around each run of consecutive half sweeps between two checks
+    gfx.set_sweep_sampling(1)
+    inst = run(gfx.PREC_MIXED)
+    in_step = {}
+    for name in ("packed32", "fixed64"):
+        mean, mn, n = gfx.sweep_sample_stats(FORMS[name])
+        if n == 0:
+            raise SystemExit(f"no sampled {name} launches in the instrumented step")
+        in_step[name] = {"ms_mean": mean, "ms_min": mn, "launches_sampled": n}
+    gfx.set_sweep_sampling(0)
+    # back-to-back launches of every form from the start g (HIP events on the
+    # library stream), each after 20 untimed launches of the same form
     gfx.set_trafo(g0)
     tk = {}
     for name in ("packed32", "fixed64", "fp64", "fp32", "packed64"):
+        gfx.time_sweeps(10, FORMS[name])
         tk[name] = gfx.time_sweeps(50, FORMS[name])
     # the same run with the switch at 1e-10 (FP32 rounding floor of this
     # lattice ~5e-11; the stall guard covers lattices whose floor is higher)
@@ -640,16 +655,24 @@ def measure(args):
         if sv is not None:
             variants[name] = {"variant": sv[0], "block": sv[1], "rolled": sv[2],
                               "ms_per_variant": [round(x, 5) for x in sv[3]]}
-    # the sweep kernel with the larger share of the mixed step
-    share = {"packed32": 2 * last["it32"] * tk["packed32"], "fixed64": 2 * last["it64"] * tk["fixed64"]}
+    # the sweep kernel with the larger share of the mixed step, by its launch
+    # time inside the instrumented step
+    share = {"packed32": 2 * inst["it32"] * in_step["packed32"]["ms_mean"],
+             "fixed64": 2 * inst["it64"] * in_step["fixed64"]["ms_mean"]}
     name = max(share, key=share.get)
     v = variants[name]["variant"]
     prof = profiled_variant(name, v)
-    roof = {"bound": "hbm", "achieved": gbs[name], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": gbs[name] / HBM_PEAK_GBS,
+    alg = (V // 2) * FORM_BYTES[name]
+    ach = alg / (in_step[name]["ms_mean"] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS,
             "traffic": prof["traffic_bytes"] if prof else None,
             "kernel": FORM_KERNEL[name], "form": name, "variant": v,
-            "ms_per_launch_hip_events": tk[name],
+            "ms_per_launch_in_step": in_step[name]["ms_mean"],
+            "ms_per_launch_in_step_min": in_step[name]["ms_min"],
+            "launches_sampled": in_step[name]["launches_sampled"],
+            "ms_per_launch_back_to_back": tk[name],
+            "frac_back_to_back": gbs[name] / HBM_PEAK_GBS,
             "ms_per_launch_rocprof_profile": prof["us_rocprof"] / 1e3 if prof else None,
             "frac_rocprof_profile": prof["frac_of_8TBs"] if prof else None,
             "traffic_over_algorithmic": prof["traffic_over_algorithmic"] if prof else None,
@@ -713,6 +736,11 @@ def measure(args):
                                      "F_rel": abs(plain_links["F"] - r64["F"]) / r64["F"]},
         "iterations": {"mixed_fp32": last["it32"], "mixed_fp64": last["it64"],
                        "fp64": r64["it64"], "stop_mixed": last["iter"], "stop_fp64": r64["iter"]},
+        "sweep_kernels_in_step": {n: dict(in_step[n], kernel=FORM_KERNEL[n],
+                                          GBs=(V // 2) * FORM_BYTES[n] / (in_step[n]["ms_mean"] * 1e-3) / 1e9,
+                                          frac=(V // 2) * FORM_BYTES[n] / (in_step[n]["ms_mean"] * 1e-3)
+                                          / 1e9 / HBM_PEAK_GBS)
+                                  for n in in_step},
         "sweep_kernels": {n: {"kernel": FORM_KERNEL[n], "ms": tk[n], "GBs": gbs[n],
                               "frac": gbs[n] / HBM_PEAK_GBS,
                               "bytes_per_site_update": FORM_BYTES[n],

@@ -141,6 +141,9 @@ def load():
         "gfx_time_sweeps": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "gfx_last_run_stats": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_int),
                                          C.POINTER(C.c_int)]),
+        "gfx_set_sweep_sampling": (C.c_int, [C.c_int]),
+        "gfx_sweep_sample_stats": (C.c_int, [C.c_int, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "gfx_error_string": (C.c_char_p, [C.c_int]),
         "read_input": (C.c_int, [C.c_char_p]),
         "read_gf_ildg": (C.c_int, [vp, C.c_char_p]),
@@ -488,6 +491,21 @@ def time_sweeps(nsweeps, precision=PREC_FP64):
     ms = C.c_double()
     _check(load().gfx_time_sweeps(int(nsweeps), int(precision), C.byref(ms)), "time_sweeps")
     return ms.value
+
+
+def set_sweep_sampling(on):
+    """Bracket each run of consecutive check-free half-sweep launches of the
+    following overrelax_gauge calls with a pair of HIP events (0: off)."""
+    _check(load().gfx_set_sweep_sampling(int(on)), "set_sweep_sampling")
+
+
+def sweep_sample_stats(precision):
+    """(mean ms per launch, smallest segment mean, launches) of the bracketed
+    launches of one sweep form."""
+    mean, mn, n = C.c_double(), C.c_double(), C.c_int()
+    _check(load().gfx_sweep_sample_stats(int(precision), C.byref(mean), C.byref(mn), C.byref(n)),
+           "sweep_sample_stats")
+    return mean.value, mn.value, n.value
 
 
 def last_run_stats():

@@ -379,6 +379,12 @@ int sweep_form(const State& s, bool fp32, int ndir) {
 // slot 2 - even, or with `spare` the new odd field into slot 4 - odd; the
 // block partials of a check go to cpart (odd sweep first half, even sweep
 // second half)
+static void close_sample(State& s) {
+  hipEventRecord(s.sample_ev[2 * s.sample_n + 1], s.stream);
+  s.sample_open = false;
+  s.sample_n++;
+}
+
 void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* active, int even,
               int chk, int odd = 1, bool spare = false) {
   const int form = sweep_form(s, fp32, ndir);
@@ -392,6 +398,17 @@ void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* ac
   }
   // output slot of the updated parity (-1: in place)
   const int out = (chk == 2) ? 2 - even : spare ? 4 - odd : -1;
+  if (s.sample_on) {
+    // a check sweep, a sweep run ahead of a check's result or another form
+    // ends the open segment; a plain sweep opens one if none is open
+    if (s.sample_open && (chk != 0 || spare || s.sample_form[s.sample_n] != form)) close_sample(s);
+    if (!s.sample_open && chk == 0 && !spare && s.sample_n < State::SAMPLE_MAX) {
+      hipEventRecord(s.sample_ev[2 * s.sample_n], s.stream);
+      s.sample_form[s.sample_n] = form;
+      s.sample_cnt[s.sample_n] = 0;
+      s.sample_open = true;
+    }
+  }
   if (form == FORM_F32 || form == FORM_P32) {
     float4* alt = out < 0 ? nullptr : field32(s, out);
     if (form == FORM_F32)
@@ -401,19 +418,18 @@ void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* ac
       launch_sweep_packed_f32(s.pk, field32(s, even), field32(s, odd), s.geo, parity, ndir,
                               s.omega, active, v, map, s.stream, chk, alt, part, nb);
   } else {
-    if (form == FORM_X64) {
+    double2* alt = out < 0 ? nullptr : field64(s, out);
+    if (form == FORM_X64)
       launch_sweep_fx(s.Ufx, fieldfx(s, even), fieldfx(s, odd), s.geo, parity, ndir, s.omega,
                       active, v, map, s.stream, chk, out < 0 ? nullptr : fieldfx(s, out), part, nb);
-      return;
-    }
-    double2* alt = out < 0 ? nullptr : field64(s, out);
-    if (form == FORM_P64)
+    else if (form == FORM_P64)
       launch_sweep_packed_f64(s.pk, field64(s, even), field64(s, odd), s.geo, parity, ndir,
                               s.omega, active, v, map, s.stream, chk, alt, part, nb);
     else
       launch_sweep_f64(s.U64, field64(s, even), field64(s, odd), s.geo, parity, ndir, s.omega,
                        active, v, map, s.stream, chk, alt, part, nb);
   }
+  if (s.sample_open) s.sample_cnt[s.sample_n]++;
 }
 void do_sweep(State& s, bool fp32, int parity, int ndir, const unsigned char* active) {
   do_sweep(s, fp32, parity, ndir, active, 0, 0);
@@ -685,6 +701,8 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
   s.packed_on = packed;
   s.run_iters32 = 0;
   s.run_iters64 = 0;
+  if (s.sample_open) close_sample(s);
+  s.sample_n = 0;
   const double V = (double)VOLUME;
   const int stall_checks = (s.stall_iters + checkinterval - 1) / checkinterval;
   double maxdada = 0.0, best = 1e300;
@@ -806,6 +824,7 @@ int overrelax_gauge(int maxit, double eps, int checkinterval) {
       even_done = odd_done = true;
     }
   }
+  if (s.sample_open) close_sample(s);
   to_fp64();
   s.packed_on = false;
   stop_clock_ms(s, t0);
@@ -1135,6 +1154,48 @@ int gfx_time_sweeps(int nsweeps, int precision, double* ms) {
     hipStreamSynchronize(s.stream);
   }
   *ms = (double)t / (2.0 * nsweeps);
+  return GFX_OK;
+}
+
+int gfx_set_sweep_sampling(int on) {
+  State& s = g_st;
+  if (on && s.sample_ev == nullptr) {
+    s.sample_ev = new hipEvent_t[2 * State::SAMPLE_MAX];
+    for (int k = 0; k < 2 * State::SAMPLE_MAX; k++)
+      if (hip_check(hipEventCreate(&s.sample_ev[k]), "sampling events")) {
+        for (int j = 0; j < k; j++) hipEventDestroy(s.sample_ev[j]);
+        delete[] s.sample_ev;
+        s.sample_ev = nullptr;
+        return GFX_ERR_DEVICE;
+      }
+  }
+  if (s.sample_open) close_sample(s);
+  s.sample_on = on ? 1 : 0;
+  s.sample_n = 0;
+  return GFX_OK;
+}
+
+int gfx_sweep_sample_stats(int precision, double* mean_ms, double* min_ms, int* count) {
+  State& s = g_st;
+  const int form = form_of_precision(precision);
+  if (s.sample_open) close_sample(s);
+  if (s.sample_n > 0 && hip_check(hipEventSynchronize(s.sample_ev[2 * s.sample_n - 1]), "sampling"))
+    return GFX_ERR_DEVICE;
+  double sum = 0.0, mn = 0.0;
+  int n = 0;
+  for (int k = 0; k < s.sample_n; k++) {
+    if (s.sample_form[k] != form || s.sample_cnt[k] == 0) continue;
+    float t = 0.f;
+    if (hip_check(hipEventElapsedTime(&t, s.sample_ev[2 * k], s.sample_ev[2 * k + 1]), "sampling"))
+      return GFX_ERR_DEVICE;
+    const double per = (double)t / s.sample_cnt[k];
+    mn = (n == 0 || per < mn) ? per : mn;
+    sum += t;
+    n += s.sample_cnt[k];
+  }
+  if (mean_ms) *mean_ms = n ? sum / n : 0.0;
+  if (min_ms) *min_ms = mn;
+  if (count) *count = n;
   return GFX_OK;
 }
 

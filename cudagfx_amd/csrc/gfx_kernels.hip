@@ -146,6 +146,8 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
       for (int mu = 0; mu < NDIR; mu++) star_terms_short<V, R, NT>(K, U, gop, geo, s, i, parity, mu);
     }
   } else if (ROLLED) {
+    // fixed-point links and g: the star from their integers (raw_m_q)
+    constexpr bool FXQ = std::is_same<V, int4>::value && std::is_same<LK, const int4*>::value;
     int jf[NDIR], jb[NDIR];
 #pragma unroll
     for (int mu = 0; mu < NDIR; mu++) {
@@ -161,13 +163,23 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
       raw_load<R, NT>(U, (1 - parity) * 4 + mu, S, jb[mu], lb);
       raw_load<R, false>(gop, S, jb[mu], gb);
       __builtin_amdgcn_sched_barrier(0);
-      M3<R> A = raw_m<R>(lf);
-      M3<R> B = raw_m_dagger<R>(gf);
-      m_addmul(K, A, B);
-      A = raw_m_dagger<R>(lb);
-      B = raw_m_dagger<R>(gb);
-      m_addmul(K, A, B);
+      if constexpr (FXQ) {
+        M3<R> A = raw_m_q<R>(lf);
+        M3<R> B = raw_m_dagger_q<R, false>(gf);
+        m_addmul(K, A, B);
+        A = raw_m_dagger_q<R, true>(lb);
+        B = raw_m_dagger_q<R, false>(gb);
+        m_addmul(K, A, B);
+      } else {
+        M3<R> A = raw_m<R>(lf);
+        M3<R> B = raw_m_dagger<R>(gf);
+        m_addmul(K, A, B);
+        A = raw_m_dagger<R>(lb);
+        B = raw_m_dagger<R>(gb);
+        m_addmul(K, A, B);
+      }
     }
+    if constexpr (FXQ) fx_star_unscale(K);
   } else {
 #pragma unroll
     for (int mu = 0; mu < NDIR; mu++) {

@@ -531,6 +531,63 @@ __device__ __forceinline__ void raw_rows(const Raw<PackedLinks, R>& r, Cx<R> r0[
     r1[c3] = cx<R>(v[6 + 2 * c3], v[6 + 2 * c3 + 1]);
   }
 }
+// Fixed-point operands of the FP64 polish's star taken as their integers q
+// (2^31 times each real, exact in FP64) instead of q 2^-31: a matrix's rows
+// 0, 1 then carry 2^31 and its rebuilt row 2 2^62.  raw_m_q brings row 2 to
+// 2^31 (3 complex multiplies by 2^-31 instead of 12 real ones on the
+// rows); raw_m_dagger_q gives columns 0, 1 at 2^31 and column 2 at 2^62, or
+// at 2^31 with TO31.  U g^+ and U^+(x-mu) g^+(x-mu) formed from these carry
+// 2^62 in columns 0, 1 and 2^93 in column 2 whatever the k of each product
+// term, so the star is taken off that scale once at the end.  Powers of two
+// scale every rounding exactly (the values stay far from overflow and
+// underflow): the star is the same bit for bit as from the scaled reals.
+template <typename R>
+__device__ __forceinline__ void fx_rows_q(const Raw<const int4*, R>& r, Cx<R> r0[3], Cx<R> r1[3]) {
+  const int4 a = r.v[0], b = r.v[1], c = r.v[2];
+  r0[0] = cx<R>((R)a.x, (R)a.y);
+  r0[1] = cx<R>((R)a.z, (R)a.w);
+  r0[2] = cx<R>((R)b.x, (R)b.y);
+  r1[0] = cx<R>((R)b.z, (R)b.w);
+  r1[1] = cx<R>((R)c.x, (R)c.y);
+  r1[2] = cx<R>((R)c.z, (R)c.w);
+}
+template <typename R>
+__device__ __forceinline__ M3<R> raw_m_q(const Raw<const int4*, R>& r) {
+  const R sc = R(1.0 / FX_SCALE);
+  M3<R> m;
+  fx_rows_q<R>(r, m.e[0], m.e[1]);
+  m_row2(m);
+#pragma unroll
+  for (int c = 0; c < 3; c++) m.e[2][c] = cx<R>(m.e[2][c].re * sc, m.e[2][c].im * sc);
+  return m;
+}
+template <typename R, bool TO31>
+__device__ __forceinline__ M3<R> raw_m_dagger_q(const Raw<const int4*, R>& r) {
+  const R sc = R(1.0 / FX_SCALE);
+  Cx<R> r0[3], r1[3];
+  fx_rows_q<R>(r, r0, r1);
+  M3<R> d;
+  m_dagger_from_rows(d, r0, r1);
+  if constexpr (TO31) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) d.e[i][2] = cx<R>(d.e[i][2].re * sc, d.e[i][2].im * sc);
+  }
+  return d;
+}
+// the star accumulated from raw_m_q / raw_m_dagger_q operands, back to the
+// scale of the reals
+template <typename R>
+__device__ __forceinline__ void fx_star_unscale(M3<R>& K) {
+  const R s2 = R(1.0 / (FX_SCALE * FX_SCALE)), s3 = R(1.0 / (FX_SCALE * FX_SCALE * FX_SCALE));
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const R f = j < 2 ? s2 : s3;
+      K.e[i][j] = cx<R>(K.e[i][j].re * f, K.e[i][j].im * f);
+    }
+}
+
 template <typename R, typename LK>
 __device__ __forceinline__ M3<R> raw_m(const Raw<LK, R>& r) {
   M3<R> m;

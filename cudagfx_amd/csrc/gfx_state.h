@@ -111,6 +111,18 @@ struct State {
   // last run statistics
   double run_ms = 0.0;
   int run_iters32 = 0, run_iters64 = 0;
+
+  // launch times of the half sweeps without a check inside real runs
+  // (gfx_set_sweep_sampling): each run of consecutive such launches of one
+  // form (between two checks) is a segment bracketed by an event pair on the
+  // library stream, at most SAMPLE_MAX segments per run
+  static constexpr int SAMPLE_MAX = 2048;
+  int sample_on = 0;
+  int sample_n = 0;  // closed segments
+  bool sample_open = false;
+  hipEvent_t* sample_ev = nullptr;  // 2 * SAMPLE_MAX: start, end
+  int sample_form[SAMPLE_MAX] = {};
+  int sample_cnt[SAMPLE_MAX] = {};  // launches in the segment
 };
 
 extern State g_st;

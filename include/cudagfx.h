@@ -186,6 +186,16 @@ int gfx_time_sweeps(int nsweeps, int precision, double *ms_per_halfsweep);
  * (first sweep to the stop decision), and the number of FP32 and FP64
  * iterations it ran. */
 int gfx_last_run_stats(double *ms, int *iters_fp32, int *iters_fp64);
+/* Launch times inside real runs: with `on`, each run of consecutive
+ * half-sweep launches of one form without a fused check (the launches
+ * between two checks of overrelax_gauge) is bracketed by a pair of HIP
+ * events on the library's stream (at most 2048 such segments; the record
+ * restarts with each overrelax_gauge call and with this call). */
+int gfx_set_sweep_sampling(int on);
+/* Mean time per launch in ms over the bracketed launches of one sweep form
+ * (`precision` as for gfx_time_sweeps), the smallest segment mean, and how
+ * many launches there were; waits for the last segment. */
+int gfx_sweep_sample_stats(int precision, double *mean_ms, double *min_ms, int *count);
 /* wait for all device work of the library */
 int gfx_sync(void);
 /* nrep device copies of `bytes` bytes by a plain streaming kernel; mean ms
