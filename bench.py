@@ -47,7 +47,7 @@ FP64 polish on 32-bit fixed-point links and g) to theta < 1e-12.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-gribov] [--no-uneven]
        python bench.py --profile --plan FILE     (kernels for rocprofv3 counters)
        python bench.py --trace-step --plan FILE  (one mixed and one FP64 step, bracketed)
-       python bench.py --only gribov|uneven      (one batch workload alone)
+       python bench.py --only gribov|uneven [--persistent]  (one batch workload alone)
        python bench.py --summarize-profile FILE STATS FETCH WRITE
        python bench.py --summarize-step FILE STATS
 """
@@ -483,8 +483,10 @@ def summarize_step(plan_path, stats_dir):
 
 
 # ---------------------------------------------------------------- the bench line
-def gribov_run(gfx):
-    """The Gribov-copy workload at its stated schedule, once."""
+def gribov_run(gfx, persistent=False):
+    """The Gribov-copy workload at its stated schedule, once (persistent:
+    the check periods of single-precision stretches in persistent launches
+    instead of graph replays)."""
     G = GRIBOV
     t0 = time.perf_counter()
     u, plaq = thermalized_config(gfx, G["dims"], G["beta"], G["ntherm"], G["mtseed"])
@@ -494,6 +496,7 @@ def gribov_run(gfx):
     b = gfx.Batch()
     # the mixed form throughout: the anneal on FP32 links and g as well
     b.set_anneal_precision(gfx.PREC_MIXED)
+    b.set_persistent(int(persistent))
     base = b.add(G["dims"], u, gfx.unit_trafo())
     for _ in range(G["copies"] - 1):
         b.add_random_copy(base)
@@ -511,6 +514,7 @@ def gribov_run(gfx):
     st = b.stats()
     qb, Fb = b.best_copy(base)
     b.set_anneal_precision(gfx.PREC_FP64)
+    b.set_persistent(0)
     b.close()
     gfx.finalize_MT()
     V = int(np.prod(G["dims"]))
@@ -521,7 +525,8 @@ def gribov_run(gfx):
                         "best copy by F",
             "rc": rc, "setup_s": setup, "sa_s": t_sa, "sa_ms_per_step": 1e3 * t_sa / G["steps"],
             "sa_site_sweeps_per_s": G["copies"] * V * 4 * G["steps"] / t_sa,
-            "or_s": t_or, "or_site_updates": st["site_updates"],
+            "or_s": t_or, "or_persistent_launches": st["persistent_launches"],
+            "or_graph_launches": st["graph_launches"], "or_site_updates": st["site_updates"],
             "or_site_updates_per_s": st["site_updates"] / t_or,
             "iters_fp32": st["iters_fp32"], "iters_fp64": st["iters_fp64"],
             "stop_min": int(min(stops)), "stop_max": int(max(stops)), "stop_mean": float(np.mean(stops)),
@@ -530,9 +535,10 @@ def gribov_run(gfx):
             "plaquette": plaq}
 
 
-def uneven_run(gfx):
+def uneven_run(gfx, persistent=False):
     """The uneven-size batch, once: 16 heatbath-thermalized configurations
-    of four shapes fixed together in the mixed form."""
+    of four shapes fixed together in the mixed form (persistent: as in
+    gribov_run)."""
     U = UNEVEN
     t0 = time.perf_counter()
     cases = []
@@ -542,6 +548,7 @@ def uneven_run(gfx):
         cases.append((dims, u, gfx.random_init_trafo_host(int(np.prod(dims)))))
     gfx.set_omega(OMEGA)
     b = gfx.Batch()
+    b.set_persistent(int(persistent))
     for dims, u, g in cases:
         b.add(dims, u, g)
     gfx.sync()
@@ -552,12 +559,14 @@ def uneven_run(gfx):
     wall = time.perf_counter() - t1
     res = [b.result(q) for q in range(len(cases))]
     st = b.stats()
+    b.set_persistent(0)
     b.close()
     return {"workload": "16 configurations 4x16^4, 4x24^4, 4x32^3x64, 4x48^4 (31.2 M sites), beta "
                         "5.8/6.0/6.2 in turn, 200 heatbath sweeps each from per-configuration "
                         "twister seeds; mixed Landau OR (switch 1e-6) to theta<1e-12 in one batch",
             "rc": rc, "setup_s": setup, "wall_s": wall, "site_updates": st["site_updates"],
             "site_updates_per_s": st["site_updates"] / wall, "graph_launches": st["graph_launches"],
+            "persistent_launches": st["persistent_launches"],
             "iters_fp32": st["iters_fp32"], "iters_fp64": st["iters_fp64"],
             "converged": int(sum(1 for r in res if r[2] < EPS and r[3] >= 0)),
             "stops": [int(r[3]) for r in res], "F": [r[0] for r in res],
@@ -577,9 +586,9 @@ def measure(args):
     if args.trace_step:
         return trace_step(gfx, args.plan)
     if args.only == "gribov":
-        return {"gribov": gribov_run(gfx)}
+        return {"gribov": gribov_run(gfx, args.persistent)}
     if args.only == "uneven":
-        return {"uneven": uneven_run(gfx)}
+        return {"uneven": uneven_run(gfx, args.persistent)}
     V = int(np.prod(DIMS))
     t_setup = time.perf_counter()
     u, plaq = thermalized_config(gfx)
@@ -797,6 +806,8 @@ def main():
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--only", choices=("gribov", "uneven"),
                     help="run one of the batch workloads alone (for a kernel trace)")
+    ap.add_argument("--persistent", action="store_true",
+                    help="with --only: persistent launches instead of graph replays")
     ap.add_argument("--profile", action="store_true",
                     help="kernels for rocprofv3 counters: every sweep variant, checks, stochastic "
                          "kernels, and 512 MiB streaming copies as the byte-counter reference")

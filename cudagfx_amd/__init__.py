@@ -120,6 +120,8 @@ def load():
         "gfx_batch_add_copy": (C.c_int, [C.c_int, vp]),
         "gfx_batch_add_random_copy": (C.c_int, [C.c_int]),
         "gfx_batch_run": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int]),
+        "gfx_batch_set_persistent": (C.c_int, [C.c_int]),
+        "gfx_batch_persistent_launches": (C.c_int, []),
         "gfx_batch_result": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "gfx_batch_get_trafo": (C.c_int, [C.c_int, vp]),
@@ -399,7 +401,13 @@ class Batch:
         ms, gl, a, b, su = C.c_double(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
         load().gfx_batch_stats(C.byref(ms), C.byref(gl), C.byref(a), C.byref(b), C.byref(su))
         return dict(ms=ms.value, graph_launches=gl.value, iters_fp32=a.value, iters_fp64=b.value,
-                    site_updates=su.value)
+                    site_updates=su.value,
+                    persistent_launches=load().gfx_batch_persistent_launches())
+
+    def set_persistent(self, on):
+        """Check periods of single-precision stretches in persistent launches
+        (see gfx_batch_set_persistent)."""
+        _check(load().gfx_batch_set_persistent(int(on)), "batch_set_persistent")
 
     def variant(self, precision=PREC_FP64):
         v = C.c_int()

@@ -85,9 +85,16 @@ struct Batch {
   bool anneal_fp32 = false;    // gfx_batch_anneal on FP32 links and g
   int tuned_links = -1;        // the link form (BLINK_*) the variants were timed on
   float tune_ms[2][SWEEP_VARIANTS] = {};
+  // check periods in persistent launches (gfx_batch_set_persistent) where
+  // every lattice still sweeping is in one precision
+  bool persistent = false;
+  int* d_changed = nullptr;  // a lattice stopped or left FP32 in the launch
+  int* d_done = nullptr;     // check periods the launch ran
+  int* h_done = nullptr;
   // statistics of the last run
   double run_ms = 0.0;
   int graph_launches = 0, iters32 = 0, iters64 = 0;
+  int persistent_launches = 0;
   double site_updates = 0.0;
 };
 static Batch g_b;
@@ -129,6 +136,10 @@ static void batch_free_maps(Batch& b) {
   hipFree(b.d_results);
   hipFree(b.d_fres);
   hipFree(b.d_iter);
+  hipFree(b.d_changed);
+  hipFree(b.d_done);
+  if (b.h_done) hipHostFree(b.h_done);
+  b.d_changed = b.d_done = b.h_done = nullptr;
   if (b.h_act) hipHostFree(b.h_act);
   if (b.h_sw) hipHostFree(b.h_sw);
   if (b.h_results) hipHostFree(b.h_results);
@@ -225,11 +236,14 @@ static int batch_build(Batch& b) {
       dev_alloc((void**)&b.d_sw, sizeof(int) * n, "batch switches") ||
       dev_alloc((void**)&b.d_results, sizeof(double) * 4 * n, "batch results") ||
       dev_alloc((void**)&b.d_fres, sizeof(double) * 4 * n, "batch results") ||
-      dev_alloc((void**)&b.d_iter, sizeof(int), "batch iter"))
+      dev_alloc((void**)&b.d_iter, sizeof(int), "batch iter") ||
+      dev_alloc((void**)&b.d_changed, sizeof(int), "batch flags") ||
+      dev_alloc((void**)&b.d_done, sizeof(int), "batch flags"))
     return GFX_ERR_ALLOC;
   if (hipHostMalloc((void**)&b.h_act, sizeof(int) * 2 * n, 0) != hipSuccess ||
       hipHostMalloc((void**)&b.h_sw, sizeof(int) * n, 0) != hipSuccess ||
-      hipHostMalloc((void**)&b.h_results, sizeof(double) * 4 * n, 0) != hipSuccess)
+      hipHostMalloc((void**)&b.h_results, sizeof(double) * 4 * n, 0) != hipSuccess ||
+      hipHostMalloc((void**)&b.h_done, sizeof(int), 0) != hipSuccess)
     return GFX_ERR_ALLOC;
   b.built = true;
   return GFX_OK;
@@ -391,6 +405,10 @@ static void batch_half(Batch& b, int pr, int parity, int chk) {
                      links_of(b));
 }
 
+// check periods per persistent launch at most (the host sees the census
+// between launches)
+static const int PERSIST_PERIODS = 32;
+
 struct RunParams {
   double eps, switch_theta;
   int ck, maxit, periods, stall_checks;
@@ -399,10 +417,7 @@ struct RunParams {
 // the end of the check of iteration *d_iter: the even sweeps of the next
 // iteration carrying the check, the per-lattice grid pass, and the
 // conversions of the lattices that leave FP32
-static void batch_check_tail(Batch& b, const bool use[2], const RunParams& rp) {
-  State& s = g_st;
-  for (int pr = 0; pr < 2; pr++)
-    if (use[pr]) batch_half(b, pr, 0, 2);
+static BatchCheck batch_check_args(Batch& b) {
   BatchCheck c;
   for (int pr = 0; pr < 2; pr++) {
     c.pod[pr] = b.d_pod[pr];
@@ -412,7 +427,6 @@ static void batch_check_tail(Batch& b, const bool use[2], const RunParams& rp) {
     c.act[pr] = b.d_act[pr];
     c.inmap[pr] = b.d_inmap[pr];
   }
-  (void)use;
   c.conv = b.d_conv;
   c.sw = b.d_sw;
   c.results = b.d_results;
@@ -420,15 +434,27 @@ static void batch_check_tail(Batch& b, const bool use[2], const RunParams& rp) {
   c.best = b.d_best;
   c.stall = b.d_stall;
   c.links = links_of(b);
+  return c;
+}
+
+// the conversions of the lattices that left FP32 at the last check
+static void batch_convert_left(Batch& b) {
+  State& s = g_st;
+  const int k = block_index(batch_convert_block());
+  launch_batch_convert(b.d_probs[1], b.d_probs[0], b.d_bmap[k][0], b.nbmap[k], b.d_conv, s.stream,
+                       b.fixed);
+  launch_fill_i32(b.d_conv, (int)b.probs.size(), 0, s.stream);
+}
+
+static void batch_check_tail(Batch& b, const bool use[2], const RunParams& rp) {
+  State& s = g_st;
+  for (int pr = 0; pr < 2; pr++)
+    if (use[pr]) batch_half(b, pr, 0, 2);
+  const BatchCheck c = batch_check_args(b);
   const int n = (int)b.probs.size();
   launch_batch_finalize(c, n, 4, rp.eps, rp.switch_theta, rp.maxit, rp.stall_checks, s.stall_gate,
                         s.stream);
-  if (use[1]) {
-    const int k = block_index(batch_convert_block());
-    launch_batch_convert(b.d_probs[1], b.d_probs[0], b.d_bmap[k][0], b.nbmap[k], b.d_conv,
-                         s.stream, b.fixed);
-    launch_fill_i32(b.d_conv, n, 0, s.stream);
-  }
+  if (use[1]) batch_convert_left(b);
 }
 
 // lattices sweeping in each precision (reads the device flags)
@@ -549,13 +575,70 @@ static int batch_loop(Batch& b, bool mixed, const RunParams& rp, int* it_end) {
     }
     i++;
   };
+  // persistent launches of the check periods while every lattice still
+  // sweeping is in precision pr: each runs until a lattice stops or leaves
+  // FP32 (its conversion follows the launch) or PERSIST_PERIODS periods
+  auto persistent_stretch = [&](int pr) {
+    const Batch::CMap& m = batch_map(b, pr, b.variant[pr]);
+    const long version0 = b.census_version;
+    launch_fill_i32(b.d_iter, 1, i + ck - 1, s.stream);
+    while (rc == GFX_OK && i + ck <= rp.maxit) {
+      PersistArgs a{};
+      a.bmap = m.d;
+      a.nblocks = m.n;
+      a.pod = b.d_pod[pr];
+      a.pev = b.d_pev[pr];
+      a.nprob = n;
+      a.ck = ck;
+      a.periods = std::min(PERSIST_PERIODS, (rp.maxit - i) / ck);
+      a.eps = rp.eps;
+      a.switch_theta = rp.switch_theta;
+      a.maxit = rp.maxit;
+      a.stall_checks = rp.stall_checks;
+      a.stall_gate = s.stall_gate;
+      a.changed = b.d_changed;
+      a.done = b.d_done;
+      launch_fill_i32(b.d_changed, 1, 0, s.stream);
+      launch_fill_i32(b.d_done, 1, 0, s.stream);
+      if (hip_check(launch_batch_persistent(pr == 1, links_of(b), b.variant[pr], a,
+                                            batch_check_args(b), s.omega, s.stream),
+                    "persistent launch")) {
+        rc = GFX_ERR_DEVICE;
+        break;
+      }
+      b.persistent_launches++;
+      hipMemcpyAsync(b.h_done, b.d_done, sizeof(int), hipMemcpyDeviceToHost, s.stream);
+      if (pr == 1) batch_convert_left(b);
+      int c2[2];
+      if (census(b, c2)) {
+        rc = GFX_ERR_DEVICE;
+        break;
+      }
+      const int done = *b.h_done;
+      if (done < 1) {  // a launch runs one period at least
+        rc = GFX_ERR_DEVICE;
+        break;
+      }
+      i += done * ck;
+      if (pr == 1) b.iters32 += done * ck; else b.iters64 += done * ck;
+      after_check(i, true);
+      cnt[0] = c2[0];
+      cnt[1] = c2[1];
+      if (cnt[0] + cnt[1] == 0 || b.census_version != version0) break;
+    }
+  };
   while (rc == GFX_OK && i < rp.maxit && cnt[0] + cnt[1] > 0) {
     // eager until tuned (for the precisions in use) and just after a check
     do {
       eager();
     } while (rc == GFX_OK && i < rp.maxit && cnt[0] + cnt[1] > 0 &&
              !(even_done && (cnt[0] == 0 || tune_v[0] < 0) && (cnt[1] == 0 || tune_v[1] < 0)));
-    if (rc != GFX_OK || cnt[0] + cnt[1] == 0 || i + rp.periods * ck > rp.maxit) continue;
+    if (rc != GFX_OK || cnt[0] + cnt[1] == 0) continue;
+    if (b.persistent && (cnt[0] > 0) != (cnt[1] > 0) && i + ck <= rp.maxit) {
+      persistent_stretch(cnt[0] > 0 ? 0 : 1);
+      continue;
+    }
+    if (i + rp.periods * ck > rp.maxit) continue;
     // graph of `periods` check periods for the precisions in use
     const bool use[2] = {cnt[0] > 0, cnt[1] > 0};
     // maps are uploaded before the capture (no copy may enter the graph)
@@ -713,7 +796,7 @@ int gfx_batch_run(int maxit, double eps, int checkinterval, int precision, doubl
     b.tuned[0] = b.tuned[1] = false;
     b.tuned_links = links_of(b);
   }
-  b.graph_launches = b.iters32 = b.iters64 = 0;
+  b.graph_launches = b.iters32 = b.iters64 = b.persistent_launches = 0;
   for (auto& p : b.probs) {
     p.stop_iter = p.switch_iter = -1;
     p.idle = 0;
@@ -902,6 +985,14 @@ int gfx_batch_stats(double* ms, int* graph_launches, int* iters32, int* iters64,
   if (site_updates) *site_updates = g_b.site_updates;
   return GFX_OK;
 }
+
+int gfx_batch_set_persistent(int on) {
+  if (on && !persistent_supported()) return GFX_ERR_DEVICE;
+  g_b.persistent = on != 0;
+  return GFX_OK;
+}
+
+int gfx_batch_persistent_launches(void) { return g_b.persistent_launches; }
 
 // the sweep variant picked for the batch in each precision (index 0 FP64, 1 FP32)
 int gfx_batch_variant(int precision, int* variant, double* ms_per_variant) {

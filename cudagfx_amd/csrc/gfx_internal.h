@@ -111,11 +111,29 @@ struct BatchCheck {
   int* conv;
   int* sw;
   double* results;
-  const int* iter;
+  int* iter;
   double* best;
   int* stall;
   int links;  // BLINK_*: the FP64 stops take the packing or fixed-point bound
 };
+// the persistent form of the check periods (gfx_persist.hip)
+struct PersistArgs {
+  const int2* bmap;  // the workgroup map of the precision swept
+  int nblocks;
+  double* pod;  // check partials of the odd sweeps of that precision
+  double* pev;  // and of the even ones
+  int nprob;
+  int ck;       // iterations per check period
+  int periods;  // check periods at most
+  double eps, switch_theta;
+  int maxit, stall_checks;
+  double stall_gate;
+  int* changed;  // set when a lattice stops or leaves FP32 (cleared by the host)
+  int* done;     // check periods run
+};
+hipError_t launch_batch_persistent(bool fp32, int links, int variant, const PersistArgs& a,
+                                   const BatchCheck& c, double omega, hipStream_t st);
+bool persistent_supported();
 void launch_batch_finalize(const BatchCheck& c, int nprob, int ndir, double eps,
                            double switch_theta, int maxit, int stall_checks, double stall_gate,
                            hipStream_t st);

@@ -280,6 +280,18 @@ int gfx_batch_get_trafo(int q, gfx_su3 *trafo);
 int gfx_batch_stats(double *ms, int *graph_launches, int *iters_fp32, int *iters_fp64,
                     double *site_updates);
 int gfx_batch_variant(int precision, int *variant, double *ms_per_variant);
+/* Persistent launches for gfx_batch_run (on = 1; default 0, kept across
+ * gfx_batch_create): while every lattice still sweeping is in one precision,
+ * the check periods run inside one cooperative launch whose workgroups stay
+ * resident, take the sweep blocks of all lattices from the workgroup map and
+ * meet at a grid barrier between the half sweeps and around each check; the
+ * launch returns when a lattice stops or leaves FP32, or after 32 periods.
+ * Stretches with lattices in both precisions keep the graph replays.  Same
+ * results as the graph replays.  GFX_ERR_DEVICE if the GPU has no
+ * cooperative launch. */
+int gfx_batch_set_persistent(int on);
+/* persistent launches of the last gfx_batch_run */
+int gfx_batch_persistent_launches(void);
 /* simulated annealing of every lattice's g (simannealing_gauge per lattice,
  * simulated_annealing.cu:309-476): nsteps steps at gfx_sa_temperature(i,
  * Tmin, Tmax, expo, nsteps), each an even and an odd heatbath half sweep
