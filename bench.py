@@ -32,7 +32,8 @@ gribov: the 64-copy Gribov workload at its stated schedule, run once: 64
 Haar-random copies of one 24^4 beta = 6.0 configuration, simulated annealing
 from T = 0.45 to 0.01 over 4000 steps of one SA sweep and 3 microcanonical
 sweeps (in the mixed form: FP32 links and g), then mixed OR to theta < 1e-12
-and the copy with the largest F.
+and the copy with the largest F; then the plain FP64 OR from the same
+annealed copies, timed beside it.
 
 cpu_baseline: an OpenMP CPU half sweep built from the reference's own host
 SU(3) routines (su3manip.c, complex.c) on its 18-real AoS layout
@@ -506,6 +507,8 @@ def gribov_run(gfx, persistent=False):
     b.anneal(G["steps"], G["Tmin"], G["Tmax"], -1.0, G["micro"], 0, steps_per_graph=20)
     gfx.sync()
     t_sa = time.perf_counter() - t1
+    # the annealed copies, so that the FP64 OR starts where the mixed one did
+    annealed = [b.trafo(q) for q in range(G["copies"])]
     t2 = time.perf_counter()
     rc = b.run(20000, EPS, CK, gfx.PREC_MIXED, G["switch"], 8)
     gfx.sync()
@@ -513,6 +516,17 @@ def gribov_run(gfx, persistent=False):
     res = [b.result(q) for q in range(G["copies"])]
     st = b.stats()
     qb, Fb = b.best_copy(base)
+    # the plain FP64 OR of the same annealed copies, for comparison
+    for q, g in enumerate(annealed):
+        b.set_trafo(q, g)
+    del annealed
+    gfx.sync()
+    t3 = time.perf_counter()
+    rc64 = b.run(20000, EPS, CK, gfx.PREC_FP64, G["switch"], 8)
+    gfx.sync()
+    t_or64 = time.perf_counter() - t3
+    res64 = [b.result(q) for q in range(G["copies"])]
+    qb64, Fb64 = b.best_copy(base)
     b.set_anneal_precision(gfx.PREC_FP64)
     b.set_persistent(0)
     b.close()
@@ -532,6 +546,12 @@ def gribov_run(gfx, persistent=False):
             "stop_min": int(min(stops)), "stop_max": int(max(stops)), "stop_mean": float(np.mean(stops)),
             "converged": int(sum(1 for r in res if r[2] < EPS and r[3] >= 0)),
             "best_copy": qb, "best_F": Fb, "F_min": float(min(r[0] for r in res)),
+            "or_fp64": {"rc": rc64, "or_s": t_or64, "speedup_mixed": t_or64 / t_or,
+                        "stop_mean": float(np.mean([r[3] for r in res64])),
+                        "converged": int(sum(1 for r in res64 if r[2] < EPS and r[3] >= 0)),
+                        "best_copy": qb64, "best_F": Fb64,
+                        "max_F_rel_diff_per_copy": float(max(abs(a[0] - c[0]) / c[0]
+                                                             for a, c in zip(res, res64)))},
             "plaquette": plaq}
 
 

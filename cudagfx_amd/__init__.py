@@ -125,6 +125,7 @@ def load():
         "gfx_batch_result": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "gfx_batch_get_trafo": (C.c_int, [C.c_int, vp]),
+        "gfx_batch_set_trafo": (C.c_int, [C.c_int, vp]),
         "gfx_batch_stats": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_double)]),
@@ -396,6 +397,14 @@ class Batch:
         g = np.zeros(18 * LX * LY * LZ * LT)
         _check(load().gfx_batch_get_trafo(int(q), _ptr(g)), "batch_get_trafo")
         return g
+
+    def set_trafo(self, q, g):
+        """Lattice q's g for the next run or anneal (V su3, host layout)."""
+        LX, LY, LZ, LT = self.dims[q]
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        if g.size != 18 * LX * LY * LZ * LT:
+            raise ValueError("set_trafo: g has the wrong size for lattice %d" % q)
+        _check(load().gfx_batch_set_trafo(int(q), _ptr(g)), "batch_set_trafo")
 
     def stats(self):
         ms, gl, a, b, su = C.c_double(), C.c_int(), C.c_int(), C.c_int(), C.c_double()

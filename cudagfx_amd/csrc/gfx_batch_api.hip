@@ -973,6 +973,12 @@ int gfx_batch_get_trafo(int q, gfx_su3* trafo) {
   return download_aos_geo(g_st, g_b.probs[q].geo, g_b.probs[q].g64, (double*)trafo, 1);
 }
 
+// the g lattice q starts the next gfx_batch_run or gfx_batch_anneal from
+int gfx_batch_set_trafo(int q, gfx_su3* trafo) {
+  if (q < 0 || q >= (int)g_b.probs.size()) return GFX_ERR_STATE;
+  return upload_aos_geo(g_st, g_b.probs[q].geo, (const double*)trafo, g_b.probs[q].g64, 1);
+}
+
 // wall time of the last gfx_batch_run, graph launches, iterations in each
 // precision and the site updates done (sum over lattices of V x iterations
 // while the lattice was being updated)

@@ -277,6 +277,9 @@ int gfx_batch_run(int maxit, double eps, int checkinterval, int precision, doubl
                   int periods_per_graph);
 int gfx_batch_result(int q, double *F, double *dAdA, double *maxdAdA, int *iter);
 int gfx_batch_get_trafo(int q, gfx_su3 *trafo);
+/* replaces lattice q's g (V su3, host layout): the start of the next
+ * gfx_batch_run or gfx_batch_anneal */
+int gfx_batch_set_trafo(int q, gfx_su3 *trafo);
 int gfx_batch_stats(double *ms, int *graph_launches, int *iters_fp32, int *iters_fp64,
                     double *site_updates);
 int gfx_batch_variant(int precision, int *variant, double *ms_per_variant);

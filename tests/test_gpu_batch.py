@@ -42,6 +42,32 @@ def test_uneven_batch_matches_single_runs(gfx, orc, ck, periods):
         assert res[q][2] < EPS12 and abs(res[q][0] - hist[-1, 0]) < 1e-12
 
 
+def test_batch_set_trafo_restarts_a_lattice(gfx, orc):
+    """set_trafo replaces a lattice's g exactly; a second run from it ends
+    where a fresh batch from that g ends."""
+    gfx.set_omega(1.7)
+    dims = (8, 8, 8, 8)
+    V = orc.volume(dims)
+    u = smooth_config(orc, dims, 900, eps=0.3)
+    g0 = orc.random_init_trafo(V, 901)
+    g1 = orc.random_init_trafo(V, 902)
+    b = gfx.Batch()
+    b.add(dims, u, g0)
+    assert b.run(4000, EPS12, 5, gfx.PREC_FP64, 1e-6, 2) == 0
+    b.set_trafo(0, g1)
+    assert np.array_equal(b.trafo(0), g1)
+    assert b.run(4000, EPS12, 5, gfx.PREC_FP64, 1e-6, 2) == 0
+    r_again, g_again = b.result(0), b.trafo(0)
+    b.close()
+    b = gfx.Batch()
+    b.add(dims, u, g1)
+    assert b.run(4000, EPS12, 5, gfx.PREC_FP64, 1e-6, 2) == 0
+    r_fresh, g_fresh = b.result(0), b.trafo(0)
+    b.close()
+    assert r_again[3] == r_fresh[3]
+    assert np.abs(g_again - g_fresh).max() < 1e-13
+
+
 def test_gribov_copies_share_links(gfx, orc):
     """Copies are the same links with different initial g: each copy equals
     the single run from its own start."""
