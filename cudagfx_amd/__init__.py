@@ -131,6 +131,7 @@ def load():
                                       C.POINTER(C.c_double)]),
         "gfx_batch_variant": (C.c_int, [C.c_int, C.POINTER(C.c_int), vp]),
         "gfx_su2_sample": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int, vp]),
+        "gfx_su2_sample4": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int, vp]),
         "gfx_batch_anneal": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                        C.c_int, C.c_int]),
         "gfx_batch_best_copy": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
@@ -475,6 +476,14 @@ def su2_sample(sampler, beta, wnorm, n):
     _check(load().gfx_su2_sample(int(sampler), float(beta), float(wnorm), int(n), _ptr(out)),
            "su2_sample")
     return out
+
+
+def su2_sample4(sampler, beta, wnorm, n):
+    """The same draws with all four quaternion components, shape (n, 4)."""
+    out = np.zeros(4 * int(n))
+    _check(load().gfx_su2_sample4(int(sampler), float(beta), float(wnorm), int(n), _ptr(out)),
+           "su2_sample4")
+    return out.reshape(int(n), 4)
 
 
 def micro_sweeps(n, ndir=4):

@@ -379,9 +379,11 @@ void launch_link_heatbath(double2* U, const Geom& geo, int parity, int nu, doubl
 
 // the SU(2) samplers alone, for distribution tests: sample i draws alpha
 // for w = (wnorm, 0, 0, 0) from the random fields exactly as subgroup 0 of
-// site i in k_sa_sweep; out[i] = alpha_0 (NaN when Kennedy-Pendleton
-// rejected every trial and the subgroup would stay unchanged)
-template <int SAMPLER>
+// site i in k_sa_sweep; ALL: out[4i..4i+3] = alpha (quaternion components
+// in the reference's order), else out[i] = alpha_0 (NaN when
+// Kennedy-Pendleton rejected every trial and the subgroup would stay
+// unchanged)
+template <int SAMPLER, bool ALL>
 __global__ __launch_bounds__(HB_BLK) void k_su2_sample(double beta, double wnorm, int n,
                                                        const float* __restrict__ gauss,
                                                        const float* __restrict__ unif,
@@ -390,22 +392,33 @@ __global__ __launch_bounds__(HB_BLK) void k_su2_sample(double beta, double wnorm
   if (i >= n) return;
   const double w[4] = {wnorm, 0.0, 0.0, 0.0};
   double al[4];
-  if (SAMPLER == 0) {
+  bool ok = true;
+  if (SAMPLER == 0)
     sample_ref(al, w, beta, gauss, unif, i);
-    out[i] = al[0];
+  else
+    ok = sample_kp(al, w, beta, gauss + (size_t)KP_NG * i, unif + (size_t)KP_NU * i);
+  if (ALL) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[4 * (size_t)i + k] = ok ? al[k] : __builtin_nan("");
   } else {
-    const bool ok = sample_kp(al, w, beta, gauss + (size_t)KP_NG * i, unif + (size_t)KP_NU * i);
     out[i] = ok ? al[0] : __builtin_nan("");
   }
 }
 
 void launch_su2_sample(int sampler, double beta, double wnorm, int n, const float* gauss,
-                       const float* unif, double* out, hipStream_t st) {
+                       const float* unif, double* out, hipStream_t st, bool all) {
   const dim3 grid((n + HB_BLK - 1) / HB_BLK);
-  if (sampler == 0)
-    hipLaunchKernelGGL(k_su2_sample<0>, grid, dim3(HB_BLK), 0, st, beta, wnorm, n, gauss, unif, out);
-  else
-    hipLaunchKernelGGL(k_su2_sample<1>, grid, dim3(HB_BLK), 0, st, beta, wnorm, n, gauss, unif, out);
+  if (sampler == 0) {
+    if (all)
+      hipLaunchKernelGGL((k_su2_sample<0, true>), grid, dim3(HB_BLK), 0, st, beta, wnorm, n, gauss, unif, out);
+    else
+      hipLaunchKernelGGL((k_su2_sample<0, false>), grid, dim3(HB_BLK), 0, st, beta, wnorm, n, gauss, unif, out);
+  } else {
+    if (all)
+      hipLaunchKernelGGL((k_su2_sample<1, true>), grid, dim3(HB_BLK), 0, st, beta, wnorm, n, gauss, unif, out);
+    else
+      hipLaunchKernelGGL((k_su2_sample<1, false>), grid, dim3(HB_BLK), 0, st, beta, wnorm, n, gauss, unif, out);
+  }
 }
 
 template <typename V, typename R>

@@ -33,7 +33,7 @@ int kp_gauss_per_site();
 int kp_unif_per_site();
 void launch_haar_trafo(double2* g, const Geom& geo, const float* gauss, hipStream_t st);
 void launch_su2_sample(int sampler, double beta, double wnorm, int n, const float* gauss,
-                       const float* unif, double* out, hipStream_t st);
+                       const float* unif, double* out, hipStream_t st, bool all);
 
 static const int MT_COUNT = 4096;
 
@@ -352,22 +352,32 @@ int gfx_batch_add_random_copy(int base) {
 // n draws of the SU(2) sampler (0 reference rule, 1 Kennedy-Pendleton) for
 // w = (wnorm, 0, 0, 0) at inverse temperature beta from the current twister
 // fields (grown if needed), then one refill; a0[i] = alpha_0
-int gfx_su2_sample(int sampler, double beta, double wnorm, int n, double* a0) {
+static int su2_sample(int sampler, double beta, double wnorm, int n, double* out, bool all) {
   State& s = g_st;
-  if (n <= 0 || !a0) return GFX_ERR_STATE;
+  if (n <= 0 || !out) return GFX_ERR_STATE;
   const long ng = (long)(sampler ? kp_gauss_per_site() : 1) * n;
   const long nu = (long)(sampler ? kp_unif_per_site() : 4) * n;
   if (nu > (1L << 30)) return GFX_ERR_GEOMETRY;
   int rc = mt_prepare(s, (int)ng, (int)nu);
   if (rc) return rc;
+  const size_t bytes = sizeof(double) * (size_t)n * (all ? 4 : 1);
   double* d = nullptr;
-  if (hipMalloc((void**)&d, sizeof(double) * n) != hipSuccess) return GFX_ERR_ALLOC;
-  launch_su2_sample(sampler ? 1 : 0, beta, wnorm, n, s.rnd_gauss, s.rnd_unif, d, s.stream);
-  hipMemcpyAsync(a0, d, sizeof(double) * n, hipMemcpyDeviceToHost, s.stream);
+  if (hipMalloc((void**)&d, bytes) != hipSuccess) return GFX_ERR_ALLOC;
+  launch_su2_sample(sampler ? 1 : 0, beta, wnorm, n, s.rnd_gauss, s.rnd_unif, d, s.stream, all);
+  hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, s.stream);
   mt_refill(s, 1);
   rc = hip_check(hipStreamSynchronize(s.stream), "su2 sample");
   hipFree(d);
   return rc;
+}
+
+int gfx_su2_sample(int sampler, double beta, double wnorm, int n, double* a0) {
+  return su2_sample(sampler, beta, wnorm, n, a0, false);
+}
+
+// the same draws, all four quaternion components: alpha[4i..4i+3]
+int gfx_su2_sample4(int sampler, double beta, double wnorm, int n, double* alpha) {
+  return su2_sample(sampler, beta, wnorm, n, alpha, true);
 }
 
 // n microcanonical sweeps (even + odd) of g; F is unchanged by each

@@ -245,6 +245,11 @@ double gfx_sa_temperature(int i, double Tmin, double Tmax, double expo, int N);
  * fields, which are refilled afterwards; a0[i] = alpha_0 (NaN where the
  * Kennedy-Pendleton trials all failed).  For distribution tests. */
 int gfx_su2_sample(int sampler, double beta, double wnorm, int n, double *a0);
+/* the same, all four components of each draw: alpha[4i..4i+3] = (alpha_0,
+ * alpha_1, alpha_2, alpha_3) in the reference's component order
+ * (simulated_annealing.cu:47-55), NaN where the Kennedy-Pendleton trials
+ * all failed */
+int gfx_su2_sample4(int sampler, double beta, double wnorm, int n, double *alpha);
 /* nsweep link-heatbath sweeps of the Wilson action at beta (Kennedy-Pendleton
  * SU(2) subgroups, random numbers from the twisters) */
 int gfx_heatbath_sweeps(int nsweep, double beta);

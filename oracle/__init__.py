@@ -257,12 +257,15 @@ def mt_generate(params, state, nper, initialized, want_words=False):
 
 
 # ---------------------------------------------------------------- SA sampler
-def sa_sample_ref(gauss, unif, beta, wnorm, n):
-    """alpha_0 of dev_heatbath_su2 for w = (wnorm, 0, 0, 0), draw i using
+def sa_sample_ref(gauss, unif, beta, wnorm, n, components=1):
+    """alpha of dev_heatbath_su2 for w = (wnorm, 0, 0, 0), draw i using
     gauss[i] and unif[4i..4i+3] (simulated_annealing.cu:1-73): dev_get_a0 in
     float arithmetic (y = (|g|^2 - log(1 - u0)) / (|w|^2 beta), accepted when
     2 u1^2 <= 2 - y, then a0 = 1 - y), else alpha = w.  With v = w/|w| =
-    (1, 0, 0, 0) the product a v is a itself, so alpha_0 = a0."""
+    (1, 0, 0, 0) the product a v is a itself.  components=1: alpha_0 (n,);
+    components=4: (n, 4), the direction part as simulated_annealing.cu:47-55
+    forms it: norm = sqrt(1 - a0^2), cos_theta = 2 u2 - 1, phi = 2 pi u3,
+    (a1, a2, a3) = norm (sin_theta cos phi, sin_theta sin phi, cos_theta)."""
     f32 = np.float32
     g = np.abs(np.asarray(gauss[:n], dtype=f32))
     u = np.asarray(unif[:4 * n], dtype=f32).reshape(n, 4)
@@ -270,4 +273,16 @@ def sa_sample_ref(gauss, unif, beta, wnorm, n):
     y = g * g + b
     y = (y.astype(np.float64) / (wnorm * wnorm * beta)).astype(f32)
     acc = f32(2.0) * (u[:, 1] * u[:, 1]) <= f32(2.0) - y
-    return np.where(acc, (f32(1.0) - y).astype(np.float64), wnorm)
+    a0 = np.where(acc, (f32(1.0) - y).astype(np.float64), wnorm)
+    if components == 1:
+        return a0
+    norm = np.sqrt(np.maximum(0.0, 1.0 - a0 * a0))
+    ct = 2.0 * u[:, 2].astype(np.float64) - 1.0
+    st = np.sqrt(1.0 - ct * ct)
+    phi = 6.2831853071795862 * u[:, 3].astype(np.float64)
+    out = np.zeros((n, 4))
+    out[:, 0] = a0
+    out[:, 1] = np.where(acc, norm * st * np.cos(phi), 0.0)
+    out[:, 2] = np.where(acc, norm * st * np.sin(phi), 0.0)
+    out[:, 3] = np.where(acc, norm * ct, 0.0)
+    return out

@@ -229,6 +229,42 @@ def test_kennedy_pendleton_sampler_density(gfx, a):
     assert res.pvalue > 1e-3, res
 
 
+@pytest.mark.parametrize("a", [0.7, 3.0, 12.0])
+def test_kennedy_pendleton_sampler_directions(gfx, a):
+    """All four components of the Kennedy-Pendleton draws: unit quaternions
+    whose direction (alpha_1, alpha_2, alpha_3)/|.| is uniform on the sphere
+    and independent of alpha_0; with the alpha_0 law above this is the SU(2)
+    conditional exp(a alpha_0) of the Haar measure, the distribution that
+    gives the heatbath detailed balance."""
+    from scipy import stats
+    from test_oracle import _sphere_ks
+    gfx.set_mt_params(synthetic_params(93).ravel())
+    gfx.set_mt_seed(93)
+    al = gfx.su2_sample4(1, a / 2.0, 2.0, 200000)
+    gfx.finalize_MT()
+    acc = al[~np.isnan(al[:, 0])]
+    assert np.abs(np.sum(acc * acc, axis=1) - 1.0).max() < 1e-12
+    vec = acc[:, 1:] / np.linalg.norm(acc[:, 1:], axis=1)[:, None]
+    p3, pphi = _sphere_ks(vec[:, 2], np.arctan2(vec[:, 1], vec[:, 0]))
+    assert p3 > 1e-3 and pphi > 1e-3, (p3, pphi)
+    assert abs(stats.pearsonr(acc[:, 0], vec[:, 2])[0]) < 0.01
+
+
+def test_reference_sampler_components_follow_its_rule(gfx, orc):
+    """All four components of the reference rule's draws against the
+    oracle's restatement on the same twister numbers."""
+    gfx.set_mt_params(synthetic_params(94).ravel())
+    gfx.set_mt_seed(94)
+    n, beta, wnorm = 50000, 0.8, 1.7
+    gfx.su2_sample(0, beta, wnorm, n)  # sizes the fields, then refills them
+    gauss, unif = gfx.get_mt_fields()[:2]
+    al = gfx.su2_sample4(0, beta, wnorm, n)
+    gfx.finalize_MT()
+    want = orc.sa_sample_ref(gauss, unif, beta, wnorm, n, components=4)
+    same = np.all(np.abs(al - want) < 1e-5, axis=1)
+    assert same.mean() > 0.9999, same.mean()
+
+
 def test_reference_sampler_follows_its_rule(gfx, orc):
     """The reference rule (simulated_annealing.cu:1-73) on the same twister
     numbers as the oracle's restatement."""

@@ -483,6 +483,42 @@ def test_reference_sa_rule_density(orc):
     assert stats.kstest(acc, cdf_of(kp)).pvalue < 1e-6
 
 
+def _sphere_ks(n3, phi):
+    """Kolmogorov-Smirnov p-values of a direction's z component against the
+    uniform law on [-1, 1] and of its azimuth against the uniform law on
+    (-pi, pi]: together they say the direction is uniform on the sphere."""
+    from scipy import stats
+    return (stats.kstest(n3, stats.uniform(loc=-1.0, scale=2.0).cdf).pvalue,
+            stats.kstest(phi, stats.uniform(loc=-np.pi, scale=2.0 * np.pi).cdf).pvalue)
+
+
+def test_reference_sa_direction_isotropic(orc):
+    """The direction part of the reference SA sampler's alpha
+    (simulated_annealing.cu:47-55) as the oracle restates it: for an
+    accepted draw alpha is a unit quaternion, and (alpha_1, alpha_2,
+    alpha_3)/|.| is uniform on the sphere and independent of alpha_0, so that
+    with the a0 law pinned above the draw follows the SU(2) conditional
+    exp(kb Re tr alpha) of the Haar measure (which is what gives the heatbath
+    detailed balance).  A planted mistake (cos_theta = u2 instead of 2 u2 - 1)
+    is caught."""
+    from scipy import stats
+    rng = np.random.default_rng(6)
+    n, beta, wnorm = 200000, 0.9, 2.0
+    gauss = rng.standard_normal(n).astype(np.float32)
+    unif = (1.0 - rng.random(4 * n)).astype(np.float32)
+    al = orc.sa_sample_ref(gauss, unif, beta, wnorm, n, components=4)
+    assert np.array_equal(al[:, 0], orc.sa_sample_ref(gauss, unif, beta, wnorm, n))
+    acc = al[al[:, 0] != wnorm]
+    assert np.abs(np.sum(acc * acc, axis=1) - 1.0).max() < 1e-12
+    vec = acc[:, 1:] / np.linalg.norm(acc[:, 1:], axis=1)[:, None]
+    p3, pphi = _sphere_ks(vec[:, 2], np.arctan2(vec[:, 1], vec[:, 0]))
+    assert p3 > 1e-3 and pphi > 1e-3, (p3, pphi)
+    assert abs(stats.pearsonr(acc[:, 0], vec[:, 2])[0]) < 0.01
+    # planted: cos_theta drawn from [0, 1) only
+    bad_ct = unif.reshape(n, 4)[:, 2].astype(np.float64)[al[:, 0] != wnorm]
+    assert _sphere_ks(bad_ct, np.arctan2(vec[:, 1], vec[:, 0]))[0] < 1e-6
+
+
 def packed_round(u2, lo=True):
     """The packed links of the mixed Landau run restated in numpy: each real
     u of the 2-row links as hi = rint(u 2^23) clamped to 24 bits and, with
