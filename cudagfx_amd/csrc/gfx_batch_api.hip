@@ -891,11 +891,36 @@ int gfx_batch_anneal(int nsteps, double Tmin, double Tmax, double expo, int nmic
     set_pointers(b);
   }
   const BatchProblem* probs = b.d_probs[f32 ? 1 : 0];
-  auto one_step = [&]() {
+  if ((rc = mt_second_set(s))) {
+    hipFree(d_betas);
+    hipFree(d_step);
+    return rc;
+  }
+  // The SA half sweep of parity p reads fields[p]; the numbers of the next
+  // SA half sweep are made into the other fields on the twister stream while
+  // this one (and, after the odd one, the microcanonical sweeps, which draw
+  // no numbers) runs, once the half sweep before has read them.  Refills stay
+  // in order on one stream, so every half sweep gets the numbers it got with
+  // a refill after each.  A stretch of steps starts with fields[0] holding
+  // the current numbers and ends with the twister stream joined back (an
+  // even number of half sweeps leaves the current numbers in fields[0]).
+  float* fg[2] = {s.rnd_gauss, s.rnd_gauss2};
+  float* fu[2] = {s.rnd_unif, s.rnd_unif2};
+  auto fork = [&]() {
+    hipEventRecord(s.ev_filled[0], s.stream);
+    hipStreamWaitEvent(s.mt_stream, s.ev_filled[0], 0);
+  };
+  auto join = [&]() { hipStreamWaitEvent(s.stream, s.ev_filled[0], 0); };
+  auto one_step = [&](bool first) {
     for (int p = 0; p < 2; p++) {
-      launch_batch_sa_sweep(probs, map, nblk, b.d_roff, p, d_betas, d_step, sampler, s.rnd_gauss,
-                            s.rnd_unif, s.stream, f32);
-      mt_refill(s, 1);
+      const bool start = first && p == 0;
+      if (!start) hipStreamWaitEvent(s.stream, s.ev_filled[p], 0);
+      launch_batch_sa_sweep(probs, map, nblk, b.d_roff, p, d_betas, d_step, sampler, fg[p], fu[p],
+                            s.stream, f32);
+      hipEventRecord(s.ev_used[p], s.stream);
+      if (!start) hipStreamWaitEvent(s.mt_stream, s.ev_used[1 - p], 0);
+      mt_refill_into(s, fg[1 - p], fu[1 - p], s.mt_stream);
+      hipEventRecord(s.ev_filled[1 - p], s.mt_stream);
     }
     for (int m = 0; m < nmicro; m++)
       for (int p = 0; p < 2; p++) launch_batch_micro_sweep(probs, map, nblk, p, s.stream, f32);
@@ -908,7 +933,9 @@ int gfx_batch_anneal(int nsteps, double Tmin, double Tmax, double expo, int nmic
     hipGraphExec_t exec = nullptr;
     if (hip_check(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal), "capture")) rc = GFX_ERR_DEVICE;
     if (rc == GFX_OK) {
-      for (int k2 = 0; k2 < steps_per_graph; k2++) one_step();
+      fork();
+      for (int k2 = 0; k2 < steps_per_graph; k2++) one_step(k2 == 0);
+      join();
       if (hip_check(hipStreamEndCapture(s.stream, &graph), "end capture") ||
           hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate"))
         rc = GFX_ERR_DEVICE;
@@ -921,9 +948,10 @@ int gfx_batch_anneal(int nsteps, double Tmin, double Tmax, double expo, int nmic
     if (exec) hipGraphExecDestroy(exec);
     if (graph) hipGraphDestroy(graph);
   }
-  while (rc == GFX_OK && done < nsteps) {
-    one_step();
-    done++;
+  if (rc == GFX_OK && done < nsteps) {
+    fork();
+    for (bool first = true; done < nsteps; first = false, done++) one_step(first);
+    join();
   }
   if (f32)
     for (const BProbHost& p : b.probs) launch_g32_to_g64(p.g32, p.g64, p.geo, s.stream);

@@ -142,6 +142,10 @@ int upload_aos_geo(State& s, const Geom& geo, const double* host, double2* dst, 
 // twisters (gfx_stoch_api.hip): seed if needed and size the fields; refill
 int mt_prepare(State& s, int need_g, int need_u);
 void mt_refill(State& s, int initialized);
+// the next refill's numbers into the given fields on stream st; the second
+// set of fields (rnd_gauss2, rnd_unif2), the twister stream and its events
+void mt_refill_into(State& s, float* gauss, float* unif, hipStream_t st);
+int mt_second_set(State& s);
 int download_aos_geo(State& s, const Geom& geo, const double2* src, double* host, int nmat);
 
 }  // namespace gfx

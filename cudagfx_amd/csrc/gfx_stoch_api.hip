@@ -134,7 +134,7 @@ int mt_prepare(State& s, int need_g, int need_u) {
 
 // the twister output of one refill into the given fields on stream st
 // (the same kernels and state as mt_refill)
-static void mt_refill_into(State& s, float* gauss, float* unif, hipStream_t st) {
+void mt_refill_into(State& s, float* gauss, float* unif, hipStream_t st) {
   launch_mt_generate(s.mt_params, s.mt_state, gauss, s.n_per_gauss, 1, st);
   launch_box_muller(gauss, s.n_per_gauss, st);
   launch_mt_generate(s.mt_params, s.mt_state, unif, s.n_per_unif, 1, st);
@@ -142,7 +142,7 @@ static void mt_refill_into(State& s, float* gauss, float* unif, hipStream_t st) 
 
 // the second set of fields (as large as the first), the refill stream and
 // its events
-static int mt_second_set(State& s) {
+int mt_second_set(State& s) {
   if (!s.mt_stream) {
     if (hipStreamCreateWithFlags(&s.mt_stream, hipStreamNonBlocking) != hipSuccess) return GFX_ERR_DEVICE;
     for (int k = 0; k < 2; k++) {
