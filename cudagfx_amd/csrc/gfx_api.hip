@@ -322,7 +322,8 @@ void free_all(State& s) {
 
 int set_lattice_from_globals(State& s) {
   if (!geometry_ok(LX, LY, LZ, T)) {
-    printf("Error: lattice %d x %d x %d x %d not supported (extents must be even). Aborting...\n", LX, LY, LZ, T);
+    printf("Error: lattice %d x %d x %d x %d not supported (extents must be even, at most 2^29 sites). Aborting...\n",
+           LX, LY, LZ, T);
     return GFX_ERR_GEOMETRY;
   }
   VOLUME = LX * LY * LZ * T;

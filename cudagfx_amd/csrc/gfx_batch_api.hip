@@ -723,8 +723,8 @@ int gfx_batch_size(void) { return (int)g_b.probs.size(); }
 // a lattice with its own links; returns its index (>= 0) or -error
 int gfx_batch_add(int l0, int l1, int l2, int l3, gfx_su3* gf, gfx_su3* trafo) {
   State& s = g_st;
-  if (ensure_common(0)) return -GFX_ERR_DEVICE;
   if (!geometry_ok(l0, l1, l2, l3)) return -GFX_ERR_GEOMETRY;
+  if (ensure_common(0)) return -GFX_ERR_DEVICE;
   BProbHost p;
   p.geo = make_geom(l0, l1, l2, l3);
   p.ushare = (int)g_b.probs.size();

@@ -67,6 +67,23 @@ def test_setting_validation(gfx):
     assert lib.gfx_set_fp32_stall(300, 1e-8) == 0
 
 
+def test_geometry_is_checked_before_device_work(gfx):
+    """Odd or missing extents and lattices above 2^29 sites (the kernels
+    address a parity's plane elements with 32-bit byte offsets) are refused
+    with GFX_ERR_GEOMETRY before any device call, by every entry point that
+    takes a lattice."""
+    lib = gfx.load()
+    keep = gfx.lattice()
+    try:
+        for dims in [(5, 4, 4, 4), (4, 4, 4, 3), (0, 4, 4, 4), (1024, 1024, 32, 32)]:
+            gfx.set_lattice(*dims)
+            assert lib.init_gaugefixing(None, None) == 100, dims
+            assert lib.init_thermalization(None) == 100, dims
+            assert lib.gfx_batch_add(*dims, None, None) == -100, dims
+    finally:
+        gfx.set_lattice(*keep)
+
+
 def test_driver_binary_built():
     assert os.access(os.path.join(ROOT, "cudagfx_amd", "bin", "cudagaugefix"), os.X_OK)
 
