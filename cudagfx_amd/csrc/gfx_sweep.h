@@ -60,9 +60,9 @@ __device__ __forceinline__ void block_reduce3(double& a, double& b, double& c) {
 //   BLOCK  workgroup size (256 or 64)
 //   NT     non-temporal loads of the links
 //   ROLLED the direction loop of the star is kept as a loop: one direction's
-//          operands are live at a time (FP64: 130 VGPRs, 3 waves/SIMD)
+//          operands are live at a time (FP64: 162 VGPRs, 3 waves/SIMD)
 //          instead of the fully unrolled form (more load-level parallelism
-//          per wave, 332 registers, 1 wave/SIMD).
+//          per wave, 408 VGPRs and AGPRs, 1 wave/SIMD).
 // The FP64 sweep follows the reference's operation order.  The FP32 sweep
 // (mixed path, no bitwise target) is bound by its arithmetic as much as by
 // HBM, and uses a shorter but equal form: star terms from two rows of each
