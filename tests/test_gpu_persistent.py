@@ -94,3 +94,32 @@ def test_persistent_mixed_batch(gfx, orc):
         assert res_p[q][2] < EPS12
         uf = orc.g_trafo(u, gs_p[q], dims)
         assert orc.dAdA(uf, dims)[1] < EPS12 * (1 + 1e-9)
+
+
+def test_persistent_run_cut_at_maxit(gfx, orc):
+    """A run cut at maxit (no lattice reaches eps; 203 is not a whole number
+    of check periods, so the last iterations run eagerly after the
+    persistent launches) leaves every lattice where the graph replays leave
+    it."""
+    gfx.set_omega(1.7)
+    shapes = [(8, 8, 8, 8), (4, 4, 4, 8)]
+    cases = [(dims, smooth_config(orc, dims, 800 + k, eps=0.3),
+              orc.random_init_trafo(orc.volume(dims), 810 + k)) for k, dims in enumerate(shapes)]
+    out = []
+    for persistent in (0, 1):
+        b = gfx.Batch()
+        b.set_persistent(persistent)
+        for dims, u, g in cases:
+            b.add(dims, u, g)
+        print(f"batch run cut at maxit 203, persistent {persistent}", flush=True)
+        rc = b.run(203, 1e-30, 5, gfx.PREC_FP64, 1e-6, 2)
+        out.append((rc, [b.result(q) for q in range(2)], [b.trafo(q) for q in range(2)], b.stats()))
+        b.set_persistent(0)
+        b.close()
+    (rc_g, res_g, gs_g, st_g), (rc_p, res_p, gs_p, st_p) = out
+    assert rc_g == rc_p == 300  # GFX_ERR_NOCONV
+    assert st_p["persistent_launches"] > 0
+    for q in range(2):
+        assert res_p[q][3] == res_g[q][3] == -1
+        assert abs(res_p[q][0] - res_g[q][0]) < 1e-14
+        assert np.abs(gs_p[q] - gs_g[q]).max() < 1e-13
