@@ -22,10 +22,11 @@ namespace cg = cooperative_groups;
 
 // The grid is a whole number of rounds of the 8 XCDs, so item w of a map
 // runs on XCD w % 8 as it would as workgroup w of a plain launch: the map's
-// dealing of blocks to XCDs holds.  With the three forms of the sweep item
-// (plain, odd check, even check) in one kernel, the registers are allocated
-// for all of them: the fixed-point FP64 form takes 225 VGPRs (2 waves per
-// SIMD) where its plain launches take 168-170 (3 waves).
+// dealing of blocks to XCDs holds.  The three forms of the sweep item
+// (plain, odd check, even check) share one call site with the form chosen
+// at run time: with a call site per form the compiler allocated 223-226
+// VGPRs for the fixed-point FP64 form (2 waves per SIMD) where its plain
+// launches take 168-170.
 template <typename V, typename R, int BLOCK, bool ROLLED, bool PK>
 __global__ __launch_bounds__(BLOCK) void k_batch_persistent(PersistArgs a, BatchCheck c, R c1,
                                                             R c2) {
@@ -35,35 +36,24 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(PersistArgs a, Batch
   const int* act = c.act[pr];
   const int* poff = c.poff[pr];
   for (int per = 0; per < a.periods; per++) {
-    for (int j = 0; j < a.ck; j++) {
-      // the even half of a period's first iteration ran as the check sweep
-      // of the period before
-      if (j > 0) {
-        for (int w = blockIdx.x; w < a.nblocks; w += gridDim.x)
-          batch_sweep_item<V, R, 4, BLOCK, ROLLED, false, 0, PK>(probs, a.bmap[w], act, poff, 0,
-                                                                 c1, c2, a.pod);
-        grid.sync();
-      }
-      if (j == a.ck - 1) {
-        for (int w = blockIdx.x; w < a.nblocks; w += gridDim.x) {
-          batch_sweep_item<V, R, 4, BLOCK, ROLLED, false, 1, PK>(probs, a.bmap[w], act, poff, 1,
-                                                                 c1, c2, a.pod);
-          __syncthreads();  // the block reduction's LDS is used again by the next item
-        }
-      } else {
-        for (int w = blockIdx.x; w < a.nblocks; w += gridDim.x)
-          batch_sweep_item<V, R, 4, BLOCK, ROLLED, false, 0, PK>(probs, a.bmap[w], act, poff, 1,
-                                                                 c1, c2, a.pod);
+    // the 2 ck half sweeps of a period, one body for all (one call site:
+    // registers as in the plain launches): h = 0 the odd half of its first
+    // iteration (the even half ran as the check sweep of the period
+    // before), then even and odd halves, the odd half of the last iteration
+    // carrying the check (h = 2 ck - 2), and the even half of the next
+    // iteration carrying the check (h = 2 ck - 1)
+    for (int h = 0; h < 2 * a.ck; h++) {
+      const int parity = (h & 1) ? 0 : 1;
+      const int chk = h == 2 * a.ck - 2 ? 1 : h == 2 * a.ck - 1 ? 2 : 0;
+      double* partials = chk == 2 ? a.pev : a.pod;
+      for (int w = blockIdx.x; w < a.nblocks; w += gridDim.x) {
+        batch_sweep_item<V, R, 4, BLOCK, ROLLED, false, 3, PK>(probs, a.bmap[w], act, poff, parity,
+                                                               c1, c2, partials, chk);
+        // the block reduction's LDS is used again by the next item
+        if (chk) __syncthreads();
       }
       grid.sync();
     }
-    // the even half of the next iteration carrying the check
-    for (int w = blockIdx.x; w < a.nblocks; w += gridDim.x) {
-      batch_sweep_item<V, R, 4, BLOCK, ROLLED, false, 2, PK>(probs, a.bmap[w], act, poff, 0, c1,
-                                                             c2, a.pev);
-      __syncthreads();
-    }
-    grid.sync();
     for (int q = blockIdx.x; q < a.nprob; q += gridDim.x) {
       batch_finalize_lattice(q, c, 4, a.eps, a.switch_theta, a.maxit, a.stall_checks,
                              a.stall_gate, a.changed);

@@ -81,6 +81,8 @@ __device__ __forceinline__ void block_reduce3(double& a, double& b, double& c) {
 //   CHK = 2  even half sweep of iteration i+1: before the update, g(x)K(x)
 //            -> Delta of the even sites of state i; the new even field goes
 //            to another buffer (gout), so a stop leaves state i intact.
+//   CHK = 3  one of the three, given at run time by the argument chk (one
+//            body for all three: the persistent form's single call site)
 // Each block writes (sum F, sum dAdA, max dAdA) of its sites to
 // partials[3 * block].
 template <typename R>
@@ -109,7 +111,7 @@ __device__ __forceinline__ void site_check(const M3<R>& G, const M3<R>& K, doubl
 template <typename V, typename R, int NDIR, bool ROLLED, bool NT, int CHK, typename LK, bool FSHORT = false>
 __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, const V* gin, V* gout,
                                            const Geom& geo, const Site& s, int i, int parity, R c1,
-                                           R c2, double& F, double& d) {
+                                           R c2, double& F, double& d, int chk = CHK) {
   const int S = geo.stride;
   // star K = sum_mu U_mu(x) g^+(x+mu) + U_mu^+(x-mu) g^+(x-mu)
   M3<R> K;
@@ -184,7 +186,7 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
     }
   }
   M3<R> G = load_m<V, R>(gin, S, i);
-  if (CHK == 2) site_check(G, K, F, d);
+  if (CHK == 2 || (CHK == 3 && chk == 2)) site_check(G, K, F, d);
   M3<R> h = G;
   cm_relax(h, K);
   m_normalize(h);
@@ -198,17 +200,18 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
     overrelax_apply(G, h, c1, c2);
     m_normalize(G);
   }
-  if (CHK == 1) site_check(G, K, F, d);
+  if (CHK == 1 || (CHK == 3 && chk == 1)) site_check(G, K, F, d);
   Store<V>::store(gout, S, i, G);
 }
 
 template <int CHK>
-__device__ __forceinline__ void store_check_partials(double F, double d, double* partials, int slot) {
+__device__ __forceinline__ void store_check_partials(double F, double d, double* partials, int slot,
+                                                     int chk = CHK) {
   double mx = d;
   block_reduce3(F, d, mx);
   if (threadIdx.x == 0) {
     double* o = partials + 3 * (size_t)slot;
-    o[0] = (CHK == 1) ? F : 0.0;
+    o[0] = (CHK == 1 || (CHK == 3 && chk == 1)) ? F : 0.0;
     o[1] = d;
     o[2] = mx;
   }
@@ -257,38 +260,39 @@ __device__ __forceinline__ void or_sweep_block(LK U, V* ge, V* go, V* galt, cons
 // single-lattice polish, which the fixed-point stop bound assumes.  PK: the
 // links are read from the packed planes P.pk (FP32: the 21-bit planes,
 // FP64: high + low parts).
-// One block of the map: m = (problem, block inside the problem).  Also the
-// work item of the persistent form (k_batch_persistent), whose finalize
-// changes probs, active and the partials between the items it runs: no
-// restrict qualifiers here.
+// batch_sweep_item: one block of the map, m = (problem, block inside the
+// problem).  Also the work item of the persistent form (k_batch_persistent,
+// CHK = 3 with the check form in chk), whose per-lattice check changes
+// probs, active and the partials between the items it runs: no restrict
+// qualifiers here.
 template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK, bool PK>
 __device__ __forceinline__ void batch_sweep_item(const BatchProblem* probs, int2 m,
                                                  const int* active, const int* poff, int parity,
-                                                 R c1, R c2, double* partials) {
+                                                 R c1, R c2, double* partials, int chk = CHK) {
   if (!active[m.x]) return;
   const BatchProblem& P = probs[m.x];
   const Geom& geo = P.geo;
   const V* U = static_cast<const V*>(P.U);
   V* gme = static_cast<V*>(P.g[parity]);
   const V* gop = static_cast<const V*>(P.g[1 - parity]);
-  V* gout = (CHK == 2) ? static_cast<V*>(P.galt) : gme;
+  V* gout = (CHK == 2 || (CHK == 3 && chk == 2)) ? static_cast<V*>(P.galt) : gme;
   const int i = m.y * BLOCK + threadIdx.x;
   double F = 0.0, d = 0.0;
   constexpr bool FSHORT = !std::is_same<V, int4>::value;
   auto site = [&](const Site& s) {
     if constexpr (PK)
       sweep_site<V, R, NDIR, ROLLED, NT, CHK, PackedLinks, true>(P.pk, gop, gme, gout, geo, s, i,
-                                                                 parity, c1, c2, F, d);
+                                                                 parity, c1, c2, F, d, chk);
     else
       sweep_site<V, R, NDIR, ROLLED, NT, CHK, const V*, FSHORT>(U, gop, gme, gout, geo, s, i,
-                                                                parity, c1, c2, F, d);
+                                                                parity, c1, c2, F, d, chk);
   };
-  if (CHK == 0) {
+  if (CHK == 0 || (CHK == 3 && chk == 0)) {
     if (i >= geo.nh) return;
     site(site_coords(geo, i, parity));
   } else {
     if (i < geo.nh) site(site_coords(geo, i, parity));
-    store_check_partials<CHK>(F, d, partials, poff[m.x] + m.y);
+    store_check_partials<CHK>(F, d, partials, poff[m.x] + m.y, chk);
   }
 }
 
