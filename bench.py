@@ -49,6 +49,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-grib
        python bench.py --profile --plan FILE     (kernels for rocprofv3 counters)
        python bench.py --trace-step --plan FILE  (one mixed and one FP64 step, bracketed)
        python bench.py --only gribov|uneven [--persistent]  (one batch workload alone)
+       python bench.py --only coulomb            (Coulomb 48^3x96: FP64, mixed at switch 1e-6, 1e-10)
        python bench.py --summarize-profile FILE STATS FETCH WRITE
        python bench.py --summarize-step FILE STATS
 """
@@ -555,6 +556,49 @@ def gribov_run(gfx, persistent=False):
             "plaquette": plaq}
 
 
+def coulomb_run(gfx):
+    """Coulomb gauge of a 48^3 x 96 beta = 6.2 configuration (100 heatbath
+    sweeps from a cold start), every time slice to max dAdA < 1e-12 with
+    checks every 10 iterations, all 96 slices per launch: plain FP64, then
+    mixed with the switch at 1e-6 and at 1e-10 (FP32 slices on the 21-bit
+    links, FP64 slices polished on the fixed-point links and g), each from
+    the same initial g."""
+    dims = (48, 48, 48, 96)
+    V = int(np.prod(dims))
+    t0 = time.perf_counter()
+    u, plaq = thermalized_config(gfx, dims, 6.2, 100, 1001)
+    gfx.plant_seeds(1002)
+    g = gfx.random_init_trafo_host(V)
+    gfx.set_lattice(*dims)
+    gfx.init_gaugefixing(u, g)
+    gfx.set_omega(OMEGA)
+    setup = time.perf_counter() - t0
+    out = {"workload": "Coulomb gauge 48^3x96 beta=6.2 (100 heatbath sweeps), omega=1.7, per-slice "
+                       "theta<1e-12, ck=10, all 96 slices per launch", "setup_s": setup, "plaquette": plaq}
+    F64 = None
+    for name, prec, switch in (("fp64", gfx.PREC_FP64, 1e-6), ("mixed", gfx.PREC_MIXED, 1e-6),
+                               ("mixed_switch_1e-10", gfx.PREC_MIXED, 1e-10)):
+        gfx.set_trafo(g)
+        gfx.set_precision(prec, switch)
+        gfx.sync()
+        t1 = time.perf_counter()
+        rc, iters = gfx.coulomb_gauge(30000, EPS, CK)
+        gfx.sync()
+        wall = time.perf_counter() - t1
+        F, D, M = gfx.functional_slices()
+        if F64 is None:
+            F64 = F
+        out[name] = {"rc": rc, "wall_s": wall, "iters_min": int(np.min(iters)),
+                     "iters_mean": float(np.mean(iters)), "iters_max": int(np.max(iters)),
+                     "max_theta": float(np.max(M)),
+                     "F_rel_vs_fp64": float(np.max(np.abs(F - F64) / F64))}
+    gfx.set_precision(gfx.PREC_FP64)
+    gfx.finalize_gaugefixing()
+    for name in ("mixed", "mixed_switch_1e-10"):
+        out[name]["speedup"] = out["fp64"]["wall_s"] / out[name]["wall_s"]
+    return out
+
+
 def uneven_run(gfx, persistent=False):
     """The uneven-size batch, once: 16 heatbath-thermalized configurations
     of four shapes fixed together in the mixed form (persistent: as in
@@ -609,6 +653,8 @@ def measure(args):
         return {"gribov": gribov_run(gfx, args.persistent)}
     if args.only == "uneven":
         return {"uneven": uneven_run(gfx, args.persistent)}
+    if args.only == "coulomb":
+        return {"coulomb": coulomb_run(gfx)}
     V = int(np.prod(DIMS))
     t_setup = time.perf_counter()
     u, plaq = thermalized_config(gfx)
@@ -824,7 +870,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gribov", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
-    ap.add_argument("--only", choices=("gribov", "uneven"),
+    ap.add_argument("--only", choices=("gribov", "uneven", "coulomb"),
                     help="run one of the batch workloads alone (for a kernel trace)")
     ap.add_argument("--persistent", action="store_true",
                     help="with --only: persistent launches instead of graph replays")
