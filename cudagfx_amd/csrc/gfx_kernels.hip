@@ -27,8 +27,17 @@
 
 namespace gfx {
 
+// The check-carrying sweeps of the fixed-point polish compile to 170 VGPRs,
+// two above the 168 at which a SIMD holds 3 waves: they ask for 3 waves per
+// SIMD (the second bound is amdgpu_waves_per_eu), as the plain polish runs.
+template <typename V, int CHK>
+constexpr int sweep_min_waves() {
+  return (std::is_same<V, int4>::value && CHK != 0) ? 3 : 1;
+}
+
 template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK>
-__global__ __launch_bounds__(BLOCK) void k_or_sweep(const V* __restrict__ U, V* ge, V* go, V* galt,
+__global__ __launch_bounds__(BLOCK, (sweep_min_waves<V, CHK>())) void k_or_sweep(
+    const V* __restrict__ U, V* ge, V* go, V* galt,
                                                      Geom geo, int parity, R c1, R c2,
                                                      const unsigned char* __restrict__ active,
                                                      const int* __restrict__ bmap,
@@ -55,7 +64,7 @@ __global__ __launch_bounds__(BLOCK) void k_or_sweep_packed(PackedLinks U, V* ge,
 
 // ------------------------------------------------------------ batched sweep
 template <typename V, typename R, int NDIR, int BLOCK, bool ROLLED, bool NT, int CHK, bool PK>
-__global__ __launch_bounds__(BLOCK) void k_batch_sweep(const BatchProblem* __restrict__ probs,
+__global__ __launch_bounds__(BLOCK, (sweep_min_waves<V, CHK>())) void k_batch_sweep(const BatchProblem* __restrict__ probs,
                                                         const int2* __restrict__ bmap,
                                                         const int* __restrict__ active,
                                                         const int* __restrict__ poff, int parity,
