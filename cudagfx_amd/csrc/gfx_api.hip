@@ -346,6 +346,7 @@ int functional_now(State& s, bool fp32, int ndir, int nseg, double eps, int even
 
 void reset_tuning(State& s) {
   s.hb_tiling = -1;
+  s.sa_tiling = -1;
   for (int a = 0; a < NFORM; a++)
     for (int b = 0; b < 2; b++) {
       s.tuned[a][b] = false;

@@ -149,8 +149,11 @@ __device__ __forceinline__ void cm_heatbath(M3<R>& h, const M3<R>& K, double bet
     }
 }
 
-// star K of g at site i (same as the overrelaxation sweep)
-template <int NDIR, typename V = double2, typename R = double>
+// star K of g at site i (same as the overrelaxation sweep).  NT: the links
+// are loaded non-temporally (a single lattice reads each once per half
+// sweep, and L2 keeps the neighbour g instead); Gribov copies sharing their
+// links load them through the caches
+template <int NDIR, bool NT, typename V = double2, typename R = double>
 __device__ __forceinline__ M3<R> star_of(const V* __restrict__ U, const V* __restrict__ gop,
                                          const Geom& geo, const Site& s, int i, int parity) {
   M3<R> K;
@@ -161,20 +164,25 @@ __device__ __forceinline__ M3<R> star_of(const V* __restrict__ U, const V* __res
   // short form (the stochastic updates have no bitwise target)
 #pragma unroll 1
   for (int mu = 0; mu < NDIR; mu++)
-    star_terms_short<V, R, false, const V*, 1>(K, U, gop, geo, s, i, parity, mu);
+    star_terms_short<V, R, NT, const V*, 1>(K, U, gop, geo, s, i, parity, mu);
   return K;
 }
 
+// bmap: launch position -> block of HB_BLK sites (a sweep workgroup order
+// dealt to the XCDs); the sites of one parity are updated from the other
+// parity only and draw the numbers of their own index, so the order leaves
+// the result unchanged
 template <int SAMPLER, int NDIR>
 __global__ __launch_bounds__(HB_BLK) void k_sa_sweep(const double2* __restrict__ U,
                                                      double2* __restrict__ g, Geom geo, int parity,
                                                      double beta, const float* __restrict__ gauss,
-                                                     const float* __restrict__ unif) {
-  const int i = blockIdx.x * HB_BLK + threadIdx.x;
+                                                     const float* __restrict__ unif,
+                                                     const int* __restrict__ bmap) {
+  const int i = bmap[blockIdx.x] * HB_BLK + threadIdx.x;
   if (i >= geo.nh) return;
   const Site s = site_coords(geo, i, parity);
   const int S = geo.stride;
-  const M3<double> K = star_of<NDIR>(U, field_ptr(static_cast<const double2*>(g), 1 - parity, S),
+  const M3<double> K = star_of<NDIR, true>(U, field_ptr(static_cast<const double2*>(g), 1 - parity, S),
                                      geo, s, i, parity);
   double2* gme = field_ptr(g, parity, S);
   M3<double> h = load_m<double2, double>(gme, S, i);
@@ -204,12 +212,12 @@ __device__ __forceinline__ void micro_reflect(M3<R>& h, const M3<R>& K) {
 template <int NDIR>
 __global__ __launch_bounds__(HB_BLK) void k_micro_sweep(const double2* __restrict__ U,
                                                         double2* __restrict__ g, Geom geo,
-                                                        int parity) {
-  const int i = blockIdx.x * HB_BLK + threadIdx.x;
+                                                        int parity, const int* __restrict__ bmap) {
+  const int i = bmap[blockIdx.x] * HB_BLK + threadIdx.x;
   if (i >= geo.nh) return;
   const Site s = site_coords(geo, i, parity);
   const int S = geo.stride;
-  const M3<double> K = star_of<NDIR>(U, field_ptr(static_cast<const double2*>(g), 1 - parity, S),
+  const M3<double> K = star_of<NDIR, true>(U, field_ptr(static_cast<const double2*>(g), 1 - parity, S),
                                      geo, s, i, parity);
   double2* gme = field_ptr(g, parity, S);
   M3<double> h = load_m<double2, double>(gme, S, i);
@@ -238,7 +246,7 @@ __global__ __launch_bounds__(HB_BLK) void k_batch_sa_sweep(const BatchProblem* _
   if (i >= geo.nh) return;
   const Site s = site_coords(geo, i, parity);
   V* gme = static_cast<V*>(P.g[parity]);
-  const M3<R> K = star_of<4, V, R>(static_cast<const V*>(P.U), static_cast<const V*>(P.g[1 - parity]),
+  const M3<R> K = star_of<4, false, V, R>(static_cast<const V*>(P.U), static_cast<const V*>(P.g[1 - parity]),
                                    geo, s, i, parity);
   M3<R> h = load_m<V, R>(gme, geo.stride, i);
   cm_heatbath<SAMPLER>(h, K, betas[*step], gauss, unif, roff[m.x] + i);
@@ -257,7 +265,7 @@ __global__ __launch_bounds__(HB_BLK) void k_batch_micro_sweep(const BatchProblem
   if (i >= geo.nh) return;
   const Site s = site_coords(geo, i, parity);
   V* gme = static_cast<V*>(P.g[parity]);
-  const M3<R> K = star_of<4, V, R>(static_cast<const V*>(P.U), static_cast<const V*>(P.g[1 - parity]),
+  const M3<R> K = star_of<4, false, V, R>(static_cast<const V*>(P.U), static_cast<const V*>(P.g[1 - parity]),
                                    geo, s, i, parity);
   M3<R> h = load_m<V, R>(gme, geo.stride, i);
   micro_reflect(h, K);
@@ -346,28 +354,28 @@ void launch_haar_trafo(double2* g, const Geom& geo, const float* gauss, hipStrea
 
 void launch_sa_sweep(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
                      double beta, int sampler, const float* gauss, const float* unif,
-                     hipStream_t st) {
+                     const int* bmap, hipStream_t st) {
   dim3 grid((geo.nh + HB_BLK - 1) / HB_BLK);
   if (sampler == 0) {
     if (ndir == 4)
-      hipLaunchKernelGGL((k_sa_sweep<0, 4>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity, beta, gauss, unif);
+      hipLaunchKernelGGL((k_sa_sweep<0, 4>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity, beta, gauss, unif, bmap);
     else
-      hipLaunchKernelGGL((k_sa_sweep<0, 3>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity, beta, gauss, unif);
+      hipLaunchKernelGGL((k_sa_sweep<0, 3>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity, beta, gauss, unif, bmap);
   } else {
     if (ndir == 4)
-      hipLaunchKernelGGL((k_sa_sweep<1, 4>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity, beta, gauss, unif);
+      hipLaunchKernelGGL((k_sa_sweep<1, 4>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity, beta, gauss, unif, bmap);
     else
-      hipLaunchKernelGGL((k_sa_sweep<1, 3>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity, beta, gauss, unif);
+      hipLaunchKernelGGL((k_sa_sweep<1, 3>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity, beta, gauss, unif, bmap);
   }
 }
 
 void launch_micro_sweep(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
-                        hipStream_t st) {
+                        const int* bmap, hipStream_t st) {
   dim3 grid((geo.nh + HB_BLK - 1) / HB_BLK);
   if (ndir == 4)
-    hipLaunchKernelGGL((k_micro_sweep<4>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity);
+    hipLaunchKernelGGL((k_micro_sweep<4>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity, bmap);
   else
-    hipLaunchKernelGGL((k_micro_sweep<3>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity);
+    hipLaunchKernelGGL((k_micro_sweep<3>), grid, dim3(HB_BLK), 0, st, U, g, geo, parity, bmap);
 }
 
 void launch_link_heatbath(double2* U, const Geom& geo, int parity, int nu, double beta,

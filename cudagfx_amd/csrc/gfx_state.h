@@ -91,6 +91,9 @@ struct State {
   // workgroup order (tiling of bmap[0]) of the link heatbath for this
   // lattice, picked by timing on its first sweep (-1: not yet)
   int hb_tiling = -1;
+  // the same for the simulated-annealing and microcanonical sweeps of g,
+  // picked by timing the first SA launches of the first annealing run
+  int sa_tiling = -1;
 
   // sweep kernel variant per [form: FORM_*][ndir
   // 3/4], picked by the autotuner on first use for the current lattice

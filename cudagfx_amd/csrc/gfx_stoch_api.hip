@@ -24,9 +24,9 @@ void launch_mt_generate(uint32_t* params, uint32_t* state, float* out, int nper,
 void launch_box_muller(float* d, int nper, hipStream_t st);
 void launch_sa_sweep(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
                      double beta, int sampler, const float* gauss, const float* unif,
-                     hipStream_t st);
+                     const int* bmap, hipStream_t st);
 void launch_micro_sweep(const double2* U, double2* g, const Geom& geo, int parity, int ndir,
-                        hipStream_t st);
+                        const int* bmap, hipStream_t st);
 void launch_link_heatbath(double2* U, const Geom& geo, int parity, int nu, double beta,
                           const float* gauss, const float* unif, const int* bmap, hipStream_t st);
 int kp_gauss_per_site();
@@ -298,17 +298,53 @@ void simannealing_gauge(void) {
     if (mt_fields(s, need_g, need_u)) return;
     mt_refill(s, 1);
   }
+  if (mt_second_set(s)) return;
   const auto t0 = std::chrono::steady_clock::now();
   const int ck = saparam.checkint > 0 ? saparam.checkint : 1;
+  // SA launch k reads fields[k % 2]; the refill that launch k+1 reads is made
+  // into fields[(k+1) % 2] on the twister stream beside launch k and the
+  // microcanonical sweeps after it (once launch k-1 has read those fields),
+  // refills in order: the numbers of a refill after each launch
+  float* fg[2] = {s.rnd_gauss, s.rnd_gauss2};
+  float* fu[2] = {s.rnd_unif, s.rnd_unif2};
+  hipEventRecord(s.ev_filled[0], s.stream);  // fields[0] hold the current numbers
+  hipStreamWaitEvent(s.mt_stream, s.ev_filled[0], 0);
+  // workgroup order: the first SA launches of the first run on this lattice
+  // try each tiling that applies
+  std::vector<int> tilings;
+  if (s.sa_tiling < 0)
+    for (int t = 0; t < TILINGS; t++)
+      if (s.tiling_ok[0][t]) tilings.push_back(t);
+  float best = 1e30f;
+  int k = 0;
   for (int i = 0; i < saparam.N; i++) {
     const double T = gfx_sa_temperature(i, saparam.Tmin, saparam.Tmax, saparam.expo, saparam.N);
     const double beta = 1.0 / T;
-    for (int p = 0; p < 2; p++) {
-      launch_sa_sweep(s.U64, s.g64, s.geo, p, ndir, beta, sampler, s.rnd_gauss, s.rnd_unif, s.stream);
-      mt_refill(s, 1);
+    for (int p = 0; p < 2; p++, k++) {
+      const int b = k & 1;
+      const bool timing = k < (int)tilings.size();
+      const int t = timing ? tilings[k] : (s.sa_tiling >= 0 ? s.sa_tiling : 0);
+      if (k > 0) hipStreamWaitEvent(s.stream, s.ev_filled[b], 0);
+      if (timing) hipEventRecord(s.ev0, s.stream);
+      launch_sa_sweep(s.U64, s.g64, s.geo, p, ndir, beta, sampler, fg[b], fu[b], s.bmap[0][t], s.stream);
+      if (timing) {
+        hipEventRecord(s.ev1, s.stream);
+        hipEventSynchronize(s.ev1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, s.ev0, s.ev1);
+        if (ms < best) {
+          best = ms;
+          s.sa_tiling = t;
+        }
+      }
+      hipEventRecord(s.ev_used[b], s.stream);
+      if (k > 0) hipStreamWaitEvent(s.mt_stream, s.ev_used[1 - b], 0);
+      mt_refill_into(s, fg[1 - b], fu[1 - b], s.mt_stream);
+      hipEventRecord(s.ev_filled[1 - b], s.mt_stream);
     }
+    const int* map = s.bmap[0][s.sa_tiling >= 0 ? s.sa_tiling : 0];
     for (int m = 0; m < g_rng.sa_nmicro; m++)
-      for (int p = 0; p < 2; p++) launch_micro_sweep(s.U64, s.g64, s.geo, p, ndir, s.stream);
+      for (int p = 0; p < 2; p++) launch_micro_sweep(s.U64, s.g64, s.geo, p, ndir, map, s.stream);
     if ((i % ck) == 0) {
       if (functional_now(s, false, ndir, 1, 0.0)) return;
       FUNC = s.h_results[0];
@@ -318,6 +354,13 @@ void simannealing_gauge(void) {
         printf("iter %6d:\t FUNC = %.16e \t dAdA = %.16e\t max(dAdA) = %.16e\n", i, FUNC, DADA, maxDADA);
     }
   }
+  // the fields of the last refill become the current ones
+  hipStreamWaitEvent(s.stream, s.ev_filled[k & 1], 0);
+  if (k & 1) {
+    std::swap(s.rnd_gauss, s.rnd_gauss2);
+    std::swap(s.rnd_unif, s.rnd_unif2);
+  }
+  hipStreamSynchronize(s.mt_stream);
   hipStreamSynchronize(s.stream);
   const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (s.verbose) printf("SA finished after %f sec\n", sec);
@@ -384,8 +427,9 @@ int gfx_su2_sample4(int sampler, double beta, double wnorm, int n, double* alpha
 int gfx_micro_sweeps(int n, int ndir) {
   State& s = g_st;
   if (s.init != 1) return GFX_ERR_STATE;
+  const int* map = s.bmap[0][s.sa_tiling >= 0 ? s.sa_tiling : 0];
   for (int k = 0; k < n; k++)
-    for (int p = 0; p < 2; p++) launch_micro_sweep(s.U64, s.g64, s.geo, p, ndir == 3 ? 3 : 4, s.stream);
+    for (int p = 0; p < 2; p++) launch_micro_sweep(s.U64, s.g64, s.geo, p, ndir == 3 ? 3 : 4, map, s.stream);
   return hip_check(hipStreamSynchronize(s.stream), "micro");
 }
 

@@ -199,11 +199,11 @@ def _host_F(orc, u, g, dims):
     return orc.gauge_functional(orc.g_trafo(u, g, dims), dims)
 
 
-@pytest.mark.parametrize("sampler", [0, 1])
-def test_batch_anneal_of_one_lattice_equals_simannealing(gfx, orc, sampler):
+@pytest.mark.parametrize("sampler,dims", [(0, (8, 8, 8, 8)), (1, (8, 8, 8, 8)), (0, (16, 16, 16, 16))])
+def test_batch_anneal_of_one_lattice_equals_simannealing(gfx, orc, sampler, dims):
     """A batch holding one lattice draws the same twister numbers in the same
-    order as simannealing_gauge, so both give the same g."""
-    dims = (8, 8, 8, 8)
+    order as simannealing_gauge, so both give the same g (at 16^4 the single
+    lattice's first SA launches run under each workgroup order it times)."""
     V = orc.volume(dims)
     u = _thermalized(gfx, dims, 6.0, 30, 41)
     g0 = orc.random_init_trafo(V, 42)
