@@ -189,14 +189,17 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
   if (CHK == 2 || (CHK == 3 && chk == 2)) site_check(G, K, F, d);
   M3<R> h = G;
   cm_relax(h, K);
-  m_normalize(h);
   // the polish on packed or fixed-point links applies the series in the
   // short form too: its star and relaxation keep the reference's order,
   // which is what pins the packing to the oracle; the series differs from
-  // it by rounding only
+  // it by rounding only.  The short form takes h as the relaxation leaves
+  // it: h is G rotated by three normalised SU(2) elements, as unitary as G,
+  // and the series' result is normalised anyway (the reference normalises
+  // h first, overrelax.cu:413)
   if constexpr (SHORT || std::is_same<LK, PackedLinks>::value || std::is_same<V, int4>::value) {
     overrelax_apply_short(G, h, c1, c2);
   } else {
+    m_normalize(h);
     overrelax_apply(G, h, c1, c2);
     m_normalize(G);
   }

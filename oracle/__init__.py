@@ -151,7 +151,8 @@ def from_2v(a):
 # ---------------------------------------------------------------- sweeps
 def halfsweep(u2v, g2v, dims, parity, ndir=4, omega=1.68, nterms=3, active=None):
     """One half sweep (overrelax.cu:335-446) on g2v in place; nterms = -3:
-    the 3-term series in the short form of the library's polish sweeps."""
+    the 3-term series in the short form of the library's polish sweeps, on
+    h as the relaxation leaves it (not normalised)."""
     nn = neighbours(dims)
     ev, od = eo_lists(dims)
     sites = ev if parity == 0 else od

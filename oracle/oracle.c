@@ -378,13 +378,15 @@ static void site_update(const double *u2v, double *g2v, const int *nn, int pos,
   m_load(gx, g2v + 12L * pos);
   m_copy(h, gx);
   m_cm_relax(h, star);
-  m_normalize(h);
   /* nterms < 0: the 3-term series in the short form of the library's
-   * polish sweeps (m_overrelax_short) */
-  if (nterms < 0)
+   * polish sweeps (m_overrelax_short), which takes h as the relaxation
+   * leaves it; the reference normalises h first (overrelax.cu:413) */
+  if (nterms < 0) {
     m_overrelax_short(gx, h, omega);
-  else
+  } else {
+    m_normalize(h);
     m_overrelax(gx, h, omega, nterms);
+  }
   m_normalize(gx);
   m_store(g2v + 12L * pos, gx);
 }

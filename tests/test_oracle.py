@@ -211,6 +211,27 @@ def test_short_series_equals_reference_series(orc, omega):
             assert np.abs(a - b).max() > 1e-11
 
 
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (4, 6, 4, 2)])
+def test_short_sweep_equals_reference_sweep(orc, dims):
+    """A half sweep in the short form (nterms = -3: the series in its short
+    form on h as the relaxation leaves it, not normalised) equals the
+    reference's half sweep (h normalised, overrelax.cu:411-417) to rounding
+    on unitary links and g, both parities in turn; with g off the group by
+    1e-6 (the short form relies on the unitarity of g) the two differ by
+    more than rounding, so the comparison can tell them apart."""
+    V = orc.volume(dims)
+    u2 = orc.su3_to_2v(orc.random_init_gauge(V, 4401))
+    g0 = orc.su3_to_2v(orc.random_init_trafo(V, 4402))
+    for off, lo, hi in ((0.0, 0.0, 1e-13), (1e-6, 1e-9, 1.0)):
+        g = g0 * (1.0 + off * np.cos(np.arange(g0.size)))
+        a, b = g.copy(), g.copy()
+        for parity in (0, 1):
+            orc.halfsweep(u2, a, dims, parity, 4, 1.7)
+            orc.halfsweep(u2, b, dims, parity, 4, 1.7, nterms=-3)
+        d = np.abs(a - b).max()
+        assert lo <= d < hi, (off, d)
+
+
 @pytest.mark.parametrize("omega", [1.0, 2.0, 1.68, 1.7])
 def test_overrelax_series_closed_form(orc, omega):
     """overrelax.cu:149-177 with N = 3: g -> normalize(1 + w R + w(w-1)/2 R^2) g,
