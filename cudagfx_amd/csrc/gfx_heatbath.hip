@@ -164,7 +164,7 @@ __device__ __forceinline__ M3<R> star_of(const V* __restrict__ U, const V* __res
   // short form (the stochastic updates have no bitwise target)
 #pragma unroll 1
   for (int mu = 0; mu < NDIR; mu++)
-    star_terms_short<V, R, NT, const V*, 1>(K, U, gop, geo, s, i, parity, mu);
+    star_terms_short<V, R, NT, const V*, true>(K, U, gop, geo, s, i, parity, mu);
   return K;
 }
 

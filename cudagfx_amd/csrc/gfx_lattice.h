@@ -697,7 +697,9 @@ __device__ __forceinline__ void short_term_bwd(M3<R>& K, const Cx<R> a[2][3], Cx
 #pragma unroll
     for (int c = 0; c < 3; c++) add_term<R, SC>(K.e[r][c], p[c][r], true, c < 2 ? s1 : s2);
 }
-template <typename V, typename R, bool NT, typename LK, int NC = 0>
+// The two terms of direction mu; ALL = true loads the direction's four
+// operands before any is decoded.
+template <typename V, typename R, bool NT, typename LK, bool ALL = false>
 __device__ __forceinline__ void star_terms_short(M3<R>& K, LK U, const V* __restrict__ gop,
                                                  const Geom& geo, const Site& s, int i,
                                                  int parity, int mu) {
@@ -705,48 +707,110 @@ __device__ __forceinline__ void star_terms_short(M3<R>& K, LK U, const V* __rest
   // packed links in FP32: decoded without their scale (see short_term_fwd)
   constexpr bool SC = std::is_same<LK, PackedLinks>::value && std::is_same<R, float>::value;
   Cx<R> a[2][3], b[3][3];
-  if constexpr (NC == 0) {
+  const int jf = nb_fwd(geo, s, i, mu);
+  const int jb = nb_bwd(geo, s, i, mu);
+  if constexpr (ALL) {
+    Raw<LK, R> lf, lb;
+    Raw<const V*, R> gf, gb;
+    raw_load<R, NT>(U, parity * 4 + mu, S, i, lf);
+    raw_load<R, false>(gop, S, jf, gf);
+    raw_load<R, false>(gop, S, jb, gb);
+    raw_load<R, NT>(U, (1 - parity) * 4 + mu, S, jb, lb);
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (SC)
+      raw_rows<R, true>(lf, a[0], a[1]);
+    else
+      raw_rows<R>(lf, a[0], a[1]);
+    raw_rows<R>(gf, b[0], b[1]);
+    short_term_fwd<R, SC>(K, a, b);
+    raw_rows<R>(gb, a[0], a[1]);
+    if constexpr (SC)
+      raw_rows<R, true>(lb, b[0], b[1]);
+    else
+      raw_rows<R>(lb, b[0], b[1]);
+    short_term_bwd<R, SC>(K, a, b);
+    return;
+  }
+  if constexpr (SC)
+    link_rows<R, NT, true>(U, parity * 4 + mu, S, i, a[0], a[1]);
+  else
+    link_rows<R, NT>(U, parity * 4 + mu, S, i, a[0], a[1]);
+  Store<V>::template load<R, false>(gop, S, jf, b[0], b[1]);
+  short_term_fwd<R, SC>(K, a, b);
+  Store<V>::template load<R, false>(gop, S, jb, a[0], a[1]);
+  if constexpr (SC)
+    link_rows<R, NT, true>(U, (1 - parity) * 4 + mu, S, jb, b[0], b[1]);
+  else
+    link_rows<R, NT>(U, (1 - parity) * 4 + mu, S, jb, b[0], b[1]);
+  short_term_bwd<R, SC>(K, a, b);
+}
+
+// The whole short star, software-pipelined: the operands of directions 0
+// and 1 are loaded, then the terms of each direction mu are formed and only
+// after them are the operands of direction mu + 2 loaded into the buffer
+// direction mu freed.  An empty asm statement that takes K as read and
+// written, and hands back the site index the next loads are addressed with,
+// keeps the compiler from issuing those loads before the terms are done (the
+// fully unrolled star, left to the compiler, issues every load first: 181
+// VGPRs, 2 waves per SIMD).  The packed FP32 sweep takes 85 VGPRs this way
+// (5 waves per SIMD) and is about 10% faster at 32^4.  The terms are formed
+// in the same order as in the unrolled star, so the results are the same.
+template <typename V, typename R, bool NT, typename LK, int NDIR>
+__device__ __forceinline__ void star_short_pipelined(M3<R>& K, LK U, const V* __restrict__ gop,
+                                                     const Geom& geo, const Site& s, int i,
+                                                     int parity) {
+  const int S = geo.stride;
+  constexpr bool SC = std::is_same<LK, PackedLinks>::value && std::is_same<R, float>::value;
+  Raw<LK, R> lf[2], lb[2];
+  Raw<const V*, R> gf[2], gb[2];
+  auto load_dir = [&](int mu, int k) {
     const int jf = nb_fwd(geo, s, i, mu);
     const int jb = nb_bwd(geo, s, i, mu);
+    raw_load<R, NT>(U, parity * 4 + mu, S, i, lf[k]);
+    raw_load<R, false>(gop, S, jf, gf[k]);
+    raw_load<R, false>(gop, S, jb, gb[k]);
+    raw_load<R, NT>(U, (1 - parity) * 4 + mu, S, jb, lb[k]);
+  };
+  auto terms = [&](int k) {
+    Cx<R> a[2][3], b[3][3];
     if constexpr (SC)
-      link_rows<R, NT, true>(U, parity * 4 + mu, S, i, a[0], a[1]);
+      raw_rows<R, true>(lf[k], a[0], a[1]);
     else
-      link_rows<R, NT>(U, parity * 4 + mu, S, i, a[0], a[1]);
-    Store<V>::template load<R, false>(gop, S, jf, b[0], b[1]);
+      raw_rows<R>(lf[k], a[0], a[1]);
+    raw_rows<R>(gf[k], b[0], b[1]);
     short_term_fwd<R, SC>(K, a, b);
-    Store<V>::template load<R, false>(gop, S, jb, a[0], a[1]);
+    raw_rows<R>(gb[k], a[0], a[1]);
     if constexpr (SC)
-      link_rows<R, NT, true>(U, (1 - parity) * 4 + mu, S, jb, b[0], b[1]);
+      raw_rows<R, true>(lb[k], b[0], b[1]);
     else
-      link_rows<R, NT>(U, (1 - parity) * 4 + mu, S, jb, b[0], b[1]);
+      raw_rows<R>(lb[k], b[0], b[1]);
     short_term_bwd<R, SC>(K, a, b);
-  } else {
-    Raw<LK, R> lf[NC], lb[NC];
-    Raw<const V*, R> gf[NC], gb[NC];
+  };
+  load_dir(0, 0);
+  load_dir(1, 1);
 #pragma unroll
-    for (int k = 0; k < NC; k++) {
-      const int jf = nb_fwd(geo, s, i, mu + k);
-      const int jb = nb_bwd(geo, s, i, mu + k);
-      raw_load<R, NT>(U, parity * 4 + mu + k, S, i, lf[k]);
+  for (int mu = 0; mu < NDIR; mu++) {
+    terms(mu & 1);
+    if (mu + 2 < NDIR) {
+      // the terms of direction mu are complete before the loads of
+      // direction mu + 2 issue: the empty asm takes K as read and written
+      // and hands back the site index those loads are addressed with
+      int ii = i;
+      asm volatile(""
+                   : "+v"(ii), "+v"(K.e[0][0].re), "+v"(K.e[0][0].im), "+v"(K.e[0][1].re),
+                     "+v"(K.e[0][1].im), "+v"(K.e[0][2].re), "+v"(K.e[0][2].im),
+                     "+v"(K.e[1][0].re), "+v"(K.e[1][0].im), "+v"(K.e[1][1].re),
+                     "+v"(K.e[1][1].im), "+v"(K.e[1][2].re), "+v"(K.e[1][2].im),
+                     "+v"(K.e[2][0].re), "+v"(K.e[2][0].im), "+v"(K.e[2][1].re),
+                     "+v"(K.e[2][1].im), "+v"(K.e[2][2].re), "+v"(K.e[2][2].im));
+      const int z = ii - i;
+      const int jf = nb_fwd(geo, s, i, mu + 2) + z;
+      const int jb = nb_bwd(geo, s, i, mu + 2) + z;
+      const int k = mu & 1;
+      raw_load<R, NT>(U, parity * 4 + mu + 2, S, ii, lf[k]);
       raw_load<R, false>(gop, S, jf, gf[k]);
       raw_load<R, false>(gop, S, jb, gb[k]);
-      raw_load<R, NT>(U, (1 - parity) * 4 + mu + k, S, jb, lb[k]);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int k = 0; k < NC; k++) {
-      if constexpr (SC)
-        raw_rows<R, true>(lf[k], a[0], a[1]);
-      else
-        raw_rows<R>(lf[k], a[0], a[1]);
-      raw_rows<R>(gf[k], b[0], b[1]);
-      short_term_fwd<R, SC>(K, a, b);
-      raw_rows<R>(gb[k], a[0], a[1]);
-      if constexpr (SC)
-        raw_rows<R, true>(lb[k], b[0], b[1]);
-      else
-        raw_rows<R>(lb[k], b[0], b[1]);
-      short_term_bwd<R, SC>(K, a, b);
+      raw_load<R, NT>(U, (1 - parity) * 4 + mu + 2, S, jb, lb[k]);
     }
   }
 }

@@ -62,7 +62,9 @@ __device__ __forceinline__ void block_reduce3(double& a, double& b, double& c) {
 //   ROLLED the direction loop of the star is kept as a loop: one direction's
 //          operands are live at a time (FP64: 162 VGPRs, 3 waves/SIMD)
 //          instead of the fully unrolled form (more load-level parallelism
-//          per wave, 408 VGPRs and AGPRs, 1 wave/SIMD).
+//          per wave, 408 VGPRs and AGPRs, 1 wave/SIMD).  In FP32 the rolled
+//          variants run the software-pipelined star (star_short_pipelined:
+//          two directions' operands in flight, 85-105 VGPRs).
 // The FP64 sweep follows the reference's operation order.  The FP32 sweep
 // (mixed path, no bitwise target) is bound by its arithmetic as much as by
 // HBM, and uses a shorter but equal form: star terms from two rows of each
@@ -121,14 +123,11 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
     // each term is a product of two SU(3) matrices; form its rows 0,1 and
     // rebuild row 2 (U g^+ directly, U^+(x-mu) g^+(x-mu) as (g U)^+)
     if (ROLLED) {
-      // the FP32 rolled loop holds two directions' eight operands in flight
-      // together (FP32 registers are cheap); the batching form's FP64 one
-      // loads operand by operand
+      // FP32: the software-pipelined star (the next direction's operands in
+      // flight while a direction's terms are formed); the batching form's
+      // FP64 loop loads operand by operand
       if constexpr (std::is_same<R, float>::value) {
-#pragma unroll 1
-        for (int mu = 0; mu + 1 < NDIR; mu += 2)
-          star_terms_short<V, R, NT, LK, 2>(K, U, gop, geo, s, i, parity, mu);
-        if (NDIR & 1) star_terms_short<V, R, NT, LK, 1>(K, U, gop, geo, s, i, parity, NDIR - 1);
+        star_short_pipelined<V, R, NT, LK, NDIR>(K, U, gop, geo, s, i, parity);
       } else {
 #pragma unroll 1
         for (int mu = 0; mu < NDIR; mu++) star_terms_short<V, R, NT>(K, U, gop, geo, s, i, parity, mu);
