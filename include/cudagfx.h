@@ -205,12 +205,14 @@ int gfx_stream_copy(size_t bytes, int nrep, double *ms);
 /* Sweep-kernel autotuner: on the first sweep of a lattice in a precision and
  * gauge, every variant is timed on a scratch copy of g and the fastest is
  * kept.  Variant v (0..31): workgroup size 64 if bit 0 else 256, star loop
- * unrolled if bit 1 else rolled, non-temporal link loads if bit 2, and bits
- * 3-4 the workgroup order: 0 lexicographic, 1 and 2 chunks of 4 and 8
- * superrows (x2 rows holding whole workgroups) with x3 outermost, 3 chunks
- * in x1 as well (variants whose order does not fit the lattice are not
- * timed; the PACKED64 form has rolled variants only).  on = 0 keeps variant 0
- * or the one set by gfx_set_sweep_variant. */
+ * unrolled if bit 1 else rolled (in the FP32 forms: software-pipelined, the
+ * next direction's operands loaded while a direction's terms are formed),
+ * non-temporal link loads if bit 2, and bits 3-4 the workgroup order: 0
+ * lexicographic, 1 and 2 chunks of 4 and 8 superrows (x2 rows holding whole
+ * workgroups) with x3 outermost, 3 chunks in x1 as well (variants whose
+ * order does not fit the lattice are not timed; the PACKED64 and FIXED64
+ * forms have rolled variants only).  on = 0 keeps variant 0 or the one set
+ * by gfx_set_sweep_variant. */
 int gfx_set_autotune(int on);
 /* the chosen variant and the measured ms per half sweep of each of the 32
  * variants (0 when not timed) for precision GFX_PREC_FP64, GFX_PREC_MIXED
