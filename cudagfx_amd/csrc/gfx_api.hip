@@ -514,7 +514,7 @@ int tune_sweep(State& s, int form, int ndir) {
   int bestv = s.variant[pi][di];
   for (int v = 0; v < SWEEP_VARIANTS; v++) {
     if (!s.tiling_ok[block_index(sweep_variant_block(v))][sweep_variant_tiling(v)]) continue;
-    if ((form == FORM_P64 || form == FORM_X64) && !sweep_variant_rolled(v)) continue;
+    if (form == FORM_P64 && !sweep_variant_rolled(v)) continue;
     const int* map = sweep_map(s, v);
     for (int rep = 0; rep < 2; rep++)
       for (int p = 0; p < 2; p++) sweep(v, map, p);

@@ -691,16 +691,22 @@ void launch_sweep_packed_f64(const PackedArgs& p, double2* ge, double2* go, cons
 }
 
 // the FP64 polish on 32-bit fixed-point links and g (Landau: 4 directions;
-// Coulomb: 3, active slices only): rolled variants only (bit 1 of the
-// variant ignored; unrolled, the FP64 star spills)
+// Coulomb: 3, active slices only): bit 1 of the variant selects the star one
+// term ahead (the two operands of the next term load while a term is formed,
+// 164-166 VGPRs) instead of the rolled direction loop (the fully unrolled
+// FP64 star would spill)
 template <int NDIR, int CHK>
 static void launch_fx_chk(const SweepArgs<int4>& a, double c1, double c2, int variant,
                           hipStream_t st) {
-  switch (variant & 5) {
+  switch (variant & 7) {
     case 0: launch_sweep_v<int4, double, NDIR, 256, true, false, CHK>(a, c1, c2, st); break;
     case 1: launch_sweep_v<int4, double, NDIR, 64, true, false, CHK>(a, c1, c2, st); break;
+    case 2: launch_sweep_v<int4, double, NDIR, 256, false, false, CHK>(a, c1, c2, st); break;
+    case 3: launch_sweep_v<int4, double, NDIR, 64, false, false, CHK>(a, c1, c2, st); break;
     case 4: launch_sweep_v<int4, double, NDIR, 256, true, true, CHK>(a, c1, c2, st); break;
-    default: launch_sweep_v<int4, double, NDIR, 64, true, true, CHK>(a, c1, c2, st); break;
+    case 5: launch_sweep_v<int4, double, NDIR, 64, true, true, CHK>(a, c1, c2, st); break;
+    case 6: launch_sweep_v<int4, double, NDIR, 256, false, true, CHK>(a, c1, c2, st); break;
+    default: launch_sweep_v<int4, double, NDIR, 64, false, true, CHK>(a, c1, c2, st); break;
   }
 }
 template <int NDIR>
@@ -777,25 +783,17 @@ static void launch_bsweep_v(const BSweepArgs& a, R c1, R c2, hipStream_t st) {
 
 template <typename V, typename R, int CHK>
 static void launch_bsweep_chk(const BSweepArgs& a, R c1, R c2, int variant, hipStream_t st) {
-  if constexpr (std::is_same<V, int4>::value) {
-    // fixed-point fields: rolled variants only, as the single-lattice polish
-    switch (variant & 5) {
-      case 0: launch_bsweep_v<V, R, 256, true, false, CHK>(a, c1, c2, st); break;
-      case 1: launch_bsweep_v<V, R, 64, true, false, CHK>(a, c1, c2, st); break;
-      case 4: launch_bsweep_v<V, R, 256, true, true, CHK>(a, c1, c2, st); break;
-      default: launch_bsweep_v<V, R, 64, true, true, CHK>(a, c1, c2, st); break;
-    }
-  } else {
-    switch (variant & 7) {
-      case 0: launch_bsweep_v<V, R, 256, true, false, CHK>(a, c1, c2, st); break;
-      case 1: launch_bsweep_v<V, R, 64, true, false, CHK>(a, c1, c2, st); break;
-      case 2: launch_bsweep_v<V, R, 256, false, false, CHK>(a, c1, c2, st); break;
-      case 3: launch_bsweep_v<V, R, 64, false, false, CHK>(a, c1, c2, st); break;
-      case 4: launch_bsweep_v<V, R, 256, true, true, CHK>(a, c1, c2, st); break;
-      case 5: launch_bsweep_v<V, R, 64, true, true, CHK>(a, c1, c2, st); break;
-      case 6: launch_bsweep_v<V, R, 256, false, true, CHK>(a, c1, c2, st); break;
-      default: launch_bsweep_v<V, R, 64, false, true, CHK>(a, c1, c2, st); break;
-    }
+  // on fixed-point fields bit 1 selects the star one term ahead instead of
+  // the rolled direction loop, as in the single-lattice polish
+  switch (variant & 7) {
+    case 0: launch_bsweep_v<V, R, 256, true, false, CHK>(a, c1, c2, st); break;
+    case 1: launch_bsweep_v<V, R, 64, true, false, CHK>(a, c1, c2, st); break;
+    case 2: launch_bsweep_v<V, R, 256, false, false, CHK>(a, c1, c2, st); break;
+    case 3: launch_bsweep_v<V, R, 64, false, false, CHK>(a, c1, c2, st); break;
+    case 4: launch_bsweep_v<V, R, 256, true, true, CHK>(a, c1, c2, st); break;
+    case 5: launch_bsweep_v<V, R, 64, true, true, CHK>(a, c1, c2, st); break;
+    case 6: launch_bsweep_v<V, R, 256, false, true, CHK>(a, c1, c2, st); break;
+    default: launch_bsweep_v<V, R, 64, false, true, CHK>(a, c1, c2, st); break;
   }
 }
 

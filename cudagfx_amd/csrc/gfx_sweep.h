@@ -171,6 +171,43 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
       }
     }
     if constexpr (FXQ) fx_star_unscale(K);
+  } else if constexpr (std::is_same<V, int4>::value && std::is_same<LK, const int4*>::value) {
+    // fixed-point links and g, one star term ahead: the two operands of term
+    // t + 1 load while term t is formed (the terms in the rolled loop's
+    // order; star_short_pipelined explains the empty asm)
+    Raw<LK, R> lr[2];
+    Raw<const V*, R> gr[2];
+    auto load_term = [&](int t, int k, int z) {
+      const int mu = t >> 1;
+      if ((t & 1) == 0) {
+        raw_load<R, NT>(U, parity * 4 + mu, S, i + z, lr[k]);
+        raw_load<R, false>(gop, S, nb_fwd(geo, s, i, mu) + z, gr[k]);
+      } else {
+        const int jb = nb_bwd(geo, s, i, mu) + z;
+        raw_load<R, NT>(U, (1 - parity) * 4 + mu, S, jb, lr[k]);
+        raw_load<R, false>(gop, S, jb, gr[k]);
+      }
+    };
+    load_term(0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < 2 * NDIR; t++) {
+      const int k = t & 1;
+      if (t + 1 < 2 * NDIR) {
+        int ii = i;
+        asm volatile(""
+                     : "+v"(ii), "+v"(K.e[0][0].re), "+v"(K.e[0][0].im), "+v"(K.e[0][1].re),
+                       "+v"(K.e[0][1].im), "+v"(K.e[0][2].re), "+v"(K.e[0][2].im),
+                       "+v"(K.e[1][0].re), "+v"(K.e[1][0].im), "+v"(K.e[1][1].re),
+                       "+v"(K.e[1][1].im), "+v"(K.e[1][2].re), "+v"(K.e[1][2].im),
+                       "+v"(K.e[2][0].re), "+v"(K.e[2][0].im), "+v"(K.e[2][1].re),
+                       "+v"(K.e[2][1].im), "+v"(K.e[2][2].re), "+v"(K.e[2][2].im));
+        load_term(t + 1, 1 - k, ii - i);
+      }
+      const M3<R> A = k ? raw_m_dagger_q<R, true>(lr[k]) : raw_m_q<R>(lr[k]);
+      const M3<R> B = raw_m_dagger_q<R, false>(gr[k]);
+      m_addmul(K, A, B);
+    }
+    fx_star_unscale(K);
   } else {
 #pragma unroll
     for (int mu = 0; mu < NDIR; mu++) {
