@@ -162,6 +162,11 @@ __device__ __forceinline__ M3<R> star_of(const V* __restrict__ U, const V* __res
   // unrolled, these kernels spilled to scratch at 1 wave/SIMD), the four
   // operands of a direction loaded before any is decoded, products in the
   // short form (the stochastic updates have no bitwise target)
+  // (the software-pipelined star of the FP32 OR sweeps takes the batched
+  // anneal's FP32 kernels from 88 to 107-126 VGPRs, 4 instead of 5 waves per
+  // SIMD: with the links shared by the copies and served from L2 they are
+  // bound by their arithmetic, and the 4000-step anneal took 23.8 instead
+  // of 23.1-23.4 s)
 #pragma unroll 1
   for (int mu = 0; mu < NDIR; mu++)
     star_terms_short<V, R, NT, const V*, true>(K, U, gop, geo, s, i, parity, mu);
