@@ -518,14 +518,16 @@ int tune_sweep(State& s, int form, int ndir) {
     const int* map = sweep_map(s, v);
     for (int rep = 0; rep < 2; rep++)
       for (int p = 0; p < 2; p++) sweep(v, map, p);
+    // 6 timed sweeps: the fastest variants of a form lie within 2-3% of
+    // each other, about the spread of a 3-sweep timing
     hipEventRecord(s.ev0, s.stream);
-    for (int rep = 0; rep < 3; rep++)
+    for (int rep = 0; rep < 6; rep++)
       for (int p = 0; p < 2; p++) sweep(v, map, p);
     hipEventRecord(s.ev1, s.stream);
     if (hip_check(hipEventSynchronize(s.ev1), "tune")) break;
     float t = 0.f;
     hipEventElapsedTime(&t, s.ev0, s.ev1);
-    t /= 6.0f;
+    t /= 12.0f;
     s.tune_ms[pi][di][v] = t;
     if (t < best) {
       best = t;
