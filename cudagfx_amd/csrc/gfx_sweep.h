@@ -60,11 +60,13 @@ __device__ __forceinline__ void block_reduce3(double& a, double& b, double& c) {
 //   BLOCK  workgroup size (256 or 64)
 //   NT     non-temporal loads of the links
 //   ROLLED the direction loop of the star is kept as a loop: one direction's
-//          operands are live at a time (FP64: 162 VGPRs, 3 waves/SIMD)
-//          instead of the fully unrolled form (more load-level parallelism
-//          per wave, 408 VGPRs and AGPRs, 1 wave/SIMD).  In FP32 the rolled
-//          variants run the software-pipelined star (star_short_pipelined:
-//          two directions' operands in flight, 85-105 VGPRs).
+//          operands are live at a time (FP64: 162 VGPRs, 3 waves/SIMD).  The
+//          other variants: on FP64 and fixed-point links and g the star one
+//          term ahead (168 and 164 VGPRs, 3 waves/SIMD); in the FP32 forms
+//          the fully unrolled star (every operand in flight, 181 VGPRs, 2
+//          waves/SIMD).  In FP32 the rolled variants run the software-
+//          pipelined star (star_short_pipelined: two directions' operands in
+//          flight, 85-105 VGPRs).
 // The FP64 sweep follows the reference's operation order.  The FP32 sweep
 // (mixed path, no bitwise target) is bound by its arithmetic as much as by
 // HBM, and uses a shorter but equal form: star terms from two rows of each
@@ -171,10 +173,12 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
       }
     }
     if constexpr (FXQ) fx_star_unscale(K);
-  } else if constexpr (std::is_same<V, int4>::value && std::is_same<LK, const int4*>::value) {
-    // fixed-point links and g, one star term ahead: the two operands of term
-    // t + 1 load while term t is formed (the terms in the rolled loop's
-    // order; star_short_pipelined explains the empty asm)
+  } else if constexpr ((std::is_same<V, int4>::value || std::is_same<V, double2>::value) &&
+                       std::is_same<LK, const V*>::value) {
+    // fixed-point or FP64 links and g, one star term ahead: the two operands
+    // of term t + 1 load while term t is formed (the terms in the rolled
+    // loop's order; star_short_pipelined explains the empty asm)
+    constexpr bool FXQ = std::is_same<V, int4>::value;
     Raw<LK, R> lr[2];
     Raw<const V*, R> gr[2];
     auto load_term = [&](int t, int k, int z) {
@@ -203,11 +207,17 @@ __device__ __forceinline__ void sweep_site(LK U, const V* __restrict__ gop, cons
                        "+v"(K.e[2][1].im), "+v"(K.e[2][2].re), "+v"(K.e[2][2].im));
         load_term(t + 1, 1 - k, ii - i);
       }
-      const M3<R> A = k ? raw_m_dagger_q<R, true>(lr[k]) : raw_m_q<R>(lr[k]);
-      const M3<R> B = raw_m_dagger_q<R, false>(gr[k]);
-      m_addmul(K, A, B);
+      if constexpr (FXQ) {
+        const M3<R> A = k ? raw_m_dagger_q<R, true>(lr[k]) : raw_m_q<R>(lr[k]);
+        const M3<R> B = raw_m_dagger_q<R, false>(gr[k]);
+        m_addmul(K, A, B);
+      } else {
+        const M3<R> A = k ? raw_m_dagger<R>(lr[k]) : raw_m<R>(lr[k]);
+        const M3<R> B = raw_m_dagger<R>(gr[k]);
+        m_addmul(K, A, B);
+      }
     }
-    fx_star_unscale(K);
+    if constexpr (FXQ) fx_star_unscale(K);
   } else {
 #pragma unroll
     for (int mu = 0; mu < NDIR; mu++) {
